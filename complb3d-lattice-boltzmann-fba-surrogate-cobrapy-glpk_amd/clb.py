@@ -1,0 +1,454 @@
+"""Python binding of the C ABI in include/clb.h (libclb.so, built in this directory).
+
+This is the host-side mirror used by the tests and the benchmark. It loads the HIP
+library and nothing else: there is no CPU fallback, and every call that needs the GPU
+raises ClbError when the library or the device is missing.
+
+Names follow the reference's operator surface where one exists:
+  * ``EquilibriumChemistry`` mirrors the setters / calculate_species_concentrations of
+    EquilibriumChemistry<T> (src/complab3d_processors_part4_eqsolver.hh:186-1214), solved
+    by the GPU Newton kernel.
+  * ``Lattice.step`` runs the reference's per-iteration order (src/complab.cpp:1252-1849).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libclb.so")
+
+CLB_OK = 0
+PREC_FP64, PREC_FP32 = 0, 1
+KIND_NONE, KIND_BB, KIND_FLUID = 0, 1, 2
+BC_DIRICHLET, BC_NEUMANN = 0, 1
+KINETICS_NONE, KINETICS_PLUGIN = 0, 1
+WARM_REF, WARM_USER, WARM_MLP = 0, 1, 2
+
+
+class ClbError(RuntimeError):
+    pass
+
+
+class _Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int), ("nx", ctypes.c_int),
+                ("ny", ctypes.c_int), ("nz", ctypes.c_int)]
+
+
+class Perf(ctypes.Structure):
+    _fields_ = [("flow_ms", ctypes.c_double), ("flow_launches", ctypes.c_longlong),
+                ("ade_ms", ctypes.c_double), ("ade_launches", ctypes.c_longlong),
+                ("spec_ms", ctypes.c_double), ("spec_launches", ctypes.c_longlong),
+                ("mlp_ms", ctypes.c_double), ("mlp_launches", ctypes.c_longlong),
+                ("fluid_nodes", ctypes.c_longlong), ("reactive_nodes", ctypes.c_longlong)]
+
+
+_lib = None
+
+
+def load():
+    """Load libclb.so (raises ClbError if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ClbError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build() or make)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    c_int, c_double, c_long, c_u64 = ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_uint64
+    vp = ctypes.c_void_p
+    sig = {
+        "clb_create": [P(_Config), P(vp)],
+        "clb_destroy": [vp],
+        "clb_sync": [vp],
+        "clb_device_name": [ctypes.c_char_p, c_int],
+        "clb_upload_mask": [vp, vp],
+        "clb_download_mask": [vp, vp],
+        "clb_set_dynamics": [vp, vp, vp],
+        "clb_flow_init": [vp, c_double, c_double],
+        "clb_flow_steps": [vp, c_int, vp],
+        "clb_flow_download_velocity": [vp, vp, vp],
+        "clb_set_solutes": [vp, c_int, vp, vp, vp],
+        "clb_solutes_init": [vp, vp],
+        "clb_couple_velocity": [vp],
+        "clb_set_velocity": [vp, vp],
+        "clb_set_biomass": [vp, c_int, vp],
+        "clb_download_conc": [vp, vp],
+        "clb_download_biomass": [vp, vp],
+        "clb_set_kinetics": [vp, c_int, c_double, c_int, c_int],
+        "clb_set_tableau": [vp, c_int, c_int, vp, vp, vp, c_double, c_int],
+        "clb_set_mask_update": [vp, c_int, c_double, c_int, c_int],
+        "clb_step": [vp, c_int],
+        "clb_speciation_stats": [vp, vp],
+        "clb_batch_load": [vp, c_long, vp, c_int, vp, vp],
+        "clb_batch_solve": [vp, c_int, vp],
+        "clb_batch_fetch": [vp, vp, vp, vp, vp],
+        "clb_mlp_config": [vp, c_int, c_int, c_int],
+        "clb_mlp_set_params": [vp, vp],
+        "clb_mlp_get_params": [vp, vp],
+        "clb_mlp_set_scaling": [vp, vp, vp, vp, vp],
+        "clb_mlp_forward": [vp, c_long, vp, vp, c_int],
+        "clb_mlp_grad": [vp, c_long, vp, vp, vp, vp],
+        "clb_mlp_train": [vp, c_long, vp, vp, c_int, c_int, c_double, c_u64, vp],
+        "clb_mlp_train_from_batch": [vp, c_int, c_int, c_double, c_u64, vp],
+        "clb_set_timing": [vp, c_int],
+        "clb_get_perf": [vp, P(Perf)],
+        "clb_reset_perf": [vp],
+        "clb_gen_sphere_pack": [c_int, c_int, c_int, c_double, c_double, c_double, c_u64, c_int, c_int, c_int, vp],
+        "clb_gen_tableau": [c_int, c_int, c_double, c_double, c_int, c_int, c_u64, c_double, c_double, vp, vp],
+        "clb_gen_totals_field": [c_int, c_int, c_int, c_int, c_double, c_double, c_double, c_u64, vp],
+        "clb_gen_biomass_seed": [c_long, vp, c_int, c_double, c_double, c_double, c_u64, c_u64, vp],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = c_int
+    lib.clb_last_error.argtypes = [vp]
+    lib.clb_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names the header declares, for the ABI test."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "clb.h")
+    with open(hdr) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(clb_[a-z0-9_]+)\s*\(", text)))
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------------ generators (host)
+def gen_sphere_pack(nx, ny, nz, rmin=3.0, rmax=5.0, porosity=0.45, seed=0, pore=2, solid=0, bb=1):
+    lib = load()
+    m = np.zeros(nx * ny * nz, dtype=np.uint8)
+    rc = lib.clb_gen_sphere_pack(nx, ny, nz, rmin, rmax, porosity, seed, pore, solid, bb, _ptr(m))
+    if rc != CLB_OK:
+        raise ClbError(f"clb_gen_sphere_pack failed ({rc})")
+    return m
+
+
+def gen_tableau(nc, ns, logk_min=-14.0, logk_max=14.0, smin=-2, smax=2, seed=3, log_t_min=-6.0, log_t_max=-2.0):
+    lib = load()
+    S = np.zeros(ns * nc)
+    lk = np.zeros(ns)
+    rc = lib.clb_gen_tableau(nc, ns, logk_min, logk_max, smin, smax, seed, log_t_min, log_t_max, _ptr(S), _ptr(lk))
+    if rc != CLB_OK:
+        raise ClbError(f"clb_gen_tableau failed ({rc})")
+    return S.reshape(ns, nc), lk
+
+
+def gen_totals_field(nx, ny, nz, nc, sigma=4.0, log_t_min=-6.0, log_t_max=-2.0, seed=4):
+    lib = load()
+    t = np.zeros(nx * ny * nz * nc)
+    rc = lib.clb_gen_totals_field(nx, ny, nz, nc, sigma, log_t_min, log_t_max, seed, _ptr(t))
+    if rc != CLB_OK:
+        raise ClbError(f"clb_gen_totals_field failed ({rc})")
+    return t.reshape(-1, nc)
+
+
+def gen_biomass_seed(mask, pore=2, fraction=0.01, bmin=10.0, bmax=100.0, seed_sites=1, seed_density=2):
+    lib = load()
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    b = np.zeros(m.size)
+    rc = lib.clb_gen_biomass_seed(m.size, _ptr(m), pore, fraction, bmin, bmax, seed_sites, seed_density, _ptr(b))
+    if rc != CLB_OK:
+        raise ClbError(f"clb_gen_biomass_seed failed ({rc})")
+    return b
+
+
+def device_name():
+    lib = load()
+    buf = ctypes.create_string_buffer(256)
+    if lib.clb_device_name(buf, 256) != CLB_OK:
+        raise ClbError("no HIP device")
+    return buf.value.decode()
+
+
+# ------------------------------------------------------------------ context
+class Lattice:
+    """One GPU's lattice: mask, D3Q19 flow, D3Q7 solutes, reactions, batched speciation."""
+
+    def __init__(self, nx, ny, nz, precision=PREC_FP64, device=0):
+        self.lib = load()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.N = nx * ny * nz
+        cfg = _Config(device, precision, nx, ny, nz)
+        h = ctypes.c_void_p()
+        rc = self.lib.clb_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != CLB_OK:
+            raise ClbError(f"clb_create failed ({rc}): is a GPU visible and the library built for it?")
+        self.h = h
+        self.S = 0
+        self.nb = 0
+        self.nc = 0
+
+    def close(self):
+        if self.h:
+            self.lib.clb_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != CLB_OK:
+            msg = self.lib.clb_last_error(self.h)
+            raise ClbError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # geometry / dynamics
+    def upload_mask(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+        assert m.size == self.N
+        self._chk(self.lib.clb_upload_mask(self.h, _ptr(m)), "upload_mask")
+
+    def download_mask(self):
+        m = np.zeros(self.N, dtype=np.uint8)
+        self._chk(self.lib.clb_download_mask(self.h, _ptr(m)), "download_mask")
+        return m
+
+    def set_dynamics(self, kind, flow_omega):
+        k = np.ascontiguousarray(kind, dtype=np.int32)
+        o = _f64(flow_omega)
+        assert k.size == 256 and o.size == 256
+        self._chk(self.lib.clb_set_dynamics(self.h, _ptr(k), _ptr(o)), "set_dynamics")
+
+    # flow
+    def flow_init(self, rho_in, rho_out):
+        self._chk(self.lib.clb_flow_init(self.h, rho_in, rho_out), "flow_init")
+
+    def flow_steps(self, n, energy=False):
+        e = np.zeros(n) if energy else None
+        self._chk(self.lib.clb_flow_steps(self.h, n, _ptr(e)), "flow_steps")
+        return e
+
+    def velocity(self):
+        u = np.zeros(3 * self.N)
+        r = np.zeros(self.N)
+        self._chk(self.lib.clb_flow_download_velocity(self.h, _ptr(u), _ptr(r)), "velocity")
+        return u.reshape(self.N, 3), r
+
+    # solutes
+    def set_solutes(self, omega, bc_type, bc_value):
+        o = _f64(omega).ravel()
+        S = o.size // 256
+        bt = np.ascontiguousarray(bc_type, dtype=np.int32).ravel()
+        bv = _f64(bc_value).ravel()
+        assert bt.size == 2 * S and bv.size == 2 * S
+        self._chk(self.lib.clb_set_solutes(self.h, S, _ptr(o), _ptr(bt), _ptr(bv)), "set_solutes")
+        self.S = S
+
+    def solutes_init(self, c0):
+        c = _f64(c0)
+        self._chk(self.lib.clb_solutes_init(self.h, _ptr(c)), "solutes_init")
+
+    def couple_velocity(self):
+        self._chk(self.lib.clb_couple_velocity(self.h), "couple_velocity")
+
+    def set_velocity(self, u):
+        uu = _f64(u).ravel()
+        assert uu.size == 3 * self.N
+        self._chk(self.lib.clb_set_velocity(self.h, _ptr(uu)), "set_velocity")
+
+    def set_biomass(self, b):
+        bb = _f64(b).ravel()
+        nb = bb.size // self.N
+        self._chk(self.lib.clb_set_biomass(self.h, nb, _ptr(bb)), "set_biomass")
+        self.nb = nb
+
+    def conc(self):
+        c = np.zeros(self.S * self.N)
+        self._chk(self.lib.clb_download_conc(self.h, _ptr(c)), "download_conc")
+        return c.reshape(self.S, self.N)
+
+    def biomass(self):
+        b = np.zeros(max(self.nb, 1) * self.N)
+        self._chk(self.lib.clb_download_biomass(self.h, _ptr(b)), "download_biomass")
+        return b.reshape(-1, self.N)[: self.nb]
+
+    def set_kinetics(self, kind, dt, solid_code=0, bb_code=1):
+        self._chk(self.lib.clb_set_kinetics(self.h, kind, dt, solid_code, bb_code), "set_kinetics")
+
+    def set_tableau(self, S, logK, comp_species=None, tol=1e-10, max_iter=200):
+        S = _f64(S)
+        ns, nc = S.shape
+        lk = _f64(logK)
+        cs = None if comp_species is None else np.ascontiguousarray(comp_species, dtype=np.int32)
+        self._chk(self.lib.clb_set_tableau(self.h, nc, ns, _ptr(S.ravel()), _ptr(lk), _ptr(cs), tol, max_iter),
+                  "set_tableau")
+        self.nc = nc
+
+    def set_mask_update(self, enable, threshold, pore_code=2, biofilm_code=3):
+        self._chk(self.lib.clb_set_mask_update(self.h, int(enable), threshold, pore_code, biofilm_code),
+                  "set_mask_update")
+
+    def step(self, n=1):
+        self._chk(self.lib.clb_step(self.h, n), "step")
+
+    def speciation_stats(self):
+        s = np.zeros(3, dtype=np.int64)
+        self._chk(self.lib.clb_speciation_stats(self.h, _ptr(s)), "speciation_stats")
+        return s
+
+    # batched speciation
+    def batch_load(self, totals, var_species=None, logk_var=None):
+        t = _f64(totals)
+        n = t.shape[0]
+        nvar = 0 if var_species is None else len(var_species)
+        vs = None if nvar == 0 else np.ascontiguousarray(var_species, dtype=np.int32)
+        lk = None if nvar == 0 else _f64(logk_var)
+        self._chk(self.lib.clb_batch_load(self.h, n, _ptr(t.ravel()), nvar, _ptr(vs),
+                                          None if lk is None else _ptr(lk.ravel())), "batch_load")
+        self.bn = n
+
+    def batch_solve(self, warm=WARM_REF, logc_in=None):
+        lc = None if logc_in is None else _f64(logc_in).ravel()
+        self._chk(self.lib.clb_batch_solve(self.h, warm, _ptr(lc)), "batch_solve")
+
+    def batch_fetch(self):
+        n, nc = self.bn, self.nc
+        lc = np.zeros(n * nc)
+        it = np.zeros(n, dtype=np.int32)
+        r = np.zeros(n)
+        cv = np.zeros(n, dtype=np.uint8)
+        self._chk(self.lib.clb_batch_fetch(self.h, _ptr(lc), _ptr(it), _ptr(r), _ptr(cv)), "batch_fetch")
+        return lc.reshape(n, nc), it, r, cv
+
+    # MLP
+    def mlp_config(self, din, hidden, dout):
+        self._chk(self.lib.clb_mlp_config(self.h, din, hidden, dout), "mlp_config")
+        self.mlp_dims = (din, hidden, dout)
+
+    def mlp_nparams(self):
+        din, H, dout = self.mlp_dims
+        return H * din + H + H * H + H + dout * H + dout
+
+    def mlp_set_params(self, theta):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        assert t.size == self.mlp_nparams()
+        self._chk(self.lib.clb_mlp_set_params(self.h, _ptr(t)), "mlp_set_params")
+
+    def mlp_get_params(self):
+        t = np.zeros(self.mlp_nparams(), dtype=np.float32)
+        self._chk(self.lib.clb_mlp_get_params(self.h, _ptr(t)), "mlp_get_params")
+        return t
+
+    def mlp_set_scaling(self, xmin, xmax, ymin, ymax):
+        a = [_f64(v) for v in (xmin, xmax, ymin, ymax)]
+        self._chk(self.lib.clb_mlp_set_scaling(self.h, *[_ptr(v) for v in a]), "mlp_set_scaling")
+
+    def mlp_forward(self, X, bf16=False):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n = X.shape[0]
+        Y = np.zeros((n, self.mlp_dims[2]), dtype=np.float32)
+        self._chk(self.lib.clb_mlp_forward(self.h, n, _ptr(X), _ptr(Y), int(bf16)), "mlp_forward")
+        return Y
+
+    def mlp_grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        g = np.zeros(self.mlp_nparams(), dtype=np.float32)
+        loss = ctypes.c_double()
+        self._chk(self.lib.clb_mlp_grad(self.h, X.shape[0], _ptr(X), _ptr(T), _ptr(g), ctypes.byref(loss)),
+                  "mlp_grad")
+        return g, loss.value
+
+    def mlp_train(self, X, T, epochs, batch, lr, seed=7):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        T = np.ascontiguousarray(T, dtype=np.float32)
+        loss = ctypes.c_double()
+        self._chk(self.lib.clb_mlp_train(self.h, X.shape[0], _ptr(X), _ptr(T), epochs, batch, lr, seed,
+                                         ctypes.byref(loss)), "mlp_train")
+        return loss.value
+
+    def mlp_train_from_batch(self, epochs, batch, lr, seed=7):
+        loss = ctypes.c_double()
+        self._chk(self.lib.clb_mlp_train_from_batch(self.h, epochs, batch, lr, seed, ctypes.byref(loss)),
+                  "mlp_train_from_batch")
+        return loss.value
+
+    # measurement
+    def set_timing(self, on):
+        self._chk(self.lib.clb_set_timing(self.h, int(on)), "set_timing")
+
+    def perf(self):
+        p = Perf()
+        self._chk(self.lib.clb_get_perf(self.h, ctypes.byref(p)), "get_perf")
+        return p
+
+    def reset_perf(self):
+        self._chk(self.lib.clb_reset_perf(self.h), "reset_perf")
+
+    def sync(self):
+        self._chk(self.lib.clb_sync(self.h), "sync")
+
+
+class EquilibriumChemistry:
+    """GPU-backed mirror of EquilibriumChemistry<T>
+    (src/complab3d_processors_part4_eqsolver.hh:186-1271): same setters, same defaults
+    (maxIterations 200, tolerance 1e-8), same convergence judge (PCF residual 2-norm <
+    tolerance); calculate_species_concentrations takes a batch of species vectors."""
+
+    def __init__(self, lattice=None):
+        self.lat = lattice if lattice is not None else Lattice(4, 1, 1)
+        self.species = []
+        self.components = []
+        self.logK = None
+        self.S = None
+        self.maxIterations = 200
+        self.tolerance = 1e-8
+        self.lastIterations = None
+        self.lastResidual = None
+        self.lastConverged = None
+
+    def setSpeciesNames(self, names):
+        self.species = [n for n in names if n not in ("H2O", "h2o")]
+
+    def setComponentNames(self, names):
+        self.components = list(names)
+
+    def setLogK(self, logK):
+        self.logK = np.asarray(logK, dtype=np.float64)
+
+    def setStoichiometryMatrix(self, S):
+        self.S = np.asarray(S, dtype=np.float64)
+
+    def setMaxIterations(self, n):
+        self.maxIterations = int(n)
+
+    def setTolerance(self, tol):
+        self.tolerance = float(tol)
+
+    def _comp_species(self):
+        return [self.species.index(c) if c in self.species else -1 for c in self.components]
+
+    def calculate_species_concentrations(self, conc):
+        """conc: (n, ns) species concentrations -> (n, ns) equilibrium concentrations."""
+        conc = np.atleast_2d(np.asarray(conc, dtype=np.float64))
+        S, lk = self.S, self.logK
+        ns, nc = S.shape
+        eq = np.array([np.any(np.abs(S[i]) > 1e-10) for i in range(ns)])
+        cs = np.clip(conc, 1e-30, 10.0)
+        T = np.maximum(cs[:, eq] @ S[eq], 1e-30)
+        guess = np.empty((conc.shape[0], nc))
+        for j, sp in enumerate(self._comp_species()):
+            guess[:, j] = np.log10(cs[:, sp]) if sp >= 0 else -7.0
+        self.lat.set_tableau(S, lk, self._comp_species(), self.tolerance, self.maxIterations)
+        self.lat.batch_load(T)
+        self.lat.batch_solve(WARM_USER, guess)
+        x, it, r, cv = self.lat.batch_fetch()
+        self.lastIterations, self.lastResidual, self.lastConverged = it, r, cv.astype(bool)
+        lc = np.clip(lk[None, :] + x @ S.T, -30.0, 1.0)
+        out = np.where(eq[None, :], 10.0 ** lc, np.where(conc > 1e-30, np.minimum(conc, 10.0), 1e-30))
+        return np.clip(out, 1e-30, 10.0)

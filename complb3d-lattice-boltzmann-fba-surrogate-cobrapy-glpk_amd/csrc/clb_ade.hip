@@ -1,0 +1,312 @@
+// clb_ade.hip -- D3Q7 advection-diffusion for every solute, fused with the per-voxel
+// kinetics, in one pass over the populations.
+//
+// Replaces, per step of the loop at src/complab.cpp:1152-1860:
+//   * Palabos AdvectionDiffusionBGKdynamics collide (:1252) and stream (:1849) of every
+//     mobile solute, with Dirichlet / Neumann x faces (soluteDomainSetup,
+//     src/complab_functions.hh:296-340) and bounce-back / no-dynamics nodes;
+//   * dC = dC0 resets and run_kinetics + update_rxnLattices
+//     (src/complab3d_processors_part1.hh:72-127, 345-415), which call the compiled
+//     defineRxnKinetics hook per pore voxel;
+//   * the biomass threshold reclassification of updateLocalMaskNtotalLattices3D
+//     (src/complab3d_processors_part2.hh:99-210, single pore type).
+// All solutes of a node are read once, collided, reacted and written once (AA pattern).
+// Populations are stored as g_i = f_i - t_i * c_ref[s].
+//
+// HBM traffic per fluid node and step: S*7 populations read + written, 3 velocity
+// components, 1 mask byte (+ biomass read/write and the post-kinetics concentrations
+// when kinetics / speciation are on).
+#include "clb_kinetics_device.h"
+#include "clb_lattice.h"
+
+namespace clb {
+
+namespace {
+
+template <typename R, int S, bool ODD>
+__device__ __forceinline__ void load7(const R* __restrict__ g, long long N, const Nbr& nb, int n, R* p) {
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      if (!ODD) {
+        p[s * 7 + i] = g[(long long)(s * 7 + i) * N + n];
+      } else {
+        if (nb.xok(-Q7::cx[i]))
+          p[s * 7 + i] = g[(long long)(s * 7 + opp_dir(i)) * N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])];
+        else
+          p[s * 7 + i] = R(0);
+      }
+    }
+}
+
+template <typename R, int S, bool ODD>
+__device__ __forceinline__ void store7(R* __restrict__ g, long long N, const Nbr& nb, int n, const R* p) {
+#pragma unroll
+  for (int s = 0; s < S; ++s)
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      if (!ODD) {
+        g[(long long)(s * 7 + opp_dir(i)) * N + n] = p[s * 7 + i];
+      } else {
+        if (nb.xok(Q7::cx[i])) g[(long long)(s * 7 + i) * N + nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])] = p[s * 7 + i];
+      }
+    }
+}
+
+// Collide (with boundary completion), kinetics, optional post-kinetics concentration
+// output and biomass threshold rule for one fluid node. rho_bc[s] is the Neumann density
+// (interior neighbour's pre-collision density) for boundary nodes.
+template <typename R, int S, bool KIN, bool CBUF>
+__device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int n, uint8_t code, R* p, const R* rho_bc) {
+  const Geom& g = a.geo;
+  if (x == 0 || x == g.nx - 1) {
+    const int side = (x == 0) ? 0 : 1;
+    const int miss = (x == 0) ? 1 : 2;  // population entering the domain
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const R cb = (a.bc_type[2 * s + side] == CLB_BC_DIRICHLET) ? a.bc_value[2 * s + side] : rho_bc[s];
+      R others = 0;
+#pragma unroll
+      for (int i = 0; i < 7; ++i)
+        if (i != miss) others += p[s * 7 + i];
+      p[s * 7 + miss] = (cb - a.c_ref[s]) - others;
+    }
+  }
+  const R ux = a.u[n], uy = a.u[g.N + n], uz = a.u[2 * g.N + n];
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    R rs = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) rs += p[s * 7 + i];
+    const R rho = rs + a.c_ref[s];
+    const R om = a.omega[s * 256 + code];
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const R cu = R(Q7::cx[i]) * ux + R(Q7::cy[i]) * uy + R(Q7::cz[i]) * uz;
+      const R eq = R(Q7::w(i)) * rs + R(4.0 * Q7::w(i)) * rho * cu;
+      p[s * 7 + i] += om * (eq - p[s * 7 + i]);
+    }
+  }
+  const bool react = (x > 0 && x < g.nx - 1 && code != a.solid_code && code != a.bb_code);
+  if (KIN && react) {
+    KVec C(S, 0.0), Bv(a.nb, 0.0), sr(S, 0.0), br(a.nb, 0.0);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      R rs = 0;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) rs += p[s * 7 + i];
+      double c0 = (double)rs + (double)a.c_ref[s];
+      if (c0 < 1e-12) c0 = 0;
+      C[s] = c0;
+    }
+    for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
+      if (b < a.nb) {
+        double b0 = (double)a.bmass[(long long)b * g.N + n];
+        if (b0 < 1e-12) b0 = 0;
+        Bv[b] = b0;
+      }
+    defineRxnKinetics(Bv, C, sr, br, (plb::plint)code);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const double dC = sr[s] * a.dt;
+      if (dC > 1e-12 || dC < -1e-12) {
+#pragma unroll
+        for (int i = 0; i < 7; ++i) p[s * 7 + i] += (R)(dC * Q7::w(i));
+      }
+    }
+    for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
+      if (b < a.nb) {
+        const double dB = br[b] * a.dt;
+        if (dB > 1e-12 || dB < -1e-12) {
+          R* bp = &a.bmass[(long long)b * g.N + n];
+          *bp = (R)((double)*bp + dB);
+        }
+      }
+  }
+  if (react && a.mask_update && a.nb > 0) {
+    double btot = 0;
+    for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
+      if (b < a.nb) {
+        const double bv = (double)a.bmass[(long long)b * g.N + n];
+        if (bv > 1e-14) btot += bv;
+      }
+    if (code == a.pore_code && btot >= a.bthreshold) a.mask[n] = (uint8_t)a.biofilm_code;
+    else if (code == a.biofilm_code && btot < a.bthreshold) a.mask[n] = (uint8_t)a.pore_code;
+  }
+  if (CBUF && react) {
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      R rs = 0;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) rs += p[s * 7 + i];
+      a.cbuf[(long long)s * g.N + n] = (double)rs + (double)a.c_ref[s];
+    }
+  }
+}
+
+// One thread per (x, y, z) with 1 <= x <= nx-2. The thread of x = 1 also updates the
+// inlet node x = 0 and the thread of x = nx-2 the outlet node x = nx-1, after it has taken
+// the interior node's pre-collision density, which the Neumann face needs (the density
+// FlatAdiabaticBoundaryFunctional3D copies after the previous stream).
+template <typename R, int S, bool ODD, bool KIN, bool CBUF>
+__global__ void __launch_bounds__(256) ade_step_kernel(AdeArgs<R> a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  const Geom& g = a.geo;
+  if (x < 1 || x > g.nx - 2 || y >= g.ny) return;
+  const int n = x + g.nx * (y + g.ny * z);
+  const uint8_t code = a.mask[n];
+  const bool fluid = a.kind[code] == CLB_KIND_FLUID;
+  R rho_in[S];
+  if (fluid) {
+    const Nbr nb(x, y, z, g);
+    R p[S * 7];
+    load7<R, S, ODD>(a.g, g.N, nb, n, p);
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      R rs = 0;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) rs += p[s * 7 + i];
+      rho_in[s] = rs + a.c_ref[s];
+    }
+    ade_node<R, S, KIN, CBUF>(a, x, n, code, p, nullptr);
+    store7<R, S, ODD>(a.g, g.N, nb, n, p);
+  }
+  if (x == 1 || x == g.nx - 2) {
+    const int xb = (x == 1) ? 0 : g.nx - 1;
+    const int nbn = xb + g.nx * (y + g.ny * z);
+    const uint8_t cb = a.mask[nbn];
+    if (a.kind[cb] == CLB_KIND_FLUID) {
+      const Nbr nb(xb, y, z, g);
+      R q[S * 7];
+      load7<R, S, ODD>(a.g, g.N, nb, nbn, q);
+      if (!fluid) {
+        // interior neighbour is a wall: the Neumann face keeps its own density
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+          R rs = 0;
+#pragma unroll
+          for (int i = 0; i < 7; ++i) rs += q[s * 7 + i];
+          rho_in[s] = rs + a.c_ref[s];
+        }
+      }
+      ade_node<R, S, false, false>(a, xb, nbn, cb, q, rho_in);
+      store7<R, S, ODD>(a.g, g.N, nb, nbn, q);
+    }
+  }
+}
+
+template <typename R>
+__global__ void ade_init_kernel(R* g, Geom geo, int S, const double* c0, const R* c_ref) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= geo.N) return;
+  for (int s = 0; s < S; ++s)
+#pragma unroll
+    for (int i = 0; i < 7; ++i) g[(long long)(s * 7 + i) * geo.N + n] = (R)(Q7::w(i) * (c0[s] - (double)c_ref[s]));
+}
+
+template <typename R, bool ODD>
+__device__ double node_density(const AdeArgs<R>& a, int s, int x, int y, int z) {
+  const Geom& g = a.geo;
+  const Nbr nb(x, y, z, g);
+  const int n = x + g.nx * (y + g.ny * z);
+  double r = 0;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    R v;
+    if (!ODD) v = a.g[(long long)(s * 7 + i) * g.N + n];
+    else v = nb.xok(-Q7::cx[i]) ? a.g[(long long)(s * 7 + opp_dir(i)) * g.N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])]
+                                : R(0);
+    r += (double)v;
+  }
+  return r + (double)a.c_ref[s];
+}
+
+// Concentration of every node (sum of the pre-collision populations of the next step).
+// Fluid nodes of the x faces report what their boundary prescribes: the Dirichlet value,
+// or for a Neumann face the density of the interior neighbour (its own when that is a
+// wall), since their entering population is only rebuilt at the next collision.
+template <typename R, bool ODD>
+__global__ void ade_conc_kernel(AdeArgs<R> a, double* c) {
+  const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const Geom& g = a.geo;
+  if (n64 >= g.N) return;
+  const int n = (int)n64;
+  const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+  const bool face = (x == 0 || x == g.nx - 1) && a.kind[a.mask[n]] == CLB_KIND_FLUID;
+  for (int s = 0; s < a.S; ++s) {
+    double r;
+    if (!face) {
+      r = node_density<R, ODD>(a, s, x, y, z);
+    } else {
+      const int side = (x == 0) ? 0 : 1;
+      if (a.bc_type[2 * s + side] == CLB_BC_DIRICHLET) {
+        r = (double)a.bc_value[2 * s + side];
+      } else {
+        const int xi = (x == 0) ? 1 : g.nx - 2;
+        const bool inner_fluid = a.kind[a.mask[xi + g.nx * (y + g.ny * z)]] == CLB_KIND_FLUID;
+        r = node_density<R, ODD>(a, s, inner_fluid ? xi : x, y, z);
+      }
+    }
+    c[(long long)s * g.N + n] = r;
+  }
+}
+
+template <typename R, int S>
+void launch_ade_step_s(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t st) {
+  dim3 grid((a.geo.nx + block.x - 1) / block.x, (a.geo.ny + block.y - 1) / block.y, a.geo.nz);
+  const bool kin = a.kinetics != 0;
+  const bool cb = a.cbuf != nullptr;
+#define CLB_ADE_LAUNCH(O, K, C) hipLaunchKernelGGL((ade_step_kernel<R, S, O, K, C>), grid, block, 0, st, a)
+  if (odd) {
+    if (kin) { if (cb) CLB_ADE_LAUNCH(true, true, true); else CLB_ADE_LAUNCH(true, true, false); }
+    else { if (cb) CLB_ADE_LAUNCH(true, false, true); else CLB_ADE_LAUNCH(true, false, false); }
+  } else {
+    if (kin) { if (cb) CLB_ADE_LAUNCH(false, true, true); else CLB_ADE_LAUNCH(false, true, false); }
+    else { if (cb) CLB_ADE_LAUNCH(false, false, true); else CLB_ADE_LAUNCH(false, false, false); }
+  }
+#undef CLB_ADE_LAUNCH
+}
+
+}  // namespace
+
+template <typename R>
+void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c_ref, hipStream_t s) {
+  const int bs = 256;
+  hipLaunchKernelGGL(ade_init_kernel<R>, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, gpop, g, S, c0,
+                     c_ref);
+}
+
+template <typename R>
+void launch_ade_step(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t s) {
+  switch (a.S) {
+    case 1: launch_ade_step_s<R, 1>(a, odd, block, s); break;
+    case 2: launch_ade_step_s<R, 2>(a, odd, block, s); break;
+    case 3: launch_ade_step_s<R, 3>(a, odd, block, s); break;
+    case 4: launch_ade_step_s<R, 4>(a, odd, block, s); break;
+    case 5: launch_ade_step_s<R, 5>(a, odd, block, s); break;
+    case 6: launch_ade_step_s<R, 6>(a, odd, block, s); break;
+    case 7: launch_ade_step_s<R, 7>(a, odd, block, s); break;
+    case 8: launch_ade_step_s<R, 8>(a, odd, block, s); break;
+    default: break;
+  }
+}
+
+template <typename R>
+void launch_ade_conc(const AdeArgs<R>& a, bool odd_next, double* c, hipStream_t s) {
+  const int bs = 256;
+  dim3 grid((unsigned)((a.geo.N + bs - 1) / bs));
+  if (odd_next) hipLaunchKernelGGL((ade_conc_kernel<R, true>), grid, dim3(bs), 0, s, a, c);
+  else hipLaunchKernelGGL((ade_conc_kernel<R, false>), grid, dim3(bs), 0, s, a, c);
+}
+
+template void launch_ade_init<float>(const Geom&, int, float*, const double*, const float*, hipStream_t);
+template void launch_ade_init<double>(const Geom&, int, double*, const double*, const double*, hipStream_t);
+template void launch_ade_step<float>(const AdeArgs<float>&, bool, dim3, hipStream_t);
+template void launch_ade_step<double>(const AdeArgs<double>&, bool, dim3, hipStream_t);
+template void launch_ade_conc<float>(const AdeArgs<float>&, bool, double*, hipStream_t);
+template void launch_ade_conc<double>(const AdeArgs<double>&, bool, double*, hipStream_t);
+
+}  // namespace clb

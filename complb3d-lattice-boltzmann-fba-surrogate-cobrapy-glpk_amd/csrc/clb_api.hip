@@ -1,0 +1,1115 @@
+// clb_api.hip -- the C ABI: context, device memory, step sequencing, measurement.
+//
+// The context owns what the reference's main() owns for the hot path: the flow lattice,
+// the solute lattices with their increment state, the mask, the speciation solver
+// settings (src/complab.cpp:592, 748-829, 417-426). The step sequence of clb_step is the
+// reference's per-iteration order (src/complab.cpp:1252-1849): collide all solutes ->
+// kinetics -> speciation -> mask update -> stream, done as one fused transport/kinetics
+// kernel followed by the speciation kernel.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "clb_internal.h"
+
+using namespace clb;
+
+struct EventPool {
+  std::vector<hipEvent_t> ev;  // start/stop pairs
+  size_t used = 0;
+};
+
+struct clb_ctx {
+  clb_config cfg{};
+  Geom geo{};
+  bool fp32 = false;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  uint8_t* d_mask = nullptr;
+  uint8_t* d_kind = nullptr;
+  int h_kind[256] = {0};
+  double h_flow_omega[256] = {0};
+  void* d_flow_omega = nullptr;
+
+  // flow
+  void* d_f = nullptr;
+  long long flow_t = 0;
+  double rho_in = 1.0, rho_out = 1.0;
+  bool flow_ready = false;
+  double* d_energy = nullptr;
+  dim3 flow_block{64, 4, 1};
+
+  // solutes
+  int S = 0;
+  std::vector<double> ade_omega;
+  int bc_type[2 * CLB_MAX_SOLUTES] = {0};
+  double bc_value[2 * CLB_MAX_SOLUTES] = {0};
+  void* d_ade_omega = nullptr;
+  void* d_cref = nullptr;
+  void* d_g = nullptr;
+  void* d_u = nullptr;
+  long long ade_t = 0;
+  bool ade_ready = false;
+  int nb = 0;
+  void* d_b = nullptr;
+  dim3 ade_block{64, 4, 1};
+
+  int kin_kind = 0;
+  double dt = 1.0;
+  int solid_code = 0, bb_code = 1;
+  int mask_update = 0;
+  double bthr = 0;
+  int pore_code = 2, biofilm_code = 3;
+
+  // coupled speciation
+  bool have_tableau = false;
+  Tableau tb{};
+  double* d_cbuf = nullptr;
+  int* d_list = nullptr;
+  long long n_reactive = 0;
+  bool list_dirty = true;
+  unsigned int* d_counter = nullptr;
+  unsigned long long* d_stats = nullptr;
+  int spec_grid = 0;
+
+  // batch speciation
+  long long bn = 0;
+  double* d_bt = nullptr;
+  double* d_blk = nullptr;
+  int bnvar = 0;
+  int bvsp[4] = {0, 0, 0, 0};
+  double* d_blogc = nullptr;
+  int* d_biters = nullptr;
+  double* d_bres = nullptr;
+  uint8_t* d_bconv = nullptr;
+
+  // MLP
+  MlpDims md{0, 0, 0};
+  long long mlp_P = 0;
+  float* d_theta = nullptr;
+  float* d_adam_m = nullptr;
+  float* d_adam_v = nullptr;
+  double* d_box = nullptr;  // xmin[32] xmax[32] ymin[32] ymax[32]
+  double h_box[128] = {0};
+  bool mlp_scaled = false;
+  float* d_work = nullptr;
+  size_t work_bytes = 0;
+
+  // measurement
+  bool timing = false;
+  EventPool pools[4];  // flow, ade, spec, mlp
+  clb_perf perf{};
+};
+
+namespace {
+
+#define CLB_CHECK_HIP(ctx, expr)                                                          \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      (ctx)->err = std::string(#expr) + ": " + hipGetErrorString(e_);                     \
+      return CLB_ERR_HIP;                                                                 \
+    }                                                                                     \
+  } while (0)
+
+int fail(clb_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+template <typename T>
+int dalloc(clb_ctx* c, T** p, size_t count) {
+  if (*p) { (void)hipFree(*p); *p = nullptr; }
+  if (count == 0) return CLB_OK;
+  if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return fail(c, CLB_ERR_NOMEM, "device allocation of " + std::to_string(count * sizeof(T)) + " bytes failed");
+  }
+  return CLB_OK;
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+// Timing: record an event pair around a launch when enabled.
+struct Timed {
+  clb_ctx* c;
+  int cat;
+  Timed(clb_ctx* c_, int cat_) : c(c_), cat(cat_) {
+    if (!c->timing) return;
+    EventPool& p = c->pools[cat];
+    if (p.used + 2 > p.ev.size()) {
+      hipEvent_t a, b;
+      (void)hipEventCreate(&a);
+      (void)hipEventCreate(&b);
+      p.ev.push_back(a);
+      p.ev.push_back(b);
+    }
+    (void)hipEventRecord(p.ev[p.used], c->stream);
+  }
+  ~Timed() {
+    if (!c->timing) return;
+    EventPool& p = c->pools[cat];
+    (void)hipEventRecord(p.ev[p.used + 1], c->stream);
+    p.used += 2;
+    if (p.used >= 8192) {  // fold before the pool grows further
+      (void)hipStreamSynchronize(c->stream);
+      double ms = 0;
+      for (size_t i = 0; i < p.used; i += 2) {
+        float t = 0;
+        (void)hipEventElapsedTime(&t, p.ev[i], p.ev[i + 1]);
+        ms += t;
+      }
+      double* acc[4] = {&c->perf.flow_ms, &c->perf.ade_ms, &c->perf.spec_ms, &c->perf.mlp_ms};
+      *acc[cat] += ms;
+      p.used = 0;
+    }
+  }
+};
+
+void count_launch(clb_ctx* c, int cat) {
+  long long* cnt[4] = {&c->perf.flow_launches, &c->perf.ade_launches, &c->perf.spec_launches,
+                       &c->perf.mlp_launches};
+  *cnt[cat] += 1;
+}
+
+template <typename R>
+FlowArgs<R> flow_args(clb_ctx* c, double* energy) {
+  FlowArgs<R> a;
+  a.f = (R*)c->d_f;
+  a.mask = c->d_mask;
+  a.kind = c->d_kind;
+  a.omega = (const R*)c->d_flow_omega;
+  a.g = c->geo;
+  a.rhobar_in = (R)(c->rho_in - 1.0);
+  a.rhobar_out = (R)(c->rho_out - 1.0);
+  a.energy = energy;
+  return a;
+}
+
+template <typename R>
+AdeArgs<R> ade_args(clb_ctx* c) {
+  AdeArgs<R> a;
+  a.g = (R*)c->d_g;
+  a.mask = c->d_mask;
+  a.kind = c->d_kind;
+  a.omega = (const R*)c->d_ade_omega;
+  a.u = (const R*)c->d_u;
+  a.bmass = (R*)c->d_b;
+  a.cbuf = c->have_tableau ? c->d_cbuf : nullptr;
+  a.geo = c->geo;
+  a.S = c->S;
+  a.nb = c->nb;
+  for (int i = 0; i < 2 * CLB_MAX_SOLUTES; ++i) {
+    a.bc_type[i] = c->bc_type[i];
+    a.bc_value[i] = (R)c->bc_value[i];
+  }
+  for (int s = 0; s < CLB_MAX_SOLUTES; ++s) a.c_ref[s] = (R)0;
+  a.kinetics = c->kin_kind;
+  a.dt = c->dt;
+  a.solid_code = c->solid_code;
+  a.bb_code = c->bb_code;
+  a.mask_update = c->mask_update;
+  a.bthreshold = c->bthr;
+  a.pore_code = c->pore_code;
+  a.biofilm_code = c->biofilm_code;
+  return a;
+}
+
+int upload_tables(clb_ctx* c) {
+  uint8_t k8[256];
+  for (int i = 0; i < 256; ++i) k8[i] = (uint8_t)c->h_kind[i];
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_kind, k8, 256, hipMemcpyHostToDevice));
+  if (c->fp32) {
+    float o[256];
+    for (int i = 0; i < 256; ++i) o[i] = (float)c->h_flow_omega[i];
+    CLB_CHECK_HIP(c, hipMemcpy(c->d_flow_omega, o, sizeof(o), hipMemcpyHostToDevice));
+  } else {
+    CLB_CHECK_HIP(c, hipMemcpy(c->d_flow_omega, c->h_flow_omega, sizeof(c->h_flow_omega), hipMemcpyHostToDevice));
+  }
+  return CLB_OK;
+}
+
+template <typename R>
+int flow_steps_t(clb_ctx* c, int nsteps, double* energy) {
+  for (int t = 0; t < nsteps; ++t) {
+    double* e = nullptr;
+    if (energy) {
+      launch_fill_f64(c->d_energy, 2, 0.0, c->stream);
+      e = c->d_energy;
+    }
+    {
+      Timed tm(c, 0);
+      launch_flow_step<R>(flow_args<R>(c, e), (c->flow_t & 1) != 0, c->flow_block, c->stream);
+    }
+    count_launch(c, 0);
+    c->flow_t++;
+    if (energy) {
+      double h[2];
+      CLB_CHECK_HIP(c, hipMemcpyAsync(h, c->d_energy, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+      CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+      energy[t] = h[1] > 0 ? h[0] / h[1] : 0.0;
+    }
+  }
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+// Reactive nodes (1 <= x <= nx-2, fluid, not solid / bounce-back code), compacted per
+// wavefront so each wavefront's nodes stay contiguous in memory.
+__global__ void reactive_list_kernel(const uint8_t* mask, const uint8_t* kind, Geom g, int solid, int bb, int* list,
+                                     unsigned int* count) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  bool r = false;
+  if (n < g.N) {
+    const int x = (int)(n % g.nx);
+    const uint8_t code = mask[n];
+    r = x > 0 && x < g.nx - 1 && kind[code] == CLB_KIND_FLUID && code != solid && code != bb;
+  }
+  const unsigned long long bal = __ballot(r);
+  const int lane = threadIdx.x & 63;
+  unsigned int base = 0;
+  if (lane == 0 && bal) base = atomicAdd(count, (unsigned int)__popcll(bal));
+  base = __shfl(base, 0);
+  if (r) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)n;
+}
+
+int build_reactive_list(clb_ctx* c) {
+  int rc = dalloc(c, &c->d_list, (size_t)c->geo.N);
+  if (rc) return rc;
+  launch_fill_u32(c->d_counter, 1, 0u, c->stream);
+  hipLaunchKernelGGL(reactive_list_kernel, dim3((unsigned)((c->geo.N + 255) / 256)), dim3(256), 0, c->stream,
+                     c->d_mask, c->d_kind, c->geo, c->solid_code, c->bb_code, c->d_list, c->d_counter);
+  unsigned int cnt = 0;
+  CLB_CHECK_HIP(c, hipMemcpyAsync(&cnt, c->d_counter, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  c->n_reactive = cnt;
+  c->perf.reactive_nodes = cnt;
+  c->list_dirty = false;
+  return CLB_OK;
+}
+
+template <typename R>
+int step_t(clb_ctx* c, int nsteps) {
+  if (c->have_tableau && c->list_dirty) {
+    int rc = build_reactive_list(c);
+    if (rc) return rc;
+  }
+  for (int t = 0; t < nsteps; ++t) {
+    const bool odd = (c->ade_t & 1) != 0;
+    AdeArgs<R> a = ade_args<R>(c);
+    {
+      Timed tm(c, 1);
+      launch_ade_step<R>(a, odd, c->ade_block, c->stream);
+    }
+    count_launch(c, 1);
+    if (c->have_tableau && c->n_reactive > 0) {
+      launch_fill_u32(c->d_counter, 1, 0u, c->stream);
+      SpecArgs sa{};
+      sa.n = c->n_reactive;
+      sa.iters = (int*)c->d_list;  // node list for the coupled kernel
+      sa.counter = c->d_counter;
+      sa.stats = c->d_stats;
+      {
+        Timed tm(c, 2);
+        launch_spec_coupled(a, odd, c->tb, sa, c->spec_grid, c->stream);
+      }
+      count_launch(c, 2);
+    }
+    c->ade_t++;
+  }
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int fold_timing(clb_ctx* c) {
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  double* acc[4] = {&c->perf.flow_ms, &c->perf.ade_ms, &c->perf.spec_ms, &c->perf.mlp_ms};
+  for (int k = 0; k < 4; ++k) {
+    EventPool& p = c->pools[k];
+    for (size_t i = 0; i + 1 < p.used; i += 2) {
+      float t = 0;
+      (void)hipEventElapsedTime(&t, p.ev[i], p.ev[i + 1]);
+      *acc[k] += t;
+    }
+    p.used = 0;
+  }
+  return CLB_OK;
+}
+
+long long mlp_param_count(const MlpDims& d) {
+  return (long long)d.H * d.din + d.H + (long long)d.H * d.H + d.H + (long long)d.dout * d.H + d.dout;
+}
+
+int ensure_work(clb_ctx* c, size_t bytes) {
+  if (c->work_bytes >= bytes) return CLB_OK;
+  dfree(c->d_work);
+  c->d_work = nullptr;
+  c->work_bytes = 0;
+  if (hipMalloc((void**)&c->d_work, bytes) != hipSuccess) return fail(c, CLB_ERR_NOMEM, "work buffer allocation failed");
+  c->work_bytes = bytes;
+  return CLB_OK;
+}
+
+// Forward + backward over one batch with the FP32 kernels. Buffers in `w`:
+// H1[B*H], H2[B*H], Y[B*dout], dY[B*dout], dH2[B*H], dH1[B*H].
+int mlp_fwd_bwd(clb_ctx* c, long long B, const float* X, const float* T, float* grad, double* d_loss, float* w) {
+  const MlpDims d = c->md;
+  const int H = d.H;
+  float* H1 = w;
+  float* H2 = H1 + B * H;
+  float* Y = H2 + B * H;
+  float* dY = Y + B * d.dout;
+  float* dZ2 = dY + B * d.dout;
+  float* dZ1 = dZ2 + B * H;
+  const float* W1 = c->d_theta;
+  const float* b1 = W1 + H * d.din;
+  const float* W2 = b1 + H;
+  const float* b2 = W2 + H * H;
+  const float* W3 = b2 + H;
+  const float* b3 = W3 + d.dout * H;
+  float* gW1 = grad;
+  float* gb1 = gW1 + H * d.din;
+  float* gW2 = gb1 + H;
+  float* gb2 = gW2 + H * H;
+  float* gW3 = gb2 + H;
+  float* gb3 = gW3 + d.dout * H;
+  hipStream_t s = c->stream;
+  const int Bi = (int)B;
+  launch_gemm(0, 1, Bi, H, d.din, X, d.din, W1, d.din, H1, H, b1, 1, 0.f, s);
+  launch_gemm(0, 1, Bi, H, H, H1, H, W2, H, H2, H, b2, 1, 0.f, s);
+  launch_gemm(0, 1, Bi, d.dout, H, H2, H, W3, H, Y, d.dout, b3, 0, 0.f, s);
+  if (!T) return CLB_OK;
+  launch_mse_grad(B * d.dout, Y, T, 2.0f / (float)(B * d.dout), dY, d_loss, s);
+  launch_gemm(1, 0, d.dout, H, Bi, dY, d.dout, H2, H, gW3, H, nullptr, 0, 0.f, s);
+  launch_colsum(Bi, d.dout, dY, gb3, s);
+  launch_gemm(0, 0, Bi, H, d.dout, dY, d.dout, W3, H, dZ2, H, nullptr, 0, 0.f, s);
+  launch_tanh_backward(B * H, dZ2, H2, s);
+  launch_gemm(1, 0, H, H, Bi, dZ2, H, H1, H, gW2, H, nullptr, 0, 0.f, s);
+  launch_colsum(Bi, H, dZ2, gb2, s);
+  launch_gemm(0, 0, Bi, H, H, dZ2, H, W2, H, dZ1, H, nullptr, 0, 0.f, s);
+  launch_tanh_backward(B * H, dZ1, H1, s);
+  launch_gemm(1, 0, H, d.din, Bi, dZ1, H, X, d.din, gW1, d.din, nullptr, 0, 0.f, s);
+  launch_colsum(Bi, H, dZ1, gb1, s);
+  return CLB_OK;
+}
+
+size_t mlp_fwd_bwd_floats(const MlpDims& d, long long B) { return (size_t)B * (4 * d.H + 2 * d.dout); }
+
+// Adam over (X, T) rows [0, n), minibatches in a fixed shuffled order per epoch.
+int mlp_train_device(clb_ctx* c, long long n, const float* dX, const float* dT, int epochs, int batch, double lr,
+                     uint64_t seed, double* final_loss) {
+  const MlpDims d = c->md;
+  if (batch < 1) batch = 1;
+  if (batch > n) batch = (int)n;
+  const long long P = c->mlp_P;
+  const size_t nfl = (mlp_fwd_bwd_floats(d, batch) + P + 1) & ~(size_t)1;  // keep the double 8-byte aligned
+  int rc = ensure_work(c, nfl * sizeof(float) + sizeof(double) * 2);
+  if (rc) return rc;
+  float* w = c->d_work;
+  float* grad = w + mlp_fwd_bwd_floats(d, batch);
+  double* d_loss = (double*)(w + nfl);
+  launch_fill_f64(d_loss, 1, 0.0, c->stream);
+  (void)hipMemsetAsync(c->d_adam_m, 0, sizeof(float) * P, c->stream);
+  (void)hipMemsetAsync(c->d_adam_v, 0, sizeof(float) * P, c->stream);
+  std::mt19937_64 rng(seed);
+  const long long nbatches = n / batch;
+  std::vector<long long> order(nbatches);
+  std::iota(order.begin(), order.end(), 0);
+  long long tstep = 0;
+  double last = 0;
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::shuffle(order.begin(), order.end(), rng);
+    launch_fill_f64(d_loss, 1, 0.0, c->stream);
+    for (long long bi = 0; bi < nbatches; ++bi) {
+      const long long r0 = order[bi] * batch;
+      const float* Xp = dX + r0 * d.din;
+      const float* Tp = dT + r0 * d.dout;
+      mlp_fwd_bwd(c, batch, Xp, Tp, grad, d_loss, w);
+      ++tstep;
+      const float b1 = 0.9f, b2 = 0.999f;
+      launch_adam(P, c->d_theta, grad, c->d_adam_m, c->d_adam_v, (float)lr, b1, b2, 1e-8f,
+                  (float)std::pow(0.9, (double)tstep), (float)std::pow(0.999, (double)tstep), c->stream);
+    }
+    double h = 0;
+    CLB_CHECK_HIP(c, hipMemcpyAsync(&h, d_loss, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    last = h / (double)(nbatches * (long long)batch * d.dout);
+  }
+  if (final_loss) *final_loss = last;
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clb_create(const clb_config* cfg, clb_ctx** out) {
+  if (!cfg || !out) return CLB_ERR_ARG;
+  *out = nullptr;
+  if (cfg->nx < 4 || cfg->ny < 1 || cfg->nz < 1) return CLB_ERR_ARG;
+  if (cfg->precision != CLB_PREC_FP64 && cfg->precision != CLB_PREC_FP32) return CLB_ERR_ARG;
+  const long long N = (long long)cfg->nx * cfg->ny * cfg->nz;
+  if (N * 19 >= (1ll << 31) || N * 7 * CLB_MAX_SOLUTES >= (1ll << 40)) return CLB_ERR_UNSUP;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CLB_ERR_HIP;
+  if (cfg->device < 0 || cfg->device >= ndev) return CLB_ERR_ARG;
+  if (hipSetDevice(cfg->device) != hipSuccess) return CLB_ERR_HIP;
+  clb_ctx* c = new clb_ctx();
+  c->cfg = *cfg;
+  c->geo = Geom{cfg->nx, cfg->ny, cfg->nz, N};
+  c->fp32 = cfg->precision == CLB_PREC_FP32;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return CLB_ERR_HIP;
+  }
+  const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+  int rc = CLB_OK;
+  if (!rc) rc = dalloc(c, &c->d_mask, (size_t)N);
+  if (!rc) rc = dalloc(c, &c->d_kind, 256);
+  if (!rc) rc = dalloc(c, (uint8_t**)&c->d_flow_omega, 256 * rs);
+  if (!rc) rc = dalloc(c, &c->d_energy, 2);
+  if (!rc) rc = dalloc(c, &c->d_counter, 4);
+  if (!rc) rc = dalloc(c, &c->d_stats, 4);
+  if (rc) {
+    clb_destroy(c);
+    return rc;
+  }
+  // default material numbers: 0 no dynamics, 1 bounce-back, others fluid with tau = 0.8
+  for (int i = 0; i < 256; ++i) {
+    c->h_kind[i] = (i == 0) ? CLB_KIND_NONE : (i == 1 ? CLB_KIND_BB : CLB_KIND_FLUID);
+    c->h_flow_omega[i] = 1.0 / 0.8;
+  }
+  rc = upload_tables(c);
+  if (rc) {
+    clb_destroy(c);
+    return rc;
+  }
+  launch_fill_u64(c->d_stats, 4, 0ull, c->stream);
+  c->spec_grid = spec_occupancy_grid(8, cfg->device);
+  *out = c;
+  return CLB_OK;
+}
+
+int clb_destroy(clb_ctx* c) {
+  if (!c) return CLB_ERR_ARG;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  void* ptrs[] = {c->d_mask, c->d_kind, c->d_flow_omega, c->d_f, c->d_energy, c->d_ade_omega, c->d_cref, c->d_g,
+                  c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
+                  c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
+                  c->d_box, c->d_work};
+  for (void* p : ptrs) dfree(p);
+  for (auto& p : c->pools)
+    for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CLB_OK;
+}
+
+const char* clb_last_error(const clb_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int clb_sync(clb_ctx* c) {
+  if (!c) return CLB_ERR_ARG;
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int clb_device_name(char* buf, int len) {
+  if (!buf || len < 1) return CLB_ERR_ARG;
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return CLB_ERR_HIP;
+  std::snprintf(buf, (size_t)len, "%s (%s, %d CUs)", p.name, p.gcnArchName, p.multiProcessorCount);
+  return CLB_OK;
+}
+
+int clb_upload_mask(clb_ctx* c, const uint8_t* mask) {
+  if (!c || !mask) return CLB_ERR_ARG;
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_mask, mask, (size_t)c->geo.N, hipMemcpyHostToDevice));
+  long long fluid = 0;
+  for (long long n = 0; n < c->geo.N; ++n) fluid += (c->h_kind[mask[n]] == CLB_KIND_FLUID);
+  c->perf.fluid_nodes = fluid;
+  c->list_dirty = true;
+  return CLB_OK;
+}
+
+int clb_download_mask(clb_ctx* c, uint8_t* mask) {
+  if (!c || !mask) return CLB_ERR_ARG;
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipMemcpy(mask, c->d_mask, (size_t)c->geo.N, hipMemcpyDeviceToHost));
+  return CLB_OK;
+}
+
+int clb_set_dynamics(clb_ctx* c, const int* kind, const double* flow_omega) {
+  if (!c || !kind) return CLB_ERR_ARG;
+  for (int i = 0; i < 256; ++i) {
+    if (kind[i] < CLB_KIND_NONE || kind[i] > CLB_KIND_FLUID) return fail(c, CLB_ERR_ARG, "bad dynamics kind");
+    c->h_kind[i] = kind[i];
+    if (flow_omega) {
+      if (kind[i] == CLB_KIND_FLUID && !(flow_omega[i] > 0.0 && flow_omega[i] < 2.0))
+        return fail(c, CLB_ERR_ARG, "flow omega outside (0, 2)");
+      c->h_flow_omega[i] = flow_omega[i];
+    }
+  }
+  c->list_dirty = true;
+  return upload_tables(c);
+}
+
+int clb_flow_init(clb_ctx* c, double rho_in, double rho_out) {
+  if (!c) return CLB_ERR_ARG;
+  const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+  if (!c->d_f) {
+    int rc = dalloc(c, (uint8_t**)&c->d_f, (size_t)19 * c->geo.N * rs);
+    if (rc) return rc;
+  }
+  c->rho_in = rho_in;
+  c->rho_out = rho_out;
+  if (c->fp32) launch_flow_init<float>(c->geo, (float*)c->d_f, rho_in, rho_out, c->stream);
+  else launch_flow_init<double>(c->geo, (double*)c->d_f, rho_in, rho_out, c->stream);
+  c->flow_t = 0;
+  c->flow_ready = true;
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int clb_flow_steps(clb_ctx* c, int nsteps, double* energy) {
+  if (!c || nsteps < 0) return CLB_ERR_ARG;
+  if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "clb_flow_init first");
+  return c->fp32 ? flow_steps_t<float>(c, nsteps, energy) : flow_steps_t<double>(c, nsteps, energy);
+}
+
+int clb_flow_download_velocity(clb_ctx* c, double* u, double* rho) {
+  if (!c || (!u && !rho)) return CLB_ERR_ARG;
+  if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "clb_flow_init first");
+  double *du = nullptr, *dr = nullptr;
+  int rc = dalloc(c, &du, (size_t)3 * c->geo.N);
+  if (!rc) rc = dalloc(c, &dr, (size_t)c->geo.N);
+  if (rc) { dfree(du); dfree(dr); return rc; }
+  const bool odd_next = (c->flow_t & 1) != 0;
+  if (c->fp32) launch_flow_velocity<float>(flow_args<float>(c, nullptr), odd_next, du, dr, nullptr, c->stream);
+  else launch_flow_velocity<double>(flow_args<double>(c, nullptr), odd_next, du, dr, nullptr, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess && u) e = hipMemcpy(u, du, sizeof(double) * 3 * c->geo.N, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && rho) e = hipMemcpy(rho, dr, sizeof(double) * c->geo.N, hipMemcpyDeviceToHost);
+  dfree(du);
+  dfree(dr);
+  if (e != hipSuccess) return fail(c, CLB_ERR_HIP, hipGetErrorString(e));
+  return CLB_OK;
+}
+
+int clb_set_solutes(clb_ctx* c, int S, const double* omega, const int* bc_type, const double* bc_value) {
+  if (!c || S < 1 || S > CLB_MAX_SOLUTES || !omega || !bc_type || !bc_value) return CLB_ERR_ARG;
+  for (int s = 0; s < S; ++s)
+    for (int i = 0; i < 256; ++i)
+      if (c->h_kind[i] == CLB_KIND_FLUID && !(omega[s * 256 + i] > 0.0 && omega[s * 256 + i] < 2.0))
+        return fail(c, CLB_ERR_ARG, "solute omega outside (0, 2)");
+  for (int i = 0; i < 2 * S; ++i)
+    if (bc_type[i] != CLB_BC_DIRICHLET && bc_type[i] != CLB_BC_NEUMANN) return fail(c, CLB_ERR_ARG, "bad bc type");
+  const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+  const bool realloc_pop = (S != c->S) || !c->d_g;
+  c->S = S;
+  c->ade_omega.assign(omega, omega + (size_t)S * 256);
+  for (int i = 0; i < 2 * CLB_MAX_SOLUTES; ++i) {
+    c->bc_type[i] = (i < 2 * S) ? bc_type[i] : 0;
+    c->bc_value[i] = (i < 2 * S) ? bc_value[i] : 0.0;
+  }
+  int rc = CLB_OK;
+  if (realloc_pop) {
+    rc = dalloc(c, (uint8_t**)&c->d_g, (size_t)S * 7 * c->geo.N * rs);
+    if (!rc) rc = dalloc(c, (uint8_t**)&c->d_ade_omega, (size_t)S * 256 * rs);
+    if (!rc && !c->d_u) rc = dalloc(c, (uint8_t**)&c->d_u, (size_t)3 * c->geo.N * rs);
+    if (rc) return rc;
+    c->ade_ready = false;
+    if (c->have_tableau) {
+      rc = dalloc(c, &c->d_cbuf, (size_t)S * c->geo.N);
+      if (rc) return rc;
+    }
+  }
+  if (c->fp32) {
+    std::vector<float> o(c->ade_omega.begin(), c->ade_omega.end());
+    CLB_CHECK_HIP(c, hipMemcpy(c->d_ade_omega, o.data(), o.size() * sizeof(float), hipMemcpyHostToDevice));
+  } else {
+    CLB_CHECK_HIP(c, hipMemcpy(c->d_ade_omega, c->ade_omega.data(), c->ade_omega.size() * sizeof(double),
+                               hipMemcpyHostToDevice));
+  }
+  return CLB_OK;
+}
+
+int clb_solutes_init(clb_ctx* c, const double* c0) {
+  if (!c || !c0) return CLB_ERR_ARG;
+  if (!c->d_g) return fail(c, CLB_ERR_STATE, "clb_set_solutes first");
+  double* dc0 = nullptr;
+  int rc = dalloc(c, &dc0, (size_t)c->S);
+  if (rc) return rc;
+  const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+  if (!c->d_cref) {
+    rc = dalloc(c, (uint8_t**)&c->d_cref, CLB_MAX_SOLUTES * rs);
+    if (rc) { dfree(dc0); return rc; }
+    CLB_CHECK_HIP(c, hipMemset(c->d_cref, 0, CLB_MAX_SOLUTES * rs));
+  }
+  CLB_CHECK_HIP(c, hipMemcpy(dc0, c0, sizeof(double) * c->S, hipMemcpyHostToDevice));
+  if (c->fp32) launch_ade_init<float>(c->geo, c->S, (float*)c->d_g, dc0, (const float*)c->d_cref, c->stream);
+  else launch_ade_init<double>(c->geo, c->S, (double*)c->d_g, dc0, (const double*)c->d_cref, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  dfree(dc0);
+  if (e != hipSuccess) return fail(c, CLB_ERR_HIP, hipGetErrorString(e));
+  c->ade_t = 0;
+  c->ade_ready = true;
+  return CLB_OK;
+}
+
+int clb_couple_velocity(clb_ctx* c) {
+  if (!c) return CLB_ERR_ARG;
+  if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "no flow field");
+  if (!c->d_u) return fail(c, CLB_ERR_STATE, "clb_set_solutes first");
+  const bool odd_next = (c->flow_t & 1) != 0;
+  if (c->fp32)
+    launch_flow_velocity<float>(flow_args<float>(c, nullptr), odd_next, nullptr, nullptr, (float*)c->d_u, c->stream);
+  else
+    launch_flow_velocity<double>(flow_args<double>(c, nullptr), odd_next, nullptr, nullptr, (double*)c->d_u,
+                                 c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int clb_set_velocity(clb_ctx* c, const double* u) {
+  if (!c || !u) return CLB_ERR_ARG;
+  if (!c->d_u) return fail(c, CLB_ERR_STATE, "clb_set_solutes first");
+  const long long N = c->geo.N;
+  if (c->fp32) {
+    std::vector<float> p(3 * N);
+    for (long long n = 0; n < N; ++n)
+      for (int k = 0; k < 3; ++k) p[k * N + n] = (float)u[3 * n + k];
+    CLB_CHECK_HIP(c, hipMemcpy(c->d_u, p.data(), sizeof(float) * 3 * N, hipMemcpyHostToDevice));
+  } else {
+    std::vector<double> p(3 * N);
+    for (long long n = 0; n < N; ++n)
+      for (int k = 0; k < 3; ++k) p[k * N + n] = u[3 * n + k];
+    CLB_CHECK_HIP(c, hipMemcpy(c->d_u, p.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
+  }
+  return CLB_OK;
+}
+
+int clb_set_biomass(clb_ctx* c, int nb, const double* b) {
+  if (!c || nb < 0 || nb > CLB_MAX_BIOMASS || (nb > 0 && !b)) return CLB_ERR_ARG;
+  c->nb = nb;
+  if (nb == 0) return CLB_OK;
+  const long long n = (long long)nb * c->geo.N;
+  if (c->fp32) {
+    int rc = dalloc(c, (float**)&c->d_b, (size_t)n);
+    if (rc) return rc;
+    std::vector<float> p(b, b + n);
+    CLB_CHECK_HIP(c, hipMemcpy(c->d_b, p.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  } else {
+    int rc = dalloc(c, (double**)&c->d_b, (size_t)n);
+    if (rc) return rc;
+    CLB_CHECK_HIP(c, hipMemcpy(c->d_b, b, sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  return CLB_OK;
+}
+
+int clb_download_conc(clb_ctx* c, double* out) {
+  if (!c || !out) return CLB_ERR_ARG;
+  if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "solutes not initialised");
+  double* dc = nullptr;
+  int rc = dalloc(c, &dc, (size_t)c->S * c->geo.N);
+  if (rc) return rc;
+  const bool odd_next = (c->ade_t & 1) != 0;
+  if (c->fp32) launch_ade_conc<float>(ade_args<float>(c), odd_next, dc, c->stream);
+  else launch_ade_conc<double>(ade_args<double>(c), odd_next, dc, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(out, dc, sizeof(double) * c->S * c->geo.N, hipMemcpyDeviceToHost);
+  dfree(dc);
+  if (e != hipSuccess) return fail(c, CLB_ERR_HIP, hipGetErrorString(e));
+  return CLB_OK;
+}
+
+int clb_download_biomass(clb_ctx* c, double* out) {
+  if (!c || !out) return CLB_ERR_ARG;
+  if (c->nb == 0) return CLB_OK;
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  const long long n = (long long)c->nb * c->geo.N;
+  if (c->fp32) {
+    std::vector<float> p(n);
+    CLB_CHECK_HIP(c, hipMemcpy(p.data(), c->d_b, sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < n; ++i) out[i] = p[i];
+  } else {
+    CLB_CHECK_HIP(c, hipMemcpy(out, c->d_b, sizeof(double) * n, hipMemcpyDeviceToHost));
+  }
+  return CLB_OK;
+}
+
+int clb_set_kinetics(clb_ctx* c, int kind, double dt, int solid_code, int bb_code) {
+  if (!c || (kind != CLB_KINETICS_NONE && kind != CLB_KINETICS_PLUGIN) || !(dt > 0)) return CLB_ERR_ARG;
+  c->kin_kind = kind;
+  c->dt = dt;
+  c->solid_code = solid_code;
+  c->bb_code = bb_code;
+  c->list_dirty = true;
+  return CLB_OK;
+}
+
+int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* logK, const int* comp_species,
+                    double tol, int max_iter) {
+  if (!c) return CLB_ERR_ARG;
+  if (nc == 0) {
+    c->have_tableau = false;
+    return CLB_OK;
+  }
+  if (nc < 1 || nc > CLB_MAX_COMPONENTS || ns < 1 || ns > CLB_MAX_SPECIES || !S || !logK || !(tol > 0) ||
+      max_iter < 1)
+    return CLB_ERR_ARG;
+  Tableau tb{};
+  tb.nc = nc;
+  tb.ns = ns;
+  for (int i = 0; i < ns; ++i) {
+    tb.logK[i] = logK[i];
+    int eq = 0;
+    for (int j = 0; j < nc; ++j) {
+      tb.S[i * CLB_MAX_COMPONENTS + j] = S[i * nc + j];
+      if (std::fabs(S[i * nc + j]) > 1e-10) eq = 1;
+    }
+    tb.is_eq[i] = eq;
+  }
+  for (int j = 0; j < CLB_MAX_COMPONENTS; ++j) {
+    double mu0 = 1.0;
+    if (j < nc)
+      for (int i = 0; i < ns; ++i)
+        if (tb.is_eq[i]) {
+          const double m = S[i * nc + j];
+          if (m > 1e-10 && m < mu0) mu0 = m;
+        }
+    tb.inv_mu0[j] = 1.0 / mu0;
+    tb.comp_species[j] = (j < nc && comp_species) ? comp_species[j] : -1;
+  }
+  tb.tol = tol;
+  tb.max_iter = max_iter;
+  c->tb = tb;
+  c->have_tableau = true;
+  if (c->S > 0) {
+    int rc = dalloc(c, &c->d_cbuf, (size_t)c->S * c->geo.N);
+    if (rc) return rc;
+  }
+  c->list_dirty = true;
+  return CLB_OK;
+}
+
+int clb_set_mask_update(clb_ctx* c, int enable, double threshold, int pore_code, int biofilm_code) {
+  if (!c) return CLB_ERR_ARG;
+  c->mask_update = enable ? 1 : 0;
+  c->bthr = threshold;
+  c->pore_code = pore_code;
+  c->biofilm_code = biofilm_code;
+  return CLB_OK;
+}
+
+int clb_step(clb_ctx* c, int nsteps) {
+  if (!c || nsteps < 0) return CLB_ERR_ARG;
+  if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "clb_solutes_init first");
+  if (c->kin_kind == CLB_KINETICS_PLUGIN && c->nb > 0 && !c->d_b) return fail(c, CLB_ERR_STATE, "no biomass");
+  if (c->have_tableau && c->tb.ns != c->S) return fail(c, CLB_ERR_ARG, "tableau species count != solute count");
+  return c->fp32 ? step_t<float>(c, nsteps) : step_t<double>(c, nsteps);
+}
+
+int clb_speciation_stats(clb_ctx* c, long long* stats) {
+  if (!c || !stats) return CLB_ERR_ARG;
+  unsigned long long h[3];
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 3; ++i) stats[i] = (long long)h[i];
+  return CLB_OK;
+}
+
+// ---------------------------------------------------------------- batched speciation
+int clb_batch_load(clb_ctx* c, long n, const double* totals, int nvar, const int* var_species,
+                   const double* logk_var) {
+  if (!c || n < 1 || !totals || nvar < 0 || nvar > 4 || (nvar > 0 && (!var_species || !logk_var)))
+    return CLB_ERR_ARG;
+  if (!c->have_tableau) return fail(c, CLB_ERR_STATE, "clb_set_tableau first");
+  for (int v = 0; v < nvar; ++v)
+    if (var_species[v] < 0 || var_species[v] >= c->tb.ns) return fail(c, CLB_ERR_ARG, "bad var_species");
+  const int nc = c->tb.nc;
+  int rc = CLB_OK;
+  if (n != c->bn || !c->d_bt) {
+    rc = dalloc(c, &c->d_bt, (size_t)n * nc);
+    if (!rc) rc = dalloc(c, &c->d_blogc, (size_t)n * nc);
+    if (!rc) rc = dalloc(c, &c->d_biters, (size_t)n);
+    if (!rc) rc = dalloc(c, &c->d_bres, (size_t)n);
+    if (!rc) rc = dalloc(c, &c->d_bconv, (size_t)n);
+    if (rc) return rc;
+  }
+  rc = dalloc(c, &c->d_blk, (size_t)n * (nvar > 0 ? nvar : 1));
+  if (rc) return rc;
+  c->bn = n;
+  c->bnvar = nvar;
+  for (int v = 0; v < 4; ++v) c->bvsp[v] = (v < nvar) ? var_species[v] : 0;
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_bt, totals, sizeof(double) * n * nc, hipMemcpyHostToDevice));
+  if (nvar > 0) CLB_CHECK_HIP(c, hipMemcpy(c->d_blk, logk_var, sizeof(double) * n * nvar, hipMemcpyHostToDevice));
+  return CLB_OK;
+}
+
+int clb_batch_solve(clb_ctx* c, int warm, const double* logc_in) {
+  if (!c) return CLB_ERR_ARG;
+  if (c->bn < 1) return fail(c, CLB_ERR_STATE, "clb_batch_load first");
+  const long long n = c->bn;
+  const int nc = c->tb.nc;
+  SpecArgs sa{};
+  sa.n = n;
+  sa.totals = c->d_bt;
+  sa.logk_var = c->d_blk;
+  sa.nvar = c->bnvar;
+  for (int v = 0; v < 4; ++v) sa.var_species[v] = c->bvsp[v];
+  sa.logc = c->d_blogc;
+  sa.iters = c->d_biters;
+  sa.resid = c->d_bres;
+  sa.conv = c->d_bconv;
+  sa.counter = c->d_counter;
+  sa.stats = c->d_stats;
+  if (warm == CLB_WARM_USER) {
+    if (!logc_in) return CLB_ERR_ARG;
+    CLB_CHECK_HIP(c, hipMemcpyAsync(c->d_blogc, logc_in, sizeof(double) * n * nc, hipMemcpyHostToDevice, c->stream));
+  } else if (warm == CLB_WARM_MLP) {
+    if (!c->d_theta || !c->mlp_scaled) return fail(c, CLB_ERR_STATE, "MLP not configured / scaled");
+    if (c->md.din != nc + c->bnvar || c->md.dout != nc || c->md.H != 128)
+      return fail(c, CLB_ERR_UNSUP, "MLP shape does not match the batch (din = nc + nvar, dout = nc, H = 128)");
+    int rc = ensure_work(c, (size_t)n * (c->md.din + c->md.dout) * sizeof(float) + (size_t)n + 256);
+    if (rc) return rc;
+    Timed tm(c, 3);
+    launch_mlp_warm(c->md, n, c->d_theta, c->d_box, c->d_box + 32, c->d_box + 64, c->d_box + 96, c->tb, sa,
+                    c->d_work, c->stream);
+    count_launch(c, 3);
+  } else if (warm != CLB_WARM_REF) {
+    return CLB_ERR_ARG;
+  }
+  launch_fill_u32(c->d_counter, 1, 0u, c->stream);
+  {
+    Timed tm(c, 2);
+    launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->stream);
+  }
+  count_launch(c, 2);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int clb_batch_fetch(clb_ctx* c, double* logc, int* iters, double* resid, uint8_t* conv) {
+  if (!c) return CLB_ERR_ARG;
+  if (c->bn < 1) return fail(c, CLB_ERR_STATE, "no batch");
+  const long long n = c->bn;
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  if (logc) CLB_CHECK_HIP(c, hipMemcpy(logc, c->d_blogc, sizeof(double) * n * c->tb.nc, hipMemcpyDeviceToHost));
+  if (iters) CLB_CHECK_HIP(c, hipMemcpy(iters, c->d_biters, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (resid) CLB_CHECK_HIP(c, hipMemcpy(resid, c->d_bres, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (conv) CLB_CHECK_HIP(c, hipMemcpy(conv, c->d_bconv, n, hipMemcpyDeviceToHost));
+  return CLB_OK;
+}
+
+// ---------------------------------------------------------------- MLP
+int clb_mlp_config(clb_ctx* c, int din, int hidden, int dout) {
+  if (!c || din < 1 || din > 16 || hidden < 1 || hidden > 1024 || dout < 1 || dout > 32) return CLB_ERR_ARG;
+  c->md = MlpDims{din, hidden, dout};
+  c->mlp_P = mlp_param_count(c->md);
+  int rc = dalloc(c, &c->d_theta, (size_t)c->mlp_P);
+  if (!rc) rc = dalloc(c, &c->d_adam_m, (size_t)c->mlp_P);
+  if (!rc) rc = dalloc(c, &c->d_adam_v, (size_t)c->mlp_P);
+  if (!rc) rc = dalloc(c, &c->d_box, 128);
+  if (rc) return rc;
+  // Xavier-uniform initialisation, deterministic
+  std::vector<float> th((size_t)c->mlp_P, 0.f);
+  std::mt19937_64 rng(7);
+  auto fill = [&](size_t off, int fan_out, int fan_in) {
+    const double a = std::sqrt(6.0 / (double)(fan_in + fan_out));
+    std::uniform_real_distribution<double> U(-a, a);
+    for (long long i = 0; i < (long long)fan_out * fan_in; ++i) th[off + i] = (float)U(rng);
+  };
+  const int H = hidden;
+  fill(0, H, din);
+  fill((size_t)H * din + H, H, H);
+  fill((size_t)H * din + H + (size_t)H * H + H, dout, H);
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_theta, th.data(), sizeof(float) * th.size(), hipMemcpyHostToDevice));
+  c->mlp_scaled = false;
+  return CLB_OK;
+}
+
+int clb_mlp_set_params(clb_ctx* c, const float* theta) {
+  if (!c || !theta) return CLB_ERR_ARG;
+  if (!c->d_theta) return fail(c, CLB_ERR_STATE, "clb_mlp_config first");
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_theta, theta, sizeof(float) * c->mlp_P, hipMemcpyHostToDevice));
+  return CLB_OK;
+}
+
+int clb_mlp_get_params(clb_ctx* c, float* theta) {
+  if (!c || !theta) return CLB_ERR_ARG;
+  if (!c->d_theta) return fail(c, CLB_ERR_STATE, "clb_mlp_config first");
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipMemcpy(theta, c->d_theta, sizeof(float) * c->mlp_P, hipMemcpyDeviceToHost));
+  return CLB_OK;
+}
+
+int clb_mlp_set_scaling(clb_ctx* c, const double* xmin, const double* xmax, const double* ymin, const double* ymax) {
+  if (!c || !xmin || !xmax || !ymin || !ymax) return CLB_ERR_ARG;
+  if (!c->d_box) return fail(c, CLB_ERR_STATE, "clb_mlp_config first");
+  for (int k = 0; k < 128; ++k) c->h_box[k] = 0.0;
+  for (int k = 0; k < c->md.din; ++k) { c->h_box[k] = xmin[k]; c->h_box[32 + k] = xmax[k]; }
+  for (int k = 0; k < c->md.dout; ++k) { c->h_box[64 + k] = ymin[k]; c->h_box[96 + k] = ymax[k]; }
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_box, c->h_box, sizeof(c->h_box), hipMemcpyHostToDevice));
+  c->mlp_scaled = true;
+  return CLB_OK;
+}
+
+int clb_mlp_forward(clb_ctx* c, long n, const float* X, float* Y, int bf16) {
+  if (!c || n < 1 || !X || !Y) return CLB_ERR_ARG;
+  if (!c->d_theta) return fail(c, CLB_ERR_STATE, "clb_mlp_config first");
+  if (bf16 && (c->md.H != 128 || c->md.din > 16 || c->md.dout > 32))
+    return fail(c, CLB_ERR_UNSUP, "BF16 inference kernel needs H = 128, din <= 16, dout <= 32");
+  const MlpDims d = c->md;
+  size_t need = ((size_t)n * (d.din + d.dout) + (bf16 ? 0 : mlp_fwd_bwd_floats(d, n))) * sizeof(float);
+  int rc = ensure_work(c, need);
+  if (rc) return rc;
+  float* dX = c->d_work;
+  float* dY = dX + (size_t)n * d.din;
+  float* w = dY + (size_t)n * d.dout;
+  CLB_CHECK_HIP(c, hipMemcpyAsync(dX, X, sizeof(float) * n * d.din, hipMemcpyHostToDevice, c->stream));
+  if (bf16) {
+    Timed tm(c, 3);
+    launch_mlp_infer_bf16(d, n, c->d_theta, dX, dY, c->stream);
+    count_launch(c, 3);
+  } else {
+    mlp_fwd_bwd(c, n, dX, nullptr, nullptr, nullptr, w);
+    float* Yf = w + 2 * (size_t)n * d.H;
+    CLB_CHECK_HIP(c, hipMemcpyAsync(dY, Yf, sizeof(float) * n * d.dout, hipMemcpyDeviceToDevice, c->stream));
+  }
+  CLB_CHECK_HIP(c, hipMemcpyAsync(Y, dY, sizeof(float) * n * d.dout, hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int clb_mlp_grad(clb_ctx* c, long n, const float* X, const float* T, float* grad, double* loss) {
+  if (!c || n < 1 || !X || !T || !grad) return CLB_ERR_ARG;
+  if (!c->d_theta) return fail(c, CLB_ERR_STATE, "clb_mlp_config first");
+  const MlpDims d = c->md;
+  const size_t fl = ((size_t)n * (d.din + d.dout) + (size_t)c->mlp_P + mlp_fwd_bwd_floats(d, n) + 1) & ~(size_t)1;
+  int rc = ensure_work(c, fl * sizeof(float) + 2 * sizeof(double));
+  if (rc) return rc;
+  float* dX = c->d_work;
+  float* dT = dX + (size_t)n * d.din;
+  float* dg = dT + (size_t)n * d.dout;
+  float* w = dg + c->mlp_P;
+  double* dl = (double*)(c->d_work + fl);
+  CLB_CHECK_HIP(c, hipMemcpyAsync(dX, X, sizeof(float) * n * d.din, hipMemcpyHostToDevice, c->stream));
+  CLB_CHECK_HIP(c, hipMemcpyAsync(dT, T, sizeof(float) * n * d.dout, hipMemcpyHostToDevice, c->stream));
+  launch_fill_f64(dl, 1, 0.0, c->stream);
+  mlp_fwd_bwd(c, n, dX, dT, dg, dl, w);
+  double h = 0;
+  CLB_CHECK_HIP(c, hipMemcpyAsync(grad, dg, sizeof(float) * c->mlp_P, hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipMemcpyAsync(&h, dl, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  if (loss) *loss = h / (double)(n * d.dout);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int clb_mlp_train(clb_ctx* c, long n, const float* X, const float* T, int epochs, int batch, double lr,
+                  uint64_t seed, double* final_loss) {
+  if (!c || n < 1 || !X || !T || epochs < 0 || !(lr > 0)) return CLB_ERR_ARG;
+  if (!c->d_theta) return fail(c, CLB_ERR_STATE, "clb_mlp_config first");
+  const MlpDims d = c->md;
+  float *dX = nullptr, *dT = nullptr;
+  int rc = dalloc(c, &dX, (size_t)n * d.din);
+  if (!rc) rc = dalloc(c, &dT, (size_t)n * d.dout);
+  if (rc) { dfree(dX); dfree(dT); return rc; }
+  hipError_t e = hipMemcpy(dX, X, sizeof(float) * n * d.din, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dT, T, sizeof(float) * n * d.dout, hipMemcpyHostToDevice);
+  if (e != hipSuccess) { dfree(dX); dfree(dT); return fail(c, CLB_ERR_HIP, hipGetErrorString(e)); }
+  rc = mlp_train_device(c, n, dX, dT, epochs, batch, lr, seed, final_loss);
+  (void)hipStreamSynchronize(c->stream);
+  dfree(dX);
+  dfree(dT);
+  return rc;
+}
+
+int clb_mlp_train_from_batch(clb_ctx* c, int epochs, int batch, double lr, uint64_t seed, double* final_loss) {
+  if (!c) return CLB_ERR_ARG;
+  if (c->bn < 1) return fail(c, CLB_ERR_STATE, "no solved batch");
+  const long long n = c->bn;
+  const int nc = c->tb.nc, nvar = c->bnvar, din = nc + nvar;
+  if (!c->d_theta || c->md.din != din || c->md.dout != nc)
+    return fail(c, CLB_ERR_STATE, "clb_mlp_config(nc + nvar, H, nc) first");
+  // scaling box from the solved batch (min / max of every input and output)
+  std::vector<double> t((size_t)n * nc), lk((size_t)n * (nvar > 0 ? nvar : 1)), lc((size_t)n * nc);
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipMemcpy(t.data(), c->d_bt, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+  if (nvar > 0) CLB_CHECK_HIP(c, hipMemcpy(lk.data(), c->d_blk, sizeof(double) * n * nvar, hipMemcpyDeviceToHost));
+  CLB_CHECK_HIP(c, hipMemcpy(lc.data(), c->d_blogc, sizeof(double) * lc.size(), hipMemcpyDeviceToHost));
+  std::vector<double> xmin(din, 1e300), xmax(din, -1e300), ymin(nc, 1e300), ymax(nc, -1e300);
+  for (long long v = 0; v < n; ++v) {
+    for (int k = 0; k < din; ++k) {
+      const double raw = (k < nc) ? std::log10(std::max(t[v * nc + k], 1e-30)) : lk[v * nvar + (k - nc)];
+      xmin[k] = std::min(xmin[k], raw);
+      xmax[k] = std::max(xmax[k], raw);
+    }
+    for (int k = 0; k < nc; ++k) {
+      ymin[k] = std::min(ymin[k], lc[v * nc + k]);
+      ymax[k] = std::max(ymax[k], lc[v * nc + k]);
+    }
+  }
+  int rc = clb_mlp_set_scaling(c, xmin.data(), xmax.data(), ymin.data(), ymax.data());
+  if (rc) return rc;
+  float *dX = nullptr, *dT = nullptr;
+  rc = dalloc(c, &dX, (size_t)n * din);
+  if (!rc) rc = dalloc(c, &dT, (size_t)n * nc);
+  if (rc) { dfree(dX); dfree(dT); return rc; }
+  SpecArgs sa{};
+  sa.n = n;
+  sa.totals = c->d_bt;
+  sa.logk_var = c->d_blk;
+  sa.nvar = nvar;
+  sa.logc = c->d_blogc;
+  launch_mlp_make_dataset(c->md, n, c->tb, sa, dX, dT, c->d_box, c->d_box + 32, c->d_box + 64, c->d_box + 96,
+                          c->stream);
+  // first 70 % train (the rest is held out for validation / test by the caller)
+  const long long ntrain = (n * 7) / 10;
+  rc = mlp_train_device(c, ntrain, dX, dT, epochs, batch, lr, seed, final_loss);
+  (void)hipStreamSynchronize(c->stream);
+  dfree(dX);
+  dfree(dT);
+  return rc;
+}
+
+// ---------------------------------------------------------------- measurement
+int clb_set_timing(clb_ctx* c, int enable) {
+  if (!c) return CLB_ERR_ARG;
+  if (c->timing && !enable) fold_timing(c);
+  c->timing = enable != 0;
+  return CLB_OK;
+}
+
+int clb_get_perf(clb_ctx* c, clb_perf* p) {
+  if (!c || !p) return CLB_ERR_ARG;
+  int rc = fold_timing(c);
+  if (rc) return rc;
+  *p = c->perf;
+  return CLB_OK;
+}
+
+int clb_reset_perf(clb_ctx* c) {
+  if (!c) return CLB_ERR_ARG;
+  fold_timing(c);
+  const long long f = c->perf.fluid_nodes, r = c->perf.reactive_nodes;
+  c->perf = clb_perf{};
+  c->perf.fluid_nodes = f;
+  c->perf.reactive_nodes = r;
+  launch_fill_u64(c->d_stats, 4, 0ull, c->stream);
+  return CLB_OK;
+}
+
+}  // extern "C"

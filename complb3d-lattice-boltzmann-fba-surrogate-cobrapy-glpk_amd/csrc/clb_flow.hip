@@ -1,0 +1,214 @@
+// clb_flow.hip -- D3Q19 incompressible BGK flow through the pore mask.
+//
+// Replaces Palabos IncBGKdynamics + BounceBack + NoDynamics and the regularized local
+// pressure boundaries that NSdomainSetup installs (src/complab_functions.hh:245-294),
+// stepped by the collideAndStream loops of src/complab.cpp:597-667.
+//
+// HBM traffic per fluid node and step: 19 populations read + 19 written
+// (2*19*sizeof(R)) + 1 mask byte; bounce-back / no-dynamics nodes cost the mask byte.
+#include "clb_lattice.h"
+
+namespace clb {
+
+namespace {
+
+// fbar_eq_i = t_i [rhobar + 3 c.j + 4.5 (c.j)^2 - 1.5 j^2]   (rho0 = 1)
+template <typename R>
+__device__ __forceinline__ R feq19(int i, R rhobar, R jx, R jy, R jz, R jsq) {
+  const R cj = R(Q19::cx[i]) * jx + R(Q19::cy[i]) * jy + R(Q19::cz[i]) * jz;
+  return R(Q19::w(i)) * (rhobar + R(3) * cj + R(4.5) * cj * cj - R(1.5) * jsq);
+}
+
+// Regularized pressure boundary on an x face (side -1: x = 0, side +1: x = nx-1).
+// j_x from the mass relation over the known populations, tangential momentum zero,
+// Pi_neq from on-wall (once) and normal (twice) populations, then all 19 populations
+// rebuilt as f_eq + t_i * 9/2 * Q_i:Pi_neq.
+template <typename R, int SIDE>
+__device__ __forceinline__ void regularize(R* p, R rhobar) {
+  R onwall = 0, normal = 0;
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    if (Q19::cx[i] == 0) onwall += p[i];
+    else if (Q19::cx[i] == SIDE) normal += p[i];
+  }
+  const R jx = (SIDE < 0) ? (rhobar - onwall - R(2) * normal) : -(rhobar - onwall - R(2) * normal);
+  const R jsq = jx * jx;
+  R pxx = 0, pxy = 0, pxz = 0, pyy = 0, pyz = 0, pzz = 0;
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    if (Q19::cx[i] != 0 && Q19::cx[i] != SIDE) continue;
+    const R wgt = (Q19::cx[i] == 0) ? R(1) : R(2);
+    const R fneq = wgt * (p[i] - feq19<R>(i, rhobar, jx, R(0), R(0), jsq));
+    const R cx = Q19::cx[i], cy = Q19::cy[i], cz = Q19::cz[i];
+    pxx += cx * cx * fneq; pxy += cx * cy * fneq; pxz += cx * cz * fneq;
+    pyy += cy * cy * fneq; pyz += cy * cz * fneq; pzz += cz * cz * fneq;
+  }
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    const R cx = Q19::cx[i], cy = Q19::cy[i], cz = Q19::cz[i];
+    const R third = R(1) / R(3);
+    const R qpi = (cx * cx - third) * pxx + (cy * cy - third) * pyy + (cz * cz - third) * pzz +
+                  R(2) * (cx * cy * pxy + cx * cz * pxz + cy * cz * pyz);
+    p[i] = feq19<R>(i, rhobar, jx, R(0), R(0), jsq) + R(Q19::w(i) * 4.5) * qpi;
+  }
+}
+
+template <typename R, bool ODD>
+__device__ __forceinline__ void load19(const R* __restrict__ f, long long N, const Nbr& nb, int n, R* p) {
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    if (!ODD) {
+      p[i] = f[(long long)i * N + n];
+    } else {
+      const int o = opp_dir(i);
+      if (nb.xok(-Q19::cx[i])) p[i] = f[(long long)o * N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])];
+      else p[i] = R(0);
+    }
+  }
+}
+
+template <typename R, bool ODD>
+__device__ __forceinline__ void store19(R* __restrict__ f, long long N, const Nbr& nb, int n, const R* p) {
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    if (!ODD) {
+      f[(long long)opp_dir(i) * N + n] = p[i];
+    } else {
+      if (nb.xok(Q19::cx[i])) f[(long long)i * N + nb.at(Q19::cx[i], Q19::cy[i], Q19::cz[i])] = p[i];
+    }
+  }
+}
+
+template <typename R, bool ODD, bool ENERGY>
+__global__ void __launch_bounds__(256) flow_step_kernel(FlowArgs<R> a) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  const Geom& g = a.g;
+  double e = 0.0, cnt = 0.0;
+  if (x < g.nx && y < g.ny) {
+    const int n = x + g.nx * (y + g.ny * z);
+    const uint8_t code = a.mask[n];
+    if (a.kind[code] == CLB_KIND_FLUID) {
+      const Nbr nb(x, y, z, g);
+      R p[19];
+      load19<R, ODD>(a.f, g.N, nb, n, p);
+      if (x == 0) regularize<R, -1>(p, a.rhobar_in);
+      else if (x == g.nx - 1) regularize<R, 1>(p, a.rhobar_out);
+      R rhobar = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) {
+        rhobar += p[i];
+        jx += R(Q19::cx[i]) * p[i];
+        jy += R(Q19::cy[i]) * p[i];
+        jz += R(Q19::cz[i]) * p[i];
+      }
+      const R om = a.omega[code];
+      const R jsq = jx * jx + jy * jy + jz * jz;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) p[i] = (R(1) - om) * p[i] + om * feq19<R>(i, rhobar, jx, jy, jz, jsq);
+      store19<R, ODD>(a.f, g.N, nb, n, p);
+      if (ENERGY) { e = 0.5 * (double)jsq; cnt = 1.0; }
+    }
+  }
+  if (ENERGY) {
+    __shared__ double se[256], sc[256];
+    const int t = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nt = blockDim.x * blockDim.y;
+    se[t] = e; sc[t] = cnt;
+    __syncthreads();
+    for (int s = nt / 2; s > 0; s >>= 1) {
+      if (t < s) { se[t] += se[t + s]; sc[t] += sc[t + s]; }
+      __syncthreads();
+    }
+    if (t == 0) { atomicAdd(&a.energy[0], se[0]); atomicAdd(&a.energy[1], sc[0]); }
+  }
+}
+
+template <typename R>
+__global__ void flow_init_kernel(R* f, Geom g, double rho_in, double rho_out) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  const int x = (int)(n % g.nx);
+  const double rho = rho_in + (rho_out - rho_in) * (double)x / (double)(g.nx - 1);
+#pragma unroll
+  for (int i = 0; i < 19; ++i) f[(long long)i * g.N + n] = (R)(Q19::w(i) * (rho - 1.0));
+}
+
+// Velocity (u = j, rho0 = 1) and density from the pre-collision populations of the next
+// step; boundary fluid nodes use the pressure-boundary mass relation. Optionally writes
+// the advecting-velocity planes of the solute lattices (latticeToPassiveAdvDiff).
+template <typename R, bool ODD>
+__global__ void flow_velocity_kernel(FlowArgs<R> a, double* u, double* rho, R* ade_u) {
+  const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const Geom& g = a.g;
+  if (n64 >= g.N) return;
+  const int n = (int)n64;
+  const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+  double jx = 0, jy = 0, jz = 0, rb = 0;
+  if (a.kind[a.mask[n]] == CLB_KIND_FLUID) {
+    const Nbr nb(x, y, z, g);
+    R p[19];
+    load19<R, ODD>(a.f, g.N, nb, n, p);
+    if (x == 0 || x == g.nx - 1) {
+      const int side = (x == 0) ? -1 : 1;
+      rb = (double)((x == 0) ? a.rhobar_in : a.rhobar_out);
+      double onwall = 0, normal = 0;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) {
+        if (Q19::cx[i] == 0) onwall += (double)p[i];
+        else if (Q19::cx[i] == side) normal += (double)p[i];
+      }
+      jx = (side < 0) ? (rb - onwall - 2.0 * normal) : -(rb - onwall - 2.0 * normal);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 19; ++i) {
+        rb += (double)p[i];
+        jx += Q19::cx[i] * (double)p[i];
+        jy += Q19::cy[i] * (double)p[i];
+        jz += Q19::cz[i] * (double)p[i];
+      }
+    }
+  }
+  if (u) { u[3 * n64] = jx; u[3 * n64 + 1] = jy; u[3 * n64 + 2] = jz; }
+  if (rho) rho[n64] = rb + 1.0;
+  if (ade_u) { ade_u[n64] = (R)jx; ade_u[g.N + n64] = (R)jy; ade_u[2 * g.N + n64] = (R)jz; }
+}
+
+}  // namespace
+
+template <typename R>
+void launch_flow_init(const Geom& g, R* f, double rho_in, double rho_out, hipStream_t s) {
+  const int bs = 256;
+  hipLaunchKernelGGL(flow_init_kernel<R>, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, f, g, rho_in,
+                     rho_out);
+}
+
+template <typename R>
+void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s) {
+  dim3 grid((a.g.nx + block.x - 1) / block.x, (a.g.ny + block.y - 1) / block.y, a.g.nz);
+  if (a.energy) {
+    if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((flow_step_kernel<R, false, true>), grid, block, 0, s, a);
+  } else {
+    if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, false>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((flow_step_kernel<R, false, false>), grid, block, 0, s, a);
+  }
+}
+
+template <typename R>
+void launch_flow_velocity(const FlowArgs<R>& a, bool odd_next, double* u, double* rho, R* ade_u, hipStream_t s) {
+  const int bs = 256;
+  dim3 grid((unsigned)((a.g.N + bs - 1) / bs));
+  if (odd_next) hipLaunchKernelGGL((flow_velocity_kernel<R, true>), grid, dim3(bs), 0, s, a, u, rho, ade_u);
+  else hipLaunchKernelGGL((flow_velocity_kernel<R, false>), grid, dim3(bs), 0, s, a, u, rho, ade_u);
+}
+
+template void launch_flow_init<float>(const Geom&, float*, double, double, hipStream_t);
+template void launch_flow_init<double>(const Geom&, double*, double, double, hipStream_t);
+template void launch_flow_step<float>(const FlowArgs<float>&, bool, dim3, hipStream_t);
+template void launch_flow_step<double>(const FlowArgs<double>&, bool, dim3, hipStream_t);
+template void launch_flow_velocity<float>(const FlowArgs<float>&, bool, double*, double*, float*, hipStream_t);
+template void launch_flow_velocity<double>(const FlowArgs<double>&, bool, double*, double*, double*, hipStream_t);
+
+}  // namespace clb

@@ -1,0 +1,141 @@
+// clb_internal.h -- shared declarations of the HIP path (context, kernel launchers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/clb.h"
+
+namespace clb {
+
+// ------------------------------------------------------------------ lattice tables
+// D3Q19 velocity set; opposite pairs are (1,2), (3,4), ... (17,18).
+#define CLB_D3Q19_CX {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0}
+#define CLB_D3Q19_CY {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1}
+#define CLB_D3Q19_CZ {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1}
+// D3Q7: rest, +x, -x, +y, -y, +z, -z; weights 1/4 and 1/8 (c_s^2 = 1/4).
+#define CLB_D3Q7_CX {0, 1, -1, 0, 0, 0, 0}
+#define CLB_D3Q7_CY {0, 0, 0, 1, -1, 0, 0}
+#define CLB_D3Q7_CZ {0, 0, 0, 0, 0, 1, -1}
+
+__host__ __device__ constexpr int opp_dir(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+
+// ------------------------------------------------------------------ kernel argument blocks
+struct Geom {
+  int nx, ny, nz;
+  long long N;
+};
+
+template <typename R>
+struct FlowArgs {
+  R* f;                   // 19*N, population-major, deviation from the weights
+  const uint8_t* mask;    // N
+  const uint8_t* kind;    // 256
+  const R* omega;         // 256
+  Geom g;
+  R rhobar_in, rhobar_out;
+  double* energy;         // accumulator (sum |u|^2/2, count) or null
+};
+
+template <typename R>
+struct AdeArgs {
+  R* g;                   // S*7*N population-major; stored as f_i - t_i*c_ref[s]
+  uint8_t* mask;          // updated in place by the biomass threshold rule
+  const uint8_t* kind;    // 256
+  const R* omega;         // S*256
+  const R* u;             // 3*N (x, y, z planes)
+  R* bmass;               // nb*N
+  double* cbuf;           // S*N post-kinetics concentrations for the speciation kernel (or null)
+  Geom geo;
+  int S, nb;
+  int bc_type[2 * CLB_MAX_SOLUTES];
+  R bc_value[2 * CLB_MAX_SOLUTES];
+  R c_ref[CLB_MAX_SOLUTES];
+  int kinetics;           // 0 none, 1 plugin
+  double dt;
+  int solid_code, bb_code;
+  int mask_update;
+  double bthreshold;
+  int pore_code, biofilm_code;
+};
+
+struct Tableau {
+  int nc, ns;
+  double S[CLB_MAX_SPECIES * CLB_MAX_COMPONENTS];
+  double logK[CLB_MAX_SPECIES];
+  double inv_mu0[CLB_MAX_COMPONENTS];  // 1 / (smallest positive coefficient <= 1)
+  int is_eq[CLB_MAX_SPECIES];          // row has a nonzero entry
+  int comp_species[CLB_MAX_COMPONENTS];
+  double tol;
+  int max_iter;
+};
+
+// Batched / coupled speciation problem description.
+struct SpecArgs {
+  long long n;               // number of solves (batch) or nodes (coupled)
+  const double* totals;      // n*nc (batch) or null
+  const double* logk_var;    // n*nvar or null
+  int nvar;
+  int var_species[4];
+  double* logc;              // n*nc in/out
+  int* iters;                // n
+  double* resid;             // n
+  uint8_t* conv;             // n
+  unsigned int* counter;     // work-queue head
+  unsigned long long* stats; // solves, converged, iterations
+};
+
+// ------------------------------------------------------------------ launchers
+template <typename R>
+void launch_flow_init(const Geom& g, R* f, double rho_in, double rho_out, hipStream_t s);
+template <typename R>
+void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s);
+template <typename R>
+void launch_flow_velocity(const FlowArgs<R>& a, bool odd_next, double* u, double* rho, R* ade_u, hipStream_t s);
+
+template <typename R>
+void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c_ref, hipStream_t s);
+template <typename R>
+void launch_ade_step(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t s);
+template <typename R>
+void launch_ade_conc(const AdeArgs<R>& a, bool odd_next, double* c, hipStream_t s);
+// Speciation on the coupled step: solve at every reactive node from cbuf and add the
+// limited increments to the populations the step just wrote.
+void launch_spec_coupled(const AdeArgs<float>& a, bool odd_done, const Tableau& tb, SpecArgs sa, int grid,
+                         hipStream_t s);
+void launch_spec_coupled(const AdeArgs<double>& a, bool odd_done, const Tableau& tb, SpecArgs sa, int grid,
+                         hipStream_t s);
+
+void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, hipStream_t s);
+int spec_occupancy_grid(int nc, int device);
+
+// MLP kernels (FP32 training, BF16 MFMA inference)
+struct MlpDims {
+  int din, H, dout;
+};
+void launch_gemm(int transA, int transB, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
+                 float* C, int ldc, const float* bias, int act, float beta, hipStream_t s);
+void launch_tanh_backward(long long n, float* dz, const float* h, hipStream_t s);
+void launch_colsum(int rows, int cols, const float* A, float* out, hipStream_t s);
+void launch_mse_grad(long long n, const float* Y, const float* T, float scale, float* dY, double* loss,
+                     hipStream_t s);
+void launch_adam(long long n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
+                 float eps, float b1t, float b2t, hipStream_t s);
+void launch_mlp_infer_bf16(const MlpDims& d, long long n, const float* theta, const float* X, float* Y,
+                           hipStream_t s);
+void launch_mlp_warm(const MlpDims& d, long long n, const float* theta, const double* xmin, const double* xmax,
+                     const double* ymin, const double* ymax, const Tableau& tb, const SpecArgs& sa, float* work,
+                     hipStream_t s);
+void launch_mlp_make_dataset(const MlpDims& d, long long n, const Tableau& tb, const SpecArgs& sa, float* X,
+                             float* T, const double* xmin, const double* xmax, const double* ymin,
+                             const double* ymax, hipStream_t s);
+void launch_fill_u32(unsigned int* p, long long n, unsigned int v, hipStream_t s);
+void launch_fill_u64(unsigned long long* p, long long n, unsigned long long v, hipStream_t s);
+void launch_fill_f64(double* p, long long n, double v, hipStream_t s);
+
+}  // namespace clb
+
+#define CLB_HIP_LAUNCH_CHECK() ((void)0)

@@ -1,0 +1,410 @@
+// clb_mlp.hip -- the speciation warm-start network: FP32 training kernels (forward,
+// backward, Adam) and a fused BF16 MFMA inference kernel with FP32 accumulation.
+//
+// Network din -> H -> H -> dout, tanh, tanh, linear; inputs and outputs min-max scaled
+// to [-1, 1] (the mapminmax convention of surrogate_training/trainSurrogate.py:182-201;
+// tansig hidden layers as in src/complab3d_processors_surrogate.hh:74-295). Training
+// minimises the mean squared error with Adam.
+#include "clb_lattice.h"
+
+namespace clb {
+
+namespace {
+
+// ------------------------------------------------------------------ FP32 tiled GEMM
+// C[M x N] = act(op(A) op(B) + bias) + beta * C, op = transpose when the flag is set.
+// A is M x K (or K x M when transA), B is K x N (or N x K when transB); row-major with
+// leading dimensions lda / ldb / ldc. act: 0 none, 1 tanh. Tile 64 x 64 x 16, 256
+// threads, 4 x 4 outputs per thread.
+constexpr int TM = 64, TN = 64, TK = 16;
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
+                                                   const float* __restrict__ B, int ldb, float* __restrict__ C,
+                                                   int ldc, const float* __restrict__ bias, int act, float beta) {
+  __shared__ float As[TK][TM + 4];
+  __shared__ float Bs[TK][TN + 4];
+  const int tid = threadIdx.x;
+  const int tx = tid % 16, ty = tid / 16;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  float acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
+  for (int k0 = 0; k0 < K; k0 += TK) {
+    // load A tile: TM x TK elements, 1024 -> 4 per thread
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * 256;
+      const int mm = e / TK, kk = e % TK;
+      const int gm = m0 + mm, gk = k0 + kk;
+      float v = 0.f;
+      if (gm < M && gk < K) v = TA ? A[(long long)gk * lda + gm] : A[(long long)gm * lda + gk];
+      As[kk][mm] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int e = tid + r * 256;
+      const int nn = e / TK, kk = e % TK;
+      const int gn = n0 + nn, gk = k0 + kk;
+      float v = 0.f;
+      if (gn < N && gk < K) v = TB ? B[(long long)gn * ldb + gk] : B[(long long)gk * ldb + gn];
+      Bs[kk][nn] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < TK; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = As[kk][ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = Bs[kk][tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int gm = m0 + ty * 4 + i;
+    if (gm >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int gn = n0 + tx * 4 + j;
+      if (gn >= N) continue;
+      float v = acc[i][j];
+      if (bias) v += bias[gn];
+      if (act == 1) v = tanhf(v);
+      float* c = &C[(long long)gm * ldc + gn];
+      *c = (beta != 0.f) ? v + beta * *c : v;
+    }
+  }
+}
+
+__global__ void tanh_backward_kernel(long long n, float* dz, const float* h) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) dz[i] *= (1.f - h[i] * h[i]);
+}
+
+// out[c] = sum_r A[r, c]; one block per column chunk, shared-memory reduction.
+__global__ void colsum_kernel(int rows, int cols, const float* A, float* out) {
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (int r = threadIdx.x; r < rows; r += blockDim.x) s += A[(long long)r * cols + c];
+  __shared__ float sh[256];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = sh[0];
+}
+
+__global__ void mse_grad_kernel(long long n, const float* Y, const float* T, float scale, float* dY, double* loss) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double e2 = 0.0;
+  if (i < n) {
+    const float e = Y[i] - T[i];
+    dY[i] = scale * e;
+    e2 = (double)e * (double)e;
+  }
+  if (loss) {
+    __shared__ double sh[256];
+    sh[threadIdx.x] = e2;
+    __syncthreads();
+    for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+      if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, sh[0]);
+  }
+}
+
+__global__ void adam_kernel(long long n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
+                            float eps, float b1t, float b2t) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float mh = mi / (1.f - b1t), vh = vi / (1.f - b2t);
+  p[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+// ------------------------------------------------------------------ BF16 MFMA inference
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+  unsigned int u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);                                          // round to nearest even
+  return (unsigned short)(u >> 16);
+}
+
+constexpr int HID = 128;     // hidden width of the MFMA kernel
+constexpr int KIN = 16;      // padded input width
+constexpr int NOUT = 32;     // padded output width
+constexpr int HPAD = HID + 4;
+constexpr int KPAD = KIN + 4;
+
+// Weights staged once per block into LDS as BF16, [out][in] with padded rows.
+struct MlpSmem {
+  unsigned short w1[HID * KPAD];
+  unsigned short w2[HID * HPAD];
+  unsigned short w3[NOUT * HPAD];
+  float b1[HID], b2[HID], b3[NOUT];
+  unsigned short act[4][32 * HPAD];  // per wavefront: 32 samples x HID activations
+};
+
+__device__ __forceinline__ bf16x4 lds_bf16x4(const unsigned short* p) {
+  bf16x4 r;
+  r[0] = (short)p[0]; r[1] = (short)p[1]; r[2] = (short)p[2]; r[3] = (short)p[3];
+  return r;
+}
+
+// D[32 x 32] += A[32 x K] * W^T where A rows are samples (LDS, row stride lda) and W is
+// [n][k] (LDS, row stride ldw); lane l holds sample (l % 32), k-group (l / 32).
+template <int KSTEPS>
+__device__ __forceinline__ f32x16 mfma_tile(const unsigned short* A, int lda, const unsigned short* W, int ldw,
+                                            int n0, f32x16 acc) {
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 31, kg = lane >> 5;
+#pragma unroll
+  for (int ks = 0; ks < KSTEPS; ++ks) {
+    const int k = ks * 8 + kg * 4;
+    const bf16x4 a = lds_bf16x4(A + i * lda + k);
+    const bf16x4 b = lds_bf16x4(W + (n0 + i) * ldw + k);
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// Output element r of the 32x32 accumulator on this lane: row (sample) and column.
+__device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+// Fused inference for 32 samples per wavefront. X holds din scaled inputs per sample
+// (fp32, row-major); Y receives dout scaled outputs.
+__device__ void mlp_tile(MlpSmem& sm, int din, int dout, const float* X, float* Y, long long s0, long long n) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  unsigned short* act = sm.act[wave];
+  // stage inputs (bf16) into the activation buffer as a 32 x KIN matrix
+  for (int e = lane; e < 32 * KIN; e += 64) {
+    const int i = e / KIN, k = e % KIN;
+    const long long s = s0 + i;
+    float v = 0.f;
+    if (s < n && k < din) v = X[s * din + k];
+    act[i * HPAD + k] = f2bf(v);
+  }
+  __syncthreads();
+  // layer 1: 4 column tiles of 32 hidden units, K = 16
+  f32x16 h[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x16 z = {0};
+    h[t] = mfma_tile<KIN / 8>(act, HPAD, sm.w1, KPAD, t * 32, z);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = acc_row(r, lane), col = t * 32 + (lane & 31);
+      act[row * HPAD + col] = f2bf(tanhf(h[t][r] + sm.b1[col]));
+    }
+  __syncthreads();
+  // layer 2: K = 128
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    f32x16 z = {0};
+    h[t] = mfma_tile<HID / 8>(act, HPAD, sm.w2, HPAD, t * 32, z);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int row = acc_row(r, lane), col = t * 32 + (lane & 31);
+      act[row * HPAD + col] = f2bf(tanhf(h[t][r] + sm.b2[col]));
+    }
+  __syncthreads();
+  // layer 3: one 32-wide output tile
+  f32x16 z = {0};
+  f32x16 o = mfma_tile<HID / 8>(act, HPAD, sm.w3, HPAD, 0, z);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = acc_row(r, lane), col = lane & 31;
+    const long long s = s0 + row;
+    if (s < n && col < dout) Y[s * dout + col] = o[r] + sm.b3[col];
+  }
+  __syncthreads();
+}
+
+__device__ void stage_weights(MlpSmem& sm, const MlpDims& d, const float* theta) {
+  const float* W1 = theta;
+  const float* b1 = W1 + HID * d.din;
+  const float* W2 = b1 + HID;
+  const float* b2 = W2 + HID * HID;
+  const float* W3 = b2 + HID;
+  const float* b3 = W3 + d.dout * HID;
+  for (int e = threadIdx.x; e < HID * KPAD; e += blockDim.x) {
+    const int o = e / KPAD, k = e % KPAD;
+    sm.w1[e] = f2bf((k < d.din) ? W1[o * d.din + k] : 0.f);
+  }
+  for (int e = threadIdx.x; e < HID * HPAD; e += blockDim.x) {
+    const int o = e / HPAD, k = e % HPAD;
+    sm.w2[e] = f2bf((k < HID) ? W2[o * HID + k] : 0.f);
+  }
+  for (int e = threadIdx.x; e < NOUT * HPAD; e += blockDim.x) {
+    const int o = e / HPAD, k = e % HPAD;
+    sm.w3[e] = f2bf((o < d.dout && k < HID) ? W3[o * HID + k] : 0.f);
+  }
+  for (int e = threadIdx.x; e < HID; e += blockDim.x) { sm.b1[e] = b1[e]; sm.b2[e] = b2[e]; }
+  for (int e = threadIdx.x; e < NOUT; e += blockDim.x) sm.b3[e] = (e < d.dout) ? b3[e] : 0.f;
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) mlp_infer_kernel(MlpDims d, long long n, const float* theta, const float* X,
+                                                        float* Y) {
+  extern __shared__ unsigned char smraw[];
+  MlpSmem& sm = *reinterpret_cast<MlpSmem*>(smraw);
+  stage_weights(sm, d, theta);
+  const int wave = threadIdx.x >> 6;
+  const long long tiles = (n + 31) / 32;
+  // all four wavefronts of a block step together so the block-wide barriers match
+  for (long long t0 = (long long)blockIdx.x * 4; t0 < tiles; t0 += (long long)gridDim.x * 4) {
+    const long long t = t0 + wave;
+    mlp_tile(sm, d.din, d.dout, X, Y, t * 32, (t < tiles) ? n : 0);
+  }
+}
+
+// Warm start for the batched Newton solve: inputs are log10 totals and the per-voxel logK
+// values, scaled with the training box; outputs unscaled into log activities. A voxel with
+// any input outside the box gets NaN, which the Newton kernel replaces by the reference
+// rule.
+__global__ void mlp_features_kernel(long long n, int nc, int nvar, const double* totals, const double* logk_var,
+                                    const double* xmin, const double* xmax, float* X, unsigned char* inbox) {
+  const long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int din = nc + nvar;
+  bool ok = true;
+  for (int k = 0; k < din; ++k) {
+    const double raw = (k < nc) ? log10(fmax(totals[v * nc + k], 1e-30)) : logk_var[v * nvar + (k - nc)];
+    if (raw < xmin[k] || raw > xmax[k]) ok = false;
+    const double span = xmax[k] - xmin[k];
+    X[v * din + k] = (float)((span > 0) ? 2.0 * (raw - xmin[k]) / span - 1.0 : 0.0);
+  }
+  if (inbox) inbox[v] = ok ? 1 : 0;
+}
+
+__global__ void mlp_unscale_kernel(long long n, int dout, const float* Y, const unsigned char* inbox,
+                                   const double* ymin, const double* ymax, double* logc) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= n * dout) return;
+  const long long v = e / dout;
+  const int k = (int)(e % dout);
+  const double y = (double)Y[e];
+  logc[e] = inbox[v] ? ymin[k] + 0.5 * (y + 1.0) * (ymax[k] - ymin[k]) : __builtin_nan("");
+}
+
+// Training pairs from a solved batch: X = scaled (log10 T, logK), T = scaled log activities.
+__global__ void mlp_dataset_kernel(long long n, int nc, int nvar, const double* totals, const double* logk_var,
+                                   const double* logc, const double* xmin, const double* xmax, const double* ymin,
+                                   const double* ymax, float* X, float* T) {
+  const long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int din = nc + nvar;
+  for (int k = 0; k < din; ++k) {
+    const double raw = (k < nc) ? log10(fmax(totals[v * nc + k], 1e-30)) : logk_var[v * nvar + (k - nc)];
+    const double span = xmax[k] - xmin[k];
+    X[v * din + k] = (float)((span > 0) ? 2.0 * (raw - xmin[k]) / span - 1.0 : 0.0);
+  }
+  for (int k = 0; k < nc; ++k) {
+    const double span = ymax[k] - ymin[k];
+    T[v * nc + k] = (float)((span > 0) ? 2.0 * (logc[v * nc + k] - ymin[k]) / span - 1.0 : 0.0);
+  }
+}
+
+}  // namespace
+
+void launch_gemm(int transA, int transB, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
+                 float* C, int ldc, const float* bias, int act, float beta, hipStream_t s) {
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+  if (!transA && !transB) hipLaunchKernelGGL((gemm_kernel<false, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta);
+  else if (!transA && transB) hipLaunchKernelGGL((gemm_kernel<false, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta);
+  else if (transA && !transB) hipLaunchKernelGGL((gemm_kernel<true, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta);
+  else hipLaunchKernelGGL((gemm_kernel<true, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta);
+}
+
+void launch_tanh_backward(long long n, float* dz, const float* h, hipStream_t s) {
+  hipLaunchKernelGGL(tanh_backward_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dz, h);
+}
+void launch_colsum(int rows, int cols, const float* A, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(colsum_kernel, dim3(cols), dim3(256), 0, s, rows, cols, A, out);
+}
+void launch_mse_grad(long long n, const float* Y, const float* T, float scale, float* dY, double* loss,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(mse_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, Y, T, scale, dY, loss);
+}
+void launch_adam(long long n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
+                 float b1t, float b2t, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, p, g, m, v, lr, b1, b2, eps,
+                     b1t, b2t);
+}
+
+// The fused kernel stages ~82 KB of weights and activations in LDS (CDNA4: 160 KB per
+// CU); raise the per-launch dynamic LDS limit once.
+static void mlp_smem_attr() {
+  static bool done = false;
+  if (!done) {
+    (void)hipFuncSetAttribute((const void*)mlp_infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)sizeof(MlpSmem));
+    done = true;
+  }
+}
+
+static int mlp_grid(long long n) {
+  const long long tiles = (n + 31) / 32;
+  long long blocks = (tiles + 3) / 4;
+  if (blocks > 2048) blocks = 2048;
+  if (blocks < 1) blocks = 1;
+  return (int)blocks;
+}
+
+void launch_mlp_infer_bf16(const MlpDims& d, long long n, const float* theta, const float* X, float* Y,
+                           hipStream_t s) {
+  mlp_smem_attr();
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(mlp_grid(n)), dim3(256), sizeof(MlpSmem), s, d, n, theta, X, Y);
+}
+
+void launch_mlp_warm(const MlpDims& d, long long n, const float* theta, const double* xmin, const double* xmax,
+                     const double* ymin, const double* ymax, const Tableau& tb, const SpecArgs& sa, float* work,
+                     hipStream_t s) {
+  // work: n*din features, n*dout outputs, n in-box flags
+  float* X = work;
+  float* Y = X + n * d.din;
+  unsigned char* inbox = reinterpret_cast<unsigned char*>(Y + n * d.dout);
+  hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, tb.nc, sa.nvar,
+                     sa.totals, sa.logk_var, xmin, xmax, X, inbox);
+  mlp_smem_attr();
+  hipLaunchKernelGGL(mlp_infer_kernel, dim3(mlp_grid(n)), dim3(256), sizeof(MlpSmem), s, d, n, theta, X, Y);
+  hipLaunchKernelGGL(mlp_unscale_kernel, dim3((unsigned)((n * d.dout + 255) / 256)), dim3(256), 0, s, n, d.dout, Y,
+                     inbox, ymin, ymax, sa.logc);
+}
+
+void launch_mlp_make_dataset(const MlpDims& d, long long n, const Tableau& tb, const SpecArgs& sa, float* X, float* T,
+                             const double* xmin, const double* xmax, const double* ymin, const double* ymax,
+                             hipStream_t s) {
+  (void)d;
+  hipLaunchKernelGGL(mlp_dataset_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, tb.nc, sa.nvar,
+                     sa.totals, sa.logk_var, sa.logc, xmin, xmax, ymin, ymax, X, T);
+}
+
+}  // namespace clb

@@ -1,0 +1,171 @@
+/* clb.h -- C ABI of the MI355X path for CompLB3D's per-timestep hot path.
+ *
+ * Plain C types only: pointers, sizes and doubles. The caller owns every host buffer;
+ * the library owns device memory. Every entry point returns CLB_OK (0) or a negative
+ * CLB_ERR_* code and never aborts; clb_last_error() gives the message of the last
+ * failure on a context. One context per host thread; a context holds one GPU's share
+ * of the domain (a single device).
+ *
+ * Lattice conventions: node index n = x + nx*(y + ny*z) (x fastest); the x faces are
+ * the inlet (x = 0) and outlet (x = nx-1) planes, y and z are periodic. Mask codes are
+ * uint8 material numbers (reference defaults: 0 solid / no dynamics, 1 bounce-back,
+ * 2 pore, src/complab_functions.hh:983-988); clb_set_dynamics maps each code to a
+ * dynamics kind and relaxation frequency, the way NSdomainSetup / soluteDomainSetup
+ * assign Palabos dynamics per material number (src/complab_functions.hh:245-340).
+ */
+#ifndef CLB_H
+#define CLB_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLB_OK 0
+#define CLB_ERR_ARG (-1)     /* bad argument (null pointer, size out of range) */
+#define CLB_ERR_HIP (-2)     /* HIP runtime failure */
+#define CLB_ERR_STATE (-3)   /* call out of order (e.g. stepping before init) */
+#define CLB_ERR_NOMEM (-4)   /* device allocation failed */
+#define CLB_ERR_UNSUP (-5)   /* configuration outside what the kernels are built for */
+
+#define CLB_PREC_FP64 0      /* FP64 storage and arithmetic */
+#define CLB_PREC_FP32 1      /* FP32 storage (D3Q19 as deviations from the weights), FP32 collision */
+
+#define CLB_KIND_NONE 0      /* Palabos NoDynamics */
+#define CLB_KIND_BB 1        /* Palabos BounceBack (node-based, full way) */
+#define CLB_KIND_FLUID 2     /* IncBGKdynamics (flow) / AdvectionDiffusionBGKdynamics (solutes) */
+
+#define CLB_BC_DIRICHLET 0
+#define CLB_BC_NEUMANN 1
+
+#define CLB_KINETICS_NONE 0
+#define CLB_KINETICS_PLUGIN 1   /* the defineKinetics.hh compiled into the library */
+
+#define CLB_WARM_REF 0       /* reference rule: log10 of the component totals / component species */
+#define CLB_WARM_USER 1      /* caller-provided log activities */
+#define CLB_WARM_MLP 2       /* MLP prediction (BF16 MFMA), reference rule outside the training box */
+
+#define CLB_MAX_SOLUTES 8
+#define CLB_MAX_BIOMASS 4
+#define CLB_MAX_COMPONENTS 8
+#define CLB_MAX_SPECIES 32
+
+typedef struct clb_ctx clb_ctx;
+
+typedef struct clb_config {
+  int device;     /* HIP device ordinal */
+  int precision;  /* CLB_PREC_* */
+  int nx, ny, nz; /* lattice size including the two x boundary planes; nx >= 4 */
+} clb_config;
+
+typedef struct clb_perf {
+  double flow_ms;        /* summed device time of D3Q19 step kernels (timing enabled) */
+  long long flow_launches;
+  double ade_ms;         /* D3Q7 transport + kinetics kernels */
+  long long ade_launches;
+  double spec_ms;        /* speciation kernels */
+  long long spec_launches;
+  double mlp_ms;         /* MLP inference kernels */
+  long long mlp_launches;
+  long long fluid_nodes; /* fluid nodes of the current mask */
+  long long reactive_nodes;
+} clb_perf;
+
+/* ---- context (replaces the MultiBlockLattice3D objects main owns, src/complab.cpp:592, 748-775) */
+int clb_create(const clb_config* cfg, clb_ctx** out);
+int clb_destroy(clb_ctx* ctx);
+const char* clb_last_error(const clb_ctx* ctx);
+int clb_sync(clb_ctx* ctx);
+int clb_device_name(char* buf, int len);
+
+/* ---- geometry (readGeometry + NSdomainSetup material assignment, src/complab_functions.hh:127-294) */
+int clb_upload_mask(clb_ctx* ctx, const uint8_t* mask);          /* N codes */
+int clb_download_mask(clb_ctx* ctx, uint8_t* mask);
+int clb_set_dynamics(clb_ctx* ctx, const int* kind /*256*/, const double* flow_omega /*256*/);
+
+/* ---- D3Q19 flow (NSdomainSetup + collideAndStream loops, src/complab_functions.hh:245-294,
+ *      src/complab.cpp:597-667). Density rho_in on x = 0, rho_out on x = nx-1. */
+int clb_flow_init(clb_ctx* ctx, double rho_in, double rho_out);
+int clb_flow_steps(clb_ctx* ctx, int nsteps, double* energy /* nsteps values or NULL */);
+int clb_flow_download_velocity(clb_ctx* ctx, double* u /*3N, interleaved*/, double* rho /*N or NULL*/);
+
+/* ---- D3Q7 solutes (soluteDomainSetup, src/complab_functions.hh:296-340; omega per
+ *      substrate, src/complab.cpp:708-727). omega: S*256 (solute-major, per mask code);
+ *      bc_type / bc_value: 2*S (west, east) CLB_BC_* and Dirichlet concentration. */
+int clb_set_solutes(clb_ctx* ctx, int S, const double* omega, const int* bc_type, const double* bc_value);
+int clb_solutes_init(clb_ctx* ctx, const double* c0 /*S*/);
+int clb_couple_velocity(clb_ctx* ctx);                        /* latticeToPassiveAdvDiff, src/complab.cpp:1097-1119 */
+int clb_set_velocity(clb_ctx* ctx, const double* u /*3N interleaved*/);
+int clb_set_biomass(clb_ctx* ctx, int nb, const double* b /*nb*N*/);
+int clb_download_conc(clb_ctx* ctx, double* c /*S*N*/);
+int clb_download_biomass(clb_ctx* ctx, double* b /*nb*N*/);
+
+/* ---- reactions inside the step
+ * clb_set_kinetics: run_kinetics + update_rxnLattices (src/complab3d_processors_part1.hh:72-127,
+ *   345-415) with the compiled defineRxnKinetics; dt = ade_dt; solid_code / bb_code are
+ *   the skipped material numbers.
+ * clb_set_tableau: EquilibriumChemistry setters + run_equilibrium_biotic
+ *   (src/complab3d_processors_part4_eqsolver.hh:1355-1494). S: ns*nc row-major over the
+ *   solutes (ns == number of solutes in the coupled step); comp_species[j]: the species
+ *   index named like component j, or -1. Solved by the FP64 Newton kernel to the
+ *   reference's convergence judge (PCF residual 2-norm < tol).
+ * clb_set_mask_update: pore <-> biofilm reclassification on total biomass
+ *   (updateLocalMaskNtotalLattices3D, src/complab3d_processors_part2.hh:99-210). */
+int clb_set_kinetics(clb_ctx* ctx, int kind, double dt, int solid_code, int bb_code);
+int clb_set_tableau(clb_ctx* ctx, int nc, int ns, const double* S, const double* logK, const int* comp_species,
+                    double tol, int max_iter);
+int clb_set_mask_update(clb_ctx* ctx, int enable, double threshold, int pore_code, int biofilm_code);
+int clb_step(clb_ctx* ctx, int nsteps);                       /* collide -> kinetics -> speciation -> stream */
+int clb_speciation_stats(clb_ctx* ctx, long long* stats /*3: solves, converged, iterations*/);
+
+/* ---- batched speciation (the per-voxel solve of EquilibriumChemistry, standalone)
+ * totals: n*nc component totals; var_species[nvar] + logk_var[n*nvar]: per-voxel logK of
+ * nvar species (NULL / 0 when the tableau's logK holds everywhere); logc: n*nc log10
+ * activities (input for CLB_WARM_USER, output always); iters/resid/conv optional. */
+int clb_batch_load(clb_ctx* ctx, long n, const double* totals, int nvar, const int* var_species,
+                   const double* logk_var);
+int clb_batch_solve(clb_ctx* ctx, int warm, const double* logc_in);
+int clb_batch_fetch(clb_ctx* ctx, double* logc, int* iters, double* resid, uint8_t* conv);
+
+/* ---- MLP warm start: din -> H -> H -> dout, tanh, tanh, linear; inputs and outputs
+ * min-max scaled to [-1, 1] (surrogate_training/trainSurrogate.py:182-201 convention).
+ * theta layout (row-major, out x in): W1[H*din] b1[H] W2[H*H] b2[H] W3[dout*H] b3[dout]. */
+int clb_mlp_config(clb_ctx* ctx, int din, int hidden, int dout);
+int clb_mlp_set_params(clb_ctx* ctx, const float* theta);
+int clb_mlp_get_params(clb_ctx* ctx, float* theta);
+int clb_mlp_set_scaling(clb_ctx* ctx, const double* xmin, const double* xmax, const double* ymin,
+                        const double* ymax);
+int clb_mlp_forward(clb_ctx* ctx, long n, const float* X /*n*din, scaled*/, float* Y /*n*dout, scaled*/,
+                    int bf16);
+int clb_mlp_grad(clb_ctx* ctx, long n, const float* X, const float* T, float* grad, double* loss);
+int clb_mlp_train(clb_ctx* ctx, long n, const float* X, const float* T, int epochs, int batch, double lr,
+                  uint64_t seed, double* final_loss);
+/* Train the speciation warm start on the loaded batch's own Newton solutions:
+ * generates inputs (log10 totals + per-voxel logK) and targets (log activities) on the
+ * device, fits the scaling box, trains. */
+int clb_mlp_train_from_batch(clb_ctx* ctx, int epochs, int batch, double lr, uint64_t seed, double* final_loss);
+
+/* ---- measurement */
+int clb_set_timing(clb_ctx* ctx, int enable);
+int clb_get_perf(clb_ctx* ctx, clb_perf* perf);
+int clb_reset_perf(clb_ctx* ctx);
+
+/* ---- synthetic workloads (input generation; host code, no GPU needed)
+ * Sphere pack: overlapping spheres, radius U[rmin, rmax], centres uniform in the interior
+ * x range and periodic in y/z, added until porosity <= target; planes 1 and nx-2 copied
+ * to 0 and nx-1 (readGeometry, src/complab_functions.hh:142-155); solid voxels with a
+ * pore D3Q19 neighbour get bb_code, other solid solid_code, pore pore_code. */
+int clb_gen_sphere_pack(int nx, int ny, int nz, double rmin, double rmax, double porosity, uint64_t seed,
+                        int pore_code, int solid_code, int bb_code, uint8_t* mask);
+int clb_gen_tableau(int nc, int ns, double logk_min, double logk_max, int smin, int smax, uint64_t seed,
+                    double log_t_min, double log_t_max, double* S, double* logK);
+int clb_gen_totals_field(int nx, int ny, int nz, int nc, double sigma, double log_t_min, double log_t_max,
+                         uint64_t seed, double* totals /*N*nc*/);
+int clb_gen_biomass_seed(long N, const uint8_t* mask, int pore_code, double fraction, double bmin, double bmax,
+                         uint64_t seed_sites, uint64_t seed_density, double* b /*N*/);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CLB_H */

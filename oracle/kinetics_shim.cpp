@@ -1,0 +1,20 @@
+// kinetics_shim.cpp -- TEST INFRASTRUCTURE: compiles a defineKinetics.hh plugin as
+// host C++ and adapts it to the oracle's flattened hook type, the way the reference's
+// run_kinetics processor builds std::vector arguments per voxel
+// (src/complab3d_processors_part1.hh:79-100). The plugin path comes from
+// -DCLB_KINETICS_HEADER.
+#include <cstddef>
+#include <vector>
+
+namespace plb { typedef long long plint; }
+
+#include CLB_KINETICS_HEADER
+
+extern "C" __attribute__((visibility("default"))) void orc_kinetics_plugin(const double* B, int nb, const double* C,
+                                                                            int ns, double* subsR, double* bioR,
+                                                                            long long mask) {
+  std::vector<double> b(B, B + nb), c(C, C + ns), sr(ns, 0.0), br(nb, 0.0);
+  defineRxnKinetics(b, c, sr, br, (plb::plint)mask);
+  for (int i = 0; i < ns; ++i) subsR[i] = sr[i];
+  for (int i = 0; i < nb; ++i) bioR[i] = br[i];
+}

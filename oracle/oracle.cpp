@@ -1,0 +1,938 @@
+// oracle.cpp -- FP64 CPU restatement of CompLB3D's per-timestep hot path.
+//
+// TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP path: only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The
+// product never links, loads or calls it.
+//
+// What is restated, and from where (reference paths relative to the repository root
+// of CompLB3D):
+//   * D3Q19 incompressible BGK flow (Palabos IncBGKdynamics, used at
+//     src/complab_functions.hh:256-276), node-based full-way BounceBack / NoDynamics
+//     (src/complab_functions.hh:264-268), regularized local pressure boundaries on the
+//     x faces (src/complab_functions.hh:285-288), initial linear density profile
+//     (PressureGradient, src/complab_functions.hh:72-84).
+//   * D3Q7 advection-diffusion BGK per solute (Palabos AdvectionDiffusionBGKdynamics,
+//     src/complab_functions.hh:309-322), Dirichlet ("temperature") and Neumann
+//     (FlatAdiabaticBoundaryFunctional3D) x faces (src/complab_functions.hh:325-332).
+//   * The coupled step order collide -> kinetics -> apply -> speciation -> mask -> stream
+//     (src/complab.cpp:1252-1849).
+//   * run_kinetics / update_rxnLattices (src/complab3d_processors_part1.hh:72-127, 345-415).
+//   * run_equilibrium_biotic with its 10 % change limiter
+//     (src/complab3d_processors_part4_eqsolver.hh:1355-1494).
+//   * EquilibriumChemistry::calculate_species_concentrations, the PCF fixed point with
+//     Anderson acceleration (src/complab3d_processors_part4_eqsolver.hh:442-1214).
+//   * The reserve Newton-Raphson solver
+//     (src/complab3d_processors_part4_eqsolver_NR_reserve.hh:309-615).
+//   * Biomass threshold mask reclassification (src/complab3d_processors_part2.hh:99-210,
+//     deterministic single-pore-type form) and precipitation / dissolution node
+//     conversion (src/precipitationVOP.hh:176-204, src/dissolutionVOP.hh:469-511).
+//   * MLP forward / backward (tanh hidden layers, linear output, mean-squared error),
+//     standard back-propagation as in surrogate_training/trainSurrogate.py:132-166.
+//
+// Parity status of each part (what pins it) is listed in the test files; the Palabos
+// arithmetic is not available anywhere, so D3Q19/D3Q7 are pinned by closed forms
+// (Poiseuille, linear diffusion profile, mass conservation), not by Palabos itself.
+//
+// Storage convention: populations are stored as deviations from the lattice weights,
+// fbar_i = f_i - t_i, so rho = sum(fbar) + 1 (src/complab3d_processors_part4_eqsolver.hh:
+// 1768-1774). Node index idx = x + nx*(y + ny*z); population i of node idx lives at
+// f[i*N + idx].
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define ORC_API extern "C" __attribute__((visibility("default")))
+
+namespace {
+
+// ---------------------------------------------------------------- lattices
+const int C19[19][3] = {
+    {0, 0, 0},  {1, 0, 0},  {-1, 0, 0}, {0, 1, 0},   {0, -1, 0}, {0, 0, 1},   {0, 0, -1},
+    {1, 1, 0},  {-1, -1, 0}, {1, -1, 0}, {-1, 1, 0}, {1, 0, 1},  {-1, 0, -1}, {1, 0, -1},
+    {-1, 0, 1}, {0, 1, 1},  {0, -1, -1}, {0, 1, -1}, {0, -1, 1}};
+const double W19[19] = {1.0 / 3,  1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18, 1.0 / 18,
+                        1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36,
+                        1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+const int C7[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+const double W7[7] = {0.25, 0.125, 0.125, 0.125, 0.125, 0.125, 0.125};
+inline int opp(int i) { return i == 0 ? 0 : ((i & 1) ? i + 1 : i - 1); }
+
+// Dynamics kinds per mask code.
+enum { KIND_NONE = 0, KIND_BB = 1, KIND_FLUID = 2 };
+
+struct Grid {
+  int nx, ny, nz;
+  long N;
+  long idx(int x, int y, int z) const { return x + (long)nx * (y + (long)ny * z); }
+  int wy(int y) const { return (y + ny) % ny; }
+  int wz(int z) const { return (z + nz) % nz; }
+};
+
+// Incompressible (rho0 = 1) second-order equilibrium in deviation form:
+// fbar_eq_i = t_i [ rhobar + 3 c.j + 4.5 (c.j)^2 - 1.5 j^2 ].
+inline double feq19(int i, double rhobar, double jx, double jy, double jz, double jsq) {
+  double cj = C19[i][0] * jx + C19[i][1] * jy + C19[i][2] * jz;
+  return W19[i] * (rhobar + 3.0 * cj + 4.5 * cj * cj - 1.5 * jsq);
+}
+
+// Regularized density (pressure) boundary on an x face. side = -1 for x = 0 (unknown
+// populations have c_x = +1), +1 for x = nx-1 (unknown populations have c_x = -1).
+// Known: c_x = 0 ("on wall") and c_x = side ("normal", streamed in from the interior).
+// j_x follows from mass: rho = onwall + normal + unknown, j_x = sum c_x f; tangential
+// momentum is set to zero. Pi_neq uses on-wall populations once and normal populations
+// twice (bounce-back of the non-equilibrium part for the unknown ones), then every
+// population is rebuilt as f_eq + t_i * 9/2 * Q_i : Pi_neq.
+void regularize_pressure(double* f, double rhobar, int side) {
+  double onwall = 0, normal = 0;
+  for (int i = 0; i < 19; ++i) {
+    if (C19[i][0] == 0) onwall += f[i];
+    else if (C19[i][0] == side) normal += f[i];
+  }
+  double jx = (side < 0) ? (rhobar - onwall - 2.0 * normal) : -(rhobar - onwall - 2.0 * normal);
+  double jsq = jx * jx;
+  double pi[6] = {0, 0, 0, 0, 0, 0};  // xx xy xz yy yz zz
+  for (int i = 0; i < 19; ++i) {
+    double wgt;
+    if (C19[i][0] == 0) wgt = 1.0;
+    else if (C19[i][0] == side) wgt = 2.0;
+    else continue;
+    double fneq = f[i] - feq19(i, rhobar, jx, 0, 0, jsq);
+    double cx = C19[i][0], cy = C19[i][1], cz = C19[i][2];
+    pi[0] += wgt * cx * cx * fneq;
+    pi[1] += wgt * cx * cy * fneq;
+    pi[2] += wgt * cx * cz * fneq;
+    pi[3] += wgt * cy * cy * fneq;
+    pi[4] += wgt * cy * cz * fneq;
+    pi[5] += wgt * cz * cz * fneq;
+  }
+  for (int i = 0; i < 19; ++i) {
+    double cx = C19[i][0], cy = C19[i][1], cz = C19[i][2];
+    double qpi = (cx * cx - 1.0 / 3) * pi[0] + (cy * cy - 1.0 / 3) * pi[3] + (cz * cz - 1.0 / 3) * pi[5] +
+                 2.0 * (cx * cy * pi[1] + cx * cz * pi[2] + cy * cz * pi[4]);
+    f[i] = feq19(i, rhobar, jx, 0, 0, jsq) + W19[i] * 4.5 * qpi;
+  }
+}
+
+void bgk19(double* f, double omega, double* jout) {
+  double rhobar = 0, jx = 0, jy = 0, jz = 0;
+  for (int i = 0; i < 19; ++i) {
+    rhobar += f[i];
+    jx += C19[i][0] * f[i];
+    jy += C19[i][1] * f[i];
+    jz += C19[i][2] * f[i];
+  }
+  double jsq = jx * jx + jy * jy + jz * jz;
+  for (int i = 0; i < 19; ++i) f[i] = (1.0 - omega) * f[i] + omega * feq19(i, rhobar, jx, jy, jz, jsq);
+  if (jout) { jout[0] = jx; jout[1] = jy; jout[2] = jz; }
+}
+
+}  // namespace
+
+// =====================================================================================
+// D3Q19 flow
+// =====================================================================================
+
+// Initial state: every node at equilibrium with u = 0 and the linear density profile
+// rho(x) = rho_in + (rho_out - rho_in) x/(nx-1) (PressureGradient,
+// src/complab_functions.hh:72-84, which writes 1 - dP*3*x/(nx-1)).
+ORC_API void orc_flow_init(int nx, int ny, int nz, double rho_in, double rho_out, double* f) {
+  Grid g{nx, ny, nz, (long)nx * ny * nz};
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        double rho = rho_in + (rho_out - rho_in) * (double)x / (double)(nx - 1);
+        for (int i = 0; i < 19; ++i) f[i * g.N + g.idx(x, y, z)] = W19[i] * (rho - 1.0);
+      }
+}
+
+// nsteps of collide + stream (Palabos collideAndStream, src/complab.cpp:607-620).
+// kind[code] selects NONE / BB / FLUID, omega[code] the BGK relaxation per mask code.
+// Pressure boundaries act on the fluid nodes of the x = 0 and x = nx-1 planes; y and z
+// are periodic. energy[t] (if not null) receives the mean of |u|^2/2 over fluid nodes
+// after the collision of step t (the quantity the reference's ValueTracer follows).
+ORC_API void orc_flow_run(int nx, int ny, int nz, const uint8_t* mask, const int* kind, const double* omega,
+                          double rho_in, double rho_out, double* f, int nsteps, double* energy) {
+  Grid g{nx, ny, nz, (long)nx * ny * nz};
+  std::vector<double> ftmp(19 * g.N);
+  for (int t = 0; t < nsteps; ++t) {
+    double esum = 0;
+    long ecount = 0;
+    // stale (unknown) boundary populations keep their previous values, as in a
+    // non-periodic Palabos block; copy first so the push below can leave them alone.
+    std::memcpy(ftmp.data(), f, sizeof(double) * 19 * g.N);
+#pragma omp parallel for collapse(2) reduction(+ : esum, ecount) schedule(static)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          long n = g.idx(x, y, z);
+          double p[19];
+          for (int i = 0; i < 19; ++i) p[i] = f[i * g.N + n];
+          int k = kind[mask[n]];
+          if (k == KIND_FLUID) {
+            if (x == 0) regularize_pressure(p, rho_in - 1.0, -1);
+            else if (x == nx - 1) regularize_pressure(p, rho_out - 1.0, +1);
+            double j[3];
+            bgk19(p, omega[mask[n]], j);
+            esum += 0.5 * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
+            ecount += 1;
+          } else if (k == KIND_BB) {
+            double q[19];
+            for (int i = 0; i < 19; ++i) q[i] = p[opp(i)];
+            for (int i = 0; i < 19; ++i) p[i] = q[i];
+          }
+          // push stream
+          for (int i = 0; i < 19; ++i) {
+            int xx = x + C19[i][0];
+            if (xx < 0 || xx >= nx) continue;
+            int yy = g.wy(y + C19[i][1]), zz = g.wz(z + C19[i][2]);
+            ftmp[i * g.N + g.idx(xx, yy, zz)] = p[i];
+          }
+        }
+    f = (double*)std::memcpy(f, ftmp.data(), sizeof(double) * 19 * g.N);
+    if (energy) energy[t] = ecount ? esum / ecount : 0.0;
+  }
+}
+
+// Velocity field from post-stream populations: u = j (incompressible, rho0 = 1) on fluid
+// nodes; on boundary fluid nodes j_x comes from the pressure-boundary mass relation;
+// u = 0 on solid nodes. rho (optional) receives sum(fbar) + 1 (the prescribed value on
+// boundary fluid nodes).
+ORC_API void orc_flow_velocity(int nx, int ny, int nz, const uint8_t* mask, const int* kind, double rho_in,
+                               double rho_out, const double* f, double* u, double* rho) {
+  Grid g{nx, ny, nz, (long)nx * ny * nz};
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        long n = g.idx(x, y, z);
+        double p[19];
+        for (int i = 0; i < 19; ++i) p[i] = f[i * g.N + n];
+        double jx = 0, jy = 0, jz = 0, rb = 0;
+        if (kind[mask[n]] == KIND_FLUID) {
+          if (x == 0 || x == nx - 1) {
+            int side = (x == 0) ? -1 : 1;
+            rb = (x == 0 ? rho_in : rho_out) - 1.0;
+            double onwall = 0, normal = 0;
+            for (int i = 0; i < 19; ++i) {
+              if (C19[i][0] == 0) onwall += p[i];
+              else if (C19[i][0] == side) normal += p[i];
+            }
+            jx = (side < 0) ? (rb - onwall - 2 * normal) : -(rb - onwall - 2 * normal);
+          } else {
+            for (int i = 0; i < 19; ++i) {
+              rb += p[i];
+              jx += C19[i][0] * p[i];
+              jy += C19[i][1] * p[i];
+              jz += C19[i][2] * p[i];
+            }
+          }
+        }
+        u[3 * n + 0] = jx;
+        u[3 * n + 1] = jy;
+        u[3 * n + 2] = jz;
+        if (rho) rho[n] = rb + 1.0;
+      }
+}
+
+// =====================================================================================
+// Equilibrium speciation: PCF + Anderson (active reference solver) and reserve Newton
+// =====================================================================================
+namespace {
+
+struct Tableau {
+  int nc, ns;
+  const double* S;     // ns x nc, row-major (species rows)
+  const double* logK;  // ns
+  const int* comp_species;  // nc: species index named like component j, or -1
+};
+
+const double MIN_CONC = 1e-30, MAX_CONC = 10.0, MIN_LOG_C = -30.0, MAX_LOG_C = 1.0;
+
+bool is_eq_species(const Tableau& tb, int i) {
+  for (int j = 0; j < tb.nc; ++j)
+    if (std::fabs(tb.S[i * tb.nc + j]) > 1e-10) return true;
+  return false;
+}
+
+// calc_species (src/complab3d_processors_part4_eqsolver.hh:442-496)
+void calc_species(const Tableau& tb, const double* logC, const double* init, double* conc) {
+  for (int i = 0; i < tb.ns; ++i) {
+    if (!is_eq_species(tb, i)) {
+      conc[i] = (init[i] > MIN_CONC) ? std::min(init[i], MAX_CONC) : MIN_CONC;
+      continue;
+    }
+    double lc = tb.logK[i];
+    for (int j = 0; j < tb.nc; ++j) lc += tb.S[i * tb.nc + j] * logC[j];
+    lc = std::max(std::min(lc, MAX_LOG_C), MIN_LOG_C);
+    double c = std::pow(10.0, lc);
+    if (std::isnan(c) || std::isinf(c)) c = (init[i] > MIN_CONC) ? init[i] : MIN_CONC;
+    conc[i] = std::max(std::min(c, MAX_CONC), MIN_CONC);
+  }
+}
+
+// calc_component_totals (:520-538)
+void calc_totals(const Tableau& tb, const double* conc, double* T) {
+  for (int j = 0; j < tb.nc; ++j) {
+    double s = 0;
+    for (int i = 0; i < tb.ns; ++i) {
+      if (!is_eq_species(tb, i)) continue;
+      double cs = std::max(std::min(conc[i], MAX_CONC), MIN_CONC);
+      s += tb.S[i * tb.nc + j] * cs;
+    }
+    T[j] = std::max(s, MIN_CONC);
+  }
+}
+
+// pcf_residual (:664-726) with calc_reactive_product_sums (:585-635)
+void pcf_residual(const Tableau& tb, const double* logC, const double* T, const double* init, double* f) {
+  std::vector<double> c(tb.ns);
+  calc_species(tb, logC, init, c.data());
+  for (int j = 0; j < tb.nc; ++j) {
+    double sp = 0, sn = 0;
+    for (int i = 0; i < tb.ns; ++i) {
+      if (!is_eq_species(tb, i)) continue;
+      double mu = tb.S[i * tb.nc + j];
+      double ci = std::max(c[i], MIN_CONC);
+      if (mu > 1e-15) sp += mu * ci;
+      else if (mu < -1e-15) sn += std::fabs(mu) * ci;
+    }
+    double SR, SP;
+    if (T[j] >= 0.0) { SR = sp; SP = T[j] + sn; }
+    else { SR = std::fabs(T[j]) + sp; SP = sn; }
+    SR = std::max(SR, MIN_CONC);
+    SP = std::max(SP, MIN_CONC);
+    double mu0 = 1.0;
+    for (int i = 0; i < tb.ns; ++i) {
+      if (!is_eq_species(tb, i)) continue;
+      double mu = tb.S[i * tb.nc + j];
+      if (mu > 1e-10 && mu < mu0) mu0 = mu;
+    }
+    double fj = (1.0 / mu0) * (std::log10(SP) - std::log10(SR));
+    fj = std::max(std::min(fj, 10.0), -10.0);
+    if (std::isnan(fj) || std::isinf(fj)) fj = 0.0;
+    if (T[j] <= 1e-18) fj = 0.0;
+    f[j] = fj;
+  }
+}
+
+double norm2(const std::vector<double>& v) {
+  double s = 0;
+  for (double x : v) s += x * x;
+  return std::sqrt(s);
+}
+
+// qr_decomposition (:817-868): modified Gram-Schmidt on the columns, condition number
+// max|R_jj| / min|R_jj|.
+void qr_mgs(const std::vector<std::vector<double>>& cols, std::vector<std::vector<double>>& Q,
+            std::vector<std::vector<double>>& R, double& cond) {
+  size_t m = cols.size();
+  if (m == 0) { cond = 1.0; return; }
+  size_t n = cols[0].size();
+  Q.assign(m, std::vector<double>());
+  R.assign(m, std::vector<double>(m, 0.0));
+  double rmax = 0, rmin = std::numeric_limits<double>::max();
+  for (size_t j = 0; j < m; ++j) {
+    Q[j] = cols[j];
+    for (size_t i = 0; i < j; ++i) {
+      double d = 0;
+      for (size_t k = 0; k < n && k < cols[j].size(); ++k) d += Q[i][k] * cols[j][k];
+      R[i][j] = d;
+      for (size_t k = 0; k < n; ++k) Q[j][k] -= R[i][j] * Q[i][k];
+    }
+    R[j][j] = norm2(Q[j]);
+    if (R[j][j] > 1e-15)
+      for (size_t k = 0; k < n; ++k) Q[j][k] /= R[j][j];
+    double a = std::fabs(R[j][j]);
+    if (a > rmax) rmax = a;
+    if (a > 1e-30 && a < rmin) rmin = a;
+  }
+  cond = (rmin > 1e-30) ? rmax / rmin : std::numeric_limits<double>::max();
+}
+
+// solve_equilibrium_anderson (:952-1172). Returns converged flag; iters/resid out.
+bool solve_anderson(const Tableau& tb, const double* init, const double* T, double tol, int maxIter, int depth,
+                    double condTol, double beta, double* out, int* iters, double* resid, double* logc_out) {
+  int nc = tb.nc;
+  *iters = 0;
+  *resid = 0;
+  std::vector<double> omega(nc), f0(nc), omega_new(nc);
+  for (int j = 0; j < nc; ++j) {
+    double cc = 1e-7;
+    int idx = tb.comp_species ? tb.comp_species[j] : -1;
+    if (idx >= 0 && idx < tb.ns) cc = init[idx];
+    cc = std::max(std::min(cc, MAX_CONC), MIN_CONC);
+    omega[j] = std::log10(cc);
+  }
+  pcf_residual(tb, omega.data(), T, init, f0.data());
+  for (int j = 0; j < nc; ++j) omega_new[j] = std::max(std::min(omega[j] + f0[j], MAX_LOG_C), MIN_LOG_C);
+  std::vector<std::vector<double>> xh, fh;
+  xh.push_back(omega);
+  fh.push_back(f0);
+  omega = omega_new;
+  for (int iter = 1; iter < maxIter; ++iter) {
+    *iters = iter;
+    std::vector<double> fk(nc);
+    pcf_residual(tb, omega.data(), T, init, fk.data());
+    double fn = norm2(fk);
+    *resid = fn;
+    if (fn < tol) {
+      calc_species(tb, omega.data(), init, out);
+      if (logc_out) for (int j = 0; j < nc; ++j) logc_out[j] = omega[j];
+      return true;
+    }
+    xh.push_back(omega);
+    fh.push_back(fk);
+    int mk = std::min(depth, (int)fh.size() - 1);
+    if (mk < 1) {
+      for (int j = 0; j < nc; ++j) omega_new[j] = omega[j] + fk[j];
+    } else {
+      std::vector<std::vector<double>> dF, dX;
+      size_t hs = fh.size();
+      for (int i = 0; i < mk; ++i) {
+        size_t id = hs - mk - 1 + i;
+        std::vector<double> a(nc), b(nc);
+        for (int j = 0; j < nc; ++j) {
+          a[j] = fh[id + 1][j] - fh[id][j];
+          b[j] = xh[id + 1][j] - xh[id][j];
+        }
+        dF.push_back(a);
+        dX.push_back(b);
+      }
+      std::vector<std::vector<double>> Q, R;
+      double cond;
+      qr_mgs(dF, Q, R, cond);
+      while (cond > condTol && dF.size() > 1) {
+        dF.erase(dF.begin());
+        dX.erase(dX.begin());
+        qr_mgs(dF, Q, R, cond);
+      }
+      size_t m = dF.size();
+      std::vector<double> qtf(m), gam(m, 0.0);
+      for (size_t i = 0; i < m; ++i) {
+        double d = 0;
+        for (int k = 0; k < nc; ++k) d += Q[i][k] * fk[k];
+        qtf[i] = d;
+      }
+      for (int i = (int)m - 1; i >= 0; --i) {  // solve_upper_triangular (:891-911)
+        gam[i] = qtf[i];
+        for (size_t j = i + 1; j < m; ++j) gam[i] -= R[i][j] * gam[j];
+        if (std::fabs(R[i][i]) > 1e-30) gam[i] /= R[i][i];
+      }
+      for (int j = 0; j < nc; ++j) {
+        double dxg = 0, dfg = 0;
+        for (size_t i = 0; i < m; ++i) {
+          dxg += dX[i][j] * gam[i];
+          dfg += dF[i][j] * gam[i];
+        }
+        omega_new[j] = omega[j] - dxg + beta * (fk[j] - dfg);
+      }
+    }
+    for (int j = 0; j < nc; ++j) {
+      double v = std::max(std::min(omega_new[j], MAX_LOG_C), MIN_LOG_C);
+      if (std::isnan(omega_new[j]) || std::isinf(omega_new[j])) v = omega[j];
+      omega_new[j] = v;
+    }
+    omega = omega_new;
+    while (xh.size() > (size_t)(depth + 1)) {
+      xh.erase(xh.begin());
+      fh.erase(fh.begin());
+    }
+  }
+  calc_species(tb, omega.data(), init, out);
+  if (logc_out) for (int j = 0; j < nc; ++j) logc_out[j] = omega[j];
+  return false;
+}
+
+// calculate_species_concentrations (:1187-1214): totals, solve, NaN guard and clamp.
+bool speciate_pcf(const Tableau& tb, const double* init, double tol, int maxIter, double* out, int* iters,
+                  double* resid, double* logc_out) {
+  std::vector<double> T(tb.nc);
+  calc_totals(tb, init, T.data());
+  bool ok = solve_anderson(tb, init, T.data(), tol, maxIter, 4, 1e10, 1.0, out, iters, resid, logc_out);
+  for (int i = 0; i < tb.ns; ++i) {
+    if (std::isnan(out[i]) || std::isinf(out[i]))
+      out[i] = (init[i] > MIN_CONC) ? std::min(init[i], MAX_CONC) : MIN_CONC;
+    out[i] = std::max(std::min(out[i], MAX_CONC), MIN_CONC);
+  }
+  return ok;
+}
+
+}  // namespace
+
+// Batched PCF + Anderson speciation over n voxels. conc_in/conc_out: n x ns.
+// Mirrors EquilibriumChemistry::calculate_species_concentrations with the driver settings
+// (tol, maxIter) and depth 4, condition tolerance 1e10, beta 1 (src/complab.cpp:423-425).
+ORC_API void orc_speciate_pcf(int nc, int ns, const double* S, const double* logK, const int* comp_species, long n,
+                              const double* conc_in, double tol, int maxIter, double* conc_out, int* iters,
+                              double* resid, uint8_t* conv, double* logc_out) {
+  Tableau tb{nc, ns, S, logK, comp_species};
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long v = 0; v < n; ++v) {
+    int it;
+    double r;
+    bool ok = speciate_pcf(tb, conc_in + v * ns, tol, maxIter, conc_out + v * ns, &it, &r,
+                           logc_out ? logc_out + v * nc : nullptr);
+    if (iters) iters[v] = it;
+    if (resid) resid[v] = r;
+    if (conv) conv[v] = ok ? 1 : 0;
+  }
+}
+
+// The PCF residual f(logC) (pcf_residual, :664-726) for given totals -- the reference's
+// convergence judge, used by tests to judge any solver's output.
+ORC_API void orc_pcf_residual(int nc, int ns, const double* S, const double* logK, long n, const double* logC,
+                              const double* T, double* f) {
+  Tableau tb{nc, ns, S, logK, nullptr};
+  std::vector<double> init(ns, 0.0);
+#pragma omp parallel for schedule(static)
+  for (long v = 0; v < n; ++v) pcf_residual(tb, logC + v * nc, T + v * nc, init.data(), f + v * nc);
+}
+
+// Component totals of species vectors (calc_component_totals, :520-538).
+ORC_API void orc_component_totals(int nc, int ns, const double* S, long n, const double* conc, double* T) {
+  std::vector<double> lk(ns, 0.0);
+  Tableau tb{nc, ns, S, lk.data(), nullptr};
+  for (long v = 0; v < n; ++v) calc_totals(tb, conc + v * ns, T + v * nc);
+}
+
+// Reserve Newton-Raphson (src/complab3d_processors_part4_eqsolver_NR_reserve.hh:309-615):
+// F = S^T c - T, J = ln10 S^T diag(c) S, partial-pivot Gaussian elimination, Armijo
+// step halving (<= 40) on |F|, stall exit. Inputs are the species vectors like the
+// reference processor; totals are computed from them.
+ORC_API void orc_speciate_newton_reserve(int nc, int ns, const double* S, const double* logK,
+                                         const int* comp_species, long n, const double* conc_in, double tol,
+                                         int maxIter, double* conc_out, int* iters, double* resid, uint8_t* conv) {
+  Tableau tb{nc, ns, S, logK, comp_species};
+  const double ln10 = std::log(10.0);
+#pragma omp parallel for schedule(dynamic, 64)
+  for (long v = 0; v < n; ++v) {
+    const double* init = conc_in + v * ns;
+    double* out = conc_out + v * ns;
+    std::vector<double> T(nc), x(nc), c(ns), F(nc), xn(nc), Fn(nc), d(nc);
+    calc_totals(tb, init, T.data());
+    for (int j = 0; j < nc; ++j) {
+      double cc = 1e-7;
+      int idx = comp_species ? comp_species[j] : -1;
+      if (idx >= 0 && idx < ns) cc = init[idx];
+      cc = std::max(std::min(cc, MAX_CONC), MIN_CONC);
+      x[j] = std::log10(cc);
+    }
+    auto resid_of = [&](const std::vector<double>& lx, std::vector<double>& FF) {
+      calc_species(tb, lx.data(), init, c.data());
+      for (int j = 0; j < nc; ++j) {
+        double s = -T[j];
+        for (int i = 0; i < ns; ++i)
+          if (is_eq_species(tb, i)) s += S[i * nc + j] * c[i];
+        FF[j] = s;
+      }
+      return norm2(FF);
+    };
+    bool ok = false;
+    int it = 0;
+    double fn = 0;
+    for (int iter = 0; iter < maxIter; ++iter) {
+      it = iter + 1;
+      fn = resid_of(x, F);
+      if (fn < tol) { ok = true; break; }
+      // Jacobian at x (c from resid_of)
+      std::vector<double> A(nc * nc, 0.0), b(nc);
+      for (int j = 0; j < nc; ++j)
+        for (int k = 0; k < nc; ++k) {
+          double s = 0;
+          for (int i = 0; i < ns; ++i)
+            if (is_eq_species(tb, i)) s += ln10 * S[i * nc + j] * S[i * nc + k] * c[i];
+          A[j * nc + k] = s;
+        }
+      for (int j = 0; j < nc; ++j) b[j] = -F[j];
+      bool solved = true;
+      for (int i = 0; i < nc && solved; ++i) {  // gaussian_elimination (:414-481)
+        int mr = i;
+        double mv = std::fabs(A[i * nc + i]);
+        for (int k = i + 1; k < nc; ++k)
+          if (std::fabs(A[k * nc + i]) > mv) { mv = std::fabs(A[k * nc + i]); mr = k; }
+        if (mv < 1e-30) { solved = false; break; }
+        if (mr != i) {
+          for (int k = 0; k < nc; ++k) std::swap(A[i * nc + k], A[mr * nc + k]);
+          std::swap(b[i], b[mr]);
+        }
+        for (int k = i + 1; k < nc; ++k) {
+          double fac = A[k * nc + i] / A[i * nc + i];
+          for (int jj = i; jj < nc; ++jj) A[k * nc + jj] -= fac * A[i * nc + jj];
+          b[k] -= fac * b[i];
+        }
+      }
+      for (int i = nc - 1; i >= 0 && solved; --i) {
+        if (std::fabs(A[i * nc + i]) < 1e-30) { solved = false; break; }
+        double s = b[i];
+        for (int jj = i + 1; jj < nc; ++jj) s -= A[i * nc + jj] * d[jj];
+        d[i] = s / A[i * nc + i];
+        if (std::isnan(d[i]) || std::isinf(d[i])) solved = false;
+      }
+      if (!solved) break;
+      double alpha = 1.0, fnn = 0;
+      for (int ls = 0; ls < 40; ++ls) {
+        for (int j = 0; j < nc; ++j) xn[j] = std::max(std::min(x[j] + alpha * d[j], MAX_LOG_C), MIN_LOG_C);
+        fnn = resid_of(xn, Fn);
+        if (fnn < fn * (1.0 - 1e-4 * alpha) || alpha < 1e-15) break;
+        alpha *= 0.5;
+      }
+      x = xn;
+      double step = alpha * norm2(d);
+      if (step < tol * 1e-3 && fnn < tol * 100) { ok = true; fn = fnn; break; }
+    }
+    calc_species(tb, x.data(), init, out);
+    for (int i = 0; i < ns; ++i) {
+      if (std::isnan(out[i]) || std::isinf(out[i]))
+        out[i] = (init[i] > MIN_CONC) ? std::min(init[i], MAX_CONC) : MIN_CONC;
+      out[i] = std::max(std::min(out[i], MAX_CONC), MIN_CONC);
+    }
+    if (iters) iters[v] = it;
+    if (resid) resid[v] = fn;
+    if (conv) conv[v] = ok ? 1 : 0;
+  }
+}
+
+// =====================================================================================
+// D3Q7 transport with kinetics and speciation: the coupled step
+// =====================================================================================
+
+// Kinetics hook type: the reference's defineRxnKinetics signature flattened to arrays
+// (defineKinetics.hh:223-228): B[nb], C[ns] in; subsR[ns], bioR[nb] out; mask code.
+typedef void (*orc_kinetics_fn)(const double* B, int nb, const double* C, int ns, double* subsR, double* bioR,
+                                long long mask);
+
+struct orc_ade_params {
+  int nx, ny, nz, S;       // grid and number of solutes
+  const int* kind;         // [256] dynamics kind per mask code
+  const double* omega;     // [S*256] relaxation per solute and mask code
+  const int* bc_type;      // [2*S] west/east per solute: 0 Dirichlet, 1 Neumann
+  const double* bc_value;  // [2*S] Dirichlet concentration (Neumann: unused)
+  const double* u;         // [3*N] advecting velocity
+  int solid_code, bb_code; // mask codes skipped by reactions (no_dynamics, bounce_back)
+  // kinetics (run_kinetics + update_rxnLattices); nb = 0 disables biomass
+  orc_kinetics_fn kinetics;  // null: no kinetics
+  double dt;                 // ade_dt: rates are per second, dC = rate * dt
+  int nb;                    // number of biomass fields
+  // speciation (run_equilibrium_biotic); nc = 0 disables
+  int nc;
+  const double* eqS;         // [S*nc] tableau over all solutes (zero rows = not equilibrium)
+  const double* eqLogK;      // [S]
+  const int* eqCompSpecies;  // [nc]
+  double eqTol;
+  int eqMaxIter;
+  // biomass threshold mask update (updateLocalMaskNtotalLattices3D), 0 disables
+  int mask_update;
+  double bmass_threshold;    // thrd_frac * Bmax
+  int pore_code, biofilm_code;
+};
+
+ORC_API void orc_ade_init(long N, int S, const double* c0, double* g) {
+  for (int s = 0; s < S; ++s)
+    for (int i = 0; i < 7; ++i)
+      for (long n = 0; n < N; ++n) g[((long)s * 7 + i) * N + n] = W7[i] * (c0[s] - 1.0);
+}
+
+// Density (concentration) of every node for solute s: sum(gbar) + 1.
+ORC_API void orc_ade_density(long N, int S, const double* g, double* c) {
+  for (int s = 0; s < S; ++s)
+    for (long n = 0; n < N; ++n) {
+      double r = 0;
+      for (int i = 0; i < 7; ++i) r += g[((long)s * 7 + i) * N + n];
+      c[(long)s * N + n] = r + 1.0;
+    }
+}
+
+// nsteps coupled steps. g: S*7*N deviation populations. bmass: nb*N densities (may be
+// null when nb = 0). mask: may change when mask_update is set. bc_rho: [2*S*ny*nz]
+// Neumann state (density of the adjacent interior plane after the previous stream);
+// initialise it to the initial concentration. eq_iters (optional, N) accumulates the
+// speciation iteration counts.
+ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, double* bmass, double* bc_rho,
+                         int nsteps, long* eq_stats) {
+  const int nx = P->nx, ny = P->ny, nz = P->nz, S = P->S;
+  Grid gr{nx, ny, nz, (long)nx * ny * nz};
+  const long N = gr.N;
+  const long plane = (long)ny * nz;
+  std::vector<double> gtmp((size_t)S * 7 * N);
+  Tableau tb{P->nc, S, P->eqS, P->eqLogK, P->eqCompSpecies};
+  long solves = 0, conv = 0, itsum = 0;
+  for (int t = 0; t < nsteps; ++t) {
+    std::memcpy(gtmp.data(), g, sizeof(double) * S * 7 * N);
+#pragma omp parallel for collapse(2) reduction(+ : solves, conv, itsum) schedule(dynamic, 4)
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+          long n = gr.idx(x, y, z);
+          int code = mask[n];
+          int k = P->kind[code];
+          std::vector<double> p((size_t)S * 7);
+          for (int s = 0; s < S; ++s)
+            for (int i = 0; i < 7; ++i) p[s * 7 + i] = g[((long)s * 7 + i) * N + n];
+          if (k == KIND_FLUID) {
+            // ---- collide (src/complab.cpp:1252), boundary completion first
+            const double ux = P->u[3 * n], uy = P->u[3 * n + 1], uz = P->u[3 * n + 2];
+            for (int s = 0; s < S; ++s) {
+              double* q = &p[s * 7];
+              if (x == 0 || x == nx - 1) {
+                int side = (x == 0) ? 0 : 1;
+                double cb = (P->bc_type[2 * s + side] == 0) ? P->bc_value[2 * s + side]
+                                                            : bc_rho[((long)s * 2 + side) * plane + y + (long)ny * z];
+                int miss = (x == 0) ? 1 : 2;  // population entering the domain
+                double others = 0;
+                for (int i = 0; i < 7; ++i)
+                  if (i != miss) others += q[i];
+                q[miss] = (cb - 1.0) - others;
+              }
+              double rb = 0;
+              for (int i = 0; i < 7; ++i) rb += q[i];
+              double rho = rb + 1.0;
+              double om = P->omega[s * 256 + code];
+              for (int i = 0; i < 7; ++i) {
+                double cu = C7[i][0] * ux + C7[i][1] * uy + C7[i][2] * uz;
+                double eq = W7[i] * (rb + 4.0 * rho * cu);
+                q[i] = (1.0 - om) * q[i] + om * eq;
+              }
+            }
+            bool react = (x > 0 && x < nx - 1 && code != P->solid_code && code != P->bb_code);
+            // ---- kinetics (run_kinetics + update_rxnLattices)
+            if (react && P->kinetics) {
+              std::vector<double> C(S), B(P->nb > 0 ? P->nb : 1), sr(S, 0.0), br(P->nb > 0 ? P->nb : 1, 0.0);
+              for (int s = 0; s < S; ++s) {
+                double r = 0;
+                for (int i = 0; i < 7; ++i) r += p[s * 7 + i];
+                double c0 = r + 1.0;
+                if (c0 < 1e-12) c0 = 0;
+                C[s] = c0;
+              }
+              for (int b = 0; b < P->nb; ++b) {
+                double b0 = bmass[(long)b * N + n];
+                if (b0 < 1e-12) b0 = 0;
+                B[b] = b0;
+              }
+              P->kinetics(B.data(), P->nb, C.data(), S, sr.data(), br.data(), (long long)code);
+              for (int s = 0; s < S; ++s) {
+                double dC = sr[s] * P->dt;
+                if (dC > 1e-12 || dC < -1e-12)
+                  for (int i = 0; i < 7; ++i) p[s * 7 + i] += dC * W7[i];
+              }
+              for (int b = 0; b < P->nb; ++b) {
+                double dB = br[b] * P->dt;
+                if (dB > 1e-12 || dB < -1e-12) bmass[(long)b * N + n] += dB;
+              }
+            }
+            // ---- speciation (run_equilibrium_biotic, :1355-1494)
+            if (react && P->nc > 0) {
+              std::vector<double> conc(S), eq(S);
+              for (int s = 0; s < S; ++s) {
+                double r = 0;
+                for (int i = 0; i < 7; ++i) r += p[s * 7 + i];
+                conc[s] = std::max(r + 1.0, MIN_CONC);
+              }
+              int it;
+              double rr;
+              bool ok = speciate_pcf(tb, conc.data(), P->eqTol, P->eqMaxIter, eq.data(), &it, &rr, nullptr);
+              solves += 1;
+              conv += ok ? 1 : 0;
+              itsum += it;
+              for (int s = 0; s < S; ++s) {
+                double dC = eq[s] - conc[s];
+                double mx = std::max(1e-9, 0.1 * std::fabs(conc[s]));
+                if (std::fabs(dC) > mx) dC = (dC > 0) ? mx : -mx;
+                if (conc[s] + dC < 1e-20) dC = 1e-20 - conc[s];
+                if (std::fabs(dC) > 1e-12)
+                  for (int i = 0; i < 7; ++i) p[s * 7 + i] += dC * W7[i];
+              }
+            }
+            // ---- biomass threshold reclassification (single pore type, no CA)
+            if (react && P->mask_update && P->nb > 0) {
+              double tot = 0;
+              for (int b = 0; b < P->nb; ++b) {
+                double bv = bmass[(long)b * N + n];
+                if (bv > 1e-14) tot += bv;
+              }
+              if (code == P->pore_code && tot >= P->bmass_threshold) mask[n] = (uint8_t)P->biofilm_code;
+              else if (code == P->biofilm_code && tot < P->bmass_threshold) mask[n] = (uint8_t)P->pore_code;
+            }
+          } else if (k == KIND_BB) {
+            for (int s = 0; s < S; ++s) {
+              double q[7];
+              for (int i = 0; i < 7; ++i) q[i] = p[s * 7 + opp(i)];
+              for (int i = 0; i < 7; ++i) p[s * 7 + i] = q[i];
+            }
+          }
+          // ---- stream (src/complab.cpp:1849)
+          for (int s = 0; s < S; ++s)
+            for (int i = 0; i < 7; ++i) {
+              int xx = x + C7[i][0];
+              if (xx < 0 || xx >= nx) continue;
+              int yy = gr.wy(y + C7[i][1]), zz = gr.wz(z + C7[i][2]);
+              gtmp[((long)s * 7 + i) * N + gr.idx(xx, yy, zz)] = p[s * 7 + i];
+            }
+        }
+    std::memcpy(g, gtmp.data(), sizeof(double) * S * 7 * N);
+    // ---- Neumann faces: density of the adjacent interior plane after stream
+    for (int s = 0; s < S; ++s)
+      for (int side = 0; side < 2; ++side) {
+        if (P->bc_type[2 * s + side] != 1) continue;
+        int xi = side == 0 ? 1 : nx - 2;
+        int xb = side == 0 ? 0 : nx - 1;
+        for (int z = 0; z < nz; ++z)
+          for (int y = 0; y < ny; ++y) {
+            long n = gr.idx(xi, y, z);
+            // a wall next to the face: the face keeps its own density
+            if (P->kind[mask[n]] != KIND_FLUID) n = gr.idx(xb, y, z);
+            double r = 0;
+            for (int i = 0; i < 7; ++i) r += g[((long)s * 7 + i) * N + n];
+            bc_rho[((long)s * 2 + side) * plane + y + (long)ny * z] = r + 1.0;
+          }
+      }
+  }
+  if (eq_stats) { eq_stats[0] += solves; eq_stats[1] += conv; eq_stats[2] += itsum; }
+}
+
+// =====================================================================================
+// Mask updates
+// =====================================================================================
+
+// Precipitation / dissolution node conversion (src/precipitationVOP.hh:176-204,
+// src/dissolutionVOP.hh:469-511): pore voxels whose mineral density reaches
+// precip_max become solid (phase = phase_id); solid-mineral voxels (phase == phase_id)
+// whose mineral density drops below reopen become pore (phase 0). Returns the number
+// of changed voxels (the trigger for a flow re-solve).
+ORC_API long orc_node_conversion(long N, uint8_t* mask, uint8_t* phase, const double* mineral, int pore_code,
+                                 int solid_code, int phase_id, double precip_max, double reopen) {
+  long changed = 0;
+  for (long n = 0; n < N; ++n) {
+    if (mask[n] == pore_code && mineral[n] >= precip_max) {
+      mask[n] = (uint8_t)solid_code;
+      phase[n] = (uint8_t)phase_id;
+      ++changed;
+    } else if (mask[n] == solid_code && phase[n] == phase_id && mineral[n] < reopen) {
+      mask[n] = (uint8_t)pore_code;
+      phase[n] = 0;
+      ++changed;
+    }
+  }
+  return changed;
+}
+
+// Classify solid voxels: solid with at least one D3Q19 neighbour in the pore set
+// becomes bb_code, every other solid voxel solid_code. is_pore[code] marks pore codes.
+ORC_API void orc_classify_solid(int nx, int ny, int nz, uint8_t* mask, const uint8_t* is_pore, int solid_code,
+                                int bb_code) {
+  Grid g{nx, ny, nz, (long)nx * ny * nz};
+  std::vector<uint8_t> m(mask, mask + g.N);
+  for (int z = 0; z < nz; ++z)
+    for (int y = 0; y < ny; ++y)
+      for (int x = 0; x < nx; ++x) {
+        long n = g.idx(x, y, z);
+        if (is_pore[m[n]]) continue;
+        bool adj = false;
+        for (int i = 1; i < 19 && !adj; ++i) {
+          int xx = x + C19[i][0];
+          if (xx < 0 || xx >= nx) continue;
+          if (is_pore[m[g.idx(xx, g.wy(y + C19[i][1]), g.wz(z + C19[i][2]))]]) adj = true;
+        }
+        mask[n] = (uint8_t)(adj ? bb_code : solid_code);
+      }
+}
+
+// =====================================================================================
+// MLP 12 -> H -> H -> 8 (tanh, tanh, linear), FP64 forward and backward
+// =====================================================================================
+// Parameter layout (row-major, out x in): W1[H*din], b1[H], W2[H*H], b2[H], W3[dout*H], b3[dout].
+ORC_API void orc_mlp_forward(int din, int H, int dout, const double* theta, long B, const double* X, double* Y) {
+  const double *W1 = theta, *b1 = W1 + H * din, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + dout * H;
+#pragma omp parallel for schedule(static)
+  for (long r = 0; r < B; ++r) {
+    std::vector<double> h1(H), h2(H);
+    for (int o = 0; o < H; ++o) {
+      double s = b1[o];
+      for (int i = 0; i < din; ++i) s += W1[o * din + i] * X[r * din + i];
+      h1[o] = std::tanh(s);
+    }
+    for (int o = 0; o < H; ++o) {
+      double s = b2[o];
+      for (int i = 0; i < H; ++i) s += W2[o * H + i] * h1[i];
+      h2[o] = std::tanh(s);
+    }
+    for (int o = 0; o < dout; ++o) {
+      double s = b3[o];
+      for (int i = 0; i < H; ++i) s += W3[o * H + i] * h2[i];
+      Y[r * dout + o] = s;
+    }
+  }
+}
+
+// Loss L = (1/(B*dout)) * sum (Y - T)^2 and its gradient with respect to theta.
+ORC_API double orc_mlp_grad(int din, int H, int dout, const double* theta, long B, const double* X,
+                            const double* Tt, double* grad) {
+  const double *W1 = theta, *b1 = W1 + H * din, *W2 = b1 + H, *b2 = W2 + H * H, *W3 = b2 + H, *b3 = W3 + dout * H;
+  (void)b3;
+  const long P = (long)H * din + H + (long)H * H + H + (long)dout * H + dout;
+  std::fill(grad, grad + P, 0.0);
+  double *gW1 = grad, *gb1 = gW1 + H * din, *gW2 = gb1 + H, *gb2 = gW2 + H * H, *gW3 = gb2 + H,
+         *gb3 = gW3 + dout * H;
+  double loss = 0;
+  const double scale = 2.0 / ((double)B * dout);
+  for (long r = 0; r < B; ++r) {
+    std::vector<double> h1(H), h2(H), y(dout), dy(dout), d2(H), d1(H);
+    for (int o = 0; o < H; ++o) {
+      double s = b1[o];
+      for (int i = 0; i < din; ++i) s += W1[o * din + i] * X[r * din + i];
+      h1[o] = std::tanh(s);
+    }
+    for (int o = 0; o < H; ++o) {
+      double s = b2[o];
+      for (int i = 0; i < H; ++i) s += W2[o * H + i] * h1[i];
+      h2[o] = std::tanh(s);
+    }
+    for (int o = 0; o < dout; ++o) {
+      double s = b3[o];
+      for (int i = 0; i < H; ++i) s += W3[o * H + i] * h2[i];
+      y[o] = s;
+      double e = s - Tt[r * dout + o];
+      loss += e * e;
+      dy[o] = scale * e;
+    }
+    for (int o = 0; o < dout; ++o) {
+      gb3[o] += dy[o];
+      for (int i = 0; i < H; ++i) gW3[o * H + i] += dy[o] * h2[i];
+    }
+    for (int i = 0; i < H; ++i) {
+      double s = 0;
+      for (int o = 0; o < dout; ++o) s += W3[o * H + i] * dy[o];
+      d2[i] = s * (1.0 - h2[i] * h2[i]);
+    }
+    for (int o = 0; o < H; ++o) {
+      gb2[o] += d2[o];
+      for (int i = 0; i < H; ++i) gW2[o * H + i] += d2[o] * h1[i];
+    }
+    for (int i = 0; i < H; ++i) {
+      double s = 0;
+      for (int o = 0; o < H; ++o) s += W2[o * H + i] * d2[o];
+      d1[i] = s * (1.0 - h1[i] * h1[i]);
+    }
+    for (int o = 0; o < H; ++o) {
+      gb1[o] += d1[o];
+      for (int i = 0; i < din; ++i) gW1[o * din + i] += d1[o] * X[r * din + i];
+    }
+  }
+  return loss / ((double)B * dout);
+}
+
+ORC_API int orc_num_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

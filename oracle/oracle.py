@@ -1,0 +1,264 @@
+"""ctypes loader for the FP64 CPU checker (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, never by the product. See oracle.cpp for what each function restates.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+KIND_NONE, KIND_BB, KIND_FLUID = 0, 1, 2
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, ci, cd, cl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+    sigs = {
+        "orc_flow_init": ([ci, ci, ci, cd, cd, vp], None),
+        "orc_flow_run": ([ci, ci, ci, vp, vp, vp, cd, cd, vp, ci, vp], None),
+        "orc_flow_velocity": ([ci, ci, ci, vp, vp, cd, cd, vp, vp, vp], None),
+        "orc_speciate_pcf": ([ci, ci, vp, vp, vp, cl, vp, cd, ci, vp, vp, vp, vp, vp], None),
+        "orc_pcf_residual": ([ci, ci, vp, vp, cl, vp, vp, vp], None),
+        "orc_component_totals": ([ci, ci, vp, cl, vp, vp], None),
+        "orc_speciate_newton_reserve": ([ci, ci, vp, vp, vp, cl, vp, cd, ci, vp, vp, vp, vp], None),
+        "orc_ade_init": ([cl, ci, vp, vp], None),
+        "orc_ade_density": ([cl, ci, vp, vp], None),
+        "orc_ade_run": ([vp, vp, vp, vp, vp, ci, vp], None),
+        "orc_node_conversion": ([cl, vp, vp, vp, ci, ci, ci, cd, cd], cl),
+        "orc_classify_solid": ([ci, ci, ci, vp, vp, ci, ci], None),
+        "orc_mlp_forward": ([ci, ci, ci, vp, cl, vp, vp], None),
+        "orc_mlp_grad": ([ci, ci, ci, vp, cl, vp, vp, vp], cd),
+        "orc_kinetics_plugin": ([vp, ci, vp, ci, vp, vp, ctypes.c_longlong], None),
+        "orc_num_threads": ([], ci),
+    }
+    for name, (args, res) in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class AdeParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("S", ctypes.c_int),
+        ("kind", ctypes.c_void_p), ("omega", ctypes.c_void_p), ("bc_type", ctypes.c_void_p),
+        ("bc_value", ctypes.c_void_p), ("u", ctypes.c_void_p),
+        ("solid_code", ctypes.c_int), ("bb_code", ctypes.c_int),
+        ("kinetics", ctypes.c_void_p), ("dt", ctypes.c_double), ("nb", ctypes.c_int),
+        ("nc", ctypes.c_int), ("eqS", ctypes.c_void_p), ("eqLogK", ctypes.c_void_p),
+        ("eqCompSpecies", ctypes.c_void_p), ("eqTol", ctypes.c_double), ("eqMaxIter", ctypes.c_int),
+        ("mask_update", ctypes.c_int), ("bmass_threshold", ctypes.c_double),
+        ("pore_code", ctypes.c_int), ("biofilm_code", ctypes.c_int),
+    ]
+
+
+# ------------------------------------------------------------------ flow
+def flow_run(nx, ny, nz, mask, kind, omega, rho_in, rho_out, nsteps, energy=False, f=None):
+    lib = load()
+    N = nx * ny * nz
+    if f is None:
+        f = np.zeros(19 * N)
+        lib.orc_flow_init(nx, ny, nz, rho_in, rho_out, _p(f))
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    k = np.ascontiguousarray(kind, dtype=np.int32)
+    o = _f(omega)
+    e = np.zeros(nsteps) if energy else None
+    lib.orc_flow_run(nx, ny, nz, _p(m), _p(k), _p(o), rho_in, rho_out, _p(f), nsteps, _p(e))
+    return f, e
+
+
+def flow_velocity(nx, ny, nz, mask, kind, rho_in, rho_out, f):
+    lib = load()
+    N = nx * ny * nz
+    u = np.zeros(3 * N)
+    r = np.zeros(N)
+    m = np.ascontiguousarray(mask, dtype=np.uint8)
+    k = np.ascontiguousarray(kind, dtype=np.int32)
+    lib.orc_flow_velocity(nx, ny, nz, _p(m), _p(k), rho_in, rho_out, _p(f), _p(u), _p(r))
+    return u.reshape(N, 3), r
+
+
+# ------------------------------------------------------------------ speciation
+def speciate_pcf(S, logK, comp_species, conc, tol=1e-10, max_iter=200):
+    lib = load()
+    S = _f(S)
+    ns, nc = S.shape
+    conc = _f(conc)
+    n = conc.shape[0]
+    out = np.zeros_like(conc)
+    it = np.zeros(n, dtype=np.int32)
+    r = np.zeros(n)
+    cv = np.zeros(n, dtype=np.uint8)
+    lc = np.zeros((n, nc))
+    cs = np.ascontiguousarray(comp_species, dtype=np.int32)
+    lib.orc_speciate_pcf(nc, ns, _p(S), _p(_f(logK)), _p(cs), n, _p(conc), tol, max_iter, _p(out), _p(it),
+                         _p(r), _p(cv), _p(lc))
+    return out, it, r, cv, lc
+
+
+def speciate_newton_reserve(S, logK, comp_species, conc, tol=1e-12, max_iter=100):
+    lib = load()
+    S = _f(S)
+    ns, nc = S.shape
+    conc = _f(conc)
+    n = conc.shape[0]
+    out = np.zeros_like(conc)
+    it = np.zeros(n, dtype=np.int32)
+    r = np.zeros(n)
+    cv = np.zeros(n, dtype=np.uint8)
+    cs = np.ascontiguousarray(comp_species, dtype=np.int32)
+    lib.orc_speciate_newton_reserve(nc, ns, _p(S), _p(_f(logK)), _p(cs), n, _p(conc), tol, max_iter, _p(out),
+                                    _p(it), _p(r), _p(cv))
+    return out, it, r, cv
+
+
+def pcf_residual(S, logK, logC, T):
+    lib = load()
+    S = _f(S)
+    ns, nc = S.shape
+    logC = _f(logC)
+    T = _f(T)
+    n = logC.shape[0]
+    f = np.zeros((n, nc))
+    lib.orc_pcf_residual(nc, ns, _p(S), _p(_f(logK)), n, _p(logC), _p(T), _p(f))
+    return f
+
+
+def component_totals(S, conc):
+    lib = load()
+    S = _f(S)
+    ns, nc = S.shape
+    conc = _f(conc)
+    n = conc.shape[0]
+    T = np.zeros((n, nc))
+    lib.orc_component_totals(nc, ns, _p(S), n, _p(conc), _p(T))
+    return T
+
+
+# ------------------------------------------------------------------ transport + reactions
+_KIN_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                           ctypes.c_void_p, ctypes.c_longlong)
+
+
+def kinetics_plugin_ptr():
+    lib = load()
+    return ctypes.cast(lib.orc_kinetics_plugin, ctypes.c_void_p).value
+
+
+def kinetics_plugin(B, C, mask_code):
+    lib = load()
+    B = _f(B)
+    C = _f(C)
+    sr = np.zeros(C.size)
+    br = np.zeros(B.size)
+    lib.orc_kinetics_plugin(_p(B), B.size, _p(C), C.size, _p(sr), _p(br), mask_code)
+    return sr, br
+
+
+class AdeRun:
+    """Holds the oracle's state for a coupled transport run (populations, biomass, mask)."""
+
+    def __init__(self, nx, ny, nz, mask, kind, omega, bc_type, bc_value, u, c0, solid_code=0, bb_code=1,
+                 kinetics=False, dt=1.0, biomass=None, tableau=None, eq_tol=1e-10, eq_max_iter=200,
+                 mask_update=None):
+        self.lib = load()
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.N = N = nx * ny * nz
+        self.mask = np.ascontiguousarray(mask, dtype=np.uint8).copy()
+        self.kind = np.ascontiguousarray(kind, dtype=np.int32)
+        self.omega = _f(omega).ravel()
+        self.S = self.omega.size // 256
+        self.bc_type = np.ascontiguousarray(bc_type, dtype=np.int32).ravel()
+        self.bc_value = _f(bc_value).ravel()
+        self.u = _f(u).ravel()
+        self.g = np.zeros(self.S * 7 * N)
+        c0 = _f(c0)
+        self.lib.orc_ade_init(N, self.S, _p(c0), _p(self.g))
+        self.bc_rho = np.repeat(c0, 2 * ny * nz).copy()
+        self.bmass = None if biomass is None else _f(biomass).ravel().copy()
+        nb = 0 if biomass is None else self.bmass.size // N
+        P = AdeParams()
+        P.nx, P.ny, P.nz, P.S = nx, ny, nz, self.S
+        P.kind, P.omega = _p(self.kind).value, _p(self.omega).value
+        P.bc_type, P.bc_value, P.u = _p(self.bc_type).value, _p(self.bc_value).value, _p(self.u).value
+        P.solid_code, P.bb_code = solid_code, bb_code
+        P.kinetics = kinetics_plugin_ptr() if kinetics else None
+        P.dt, P.nb = dt, nb
+        if tableau is not None:
+            Sm, lk, cs = tableau
+            self.eqS = _f(Sm)
+            self.eqLk = _f(lk)
+            self.eqCs = np.ascontiguousarray(cs, dtype=np.int32)
+            P.nc = self.eqS.shape[1]
+            P.eqS, P.eqLogK, P.eqCompSpecies = _p(self.eqS).value, _p(self.eqLk).value, _p(self.eqCs).value
+            P.eqTol, P.eqMaxIter = eq_tol, eq_max_iter
+        else:
+            P.nc = 0
+        if mask_update is not None:
+            thr, pore, bio = mask_update
+            P.mask_update, P.bmass_threshold, P.pore_code, P.biofilm_code = 1, thr, pore, bio
+        self.P = P
+        self.eq_stats = np.zeros(3, dtype=np.int64)
+
+    def run(self, nsteps):
+        self.lib.orc_ade_run(ctypes.byref(self.P), _p(self.mask), _p(self.g), _p(self.bmass), _p(self.bc_rho),
+                             nsteps, _p(self.eq_stats))
+
+    def conc(self):
+        """Concentration per node. On fluid nodes of the x faces the entering population is
+        only rebuilt at the next collision, so these report what the boundary prescribes
+        (the Dirichlet value, or for a Neumann face the density it copies), as the
+        boundary dynamics' computeDensity does."""
+        c = np.zeros(self.S * self.N)
+        self.lib.orc_ade_density(self.N, self.S, _p(self.g), _p(c))
+        c = c.reshape(self.S, self.nz, self.ny, self.nx)
+        fluid = (self.kind[self.mask] == KIND_FLUID).reshape(self.nz, self.ny, self.nx)
+        rho = self.bc_rho.reshape(self.S, 2, self.nz, self.ny)
+        for s in range(self.S):
+            for side, xb in ((0, 0), (1, self.nx - 1)):
+                val = self.bc_value[2 * s + side] if self.bc_type[2 * s + side] == 0 else rho[s, side]
+                c[s, :, :, xb] = np.where(fluid[:, :, xb], val, c[s, :, :, xb])
+        return c.reshape(self.S, self.N)
+
+
+# ------------------------------------------------------------------ MLP
+def mlp_forward(din, H, dout, theta, X):
+    lib = load()
+    X = _f(X)
+    Y = np.zeros((X.shape[0], dout))
+    lib.orc_mlp_forward(din, H, dout, _p(_f(theta)), X.shape[0], _p(X), _p(Y))
+    return Y
+
+
+def mlp_grad(din, H, dout, theta, X, T):
+    lib = load()
+    theta = _f(theta)
+    X = _f(X)
+    T = _f(T)
+    g = np.zeros_like(theta)
+    loss = lib.orc_mlp_grad(din, H, dout, _p(theta), X.shape[0], _p(X), _p(T), _p(g))
+    return g, loss
+
+
+def num_threads():
+    return load().orc_num_threads()

@@ -1,0 +1,65 @@
+"""The C ABI library loads and exports every entry point include/clb.h declares; the
+host-side input generators (no GPU needed) are deterministic and meet their specs."""
+import ctypes
+
+import numpy as np
+
+
+def test_library_exports_every_declared_symbol(clb):
+    lib = ctypes.CDLL(clb.LIB_PATH)
+    names = clb.exported_symbols()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_sphere_pack_spec(clb, oracle):
+    nx, ny, nz = 64, 32, 32
+    m = clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.45, seed=0)
+    assert np.array_equal(m, clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.45, seed=0))
+    v = m.reshape(nz, ny, nx)
+    interior = v[:, :, 1:nx - 1]
+    phi = (interior == 2).mean()
+    assert 0.40 < phi <= 0.45
+    # inlet / outlet planes duplicate the first / last interior slice (pore vs solid)
+    assert np.array_equal(v[:, :, 0] == 2, v[:, :, 1] == 2)
+    assert np.array_equal(v[:, :, nx - 1] == 2, v[:, :, nx - 2] == 2)
+    # solid classification agrees with the checker's rule (pore D3Q19 neighbour -> 1)
+    solid = np.where(m == 2, 2, 0).astype(np.uint8)
+    is_pore = np.zeros(256, dtype=np.uint8)
+    is_pore[2] = 1
+    oracle.load().orc_classify_solid(nx, ny, nz, oracle._p(solid), oracle._p(is_pore), 0, 1)
+    assert np.array_equal(solid, m)
+    assert set(np.unique(m)) == {0, 1, 2}
+
+
+def test_tableau_generator(clb):
+    S, lk = clb.gen_tableau(8, 24, seed=3)
+    assert np.array_equal(S[:8], np.eye(8))
+    assert np.all(lk[:8] == 0)
+    assert np.all((S >= -2) & (S <= 2))
+    rows = {tuple(r) for r in S}
+    assert len(rows) == 24 and not np.any(np.all(S == 0, axis=1))
+    assert np.all((lk[8:] >= -14) & (lk[8:] <= 14))
+    S2, lk2 = clb.gen_tableau(8, 24, seed=3)
+    assert np.array_equal(S, S2) and np.array_equal(lk, lk2)
+
+
+def test_totals_field(clb):
+    t = clb.gen_totals_field(16, 16, 16, 3, sigma=2.0, seed=4)
+    assert t.shape == (4096, 3)
+    lt = np.log10(t)
+    assert lt.min() >= -6 - 1e-12 and lt.max() <= -2 + 1e-12
+    # spatially correlated: neighbours along x are much closer than random pairs
+    f = lt[:, 0].reshape(16, 16, 16)
+    assert np.mean(np.abs(np.diff(f, axis=2))) < 0.5 * np.mean(np.abs(f - f.mean()))
+
+
+def test_biomass_seed(clb):
+    m = clb.gen_sphere_pack(32, 16, 16, seed=0)
+    b = clb.gen_biomass_seed(m, fraction=0.01)
+    npore = (m == 2).sum()
+    k = (b > 0).sum()
+    assert k == round(0.01 * npore)
+    assert np.all(m[b > 0] == 2)
+    assert b[b > 0].min() >= 10 and b.max() <= 100
