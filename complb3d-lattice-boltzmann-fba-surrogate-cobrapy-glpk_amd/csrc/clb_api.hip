@@ -428,6 +428,8 @@ int mlp_train_device(clb_ctx* c, long long n, const float* dX, const float* dT, 
   for (int ep = 0; ep < epochs; ++ep) {
     std::shuffle(order.begin(), order.end(), rng);
     launch_fill_f64(d_loss, 1, 0.0, c->stream);
+    // cosine learning-rate schedule down to 2 % of the initial rate
+    const double lr_ep = lr * (0.02 + 0.98 * 0.5 * (1.0 + std::cos(3.141592653589793 * ep / std::max(epochs, 1))));
     for (long long bi = 0; bi < nbatches; ++bi) {
       const long long r0 = order[bi] * batch;
       const float* Xp = dX + r0 * d.din;
@@ -435,7 +437,7 @@ int mlp_train_device(clb_ctx* c, long long n, const float* dX, const float* dT, 
       mlp_fwd_bwd(c, batch, Xp, Tp, grad, d_loss, w);
       ++tstep;
       const float b1 = 0.9f, b2 = 0.999f;
-      launch_adam(P, c->d_theta, grad, c->d_adam_m, c->d_adam_v, (float)lr, b1, b2, 1e-8f,
+      launch_adam(P, c->d_theta, grad, c->d_adam_m, c->d_adam_v, (float)lr_ep, b1, b2, 1e-8f,
                   (float)std::pow(0.9, (double)tstep), (float)std::pow(0.999, (double)tstep), c->stream);
     }
     double h = 0;

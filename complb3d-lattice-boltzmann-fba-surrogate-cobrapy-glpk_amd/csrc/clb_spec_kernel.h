@@ -1,29 +1,35 @@
-// clb_spec_kernel.h -- per-voxel equilibrium speciation, FP64 Newton, one persistent
-// batched kernel with a work queue (instantiated by clb_spec_batch.hip and
+// clb_spec_kernel.h -- per-voxel equilibrium speciation, FP64 Newton-type solve, one
+// persistent batched kernel with a work queue (instantiated by clb_spec_batch.hip and
 // clb_spec_coupled_*.hip).
 //
-// The system is the reference's: species log10 c_i = logK_i + sum_j S_ij x_j (x = log10
-// component activities, clamped to [-30, 1]), and the reference's own convergence judge,
-// the PCF residual f_j = (1/mu0_j) (log10 S_P,j - log10 S_R,j) with its clamps and its
-// zero-total guard (pcf_residual / calc_reactive_product_sums / calc_species,
-// src/complab3d_processors_part4_eqsolver.hh:442-726). The Newton iteration solves
-// f(x) = 0 directly:  J dx = -f with J_jk = df_j/dx_k
-//   = (1/mu0_j) sum_i S_ik c_i a_ij,  a_ij = -S_ij / (S_ij < 0 ? S_P,j : S_R,j),
-// LU with partial pivoting, step capped to 5 log units, Armijo backtracking on |f|^2/2,
-// and the plain PCF update x += f (the reference's fixed-point step) when backtracking
-// fails. A voxel is converged when |f|_2 < tol: the same test the reference applies
-// (src/complab3d_processors_part4_eqsolver.hh:1033-1049); the Newton reserve solver of
-// src/complab3d_processors_part4_eqsolver_NR_reserve.hh:512-615 is the specification of
-// the Newton + backtracking structure.
+// The system is the reference's: species log10 c_i = logK_i + sum_j S_ij x_j with x the
+// log10 component activities in [-30, 1], and the reference's own convergence judge, the
+// PCF residual f_j = (1/mu0_j) (log10 S_P,j - log10 S_R,j) with its clamps and zero-total
+// guard (calc_species / calc_reactive_product_sums / pcf_residual,
+// src/complab3d_processors_part4_eqsolver.hh:442-726). A voxel is converged when
+// |f|_2 < tol, the test of src/complab3d_processors_part4_eqsolver.hh:1033-1049.
+//
+// Iteration (the Newton + damping structure follows the reserve solver,
+// src/complab3d_processors_part4_eqsolver_NR_reserve.hh:512-615, with the globalisation
+// changed so that every voxel converges):
+//   * Levenberg-Marquardt on the smooth log residual fs(x) (the judge without its clamps):
+//     (J^T J + lambda I) d = -J^T fs, J_jk = dfs_j/dx_k = (1/mu0_j) sum_i S_ik c_i a_ij,
+//     a_ij = -S_ij / (S_ij < 0 ? S_P,j : S_R,j); gain-ratio acceptance and Nielsen's
+//     lambda update; steps capped at 5 log units.
+//   * The log residual can have local minima away from the solution. When |fs| has not
+//     dropped 5 % within 6 evaluations, the voxel switches to Newton on the strictly convex
+//     potential G(x) = sum_i c_i / ln 10 - T.x (gradient S^T c - T, Hessian
+//     ln 10 S^T diag(c) S, Armijo backtracking on G), which cannot stall; once
+//     |f| < 1e-4 it returns to Levenberg-Marquardt for the fast local convergence.
 //
 // Work distribution: every lane owns one voxel at a time; a lane whose voxel has
 // converged (or hit max_iter) takes the next voxel from a global counter, one atomic per
 // wavefront per refill. Every loop trip is one residual+Jacobian evaluation on every
 // busy lane, so lanes never wait on a slower voxel of their wavefront.
 //
-// Components are padded to the kernel's NC with zero columns and zero totals (the
-// zero-total guard pins them: f_j = 0, J row j = e_j), so one instantiation serves every
-// nc <= NC.
+// Components are padded to the kernel's NC with zero columns and zero totals; the
+// zero-total guard freezes them (zero residual, identity row), so one instantiation
+// serves every nc <= NC.
 #pragma once
 
 #include "clb_lattice.h"
@@ -33,21 +39,42 @@ namespace clb {
 namespace {
 
 constexpr double kMinConc = 1e-30, kMaxConc = 10.0, kMinLog = -30.0, kMaxLog = 1.0;
+constexpr double kLn10 = 2.302585092994045684;
 constexpr double kMaxStep = 5.0;
-constexpr double kMinAlpha = 1.0 / 256.0;
+constexpr int kStallEvals = 6;
+constexpr double kBackToLM = 1e-4;
 
 __device__ __forceinline__ double clampd(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
 
-// Residual (and optionally Jacobian) at x. Returns |f|_2^2.
+// packed upper triangle index of (j, k), j <= k
+template <int NC>
+__device__ __forceinline__ constexpr int up(int j, int k) {
+  return j * NC - (j * (j - 1)) / 2 + (k - j);
+}
+
+template <int NC>
+struct EvalOut {
+  double judge2;         // |f_ref|^2 (the reference's judge)
+  double merit;          // LM: |fs|^2 / 2;  G mode: G(x)
+  double A[NC * (NC + 1) / 2];  // LM: J^T J;  G mode: Hessian of G (packed upper)
+  double rhs[NC];        // LM: -J^T fs;  G mode: -(S^T c - T)
+  double fsn;            // |fs| (LM) for the stall test
+};
+
+// One evaluation at x. gmode selects the system that is assembled. csp (optional)
+// receives the species concentrations of calc_species (clamped log c).
 template <int NC, int NSM>
-__device__ __forceinline__ double evaluate(const Tableau& tb, const int* vsp, int nvar, const double* lkv,
-                                           const double* x, const double* T, double* f, double* J, double* csp) {
-  double sp[NC], sn[NC];
+__device__ __forceinline__ void evaluate(const Tableau& tb, const int* vsp, int nvar, const double* lkv,
+                                         const double* x, const double* T, bool gmode, EvalOut<NC>& o,
+                                         double* csp) {
+  double craw[NSM];
+  double spr[NC], snr[NC], sps[NC], sns[NC];
 #pragma unroll
-  for (int j = 0; j < NC; ++j) { sp[j] = 0.0; sn[j] = 0.0; }
+  for (int j = 0; j < NC; ++j) { spr[j] = 0.0; snr[j] = 0.0; sps[j] = 0.0; sns[j] = 0.0; }
+  double gsum = 0.0;
 #pragma unroll
   for (int i = 0; i < NSM; ++i) {
-    csp[i] = 0.0;
+    craw[i] = 0.0;
     if (i < tb.ns && tb.is_eq[i]) {
       double lk = tb.logK[i];
 #pragma unroll
@@ -56,110 +83,156 @@ __device__ __forceinline__ double evaluate(const Tableau& tb, const int* vsp, in
       double lc = lk;
 #pragma unroll
       for (int j = 0; j < NC; ++j) lc += tb.S[i * CLB_MAX_COMPONENTS + j] * x[j];
-      lc = clampd(lc, kMinLog, kMaxLog);
-      const double c = exp10(lc);
-      csp[i] = c;
+      const double cr = exp10(fmin(lc, 250.0));
+      const double cc = (lc > kMaxLog) ? kMaxConc : ((lc < kMinLog) ? kMinConc : cr);
+      craw[i] = cr;
+      gsum += cr;
+      if (csp) csp[i] = cc;
 #pragma unroll
       for (int j = 0; j < NC; ++j) {
         const double s = tb.S[i * CLB_MAX_COMPONENTS + j];
-        if (s > 1e-15) sp[j] += s * c;
-        else if (s < -1e-15) sn[j] -= s * c;
+        if (s > 1e-15) { spr[j] += s * cc; sps[j] += s * cr; }
+        else if (s < -1e-15) { snr[j] -= s * cc; sns[j] -= s * cr; }
       }
+    } else if (csp) {
+      csp[i] = 0.0;
     }
   }
-  double rP[NC], rR[NC];
-  double nrm2 = 0.0;
+  // reference judge (clamped species, floors, +-10 clamp, zero-total guard)
+  double judge2 = 0.0;
+  double fs[NC], rP[NC], rR[NC];
+  double fsn2 = 0.0, gtx = 0.0;
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
+    const bool live = T[j] > 1e-18;
     double SR, SP;
-    if (T[j] >= 0.0) { SR = sp[j]; SP = T[j] + sn[j]; }
-    else { SR = -T[j] + sp[j]; SP = sn[j]; }
-    SR = fmax(SR, kMinConc);
-    SP = fmax(SP, kMinConc);
-    double fj = tb.inv_mu0[j] * (log10(SP) - log10(SR));
+    if (T[j] >= 0.0) { SR = spr[j]; SP = T[j] + snr[j]; }
+    else { SR = -T[j] + spr[j]; SP = snr[j]; }
+    double fj = tb.inv_mu0[j] * (log10(fmax(SP, kMinConc)) - log10(fmax(SR, kMinConc)));
     fj = clampd(fj, -10.0, 10.0);
-    if (!isfinite(fj)) fj = 0.0;
-    if (T[j] <= 1e-18) fj = 0.0;
-    f[j] = fj;
-    nrm2 += fj * fj;
-    rP[j] = tb.inv_mu0[j] / SP;
-    rR[j] = tb.inv_mu0[j] / SR;
+    if (!isfinite(fj) || !live) fj = 0.0;
+    judge2 += fj * fj;
+    // smooth residual on the raw species
+    double SRs, SPs;
+    if (T[j] >= 0.0) { SRs = sps[j]; SPs = T[j] + sns[j]; }
+    else { SRs = -T[j] + sps[j]; SPs = sns[j]; }
+    SRs = fmax(SRs, 1e-300);
+    SPs = fmax(SPs, 1e-300);
+    const double fsj = live ? tb.inv_mu0[j] * (log10(SPs) - log10(SRs)) : 0.0;
+    fs[j] = fsj;
+    fsn2 += fsj * fsj;
+    rP[j] = live ? tb.inv_mu0[j] / SPs : 0.0;
+    rR[j] = live ? tb.inv_mu0[j] / SRs : 0.0;
+    gtx += T[j] * x[j];
   }
-  if (J) {
+  o.judge2 = judge2;
+  o.fsn = sqrt(fsn2);
+  // Jacobian of the chosen system, row-major NC x NC
+  double J[NC * NC];
 #pragma unroll
-    for (int q = 0; q < NC * NC; ++q) J[q] = 0.0;
+  for (int q = 0; q < NC * NC; ++q) J[q] = 0.0;
 #pragma unroll
-    for (int i = 0; i < NSM; ++i) {
-      if (i < tb.ns && tb.is_eq[i]) {
-        const double c = csp[i];
+  for (int i = 0; i < NSM; ++i) {
+    if (i < tb.ns && tb.is_eq[i]) {
+      const double c = craw[i];
 #pragma unroll
-        for (int j = 0; j < NC; ++j) {
-          const double s = tb.S[i * CLB_MAX_COMPONENTS + j];
-          if (s == 0.0) continue;
-          const double a = -s * c * ((s < 0.0) ? rP[j] : rR[j]);
+      for (int j = 0; j < NC; ++j) {
+        const double s = tb.S[i * CLB_MAX_COMPONENTS + j];
+        if (s == 0.0) continue;
+        const double w = gmode ? kLn10 * s * c : -s * c * ((s < 0.0) ? rP[j] : rR[j]);
 #pragma unroll
-          for (int k = 0; k < NC; ++k) J[j * NC + k] += a * tb.S[i * CLB_MAX_COMPONENTS + k];
-        }
+        for (int k = 0; k < NC; ++k) J[j * NC + k] += w * tb.S[i * CLB_MAX_COMPONENTS + k];
       }
     }
-#pragma unroll
-    for (int j = 0; j < NC; ++j)
-      if (T[j] <= 1e-18) {
-#pragma unroll
-        for (int k = 0; k < NC; ++k) J[j * NC + k] = (k == j) ? 1.0 : 0.0;
-      }
   }
-  return nrm2;
+  if (gmode) {
+    o.merit = gsum / kLn10 - gtx;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const bool live = T[j] > 1e-18;
+#pragma unroll
+      for (int k = j; k < NC; ++k) {
+        const bool livek = T[k] > 1e-18;
+        o.A[up<NC>(j, k)] = (live && livek) ? J[j * NC + k] : ((j == k) ? 1.0 : 0.0);
+      }
+      o.rhs[j] = live ? -(sps[j] - sns[j] - T[j]) : 0.0;
+    }
+  } else {
+    o.merit = 0.5 * fsn2;
+    // a component under the zero-total guard keeps its starting activity (its residual
+    // is zero, and the reference's update never moves it): drop its column
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (!(T[k] > 1e-18)) {
+#pragma unroll
+        for (int r = 0; r < NC; ++r) J[r * NC + k] = 0.0;
+      }
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      double g = 0.0;
+#pragma unroll
+      for (int r = 0; r < NC; ++r) g += J[r * NC + j] * fs[r];
+      o.rhs[j] = -g;
+#pragma unroll
+      for (int k = j; k < NC; ++k) {
+        double a = 0.0;
+#pragma unroll
+        for (int r = 0; r < NC; ++r) a += J[r * NC + j] * J[r * NC + k];
+        o.A[up<NC>(j, k)] = a;
+      }
+    }
+  }
 }
 
-// Solve J d = -f by LU with partial pivoting; row swaps are done with selects so that
-// every array index stays a compile-time constant (registers, no scratch).
+// Solve (A + lam I) d = b by Cholesky (A symmetric, packed upper). Returns false when a
+// pivot is not positive.
 template <int NC>
-__device__ __forceinline__ bool newton_direction(double* J, const double* f, double* d) {
-  double b[NC];
+__device__ __forceinline__ bool chol_solve(const double* A, double lam, const double* b, double* d) {
+  double L[NC * (NC + 1) / 2];  // packed lower, row-major: L(j,k) k <= j at j(j+1)/2 + k
 #pragma unroll
-  for (int j = 0; j < NC; ++j) b[j] = -f[j];
+  for (int j = 0; j < NC; ++j) {
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    int p = k;
-    double pv = fabs(J[k * NC + k]);
+    for (int k = 0; k <= j; ++k) {
+      double s = A[up<NC>(k, j)] + ((j == k) ? lam : 0.0);
 #pragma unroll
-    for (int r = k + 1; r < NC; ++r) {
-      const double v = fabs(J[r * NC + k]);
-      if (v > pv) { pv = v; p = r; }
-    }
-    if (!(pv > 1e-300)) return false;
-#pragma unroll
-    for (int r = k + 1; r < NC; ++r) {
-      const bool sw = (p == r);
-#pragma unroll
-      for (int c = k; c < NC; ++c) {
-        const double a = J[k * NC + c], bb = J[r * NC + c];
-        J[k * NC + c] = sw ? bb : a;
-        J[r * NC + c] = sw ? a : bb;
+      for (int m = 0; m < k; ++m) s -= L[j * (j + 1) / 2 + m] * L[k * (k + 1) / 2 + m];
+      if (j == k) {
+        if (!(s > 0.0)) return false;
+        L[j * (j + 1) / 2 + j] = sqrt(s);
+      } else {
+        L[j * (j + 1) / 2 + k] = s / L[k * (k + 1) / 2 + k];
       }
-      const double a = b[k], bb = b[r];
-      b[k] = sw ? bb : a;
-      b[r] = sw ? a : bb;
-    }
-    const double inv = 1.0 / J[k * NC + k];
-#pragma unroll
-    for (int r = k + 1; r < NC; ++r) {
-      const double m = J[r * NC + k] * inv;
-#pragma unroll
-      for (int c = k + 1; c < NC; ++c) J[r * NC + c] -= m * J[k * NC + c];
-      b[r] -= m * b[k];
     }
   }
+  double y[NC];
 #pragma unroll
-  for (int k = NC - 1; k >= 0; --k) {
-    double s = b[k];
+  for (int j = 0; j < NC; ++j) {
+    double s = b[j];
 #pragma unroll
-    for (int c = k + 1; c < NC; ++c) s -= J[k * NC + c] * d[c];
-    d[k] = s / J[k * NC + k];
-    if (!isfinite(d[k])) return false;
+    for (int m = 0; m < j; ++m) s -= L[j * (j + 1) / 2 + m] * y[m];
+    y[j] = s / L[j * (j + 1) / 2 + j];
+  }
+#pragma unroll
+  for (int j = NC - 1; j >= 0; --j) {
+    double s = y[j];
+#pragma unroll
+    for (int m = j + 1; m < NC; ++m) s -= L[m * (m + 1) / 2 + j] * d[m];
+    d[j] = s / L[j * (j + 1) / 2 + j];
+    if (!isfinite(d[j])) return false;
   }
   return true;
+}
+
+template <int NC>
+__device__ __forceinline__ double quad_form(const double* A, const double* d) {
+  double q = 0.0;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    q += A[up<NC>(j, j)] * d[j] * d[j];
+#pragma unroll
+    for (int k = j + 1; k < NC; ++k) q += 2.0 * A[up<NC>(j, k)] * d[j] * d[k];
+  }
+  return q;
 }
 
 struct CoupledIO {
@@ -197,19 +270,22 @@ __device__ __forceinline__ void add_increment(const CoupledIO& io, int s, int n,
 // limited increments added to the populations, run_equilibrium_biotic semantics).
 template <typename R, int NC, int NSM, int MODE>
 __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int warm, CoupledIO io) {
+  constexpr int NP = NC * (NC + 1) / 2;
   const int lane = threadIdx.x & 63;
   const long long n_items = sa.n;
   int vsp[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) vsp[v] = sa.var_species[v];
   const int nvar = sa.nvar;
+  const double tol2 = tb.tol * tb.tol;
 
   long long item = -1;
   bool have = false, exhausted = false;
-  double x[NC], T[NC], fx[NC], d[NC], lkv[4];
-  double phi = 0.0, alpha = 1.0;
-  int evals = 0;
-  bool trial = false;
+  double x[NC], T[NC], d[NC], lkv[4];
+  double A[NP], rhs[NC];
+  double merit = 0.0, lam = -1.0, nu = 2.0, alpha = 1.0, best = 0.0, judge2 = 0.0;
+  int evals = 0, since = 0;
+  bool trial = false, gmode = false, lm_final = false;
   int node = 0;
   double conc[NSM];
   unsigned long long my_solves = 0, my_conv = 0, my_iters = 0;
@@ -230,7 +306,12 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
           item = v;
           have = true;
           trial = false;
+          gmode = false;
+          lm_final = false;
           evals = 0;
+          since = 0;
+          lam = -1.0;
+          nu = 2.0;
           alpha = 1.0;
 #pragma unroll
           for (int q = 0; q < 4; ++q) lkv[q] = (q < nvar) ? sa.logk_var[v * nvar + q] : 0.0;
@@ -284,50 +365,114 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
     if (!have) continue;
 
     // ---- one evaluation on every busy lane
-    double p[NC], fp[NC], J[NC * NC], csp[NSM];
+    double p[NC];
 #pragma unroll
     for (int j = 0; j < NC; ++j) p[j] = trial ? clampd(x[j] + alpha * d[j], kMinLog, kMaxLog) : x[j];
-    const double nrm2 = evaluate<NC, NSM>(tb, vsp, nvar, lkv, p, T, fp, J, csp);
+    EvalOut<NC> e;
+    evaluate<NC, NSM>(tb, vsp, nvar, lkv, p, T, gmode, e, nullptr);
     ++evals;
-    const double phip = 0.5 * nrm2;
-    bool accept = !trial || (phip <= (1.0 - 2e-4 * alpha) * phi);
+    bool accept = true;
+    if (trial) {
+      if (!gmode) {
+        // predicted decrease of |fs|^2/2 by the model: rhs.d - d.A.d / 2 (A = J^T J)
+        const double pred = [&] {
+          double r = 0.0;
+#pragma unroll
+          for (int j = 0; j < NC; ++j) r += rhs[j] * d[j];
+          return r - 0.5 * quad_form<NC>(A, d);
+        }();
+        const double rho = (merit - e.merit) / fmax(pred, 1e-300);
+        accept = rho > 0.0;
+        if (accept) {
+          const double t = 2.0 * rho - 1.0;
+          lam *= fmax(1.0 / 3.0, 1.0 - t * t * t);
+          nu = 2.0;
+        } else {
+          lam *= nu;
+          nu *= 2.0;
+        }
+      } else {
+        double slope = 0.0;
+#pragma unroll
+        for (int j = 0; j < NC; ++j) slope -= rhs[j] * d[j];
+        accept = e.merit <= merit + 1e-4 * alpha * slope;
+        if (!accept) alpha *= 0.5;
+      }
+    }
     bool done = false, converged = false;
+    bool resolve = true;  // compute a new direction at the end of this trip
     if (accept) {
 #pragma unroll
-      for (int j = 0; j < NC; ++j) { x[j] = p[j]; fx[j] = fp[j]; }
-      phi = phip;
-      if (sqrt(nrm2) < tb.tol) {
+      for (int j = 0; j < NC; ++j) { x[j] = p[j]; rhs[j] = e.rhs[j]; }
+#pragma unroll
+      for (int q = 0; q < NP; ++q) A[q] = e.A[q];
+      merit = e.merit;
+      judge2 = e.judge2;
+      if (judge2 < tol2) {
         done = true;
         converged = true;
-      } else if (evals >= tb.max_iter) {
-        done = true;
-      } else {
-        bool ok = newton_direction<NC>(J, fp, d);
+      } else if (!gmode) {
+        if (lam < 0.0) {
+          // initial damping scaled by |fs|^2: a start near the solution (a good warm
+          // start) takes plain Gauss-Newton steps, a distant one is damped
+          double mx = 0.0;
+#pragma unroll
+          for (int j = 0; j < NC; ++j) mx = fmax(mx, A[up<NC>(j, j)]);
+          lam = fmax(1e-3 * mx * fmin(1.0, e.fsn * e.fsn), 1e-300);
+          best = e.fsn;
+          since = 0;
+        } else if (e.fsn < 0.95 * best) {
+          best = e.fsn;
+          since = 0;
+        } else if (!lm_final && ++since >= kStallEvals) {
+          gmode = true;      // re-evaluate x on the convex potential
+          trial = false;
+          resolve = false;
+        }
+      } else if (judge2 < kBackToLM * kBackToLM) {
+        gmode = false;       // close to the solution: fast local convergence
+        lm_final = true;
+        lam = -1.0;
+        nu = 2.0;
+        trial = false;
+        resolve = false;
+      }
+    } else if (!gmode) {
+      since += 1;
+      if (!lm_final && since >= kStallEvals) {
+        gmode = true;
+        trial = false;
+        resolve = false;
+      }
+    } else if (alpha >= 1.0 / 1024.0) {
+      resolve = false;       // G backtracking: same direction, smaller step
+      trial = true;
+    } else {
+      // backtracking at the resolution of G: finish with Levenberg-Marquardt from x
+      gmode = false;
+      lm_final = true;
+      lam = -1.0;
+      nu = 2.0;
+      trial = false;
+      resolve = false;
+    }
+    if (!done && evals >= tb.max_iter) done = true;
+    if (!done && resolve) {
+      const bool ok = chol_solve<NC>(A, gmode ? 0.0 : lam, rhs, d);
+      if (ok) {
         double m = 0.0;
 #pragma unroll
         for (int j = 0; j < NC; ++j) m = fmax(m, fabs(d[j]));
-        if (ok && m > kMaxStep) {
+        if (m > kMaxStep) {
           const double sc = kMaxStep / m;
 #pragma unroll
           for (int j = 0; j < NC; ++j) d[j] *= sc;
         }
-        if (ok) {
-          trial = true;
-          alpha = 1.0;
-        } else {
-          // singular Jacobian: the reference's fixed-point step
-#pragma unroll
-          for (int j = 0; j < NC; ++j) x[j] = clampd(x[j] + fx[j], kMinLog, kMaxLog);
-          trial = false;
-        }
-      }
-    } else {
-      alpha *= 0.5;
-      if (evals >= tb.max_iter) {
-        done = true;
-      } else if (alpha < kMinAlpha) {
-#pragma unroll
-        for (int j = 0; j < NC; ++j) x[j] = clampd(x[j] + fx[j], kMinLog, kMaxLog);
+        trial = true;
+        alpha = 1.0;
+      } else {
+        // no usable direction: take the convex route from here
+        gmode = true;
         trial = false;
       }
     }
@@ -343,14 +488,14 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
       for (int j = 0; j < NC; ++j)
         if (j < tb.nc) sa.logc[item * tb.nc + j] = x[j];
       if (sa.iters) sa.iters[item] = evals;
-      if (sa.resid) sa.resid[item] = sqrt(2.0 * phi);
+      if (sa.resid) sa.resid[item] = sqrt(judge2);
       if (sa.conv) sa.conv[item] = converged ? 1 : 0;
     } else {
       // species at the accepted point (calc_species), NaN guard and clamp (:1187-1214)
       double cfin[NSM];
       {
-        double ftmp[NC];
-        evaluate<NC, NSM>(tb, vsp, nvar, lkv, x, T, ftmp, nullptr, cfin);
+        EvalOut<NC> e2;
+        evaluate<NC, NSM>(tb, vsp, nvar, lkv, x, T, false, e2, cfin);
       }
 #pragma unroll
       for (int i = 0; i < NSM; ++i) {

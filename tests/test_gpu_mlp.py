@@ -83,10 +83,10 @@ def test_warm_start_gain(clb, oracle):
     _, it_tr, _, cv_tr = lat.batch_fetch()
     assert cv_tr.all()
     lat.mlp_config(12, H, 8)
-    loss = lat.mlp_train_from_batch(epochs=20, batch=512, lr=2e-3, seed=7)
+    loss = lat.mlp_train_from_batch(epochs=30, batch=512, lr=2e-3, seed=7)
     assert loss < 0.01
     Tte, Lte = batch(50000, 9)
-    Tte[:10] = 1e-8                                           # outside the training box
+    Lte[:10, 0] += 1.5                                        # logK outside the training box
     lat.batch_load(Tte, var, Lte)
     lat.batch_solve(clb.WARM_REF)
     x0, it0, r0, cv0 = lat.batch_fetch()
