@@ -11,6 +11,12 @@
  *
  * Substrate order: C[0] = donor D, C[1] = acceptor A, C[2], C[3] = products.
  * With the time step dt = 1 s, k_max * dt = 1e-3 per step.
+ *
+ * Per-step stability limit, the same safeguard the reference's own kinetics file applies
+ * (defineKinetics.hh, "PER-STEP STABILITY LIMIT"): with k_max B / Ks up to 10 per step the
+ * explicit update would draw a reactant below zero, so r is capped so that neither D nor
+ * A loses more than half of what it holds in one step. Scaling the single rate keeps the
+ * stoichiometry (and so every mass balance) exact.
  */
 #ifndef DEFINE_KINETICS_HH
 #define DEFINE_KINETICS_HH
@@ -25,6 +31,8 @@ namespace DualMonod {
     const double Ks_D  = 1.0e-2;   // half saturation on the donor
     const double Ks_A  = 1.0e-2;   // half saturation on the acceptor
     const double Y     = 0.4;      // biomass yield per unit donor
+    const double dt    = 1.0;      // s, the transport time step the rates are applied over
+    const double MAX_RATE_FRACTION = 0.5;
 }
 
 void defineRxnKinetics(std::vector<double> B, std::vector<double> C,
@@ -39,7 +47,10 @@ void defineRxnKinetics(std::vector<double> B, std::vector<double> C,
     const double D  = std::max(C[0], 0.0);
     const double A  = std::max(C[1], 0.0);
     const double Bm = std::max(B[0], 0.0);
-    const double r  = DualMonod::k_max * Bm * (D / (DualMonod::Ks_D + D)) * (A / (DualMonod::Ks_A + A));
+    double r = DualMonod::k_max * Bm * (D / (DualMonod::Ks_D + D)) * (A / (DualMonod::Ks_A + A));
+
+    const double cap = DualMonod::MAX_RATE_FRACTION * std::min(D, A) / DualMonod::dt;
+    if (r > cap) r = cap;
 
     subsR[0] = -r;
     subsR[1] = -r;

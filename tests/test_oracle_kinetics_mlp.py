@@ -12,11 +12,17 @@ from conftest import d3q19_kind_omega
 
 
 def test_dual_monod_plugin_closed_form(oracle):
-    B, D, A = 37.0, 0.2, 0.05
+    B, D, A = 7.0, 0.2, 0.05
     sr, br = oracle.kinetics_plugin([B], [D, A, 0.0, 0.0], 2)
     r = 1e-3 * B * D / (1e-2 + D) * A / (1e-2 + A)
+    assert r < 0.5 * min(D, A)
     assert np.allclose(sr, [-r, -r, r, r], rtol=1e-15, atol=0)
     assert np.allclose(br, [0.4 * r], rtol=1e-15, atol=0)
+    # per-step stability cap: no reactant loses more than half of what it holds
+    B, D, A = 90.0, 0.2, 0.01
+    sr, br = oracle.kinetics_plugin([B], [D, A, 0.0, 0.0], 2)
+    assert np.allclose(sr, [-0.005, -0.005, 0.005, 0.005], rtol=1e-15, atol=0)
+    assert np.allclose(br, [0.4 * 0.005], rtol=1e-15, atol=0)
     # no reactions in solid / bounce-back voxels (mask < 2)
     sr, br = oracle.kinetics_plugin([B], [D, A, 0.0, 0.0], 1)
     assert not sr.any() and not br.any()
