@@ -91,6 +91,14 @@ def load():
         "clb_mlp_grad": [vp, c_long, vp, vp, vp, vp],
         "clb_mlp_train": [vp, c_long, vp, vp, c_int, c_int, c_double, c_u64, vp],
         "clb_mlp_train_from_batch": [vp, c_int, c_int, c_double, c_u64, vp],
+        "clb_set_immobile": [vp, vp],
+        "clb_set_abiotic": [vp, c_int],
+        "clb_set_phase": [vp, vp],
+        "clb_download_phase": [vp, vp],
+        "clb_node_conversion": [vp, c_int, c_int, c_int, c_int, c_double, c_double, vp],
+        "clb_flow_stats": [vp, vp],
+        "clb_solutes_reset": [vp, vp],
+        "clb_set_graph": [vp, c_int, c_int],
         "clb_set_timing": [vp, c_int],
         "clb_get_perf": [vp, P(Perf)],
         "clb_reset_perf": [vp],
@@ -295,6 +303,42 @@ class Lattice:
 
     def step(self, n=1):
         self._chk(self.lib.clb_step(self.h, n), "step")
+
+    def set_immobile(self, flags):
+        f = np.ascontiguousarray(flags, dtype=np.int32)
+        assert f.size == self.S
+        self._chk(self.lib.clb_set_immobile(self.h, _ptr(f)), "set_immobile")
+
+    def set_abiotic(self, mode):
+        self._chk(self.lib.clb_set_abiotic(self.h, int(mode)), "set_abiotic")
+
+    def set_phase(self, phase):
+        p = np.ascontiguousarray(phase, dtype=np.uint8).ravel()
+        assert p.size == self.N
+        self._chk(self.lib.clb_set_phase(self.h, _ptr(p)), "set_phase")
+
+    def phase(self):
+        p = np.zeros(self.N, dtype=np.uint8)
+        self._chk(self.lib.clb_download_phase(self.h, _ptr(p)), "download_phase")
+        return p
+
+    def node_conversion(self, mineral_solute, pore_code, precip_code, phase_id, precip_max, reopen):
+        ch = ctypes.c_longlong()
+        self._chk(self.lib.clb_node_conversion(self.h, mineral_solute, pore_code, precip_code, phase_id, precip_max,
+                                               reopen, ctypes.byref(ch)), "node_conversion")
+        return ch.value
+
+    def flow_stats(self):
+        s = np.zeros(4)
+        self._chk(self.lib.clb_flow_stats(self.h, _ptr(s)), "flow_stats")
+        return s
+
+    def solutes_reset(self, c0):
+        c = _f64(c0)
+        self._chk(self.lib.clb_solutes_reset(self.h, _ptr(c)), "solutes_reset")
+
+    def set_graph(self, enable, steps_per_graph=100):
+        self._chk(self.lib.clb_set_graph(self.h, int(enable), steps_per_graph), "set_graph")
 
     def speciation_stats(self):
         s = np.zeros(3, dtype=np.int64)

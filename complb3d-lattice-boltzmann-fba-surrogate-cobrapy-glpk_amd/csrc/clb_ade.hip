@@ -1,21 +1,27 @@
 // clb_ade.hip -- D3Q7 advection-diffusion for every solute, fused with the per-voxel
-// kinetics, in one pass over the populations.
+// biotic and abiotic kinetics, in one pass over the populations.
 //
 // Replaces, per step of the loop at src/complab.cpp:1152-1860:
 //   * Palabos AdvectionDiffusionBGKdynamics collide (:1252) and stream (:1849) of every
 //     mobile solute, with Dirichlet / Neumann x faces (soluteDomainSetup,
-//     src/complab_functions.hh:296-340) and bounce-back / no-dynamics nodes;
+//     src/complab_functions.hh:296-340) and bounce-back / no-dynamics nodes; immobile
+//     solutes are neither collided nor streamed (src/complab.cpp:248-266);
 //   * dC = dC0 resets and run_kinetics + update_rxnLattices
 //     (src/complab3d_processors_part1.hh:72-127, 345-415), which call the compiled
 //     defineRxnKinetics hook per pore voxel;
+//   * the abiotic block: run_abiotic_kinetics / surfaceAbioticKinetics3D +
+//     update_abiotic_rxnLattices (src/complab3d_processors_part1.hh:158-336,
+//     src/precipitationVOP.hh:53-145) with the compiled defineAbioticRxnKinetics hook;
 //   * the biomass threshold reclassification of updateLocalMaskNtotalLattices3D
 //     (src/complab3d_processors_part2.hh:99-210, single pore type).
 // All solutes of a node are read once, collided, reacted and written once (AA pattern).
-// Populations are stored as g_i = f_i - t_i * c_ref[s].
+// Populations are stored as g_i = f_i - t_i * c_ref[s]. An immobile solute keeps its
+// whole concentration in the rest population, whose AA slot is the node's own in both
+// step parities, so it costs one read and one write per node and never moves.
 //
-// HBM traffic per fluid node and step: S*7 populations read + written, 3 velocity
-// components, 1 mask byte (+ biomass read/write and the post-kinetics concentrations
-// when kinetics / speciation are on).
+// HBM traffic per fluid node and step: 7 populations read + written per mobile solute
+// (1 per immobile one), 3 velocity components, 1 mask byte (+ biomass read/write and the
+// post-kinetics concentrations when kinetics / speciation are on).
 #include "clb_kinetics_device.h"
 #include "clb_lattice.h"
 
@@ -24,47 +30,84 @@ namespace clb {
 namespace {
 
 template <typename R, int S, bool ODD>
-__device__ __forceinline__ void load7(const R* __restrict__ g, long long N, const Nbr& nb, int n, R* p) {
+__device__ __forceinline__ void load7(const AdeArgs<R>& a, const Nbr& nb, int n, R* p) {
+  const long long N = a.geo.N;
 #pragma unroll
-  for (int s = 0; s < S; ++s)
+  for (int s = 0; s < S; ++s) {
+    if (a.immobile[s]) {
+      p[s * 7] = a.g[(long long)(s * 7) * N + n];
+#pragma unroll
+      for (int i = 1; i < 7; ++i) p[s * 7 + i] = R(0);
+      continue;
+    }
 #pragma unroll
     for (int i = 0; i < 7; ++i) {
       if (!ODD) {
-        p[s * 7 + i] = g[(long long)(s * 7 + i) * N + n];
+        p[s * 7 + i] = a.g[(long long)(s * 7 + i) * N + n];
       } else {
         if (nb.xok(-Q7::cx[i]))
-          p[s * 7 + i] = g[(long long)(s * 7 + opp_dir(i)) * N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])];
+          p[s * 7 + i] = a.g[(long long)(s * 7 + opp_dir(i)) * N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])];
         else
           p[s * 7 + i] = R(0);
       }
     }
+  }
 }
 
 template <typename R, int S, bool ODD>
-__device__ __forceinline__ void store7(R* __restrict__ g, long long N, const Nbr& nb, int n, const R* p) {
+__device__ __forceinline__ void store7(const AdeArgs<R>& a, const Nbr& nb, int n, const R* p) {
+  const long long N = a.geo.N;
 #pragma unroll
-  for (int s = 0; s < S; ++s)
+  for (int s = 0; s < S; ++s) {
+    if (a.immobile[s]) {
+      a.g[(long long)(s * 7) * N + n] = p[s * 7];
+      continue;
+    }
 #pragma unroll
     for (int i = 0; i < 7; ++i) {
       if (!ODD) {
-        g[(long long)(s * 7 + opp_dir(i)) * N + n] = p[s * 7 + i];
+        a.g[(long long)(s * 7 + opp_dir(i)) * N + n] = p[s * 7 + i];
       } else {
-        if (nb.xok(Q7::cx[i])) g[(long long)(s * 7 + i) * N + nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])] = p[s * 7 + i];
+        if (nb.xok(Q7::cx[i])) a.g[(long long)(s * 7 + i) * N + nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])] = p[s * 7 + i];
       }
     }
+  }
 }
 
-// Collide (with boundary completion), kinetics, optional post-kinetics concentration
-// output and biomass threshold rule for one fluid node. rho_bc[s] is the Neumann density
-// (interior neighbour's pre-collision density) for boundary nodes.
+template <typename R>
+__device__ __forceinline__ R pop_sum(const R* q) {
+  R r = 0;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) r += q[i];
+  return r;
+}
+
+// add dC to solute s of the node (dC * t_i per population; all of it in the rest slot
+// for an immobile solute)
+template <typename R>
+__device__ __forceinline__ void inject(const AdeArgs<R>& a, int s, R* p, double dC) {
+  if (a.immobile[s]) {
+    p[s * 7] += (R)dC;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) p[s * 7 + i] += (R)(dC * Q7::w(i));
+  }
+}
+
+// Collide (with boundary completion), biotic and abiotic kinetics, optional
+// post-kinetics concentration output and biomass threshold rule for one fluid node.
+// rho_bc[s] is the Neumann density (interior neighbour's pre-collision density) for
+// boundary nodes.
 template <typename R, int S, bool KIN, bool CBUF>
-__device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int n, uint8_t code, R* p, const R* rho_bc) {
+__device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int y, int z, int n, uint8_t code, R* p,
+                                         const R* rho_bc) {
   const Geom& g = a.geo;
   if (x == 0 || x == g.nx - 1) {
     const int side = (x == 0) ? 0 : 1;
     const int miss = (x == 0) ? 1 : 2;  // population entering the domain
 #pragma unroll
     for (int s = 0; s < S; ++s) {
+      if (a.immobile[s]) continue;
       const R cb = (a.bc_type[2 * s + side] == CLB_BC_DIRICHLET) ? a.bc_value[2 * s + side] : rho_bc[s];
       R others = 0;
 #pragma unroll
@@ -76,9 +119,8 @@ __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int n, uint
   const R ux = a.u[n], uy = a.u[g.N + n], uz = a.u[2 * g.N + n];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
-    R rs = 0;
-#pragma unroll
-    for (int i = 0; i < 7; ++i) rs += p[s * 7 + i];
+    if (a.immobile[s]) continue;
+    const R rs = pop_sum(&p[s * 7]);
     const R rho = rs + a.c_ref[s];
     const R om = a.omega[s * 256 + code];
 #pragma unroll
@@ -89,14 +131,11 @@ __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int n, uint
     }
   }
   const bool react = (x > 0 && x < g.nx - 1 && code != a.solid_code && code != a.bb_code);
-  if (KIN && react) {
+  if (KIN && react && a.kinetics) {
     KVec C(S, 0.0), Bv(a.nb, 0.0), sr(S, 0.0), br(a.nb, 0.0);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-      R rs = 0;
-#pragma unroll
-      for (int i = 0; i < 7; ++i) rs += p[s * 7 + i];
-      double c0 = (double)rs + (double)a.c_ref[s];
+      double c0 = (double)pop_sum(&p[s * 7]) + (double)a.c_ref[s];
       if (c0 < 1e-12) c0 = 0;
       C[s] = c0;
     }
@@ -110,10 +149,7 @@ __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int n, uint
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const double dC = sr[s] * a.dt;
-      if (dC > 1e-12 || dC < -1e-12) {
-#pragma unroll
-        for (int i = 0; i < 7; ++i) p[s * 7 + i] += (R)(dC * Q7::w(i));
-      }
+      if (dC > 1e-12 || dC < -1e-12) inject(a, s, p, dC);
     }
     for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
       if (b < a.nb) {
@@ -123,6 +159,34 @@ __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int n, uint
           *bp = (R)((double)*bp + dB);
         }
       }
+  }
+  if (KIN && react && a.abiotic) {
+    bool gate = true;
+    if (a.abiotic == 2) {
+      // surfaceAbioticKinetics3D: only pore voxels with a wall among the 6 face neighbours
+      const Nbr nb(x, y, z, g);
+      gate = false;
+#pragma unroll
+      for (int i = 1; i < 7; ++i) {
+        const uint8_t m = a.mask[nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])];
+        if (m == a.solid_code || m == a.bb_code) gate = true;
+      }
+    }
+    if (gate) {
+      KVec C(S, 0.0), sr(S, 0.0);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        double c0 = (double)pop_sum(&p[s * 7]) + (double)a.c_ref[s];
+        if (c0 < 1e-12) c0 = 0;
+        C[s] = c0;
+      }
+      defineAbioticRxnKinetics(C, sr, (plb::plint)code);
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const double dC = sr[s] * a.dt;
+        if (dC > 1e-12 || dC < -1e-12) inject(a, s, p, dC);
+      }
+    }
   }
   if (react && a.mask_update && a.nb > 0) {
     double btot = 0;
@@ -136,12 +200,7 @@ __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int n, uint
   }
   if (CBUF && react) {
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      R rs = 0;
-#pragma unroll
-      for (int i = 0; i < 7; ++i) rs += p[s * 7 + i];
-      a.cbuf[(long long)s * g.N + n] = (double)rs + (double)a.c_ref[s];
-    }
+    for (int s = 0; s < S; ++s) a.cbuf[(long long)s * g.N + n] = (double)pop_sum(&p[s * 7]) + (double)a.c_ref[s];
   }
 }
 
@@ -163,16 +222,11 @@ __global__ void __launch_bounds__(256) ade_step_kernel(AdeArgs<R> a) {
   if (fluid) {
     const Nbr nb(x, y, z, g);
     R p[S * 7];
-    load7<R, S, ODD>(a.g, g.N, nb, n, p);
+    load7<R, S, ODD>(a, nb, n, p);
 #pragma unroll
-    for (int s = 0; s < S; ++s) {
-      R rs = 0;
-#pragma unroll
-      for (int i = 0; i < 7; ++i) rs += p[s * 7 + i];
-      rho_in[s] = rs + a.c_ref[s];
-    }
-    ade_node<R, S, KIN, CBUF>(a, x, n, code, p, nullptr);
-    store7<R, S, ODD>(a.g, g.N, nb, n, p);
+    for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&p[s * 7]) + a.c_ref[s];
+    ade_node<R, S, KIN, CBUF>(a, x, y, z, n, code, p, nullptr);
+    store7<R, S, ODD>(a, nb, n, p);
   }
   if (x == 1 || x == g.nx - 2) {
     const int xb = (x == 1) ? 0 : g.nx - 1;
@@ -181,37 +235,37 @@ __global__ void __launch_bounds__(256) ade_step_kernel(AdeArgs<R> a) {
     if (a.kind[cb] == CLB_KIND_FLUID) {
       const Nbr nb(xb, y, z, g);
       R q[S * 7];
-      load7<R, S, ODD>(a.g, g.N, nb, nbn, q);
+      load7<R, S, ODD>(a, nb, nbn, q);
       if (!fluid) {
         // interior neighbour is a wall: the Neumann face keeps its own density
 #pragma unroll
-        for (int s = 0; s < S; ++s) {
-          R rs = 0;
-#pragma unroll
-          for (int i = 0; i < 7; ++i) rs += q[s * 7 + i];
-          rho_in[s] = rs + a.c_ref[s];
-        }
+        for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&q[s * 7]) + a.c_ref[s];
       }
-      ade_node<R, S, false, false>(a, xb, nbn, cb, q, rho_in);
-      store7<R, S, ODD>(a.g, g.N, nb, nbn, q);
+      ade_node<R, S, false, false>(a, xb, y, z, nbn, cb, q, rho_in);
+      store7<R, S, ODD>(a, nb, nbn, q);
     }
   }
 }
 
 template <typename R>
-__global__ void ade_init_kernel(R* g, Geom geo, int S, const double* c0, const R* c_ref) {
+__global__ void ade_init_kernel(R* g, Geom geo, int S, const double* c0, const R* c_ref, const int* immobile) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= geo.N) return;
   for (int s = 0; s < S; ++s)
 #pragma unroll
-    for (int i = 0; i < 7; ++i) g[(long long)(s * 7 + i) * geo.N + n] = (R)(Q7::w(i) * (c0[s] - (double)c_ref[s]));
+    for (int i = 0; i < 7; ++i) {
+      const double v = immobile[s] ? ((i == 0) ? c0[s] - (double)c_ref[s] : 0.0)
+                                   : Q7::w(i) * (c0[s] - (double)c_ref[s]);
+      g[(long long)(s * 7 + i) * geo.N + n] = (R)v;
+    }
 }
 
 template <typename R, bool ODD>
 __device__ double node_density(const AdeArgs<R>& a, int s, int x, int y, int z) {
   const Geom& g = a.geo;
-  const Nbr nb(x, y, z, g);
   const int n = x + g.nx * (y + g.ny * z);
+  if (a.immobile[s]) return (double)a.g[(long long)(s * 7) * g.N + n] + (double)a.c_ref[s];
+  const Nbr nb(x, y, z, g);
   double r = 0;
 #pragma unroll
   for (int i = 0; i < 7; ++i) {
@@ -238,7 +292,7 @@ __global__ void ade_conc_kernel(AdeArgs<R> a, double* c) {
   const bool face = (x == 0 || x == g.nx - 1) && a.kind[a.mask[n]] == CLB_KIND_FLUID;
   for (int s = 0; s < a.S; ++s) {
     double r;
-    if (!face) {
+    if (!face || a.immobile[s]) {
       r = node_density<R, ODD>(a, s, x, y, z);
     } else {
       const int side = (x == 0) ? 0 : 1;
@@ -254,10 +308,89 @@ __global__ void ade_conc_kernel(AdeArgs<R> a, double* c) {
   }
 }
 
+// Write equilibrium-at-rest pre-collision populations (concentration c0[s]) for the next
+// step into the slots node n owns for that step's parity; next_odd selects the layout.
+template <typename R>
+__device__ __forceinline__ void write_rest_state(const AdeArgs<R>& a, bool next_odd, int x, int y, int z, int n,
+                                                 const double* c0) {
+  const Geom& g = a.geo;
+  const Nbr nb(x, y, z, g);
+  for (int s = 0; s < a.S; ++s) {
+    const double v = (c0 ? c0[s] : 0.0) - (double)a.c_ref[s];
+    if (a.immobile[s]) continue;  // an immobile solute keeps its amount
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const R val = (R)(Q7::w(i) * v);
+      if (!next_odd) a.g[(long long)(s * 7 + i) * g.N + n] = val;
+      else if (nb.xok(-Q7::cx[i]))
+        a.g[(long long)(s * 7 + opp_dir(i)) * g.N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])] = val;
+    }
+  }
+}
+
+// Precipitation / dissolution node conversion (precipNodeConversion3D,
+// src/precipitationVOP.hh:176-204; dissolNodeConversion3D, src/dissolutionVOP.hh:469-511).
+// The mineral is the concentration of solute `ms` (normally immobile). A pore or biofilm
+// voxel whose mineral reaches precip_max becomes a wall (precip_code) stamped with
+// phase_id; a voxel of that phase whose mineral falls below reopen becomes pore again with
+// phase 0. changed[0] counts the voxels that switched (a flow re-solve trigger).
+template <typename R, bool ODD>
+__global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int pore_code, int precip_code,
+                                       int phase_id, double precip_max, double reopen, unsigned int* changed,
+                                       R* flowf, bool flow_next_odd) {
+  const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const Geom& g = a.geo;
+  bool flip = false;
+  if (n64 < g.N) {
+    const int n = (int)n64;
+    const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+    const uint8_t m = a.mask[n];
+    const double mineral = node_density<R, ODD>(a, ms, x, y, z);
+    if (m != a.solid_code && m != a.bb_code) {
+      if (precip_max > 0.0 && mineral >= precip_max) {
+        a.mask[n] = (uint8_t)precip_code;
+        if (phase_id > 0) phase[n] = (uint8_t)phase_id;
+        flip = true;
+      }
+    } else if (reopen > 0.0 && phase_id > 0 && phase[n] == phase_id && mineral < reopen) {
+      a.mask[n] = (uint8_t)pore_code;
+      phase[n] = 0;
+      flip = true;
+      // a reopened voxel starts at rest: flow at the reference density, mobile solutes
+      // at zero (its wall slots held only bounced populations)
+      write_rest_state(a, ODD, x, y, z, n, nullptr);
+      if (flowf) {
+        const Nbr nb(x, y, z, g);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+          if (!flow_next_odd) flowf[(long long)i * g.N + n] = R(0);
+          else if (nb.xok(-Q19::cx[i]))
+            flowf[(long long)opp_dir(i) * g.N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])] = R(0);
+        }
+      }
+    }
+  }
+  const unsigned long long bal = __ballot(flip);
+  if ((threadIdx.x & 63) == 0 && bal) atomicAdd(changed, (unsigned int)__popcll(bal));
+}
+
+// stabilizeADElattice3D (src/complab3d_processors_part3.hh:261-310): every fluid (pore /
+// biofilm) node back to the equilibrium of c0 at rest; wall nodes keep their populations.
+template <typename R>
+__global__ void ade_reset_kernel(AdeArgs<R> a, bool next_odd, const double* c0) {
+  const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const Geom& g = a.geo;
+  if (n64 >= g.N) return;
+  const int n = (int)n64;
+  if (a.kind[a.mask[n]] != CLB_KIND_FLUID) return;
+  const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+  write_rest_state(a, next_odd, x, y, z, n, c0);
+}
+
 template <typename R, int S>
 void launch_ade_step_s(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t st) {
   dim3 grid((a.geo.nx + block.x - 1) / block.x, (a.geo.ny + block.y - 1) / block.y, a.geo.nz);
-  const bool kin = a.kinetics != 0;
+  const bool kin = (a.kinetics != 0) || (a.abiotic != 0);
   const bool cb = a.cbuf != nullptr;
 #define CLB_ADE_LAUNCH(O, K, C) hipLaunchKernelGGL((ade_step_kernel<R, S, O, K, C>), grid, block, 0, st, a)
   if (odd) {
@@ -273,10 +406,11 @@ void launch_ade_step_s(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t st
 }  // namespace
 
 template <typename R>
-void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c_ref, hipStream_t s) {
+void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c_ref, const int* immobile,
+                     hipStream_t s) {
   const int bs = 256;
   hipLaunchKernelGGL(ade_init_kernel<R>, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, gpop, g, S, c0,
-                     c_ref);
+                     c_ref, immobile);
 }
 
 template <typename R>
@@ -302,11 +436,38 @@ void launch_ade_conc(const AdeArgs<R>& a, bool odd_next, double* c, hipStream_t 
   else hipLaunchKernelGGL((ade_conc_kernel<R, false>), grid, dim3(bs), 0, s, a, c);
 }
 
-template void launch_ade_init<float>(const Geom&, int, float*, const double*, const float*, hipStream_t);
-template void launch_ade_init<double>(const Geom&, int, double*, const double*, const double*, hipStream_t);
+template <typename R>
+void launch_node_conversion(const AdeArgs<R>& a, bool odd_next, uint8_t* phase, int ms, int pore_code,
+                            int precip_code, int phase_id, double precip_max, double reopen, unsigned int* changed,
+                            R* flowf, bool flow_next_odd, hipStream_t s) {
+  const int bs = 256;
+  dim3 grid((unsigned)((a.geo.N + bs - 1) / bs));
+  if (odd_next)
+    hipLaunchKernelGGL((node_conversion_kernel<R, true>), grid, dim3(bs), 0, s, a, phase, ms, pore_code, precip_code,
+                       phase_id, precip_max, reopen, changed, flowf, flow_next_odd);
+  else
+    hipLaunchKernelGGL((node_conversion_kernel<R, false>), grid, dim3(bs), 0, s, a, phase, ms, pore_code,
+                       precip_code, phase_id, precip_max, reopen, changed, flowf, flow_next_odd);
+}
+
+template <typename R>
+void launch_ade_reset(const AdeArgs<R>& a, bool next_odd, const double* c0, hipStream_t s) {
+  const int bs = 256;
+  hipLaunchKernelGGL(ade_reset_kernel<R>, dim3((unsigned)((a.geo.N + bs - 1) / bs)), dim3(bs), 0, s, a, next_odd, c0);
+}
+
+template void launch_ade_reset<float>(const AdeArgs<float>&, bool, const double*, hipStream_t);
+template void launch_ade_reset<double>(const AdeArgs<double>&, bool, const double*, hipStream_t);
+template void launch_ade_init<float>(const Geom&, int, float*, const double*, const float*, const int*, hipStream_t);
+template void launch_ade_init<double>(const Geom&, int, double*, const double*, const double*, const int*,
+                                      hipStream_t);
 template void launch_ade_step<float>(const AdeArgs<float>&, bool, dim3, hipStream_t);
 template void launch_ade_step<double>(const AdeArgs<double>&, bool, dim3, hipStream_t);
 template void launch_ade_conc<float>(const AdeArgs<float>&, bool, double*, hipStream_t);
 template void launch_ade_conc<double>(const AdeArgs<double>&, bool, double*, hipStream_t);
+template void launch_node_conversion<float>(const AdeArgs<float>&, bool, uint8_t*, int, int, int, int, double,
+                                            double, unsigned int*, float*, bool, hipStream_t);
+template void launch_node_conversion<double>(const AdeArgs<double>&, bool, uint8_t*, int, int, int, int, double,
+                                             double, unsigned int*, double*, bool, hipStream_t);
 
 }  // namespace clb

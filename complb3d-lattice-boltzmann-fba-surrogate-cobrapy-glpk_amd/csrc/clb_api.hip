@@ -60,6 +60,19 @@ struct clb_ctx {
   void* d_b = nullptr;
   dim3 ade_block{64, 4, 1};
 
+  int h_immobile[CLB_MAX_SOLUTES] = {0};
+  int* d_immobile = nullptr;
+  int abiotic = 0;
+  uint8_t* d_phase = nullptr;
+  double* d_fstats = nullptr;
+
+  // captured step graphs (even + odd step pairs)
+  bool use_graph = false;
+  int graph_pairs = 50;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  bool graph_valid = false;
+
   int kin_kind = 0;
   double dt = 1.0;
   int solid_code = 0, bb_code = 1;
@@ -211,8 +224,12 @@ AdeArgs<R> ade_args(clb_ctx* c) {
     a.bc_type[i] = c->bc_type[i];
     a.bc_value[i] = (R)c->bc_value[i];
   }
-  for (int s = 0; s < CLB_MAX_SOLUTES; ++s) a.c_ref[s] = (R)0;
+  for (int s = 0; s < CLB_MAX_SOLUTES; ++s) {
+    a.c_ref[s] = (R)0;
+    a.immobile[s] = (s < c->S) ? c->h_immobile[s] : 0;
+  }
   a.kinetics = c->kin_kind;
+  a.abiotic = c->abiotic;
   a.dt = c->dt;
   a.solid_code = c->solid_code;
   a.bb_code = c->bb_code;
@@ -296,35 +313,77 @@ int build_reactive_list(clb_ctx* c) {
   return CLB_OK;
 }
 
+// One coupled step on the stream, no host synchronisation (so it can be captured).
+template <typename R>
+void enqueue_step(clb_ctx* c) {
+  const bool odd = (c->ade_t & 1) != 0;
+  AdeArgs<R> a = ade_args<R>(c);
+  {
+    Timed tm(c, 1);
+    launch_ade_step<R>(a, odd, c->ade_block, c->stream);
+  }
+  count_launch(c, 1);
+  if (c->have_tableau && c->n_reactive > 0) {
+    launch_fill_u32(c->d_counter, 1, 0u, c->stream);
+    SpecArgs sa{};
+    sa.n = c->n_reactive;
+    sa.iters = (int*)c->d_list;  // node list for the coupled kernel
+    sa.counter = c->d_counter;
+    sa.stats = c->d_stats;
+    {
+      Timed tm(c, 2);
+      launch_spec_coupled(a, odd, c->tb, sa, c->spec_grid, c->stream);
+    }
+    count_launch(c, 2);
+  }
+  c->ade_t++;
+}
+
+void drop_graph(clb_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_valid = false;
+}
+
 template <typename R>
 int step_t(clb_ctx* c, int nsteps) {
   if (c->have_tableau && c->list_dirty) {
     int rc = build_reactive_list(c);
     if (rc) return rc;
+    drop_graph(c);
   }
-  for (int t = 0; t < nsteps; ++t) {
-    const bool odd = (c->ade_t & 1) != 0;
-    AdeArgs<R> a = ade_args<R>(c);
-    {
-      Timed tm(c, 1);
-      launch_ade_step<R>(a, odd, c->ade_block, c->stream);
+  const int seg = 2 * c->graph_pairs;
+  if (c->use_graph && !c->timing && nsteps >= seg) {
+    // Replay a captured graph of `seg` steps (it starts on an even step, as the AA
+    // kernels alternate): one launch per segment instead of two or three per step.
+    if (c->ade_t & 1) {
+      enqueue_step<R>(c);
+      --nsteps;
     }
-    count_launch(c, 1);
-    if (c->have_tableau && c->n_reactive > 0) {
-      launch_fill_u32(c->d_counter, 1, 0u, c->stream);
-      SpecArgs sa{};
-      sa.n = c->n_reactive;
-      sa.iters = (int*)c->d_list;  // node list for the coupled kernel
-      sa.counter = c->d_counter;
-      sa.stats = c->d_stats;
-      {
-        Timed tm(c, 2);
-        launch_spec_coupled(a, odd, c->tb, sa, c->spec_grid, c->stream);
-      }
-      count_launch(c, 2);
+    if (!c->graph_valid) {
+      const long long t0 = c->ade_t;
+      CLB_CHECK_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (int t = 0; t < seg; ++t) enqueue_step<R>(c);
+      hipGraph_t g = nullptr;
+      CLB_CHECK_HIP(c, hipStreamEndCapture(c->stream, &g));
+      c->graph = g;
+      CLB_CHECK_HIP(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+      c->graph_valid = true;
+      c->ade_t = t0;  // capture recorded the steps; they run at replay
+      c->perf.ade_launches -= seg;
+      if (c->have_tableau && c->n_reactive > 0) c->perf.spec_launches -= seg;
     }
-    c->ade_t++;
+    while (nsteps >= seg) {
+      CLB_CHECK_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
+      c->ade_t += seg;
+      c->perf.ade_launches += seg;
+      if (c->have_tableau && c->n_reactive > 0) c->perf.spec_launches += seg;
+      nsteps -= seg;
+    }
   }
+  for (int t = 0; t < nsteps; ++t) enqueue_step<R>(c);
   CLB_CHECK_HIP(c, hipGetLastError());
   return CLB_OK;
 }
@@ -503,11 +562,12 @@ int clb_create(const clb_config* cfg, clb_ctx** out) {
 
 int clb_destroy(clb_ctx* c) {
   if (!c) return CLB_ERR_ARG;
+  drop_graph(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void* ptrs[] = {c->d_mask, c->d_kind, c->d_flow_omega, c->d_f, c->d_energy, c->d_ade_omega, c->d_cref, c->d_g,
                   c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
-                  c->d_box, c->d_work};
+                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -609,6 +669,7 @@ int clb_flow_download_velocity(clb_ctx* c, double* u, double* rho) {
 }
 
 int clb_set_solutes(clb_ctx* c, int S, const double* omega, const int* bc_type, const double* bc_value) {
+  if (c) drop_graph(c);
   if (!c || S < 1 || S > CLB_MAX_SOLUTES || !omega || !bc_type || !bc_value) return CLB_ERR_ARG;
   for (int s = 0; s < S; ++s)
     for (int i = 0; i < 256; ++i)
@@ -659,8 +720,15 @@ int clb_solutes_init(clb_ctx* c, const double* c0) {
     CLB_CHECK_HIP(c, hipMemset(c->d_cref, 0, CLB_MAX_SOLUTES * rs));
   }
   CLB_CHECK_HIP(c, hipMemcpy(dc0, c0, sizeof(double) * c->S, hipMemcpyHostToDevice));
-  if (c->fp32) launch_ade_init<float>(c->geo, c->S, (float*)c->d_g, dc0, (const float*)c->d_cref, c->stream);
-  else launch_ade_init<double>(c->geo, c->S, (double*)c->d_g, dc0, (const double*)c->d_cref, c->stream);
+  if (!c->d_immobile) {
+    rc = dalloc(c, &c->d_immobile, CLB_MAX_SOLUTES);
+    if (rc) { dfree(dc0); return rc; }
+  }
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_immobile, c->h_immobile, sizeof(c->h_immobile), hipMemcpyHostToDevice));
+  if (c->fp32)
+    launch_ade_init<float>(c->geo, c->S, (float*)c->d_g, dc0, (const float*)c->d_cref, c->d_immobile, c->stream);
+  else
+    launch_ade_init<double>(c->geo, c->S, (double*)c->d_g, dc0, (const double*)c->d_cref, c->d_immobile, c->stream);
   hipError_t e = hipStreamSynchronize(c->stream);
   dfree(dc0);
   if (e != hipSuccess) return fail(c, CLB_ERR_HIP, hipGetErrorString(e));
@@ -702,6 +770,7 @@ int clb_set_velocity(clb_ctx* c, const double* u) {
 }
 
 int clb_set_biomass(clb_ctx* c, int nb, const double* b) {
+  if (c) drop_graph(c);
   if (!c || nb < 0 || nb > CLB_MAX_BIOMASS || (nb > 0 && !b)) return CLB_ERR_ARG;
   c->nb = nb;
   if (nb == 0) return CLB_OK;
@@ -751,6 +820,7 @@ int clb_download_biomass(clb_ctx* c, double* out) {
 }
 
 int clb_set_kinetics(clb_ctx* c, int kind, double dt, int solid_code, int bb_code) {
+  if (c) drop_graph(c);
   if (!c || (kind != CLB_KINETICS_NONE && kind != CLB_KINETICS_PLUGIN) || !(dt > 0)) return CLB_ERR_ARG;
   c->kin_kind = kind;
   c->dt = dt;
@@ -762,6 +832,7 @@ int clb_set_kinetics(clb_ctx* c, int kind, double dt, int solid_code, int bb_cod
 
 int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* logK, const int* comp_species,
                     double tol, int max_iter) {
+  if (c) drop_graph(c);
   if (!c) return CLB_ERR_ARG;
   if (nc == 0) {
     c->have_tableau = false;
@@ -806,6 +877,7 @@ int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* l
 }
 
 int clb_set_mask_update(clb_ctx* c, int enable, double threshold, int pore_code, int biofilm_code) {
+  if (c) drop_graph(c);
   if (!c) return CLB_ERR_ARG;
   c->mask_update = enable ? 1 : 0;
   c->bthr = threshold;
@@ -1085,6 +1157,123 @@ int clb_mlp_train_from_batch(clb_ctx* c, int epochs, int batch, double lr, uint6
   dfree(dX);
   dfree(dT);
   return rc;
+}
+
+// ---------------------------------------------------------------- more of the step
+int clb_set_immobile(clb_ctx* c, const int* immobile) {
+  if (!c || !immobile) return CLB_ERR_ARG;
+  if (c->S < 1) return fail(c, CLB_ERR_STATE, "clb_set_solutes first");
+  for (int s = 0; s < CLB_MAX_SOLUTES; ++s) c->h_immobile[s] = (s < c->S) ? (immobile[s] ? 1 : 0) : 0;
+  drop_graph(c);
+  return CLB_OK;
+}
+
+int clb_set_abiotic(clb_ctx* c, int mode) {
+  if (!c || mode < 0 || mode > 2) return CLB_ERR_ARG;
+  c->abiotic = mode;
+  drop_graph(c);
+  return CLB_OK;
+}
+
+int clb_set_phase(clb_ctx* c, const uint8_t* phase) {
+  if (!c || !phase) return CLB_ERR_ARG;
+  if (!c->d_phase) {
+    int rc = dalloc(c, &c->d_phase, (size_t)c->geo.N);
+    if (rc) return rc;
+  }
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_phase, phase, (size_t)c->geo.N, hipMemcpyHostToDevice));
+  return CLB_OK;
+}
+
+int clb_download_phase(clb_ctx* c, uint8_t* phase) {
+  if (!c || !phase) return CLB_ERR_ARG;
+  if (!c->d_phase) return fail(c, CLB_ERR_STATE, "no phase field");
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipMemcpy(phase, c->d_phase, (size_t)c->geo.N, hipMemcpyDeviceToHost));
+  return CLB_OK;
+}
+
+int clb_node_conversion(clb_ctx* c, int mineral_solute, int pore_code, int precip_code, int phase_id,
+                        double precip_max, double reopen, long long* changed) {
+  if (!c || mineral_solute < 0 || mineral_solute >= c->S || pore_code < 0 || pore_code > 255 || precip_code < 0 ||
+      precip_code > 255 || phase_id < 0 || phase_id > 255)
+    return CLB_ERR_ARG;
+  if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "solutes not initialised");
+  if (!c->d_phase) {
+    int rc = dalloc(c, &c->d_phase, (size_t)c->geo.N);
+    if (rc) return rc;
+    CLB_CHECK_HIP(c, hipMemset(c->d_phase, 0, (size_t)c->geo.N));
+  }
+  launch_fill_u32(c->d_counter + 1, 1, 0u, c->stream);
+  const bool odd_next = (c->ade_t & 1) != 0;
+  const bool flow_odd = (c->flow_t & 1) != 0;
+  if (c->fp32)
+    launch_node_conversion<float>(ade_args<float>(c), odd_next, c->d_phase, mineral_solute, pore_code, precip_code,
+                                  phase_id, precip_max, reopen, c->d_counter + 1,
+                                  c->flow_ready ? (float*)c->d_f : nullptr, flow_odd, c->stream);
+  else
+    launch_node_conversion<double>(ade_args<double>(c), odd_next, c->d_phase, mineral_solute, pore_code,
+                                   precip_code, phase_id, precip_max, reopen, c->d_counter + 1,
+                                   c->flow_ready ? (double*)c->d_f : nullptr, flow_odd, c->stream);
+  unsigned int h = 0;
+  CLB_CHECK_HIP(c, hipMemcpyAsync(&h, c->d_counter + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  if (changed) *changed = h;
+  if (h > 0) {
+    c->list_dirty = true;
+    drop_graph(c);
+    std::vector<uint8_t> m((size_t)c->geo.N);
+    CLB_CHECK_HIP(c, hipMemcpy(m.data(), c->d_mask, (size_t)c->geo.N, hipMemcpyDeviceToHost));
+    long long fluid = 0;
+    for (long long n = 0; n < c->geo.N; ++n) fluid += (c->h_kind[m[n]] == CLB_KIND_FLUID);
+    c->perf.fluid_nodes = fluid;
+  }
+  return CLB_OK;
+}
+
+int clb_flow_stats(clb_ctx* c, double* stats) {
+  if (!c || !stats) return CLB_ERR_ARG;
+  if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "clb_flow_init first");
+  if (!c->d_fstats) {
+    int rc = dalloc(c, &c->d_fstats, 4);
+    if (rc) return rc;
+  }
+  launch_fill_f64(c->d_fstats, 4, 0.0, c->stream);
+  const bool odd_next = (c->flow_t & 1) != 0;
+  if (c->fp32) launch_flow_stats<float>(flow_args<float>(c, nullptr), odd_next, c->d_fstats, c->stream);
+  else launch_flow_stats<double>(flow_args<double>(c, nullptr), odd_next, c->d_fstats, c->stream);
+  double h[4];
+  CLB_CHECK_HIP(c, hipMemcpyAsync(h, c->d_fstats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  stats[0] = h[3] > 0 ? h[0] / h[3] : 0.0;                                  // mean |u| over fluid nodes
+  stats[1] = h[1];                                                           // max |u|
+  stats[2] = h[2] / ((double)c->geo.ny * (double)c->geo.nz);                 // mean u_x on x = nx-2
+  stats[3] = h[3];                                                           // fluid nodes counted
+  return CLB_OK;
+}
+
+int clb_solutes_reset(clb_ctx* c, const double* c0) {
+  if (!c || !c0) return CLB_ERR_ARG;
+  if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "solutes not initialised");
+  double* dc0 = nullptr;
+  int rc = dalloc(c, &dc0, (size_t)c->S);
+  if (rc) return rc;
+  CLB_CHECK_HIP(c, hipMemcpy(dc0, c0, sizeof(double) * c->S, hipMemcpyHostToDevice));
+  const bool odd_next = (c->ade_t & 1) != 0;
+  if (c->fp32) launch_ade_reset<float>(ade_args<float>(c), odd_next, dc0, c->stream);
+  else launch_ade_reset<double>(ade_args<double>(c), odd_next, dc0, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  dfree(dc0);
+  if (e != hipSuccess) return fail(c, CLB_ERR_HIP, hipGetErrorString(e));
+  return CLB_OK;
+}
+
+int clb_set_graph(clb_ctx* c, int enable, int steps_per_graph) {
+  if (!c || steps_per_graph < 2 || (steps_per_graph & 1)) return CLB_ERR_ARG;
+  c->use_graph = enable != 0;
+  if (c->graph_pairs != steps_per_graph / 2) drop_graph(c);
+  c->graph_pairs = steps_per_graph / 2;
+  return CLB_OK;
 }
 
 // ---------------------------------------------------------------- measurement

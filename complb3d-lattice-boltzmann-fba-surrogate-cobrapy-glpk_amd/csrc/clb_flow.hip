@@ -175,7 +175,62 @@ __global__ void flow_velocity_kernel(FlowArgs<R> a, double* u, double* rho, R* a
   if (ade_u) { ade_u[n64] = (R)jx; ade_u[g.N + n64] = (R)jy; ade_u[2 * g.N + n64] = (R)jz; }
 }
 
+// Velocity statistics used by the driver after a flow solve (src/complab.cpp:625-671):
+// acc[0] += sum |u| over fluid nodes, acc[1] = max |u| (as ordered bits), acc[2] += sum of
+// u_x over every node of the plane x = nx-2 (the outlet reading), acc[3] += fluid count.
+template <typename R, bool ODD>
+__global__ void flow_stats_kernel(FlowArgs<R> a, double* acc) {
+  const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const Geom& g = a.g;
+  double su = 0, ux_out = 0, cnt = 0, umax = 0;
+  if (n64 < g.N) {
+    const int n = (int)n64;
+    const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+    if (a.kind[a.mask[n]] == CLB_KIND_FLUID && x > 0 && x < g.nx - 1) {
+      const Nbr nb(x, y, z, g);
+      R p[19];
+      load19<R, ODD>(a.f, g.N, nb, n, p);
+      double jx = 0, jy = 0, jz = 0;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) {
+        jx += Q19::cx[i] * (double)p[i];
+        jy += Q19::cy[i] * (double)p[i];
+        jz += Q19::cz[i] * (double)p[i];
+      }
+      const double um = sqrt(jx * jx + jy * jy + jz * jz);
+      su = um;
+      umax = um;
+      cnt = 1;
+      if (x == g.nx - 2) ux_out = jx;
+    }
+  }
+  __shared__ double s0[256], s1[256], s2[256], s3[256];
+  const int t = threadIdx.x;
+  s0[t] = su; s1[t] = umax; s2[t] = ux_out; s3[t] = cnt;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if (t < k) { s0[t] += s0[t + k]; s1[t] = fmax(s1[t], s1[t + k]); s2[t] += s2[t + k]; s3[t] += s3[t + k]; }
+    __syncthreads();
+  }
+  if (t == 0) {
+    atomicAdd(&acc[0], s0[0]);
+    atomicMax((unsigned long long*)&acc[1], (unsigned long long)__double_as_longlong(s1[0]));
+    atomicAdd(&acc[2], s2[0]);
+    atomicAdd(&acc[3], s3[0]);
+  }
+}
+
 }  // namespace
+
+template <typename R>
+void launch_flow_stats(const FlowArgs<R>& a, bool odd_next, double* acc, hipStream_t s) {
+  const int bs = 256;
+  dim3 grid((unsigned)((a.g.N + bs - 1) / bs));
+  if (odd_next) hipLaunchKernelGGL((flow_stats_kernel<R, true>), grid, dim3(bs), 0, s, a, acc);
+  else hipLaunchKernelGGL((flow_stats_kernel<R, false>), grid, dim3(bs), 0, s, a, acc);
+}
+template void launch_flow_stats<float>(const FlowArgs<float>&, bool, double*, hipStream_t);
+template void launch_flow_stats<double>(const FlowArgs<double>&, bool, double*, hipStream_t);
 
 template <typename R>
 void launch_flow_init(const Geom& g, R* f, double rho_in, double rho_out, hipStream_t s) {

@@ -54,7 +54,9 @@ struct AdeArgs {
   int bc_type[2 * CLB_MAX_SOLUTES];
   R bc_value[2 * CLB_MAX_SOLUTES];
   R c_ref[CLB_MAX_SOLUTES];
+  int immobile[CLB_MAX_SOLUTES];  // 1: not transported, whole concentration in the rest slot
   int kinetics;           // 0 none, 1 plugin
+  int abiotic;            // 0 none, 1 plugin everywhere, 2 plugin at pore voxels next to a wall
   double dt;
   int solid_code, bb_code;
   int mask_update;
@@ -97,7 +99,16 @@ template <typename R>
 void launch_flow_velocity(const FlowArgs<R>& a, bool odd_next, double* u, double* rho, R* ade_u, hipStream_t s);
 
 template <typename R>
-void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c_ref, hipStream_t s);
+void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c_ref, const int* immobile,
+                     hipStream_t s);
+template <typename R>
+void launch_node_conversion(const AdeArgs<R>& a, bool odd_next, uint8_t* phase, int ms, int pore_code,
+                            int precip_code, int phase_id, double precip_max, double reopen, unsigned int* changed,
+                            R* flowf, bool flow_next_odd, hipStream_t s);
+template <typename R>
+void launch_ade_reset(const AdeArgs<R>& a, bool next_odd, const double* c0, hipStream_t s);
+template <typename R>
+void launch_flow_stats(const FlowArgs<R>& a, bool odd_next, double* acc, hipStream_t s);
 template <typename R>
 void launch_ade_step(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t s);
 template <typename R>

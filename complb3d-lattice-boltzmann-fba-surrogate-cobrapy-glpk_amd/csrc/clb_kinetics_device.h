@@ -7,7 +7,9 @@
 // It is included here after the system headers, inside a region where every function is
 // host+device, with the name `std` mapped onto a small namespace that supplies a
 // fixed-capacity vector and device math. The plugin's text is not modified. The plugin
-// path is given at build time with -DCLB_KINETICS_HEADER="...".
+// paths are given at build time with -DCLB_KINETICS_HEADER="..." (biotic,
+// defineRxnKinetics) and -DCLB_ABIOTIC_HEADER="..." (abiotic, defineAbioticRxnKinetics,
+// defineAbioticKinetics.hh:130-134).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -79,9 +81,14 @@ __host__ __device__ inline bool isinf(double x) { return __builtin_isinf(x); }
 __host__ __device__ inline bool isfinite(double x) { return __builtin_isfinite(x); }
 }  // namespace clb_kstd
 
+#ifndef CLB_ABIOTIC_HEADER
+#error "CLB_ABIOTIC_HEADER must name the defineAbioticKinetics.hh plugin to compile"
+#endif
+
 #define std clb_kstd
 #pragma clang force_cuda_host_device begin
 #include CLB_KINETICS_HEADER
+#include CLB_ABIOTIC_HEADER
 #pragma clang force_cuda_host_device end
 #undef std
 

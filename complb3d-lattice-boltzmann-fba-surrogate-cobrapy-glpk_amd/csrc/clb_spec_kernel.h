@@ -242,13 +242,18 @@ struct CoupledIO {
   const double* cbuf;   // S*N post-kinetics concentrations
   Geom geo;
   int odd_done;         // parity of the step whose post-collision values are in g
+  int immobile[CLB_MAX_SOLUTES];
 };
 
 template <typename R>
 __device__ __forceinline__ void add_increment(const CoupledIO& io, int s, int n, double dC) {
   R* g = (R*)io.g;
   const Geom& geo = io.geo;
-  if (!io.odd_done) {
+  if (io.immobile[s]) {
+    // immobile solute: the whole concentration sits in the node's rest slot
+    R* q = &g[(long long)(s * 7) * geo.N + n];
+    *q = (R)((double)*q + dC);
+  } else if (!io.odd_done) {
     // even step wrote f*_i of node n to slot (opp(i), n): every slot of node n
 #pragma unroll
     for (int i = 0; i < 7; ++i) {

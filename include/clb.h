@@ -119,6 +119,34 @@ int clb_set_mask_update(clb_ctx* ctx, int enable, double threshold, int pore_cod
 int clb_step(clb_ctx* ctx, int nsteps);                       /* collide -> kinetics -> speciation -> stream */
 int clb_speciation_stats(clb_ctx* ctx, long long* stats /*3: solves, converged, iterations*/);
 
+/* Immobile solutes (vec_immobile, src/complab.cpp:248-266): neither collided nor streamed,
+ * changed only by reactions. immobile: S flags; set before clb_solutes_init. */
+int clb_set_immobile(clb_ctx* ctx, const int* immobile);
+/* Abiotic kinetics with the compiled defineAbioticRxnKinetics (run_abiotic_kinetics +
+ * update_abiotic_rxnLattices, src/complab3d_processors_part1.hh:158-336): mode 0 off,
+ * 1 every pore / biofilm voxel, 2 only those with a wall among the 6 face neighbours
+ * (surfaceAbioticKinetics3D, src/precipitationVOP.hh:53-145). */
+int clb_set_abiotic(clb_ctx* ctx, int mode);
+/* Solid-phase ids per voxel (the reference's phase lattice, src/complab.cpp:272-375). */
+int clb_set_phase(clb_ctx* ctx, const uint8_t* phase);
+int clb_download_phase(clb_ctx* ctx, uint8_t* phase);
+/* Precipitation / dissolution node conversion (precipNodeConversion3D,
+ * src/precipitationVOP.hh:176-204; dissolNodeConversion3D, src/dissolutionVOP.hh:469-511):
+ * pore / biofilm voxels whose mineral (concentration of mineral_solute) reaches precip_max
+ * become precip_code with phase_id; phase_id voxels whose mineral falls below reopen become
+ * pore_code with phase 0 (and restart at rest). precip_max <= 0 or reopen <= 0 disables
+ * that direction. changed: voxels switched (nonzero => the caller re-solves the flow). */
+int clb_node_conversion(clb_ctx* ctx, int mineral_solute, int pore_code, int precip_code, int phase_id,
+                        double precip_max, double reopen, long long* changed);
+/* Velocity statistics of the flow solve (src/complab.cpp:625-671): stats[0] mean |u| over
+ * fluid nodes, [1] max |u|, [2] mean u_x over the plane x = nx-2, [3] fluid nodes counted. */
+int clb_flow_stats(clb_ctx* ctx, double* stats /*4*/);
+/* stabilizeADElattice3D (src/complab3d_processors_part3.hh:261-310): every fluid node back
+ * to the at-rest equilibrium of c0; wall nodes keep their populations. */
+int clb_solutes_reset(clb_ctx* ctx, const double* c0 /*S*/);
+/* Replay the coupled step from captured HIP graphs of steps_per_graph steps (even). */
+int clb_set_graph(clb_ctx* ctx, int enable, int steps_per_graph);
+
 /* ---- batched speciation (the per-voxel solve of EquilibriumChemistry, standalone)
  * totals: n*nc component totals; var_species[nvar] + logk_var[n*nvar]: per-voxel logK of
  * nvar species (NULL / 0 when the tableau's logK holds everywhere); logc: n*nc log10

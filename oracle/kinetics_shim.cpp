@@ -18,3 +18,12 @@ extern "C" __attribute__((visibility("default"))) void orc_kinetics_plugin(const
   for (int i = 0; i < ns; ++i) subsR[i] = sr[i];
   for (int i = 0; i < nb; ++i) bioR[i] = br[i];
 }
+
+#include CLB_ABIOTIC_HEADER
+
+extern "C" __attribute__((visibility("default"))) void orc_abiotic_plugin(const double* C, int ns, double* subsR,
+                                                                           long long mask) {
+  std::vector<double> c(C, C + ns), sr(ns, 0.0);
+  defineAbioticRxnKinetics(c, sr, (plb::plint)mask);
+  for (int i = 0; i < ns; ++i) subsR[i] = sr[i];
+}

@@ -608,6 +608,8 @@ ORC_API void orc_speciate_newton_reserve(int nc, int ns, const double* S, const 
 // (defineKinetics.hh:223-228): B[nb], C[ns] in; subsR[ns], bioR[nb] out; mask code.
 typedef void (*orc_kinetics_fn)(const double* B, int nb, const double* C, int ns, double* subsR, double* bioR,
                                 long long mask);
+// Abiotic hook (defineAbioticKinetics.hh:130-134): C[ns] in, subsR[ns] out, mask code.
+typedef void (*orc_abiotic_fn)(const double* C, int ns, double* subsR, long long mask);
 
 struct orc_ade_params {
   int nx, ny, nz, S;       // grid and number of solutes
@@ -632,6 +634,11 @@ struct orc_ade_params {
   int mask_update;
   double bmass_threshold;    // thrd_frac * Bmax
   int pore_code, biofilm_code;
+  // immobile solutes (vec_immobile, src/complab.cpp:248-266): not collided, not streamed
+  const int* immobile;       // [S] or null
+  // abiotic kinetics (run_abiotic_kinetics / surfaceAbioticKinetics3D + update)
+  orc_abiotic_fn abiotic;    // null: none
+  int abiotic_surface;       // 1: only voxels with a solid / wall face neighbour
 };
 
 ORC_API void orc_ade_init(long N, int S, const double* c0, double* g) {
@@ -680,6 +687,7 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
             // ---- collide (src/complab.cpp:1252), boundary completion first
             const double ux = P->u[3 * n], uy = P->u[3 * n + 1], uz = P->u[3 * n + 2];
             for (int s = 0; s < S; ++s) {
+              if (P->immobile && P->immobile[s]) continue;
               double* q = &p[s * 7];
               if (x == 0 || x == nx - 1) {
                 int side = (x == 0) ? 0 : 1;
@@ -728,6 +736,35 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
                 if (dB > 1e-12 || dB < -1e-12) bmass[(long)b * N + n] += dB;
               }
             }
+            // ---- abiotic kinetics (after the biotic update, src/complab.cpp:1318-1353)
+            if (react && P->abiotic) {
+              bool gate = true;
+              if (P->abiotic_surface) {
+                gate = false;
+                for (int i = 1; i < 7; ++i) {
+                  int xx = x + C7[i][0];
+                  if (xx < 0 || xx >= nx) continue;
+                  int m = mask[gr.idx(xx, gr.wy(y + C7[i][1]), gr.wz(z + C7[i][2]))];
+                  if (m == P->solid_code || m == P->bb_code) gate = true;
+                }
+              }
+              if (gate) {
+                std::vector<double> C(S), sr(S, 0.0);
+                for (int s = 0; s < S; ++s) {
+                  double r = 0;
+                  for (int i = 0; i < 7; ++i) r += p[s * 7 + i];
+                  double c0 = r + 1.0;
+                  if (c0 < 1e-12) c0 = 0;
+                  C[s] = c0;
+                }
+                P->abiotic(C.data(), S, sr.data(), (long long)code);
+                for (int s = 0; s < S; ++s) {
+                  double dC = sr[s] * P->dt;
+                  if (dC > 1e-12 || dC < -1e-12)
+                    for (int i = 0; i < 7; ++i) p[s * 7 + i] += dC * W7[i];
+                }
+              }
+            }
             // ---- speciation (run_equilibrium_biotic, :1355-1494)
             if (react && P->nc > 0) {
               std::vector<double> conc(S), eq(S);
@@ -768,9 +805,13 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
               for (int i = 0; i < 7; ++i) p[s * 7 + i] = q[i];
             }
           }
-          // ---- stream (src/complab.cpp:1849)
+          // ---- stream (src/complab.cpp:1849); immobile solutes stay in place
           for (int s = 0; s < S; ++s)
             for (int i = 0; i < 7; ++i) {
+              if (P->immobile && P->immobile[s]) {
+                gtmp[((long)s * 7 + i) * N + n] = p[s * 7 + i];
+                continue;
+              }
               int xx = x + C7[i][0];
               if (xx < 0 || xx >= nx) continue;
               int yy = gr.wy(y + C7[i][1]), zz = gr.wz(z + C7[i][2]);

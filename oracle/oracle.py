@@ -40,6 +40,7 @@ def load():
         "orc_mlp_forward": ([ci, ci, ci, vp, cl, vp, vp], None),
         "orc_mlp_grad": ([ci, ci, ci, vp, cl, vp, vp, vp], cd),
         "orc_kinetics_plugin": ([vp, ci, vp, ci, vp, vp, ctypes.c_longlong], None),
+        "orc_abiotic_plugin": ([vp, ci, vp, ctypes.c_longlong], None),
         "orc_num_threads": ([], ci),
     }
     for name, (args, res) in sigs.items():
@@ -69,6 +70,7 @@ class AdeParams(ctypes.Structure):
         ("eqCompSpecies", ctypes.c_void_p), ("eqTol", ctypes.c_double), ("eqMaxIter", ctypes.c_int),
         ("mask_update", ctypes.c_int), ("bmass_threshold", ctypes.c_double),
         ("pore_code", ctypes.c_int), ("biofilm_code", ctypes.c_int),
+        ("immobile", ctypes.c_void_p), ("abiotic", ctypes.c_void_p), ("abiotic_surface", ctypes.c_int),
     ]
 
 
@@ -165,6 +167,19 @@ def kinetics_plugin_ptr():
     return ctypes.cast(lib.orc_kinetics_plugin, ctypes.c_void_p).value
 
 
+def abiotic_plugin_ptr():
+    lib = load()
+    return ctypes.cast(lib.orc_abiotic_plugin, ctypes.c_void_p).value
+
+
+def abiotic_plugin(C, mask_code):
+    lib = load()
+    C = _f(C)
+    sr = np.zeros(C.size)
+    lib.orc_abiotic_plugin(_p(C), C.size, _p(sr), mask_code)
+    return sr
+
+
 def kinetics_plugin(B, C, mask_code):
     lib = load()
     B = _f(B)
@@ -180,7 +195,7 @@ class AdeRun:
 
     def __init__(self, nx, ny, nz, mask, kind, omega, bc_type, bc_value, u, c0, solid_code=0, bb_code=1,
                  kinetics=False, dt=1.0, biomass=None, tableau=None, eq_tol=1e-10, eq_max_iter=200,
-                 mask_update=None):
+                 mask_update=None, immobile=None, abiotic=0):
         self.lib = load()
         self.nx, self.ny, self.nz = nx, ny, nz
         self.N = N = nx * ny * nz
@@ -217,6 +232,12 @@ class AdeRun:
         if mask_update is not None:
             thr, pore, bio = mask_update
             P.mask_update, P.bmass_threshold, P.pore_code, P.biofilm_code = 1, thr, pore, bio
+        self.immobile = np.zeros(self.S, dtype=np.int32) if immobile is None else \
+            np.ascontiguousarray(immobile, dtype=np.int32)
+        P.immobile = _p(self.immobile).value
+        if abiotic:
+            P.abiotic = abiotic_plugin_ptr()
+            P.abiotic_surface = 1 if abiotic == 2 else 0
         self.P = P
         self.eq_stats = np.zeros(3, dtype=np.int64)
 
@@ -235,6 +256,8 @@ class AdeRun:
         fluid = (self.kind[self.mask] == KIND_FLUID).reshape(self.nz, self.ny, self.nx)
         rho = self.bc_rho.reshape(self.S, 2, self.nz, self.ny)
         for s in range(self.S):
+            if self.immobile[s]:
+                continue
             for side, xb in ((0, 0), (1, self.nx - 1)):
                 val = self.bc_value[2 * s + side] if self.bc_type[2 * s + side] == 0 else rho[s, side]
                 c[s, :, :, xb] = np.where(fluid[:, :, xb], val, c[s, :, :, xb])

@@ -69,3 +69,33 @@ def test_mlp_gradient_finite_differences(oracle):
         tm = theta.copy(); tm[k] -= h
         fd = (oracle.mlp_grad(din, H, dout, tp, X, T)[1] - oracle.mlp_grad(din, H, dout, tm, X, T)[1]) / (2 * h)
         assert abs(fd - g[k]) < 1e-7 * max(1.0, abs(fd))
+
+
+def test_abiotic_plugin_closed_form(oracle):
+    C = [0.2, 0.05, 0.03, 0.0]
+    sr = oracle.abiotic_plugin(C, 2)
+    R = 5e-2 * 0.03 * 0.05
+    assert np.allclose(sr, [0.0, -R, -R, R], rtol=1e-15, atol=0)
+    assert not oracle.abiotic_plugin(C, 1).any()           # walls do not react
+
+
+def test_abiotic_mass_balance_and_immobile(oracle):
+    # closed box: P1 + A -> P2 keeps P1 + P2 and A + P2 constant; P2 immobile stays where
+    # it formed (its total changes only by the reaction)
+    nx, ny, nz = 10, 6, 6
+    m = np.full((nz, ny, nx), 2, dtype=np.uint8)
+    m[:, :, 0] = 1
+    m[:, :, nx - 1] = 1
+    m = m.ravel()
+    kind, _ = d3q19_kind_omega()
+    om = np.full((4, 256), 1.0 / 0.8)
+    run = oracle.AdeRun(nx, ny, nz, m, kind, om, [1] * 8, [0.0] * 8, np.zeros((m.size, 3)), [0.0, 0.3, 0.2, 0.0],
+                        abiotic=1, immobile=[0, 0, 0, 1])
+    tot = lambda: run.g.reshape(4, 7, -1).sum(axis=(1, 2))
+    t0 = tot()
+    run.run(100)
+    d = tot() - t0
+    assert d[3] > 1e-3
+    assert abs(d[2] + d[3]) < 1e-11 and abs(d[1] + d[3]) < 1e-11 and abs(d[0]) < 1e-12
+    # the immobile product has not spread into the wall nodes
+    assert np.all(run.conc()[3][m != 2] == 0.0)
