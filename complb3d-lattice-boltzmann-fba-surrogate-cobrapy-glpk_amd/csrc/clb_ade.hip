@@ -282,6 +282,10 @@ __device__ double node_density(const AdeArgs<R>& a, int s, int x, int y, int z) 
 // Fluid nodes of the x faces report what their boundary prescribes: the Dirichlet value,
 // or for a Neumann face the density of the interior neighbour (its own when that is a
 // wall), since their entering population is only rebuilt at the next collision.
+// Wall nodes (bounce-back / no-dynamics) report 0 for mobile solutes: their slots hold the
+// neighbours' reflected populations (Palabos' BounceBack::computeDensity returns a fixed
+// fictitious density, 0 by default, not a population sum). Immobile solutes report their
+// amount everywhere, so a precipitated voxel still shows its mineral.
 template <typename R, bool ODD>
 __global__ void ade_conc_kernel(AdeArgs<R> a, double* c) {
   const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -289,10 +293,13 @@ __global__ void ade_conc_kernel(AdeArgs<R> a, double* c) {
   if (n64 >= g.N) return;
   const int n = (int)n64;
   const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
-  const bool face = (x == 0 || x == g.nx - 1) && a.kind[a.mask[n]] == CLB_KIND_FLUID;
+  const bool fluid = a.kind[a.mask[n]] == CLB_KIND_FLUID;
+  const bool face = (x == 0 || x == g.nx - 1) && fluid;
   for (int s = 0; s < a.S; ++s) {
     double r;
-    if (!face || a.immobile[s]) {
+    if (!fluid && !a.immobile[s]) {
+      r = 0.0;
+    } else if (!face || a.immobile[s]) {
       r = node_density<R, ODD>(a, s, x, y, z);
     } else {
       const int side = (x == 0) ? 0 : 1;
@@ -330,10 +337,12 @@ __device__ __forceinline__ void write_rest_state(const AdeArgs<R>& a, bool next_
 
 // Precipitation / dissolution node conversion (precipNodeConversion3D,
 // src/precipitationVOP.hh:176-204; dissolNodeConversion3D, src/dissolutionVOP.hh:469-511).
-// The mineral is the concentration of solute `ms` (normally immobile). A pore or biofilm
-// voxel whose mineral reaches precip_max becomes a wall (precip_code) stamped with
-// phase_id; a voxel of that phase whose mineral falls below reopen becomes pore again with
-// phase 0. changed[0] counts the voxels that switched (a flow re-solve trigger).
+// The mineral is the concentration of solute `ms` (normally immobile). A voxel that is
+// neither solid nor bounce-back and whose mineral reaches precip_max becomes precip_code,
+// stamped with phase_id; any other voxel stamped with phase_id whose mineral falls below
+// reopen becomes pore_code with phase 0, whatever its mask was. changed[0] counts the
+// voxels whose mask code changed (a flow re-solve trigger); a wall voxel that reopens
+// restarts at rest.
 template <typename R, bool ODD>
 __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int pore_code, int precip_code,
                                        int phase_id, double precip_max, double reopen, unsigned int* changed,
@@ -346,18 +355,21 @@ __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int
     const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
     const uint8_t m = a.mask[n];
     const double mineral = node_density<R, ODD>(a, ms, x, y, z);
-    if (m != a.solid_code && m != a.bb_code) {
-      if (precip_max > 0.0 && mineral >= precip_max) {
-        a.mask[n] = (uint8_t)precip_code;
-        if (phase_id > 0) phase[n] = (uint8_t)phase_id;
-        flip = true;
-      }
-    } else if (reopen > 0.0 && phase_id > 0 && phase[n] == phase_id && mineral < reopen) {
+    bool precip = false;
+    if (m != a.solid_code && m != a.bb_code && precip_max > 0.0 && mineral >= precip_max) {
+      a.mask[n] = (uint8_t)precip_code;
+      if (phase_id > 0) phase[n] = (uint8_t)phase_id;
+      precip = true;
+      flip = (int)m != precip_code;
+    }
+    if (!precip && reopen > 0.0 && phase_id > 0 && phase[n] == phase_id && mineral < reopen) {
       a.mask[n] = (uint8_t)pore_code;
       phase[n] = 0;
-      flip = true;
-      // a reopened voxel starts at rest: flow at the reference density, mobile solutes
-      // at zero (its wall slots held only bounced populations)
+      flip = (int)m != pore_code;
+    }
+    if (flip && a.mask[n] == (uint8_t)pore_code && a.kind[m] != CLB_KIND_FLUID) {
+      // a reopened wall voxel starts at rest: flow at the reference density, mobile
+      // solutes at zero (its wall slots held only bounced populations)
       write_rest_state(a, ODD, x, y, z, n, nullptr);
       if (flowf) {
         const Nbr nb(x, y, z, g);

@@ -843,23 +843,28 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
 // Mask updates
 // =====================================================================================
 
-// Precipitation / dissolution node conversion (src/precipitationVOP.hh:176-204,
-// src/dissolutionVOP.hh:469-511): pore voxels whose mineral density reaches
-// precip_max become solid (phase = phase_id); solid-mineral voxels (phase == phase_id)
-// whose mineral density drops below reopen become pore (phase 0). Returns the number
-// of changed voxels (the trigger for a flow re-solve).
+// Precipitation / dissolution node conversion. precipNodeConversion3D
+// (src/precipitationVOP.hh:176-204): a voxel whose mask is neither solid_code nor bb_code
+// and whose mineral density reaches precip_max becomes precip_code, stamped with phase_id
+// when phase_id > 0. dissolNodeConversion3D (src/dissolutionVOP.hh:469-511): a voxel
+// stamped with phase_id whose mineral density falls below reopen becomes pore_code with
+// phase 0, whatever its mask. precip_max <= 0 / reopen <= 0 disable a direction (the
+// reference only runs the processors that are configured). Returns the number of voxels
+// whose mask code changed (the trigger for a flow re-solve).
 ORC_API long orc_node_conversion(long N, uint8_t* mask, uint8_t* phase, const double* mineral, int pore_code,
-                                 int solid_code, int phase_id, double precip_max, double reopen) {
+                                 int solid_code, int bb_code, int precip_code, int phase_id, double precip_max,
+                                 double reopen) {
   long changed = 0;
   for (long n = 0; n < N; ++n) {
-    if (mask[n] == pore_code && mineral[n] >= precip_max) {
-      mask[n] = (uint8_t)solid_code;
-      phase[n] = (uint8_t)phase_id;
-      ++changed;
-    } else if (mask[n] == solid_code && phase[n] == phase_id && mineral[n] < reopen) {
+    const int m = mask[n];
+    if (m != solid_code && m != bb_code && precip_max > 0 && mineral[n] >= precip_max) {
+      mask[n] = (uint8_t)precip_code;
+      if (phase_id > 0) phase[n] = (uint8_t)phase_id;
+      changed += (m != precip_code);
+    } else if (reopen > 0 && phase_id > 0 && phase[n] == phase_id && mineral[n] < reopen) {
       mask[n] = (uint8_t)pore_code;
       phase[n] = 0;
-      ++changed;
+      changed += (m != pore_code);
     }
   }
   return changed;

@@ -35,7 +35,7 @@ def load():
         "orc_ade_init": ([cl, ci, vp, vp], None),
         "orc_ade_density": ([cl, ci, vp, vp], None),
         "orc_ade_run": ([vp, vp, vp, vp, vp, ci, vp], None),
-        "orc_node_conversion": ([cl, vp, vp, vp, ci, ci, ci, cd, cd], cl),
+        "orc_node_conversion": ([cl, vp, vp, vp, ci, ci, ci, ci, ci, cd, cd], cl),
         "orc_classify_solid": ([ci, ci, ci, vp, vp, ci, ci], None),
         "orc_mlp_forward": ([ci, ci, ci, vp, cl, vp, vp], None),
         "orc_mlp_grad": ([ci, ci, ci, vp, cl, vp, vp, vp], cd),
@@ -190,6 +190,19 @@ def kinetics_plugin(B, C, mask_code):
     return sr, br
 
 
+def node_conversion(mask, phase, mineral, pore_code, solid_code, bb_code, precip_code, phase_id, precip_max,
+                    reopen):
+    """Precipitation / dissolution node conversion on copies of mask and phase; returns
+    (mask, phase, number of voxels whose mask code changed)."""
+    lib = load()
+    m = np.ascontiguousarray(mask, dtype=np.uint8).copy()
+    ph = np.ascontiguousarray(phase, dtype=np.uint8).copy()
+    mi = _f(mineral).ravel()
+    n = lib.orc_node_conversion(m.size, _p(m), _p(ph), _p(mi), pore_code, solid_code, bb_code, precip_code, phase_id,
+                                precip_max, reopen)
+    return m, ph, n
+
+
 class AdeRun:
     """Holds the oracle's state for a coupled transport run (populations, biomass, mask)."""
 
@@ -249,7 +262,10 @@ class AdeRun:
         """Concentration per node. On fluid nodes of the x faces the entering population is
         only rebuilt at the next collision, so these report what the boundary prescribes
         (the Dirichlet value, or for a Neumann face the density it copies), as the
-        boundary dynamics' computeDensity does."""
+        boundary dynamics' computeDensity does. Wall nodes report 0 for mobile solutes:
+        Palabos' BounceBack::computeDensity returns a fixed fictitious density (0 by
+        default), not the sum of the reflected populations the node holds. Immobile
+        solutes report their amount everywhere (a precipitated voxel keeps its mineral)."""
         c = np.zeros(self.S * self.N)
         self.lib.orc_ade_density(self.N, self.S, _p(self.g), _p(c))
         c = c.reshape(self.S, self.nz, self.ny, self.nx)
@@ -258,6 +274,7 @@ class AdeRun:
         for s in range(self.S):
             if self.immobile[s]:
                 continue
+            c[s][~fluid] = 0.0
             for side, xb in ((0, 0), (1, self.nx - 1)):
                 val = self.bc_value[2 * s + side] if self.bc_type[2 * s + side] == 0 else rho[s, side]
                 c[s, :, :, xb] = np.where(fluid[:, :, xb], val, c[s, :, :, xb])

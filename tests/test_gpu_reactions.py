@@ -85,22 +85,25 @@ def test_node_conversion_bit_exact(clb, oracle, case):
     mask0 = lat.download_mask()
     phase0 = np.zeros(mask0.size, dtype=np.uint8)
     lat.set_phase(phase0)
+    # precipitation: non-wall voxels (interior or faces) whose mineral reaches thr -> bb, phase 7
     changed = lat.node_conversion(3, 2, 1, 7, thr, -1.0)
-    m_gpu, ph_gpu = lat.download_mask(), lat.phase()
-    m_ref, ph_ref = mask0.copy(), phase0.copy()
-    # checker: pore voxels (interior or faces) whose mineral reaches thr -> wall, phase 7
-    n_ref = oracle.load().orc_node_conversion(m_ref.size, oracle._p(m_ref), oracle._p(ph_ref),
-                                              oracle._p(np.ascontiguousarray(mineral)), 2, 1, 7, thr, -1.0)
+    m_ref, ph_ref, n_ref = oracle.node_conversion(mask0, phase0, mineral, 2, 0, 1, 1, 7, thr, -1.0)
     assert changed == n_ref > 0
-    assert np.array_equal(m_gpu, m_ref) and np.array_equal(ph_gpu, ph_ref)
-    # reopen: drop the threshold so the stamped voxels dissolve back
-    changed2 = lat.node_conversion(3, 2, 1, 7, -1.0, 10.0)
-    assert changed2 == changed
-    assert np.array_equal(lat.download_mask(), mask0)
-    assert not lat.phase().any()
+    assert np.array_equal(lat.download_mask(), m_ref) and np.array_equal(lat.phase(), ph_ref)
+    # partial reopen: the stamped voxels below the median of their mineral dissolve back
+    thr_open = float(np.median(mineral[ph_ref == 7]))
+    changed_p = lat.node_conversion(3, 2, 1, 7, -1.0, thr_open)
+    m_ref, ph_ref, n_ref = oracle.node_conversion(m_ref, ph_ref, mineral, 2, 0, 1, 1, 7, -1.0, thr_open)
+    assert changed_p == n_ref and 0 < n_ref < changed
+    assert np.array_equal(lat.download_mask(), m_ref) and np.array_equal(lat.phase(), ph_ref)
     # the step keeps running on the converted geometry
     lat.step(4)
     assert np.all(np.isfinite(lat.conc()))
+    # full reopen: a threshold above every mineral amount dissolves the rest
+    changed2 = lat.node_conversion(3, 2, 1, 7, -1.0, 2.0 * lat.conc()[3].max() + 1.0)
+    assert changed_p + changed2 == changed
+    assert np.array_equal(lat.download_mask(), mask0)
+    assert not lat.phase().any()
     lat.close()
 
 

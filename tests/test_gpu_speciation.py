@@ -151,4 +151,5 @@ def test_coupled_far_from_equilibrium_converges(clb, oracle):
     assert run.eq_stats[1] < run.eq_stats[0]
     c, stats = coupled_gpu(clb, 0, nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val, c_0, 20)
     assert stats[0] == stats[1] == run.eq_stats[0]
-    assert np.all(np.isfinite(c)) and c.min() > 0
+    # every fluid voxel keeps positive concentrations (wall voxels report 0)
+    assert np.all(np.isfinite(c)) and c[:, pack == 2].min() > 0

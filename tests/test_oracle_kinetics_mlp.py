@@ -99,3 +99,22 @@ def test_abiotic_mass_balance_and_immobile(oracle):
     assert abs(d[2] + d[3]) < 1e-11 and abs(d[1] + d[3]) < 1e-11 and abs(d[0]) < 1e-12
     # the immobile product has not spread into the wall nodes
     assert np.all(run.conc()[3][m != 2] == 0.0)
+
+
+def test_node_conversion_rules(oracle):
+    # precipNodeConversion3D (src/precipitationVOP.hh:176-204) converts every voxel that is
+    # neither solid (0) nor bounce-back (1) once its mineral reaches the threshold;
+    # dissolNodeConversion3D (src/dissolutionVOP.hh:469-511) reopens every voxel stamped
+    # with the phase whose mineral falls below the reopen threshold, whatever its mask
+    mask = np.array([0, 1, 2, 3, 2, 3, 1, 0, 2], dtype=np.uint8)
+    phase = np.array([0, 0, 0, 0, 0, 0, 7, 7, 7], dtype=np.uint8)
+    mineral = np.array([5.0, 5.0, 5.0, 5.0, 0.5, 0.5, 0.1, 2.0, 0.1])
+    m, ph, n = oracle.node_conversion(mask, phase, mineral, 2, 0, 1, 1, 7, 1.0, -1.0)
+    assert m.tolist() == [0, 1, 1, 1, 2, 3, 1, 0, 2] and ph.tolist() == [0, 0, 7, 7, 0, 0, 7, 7, 7] and n == 2
+    m, ph, n = oracle.node_conversion(m, ph, mineral, 2, 0, 1, 1, 7, -1.0, 1.0)
+    # the bb voxel (0.1) and the pore voxel already open (0.1) reopen; only the first
+    # changes its mask code; voxels at 5.0 and 2.0 stay
+    assert m.tolist() == [0, 1, 1, 1, 2, 3, 2, 0, 2] and ph.tolist() == [0, 0, 7, 7, 0, 0, 0, 7, 0] and n == 1
+    # thresholds <= 0 switch a direction off
+    m2, ph2, n = oracle.node_conversion(m, ph, mineral, 2, 0, 1, 1, 7, -1.0, -1.0)
+    assert n == 0 and np.array_equal(m2, m) and np.array_equal(ph2, ph)
