@@ -132,10 +132,12 @@ int clb_set_phase(clb_ctx* ctx, const uint8_t* phase);
 int clb_download_phase(clb_ctx* ctx, uint8_t* phase);
 /* Precipitation / dissolution node conversion (precipNodeConversion3D,
  * src/precipitationVOP.hh:176-204; dissolNodeConversion3D, src/dissolutionVOP.hh:469-511):
- * pore / biofilm voxels whose mineral (concentration of mineral_solute) reaches precip_max
- * become precip_code with phase_id; phase_id voxels whose mineral falls below reopen become
- * pore_code with phase 0 (and restart at rest). precip_max <= 0 or reopen <= 0 disables
- * that direction. changed: voxels switched (nonzero => the caller re-solves the flow). */
+ * voxels that are neither solid_code nor bb_code (clb_set_kinetics) and whose mineral
+ * (concentration of mineral_solute) reaches precip_max become precip_code, stamped with
+ * phase_id when phase_id > 0; other voxels stamped with phase_id whose mineral falls below
+ * reopen become pore_code with phase 0, whatever their mask (a reopened wall voxel restarts
+ * at rest). precip_max <= 0 or reopen <= 0 disables that direction. changed: voxels whose
+ * mask code changed (nonzero => the caller re-solves the flow). */
 int clb_node_conversion(clb_ctx* ctx, int mineral_solute, int pore_code, int precip_code, int phase_id,
                         double precip_max, double reopen, long long* changed);
 /* Velocity statistics of the flow solve (src/complab.cpp:625-671): stats[0] mean |u| over

@@ -1,0 +1,105 @@
+// host_mirror_check.cpp -- drives the C++ host interface (host/complab_gpu.hh) the way the
+// reference's C++ driver would, and writes raw float64 results for
+// tests/test_gpu_host_cpp.py to compare with the FP64 oracle.
+//
+// usage: host_mirror_check <dir>
+//   reads  <dir>/eq_in.bin   n x 4 carbonate species vectors (HCO3, H, CO3, H2CO3)
+//   writes <dir>/eq_out.bin  batched calculate_species_concentrations
+//          <dir>/eq_one.bin  the single-voxel overload on row 0
+//          <dir>/u.bin       flow velocity (3N) after 200 FP64 steps on the 32x16x16 pack
+//          <dir>/conc.bin    4 solutes (4N) after 30 coupled steps with dual-Monod kinetics
+//          <dir>/bmass.bin   biomass (N)
+//   prints one line "ok <converged> <iterations> <fluid nodes>" on success.
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "complab_gpu.hh"
+
+static std::vector<double> read_bin(const std::string& p) {
+  std::ifstream f(p, std::ios::binary | std::ios::ate);
+  if (!f) throw std::runtime_error("cannot read " + p);
+  const std::streamsize n = f.tellg();
+  f.seekg(0);
+  std::vector<double> v((size_t)n / sizeof(double));
+  f.read(reinterpret_cast<char*>(v.data()), n);
+  return v;
+}
+
+static void write_bin(const std::string& p, const std::vector<double>& v) {
+  std::ofstream f(p, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(double)));
+  if (!f) throw std::runtime_error("cannot write " + p);
+}
+
+int main(int argc, char** argv) {
+  if (argc != 2) {
+    std::fprintf(stderr, "usage: %s <dir>\n", argv[0]);
+    return 2;
+  }
+  const std::string dir = argv[1];
+  try {
+    // ---- speciation: the carbonate tableau of the reference's example 04
+    complab_gpu::EquilibriumChemistry<double> eq;
+    eq.setSpeciesNames({"HCO3", "H", "CO3", "H2CO3", "H2O"});
+    eq.setComponentNames({"HCO3", "H"});
+    eq.setLogK({0.0, 0.0, -10.33, 6.35});
+    eq.setStoichiometryMatrix({{1, 0}, {0, 1}, {1, -1}, {1, 1}});
+    eq.setTolerance(1e-12);
+    eq.setMaxIterations(100);
+    const std::vector<double> in = read_bin(dir + "/eq_in.bin");
+    const long n = (long)in.size() / 4;
+    std::vector<double> out(in.size());
+    eq.calculate_species_concentrations(n, in.data(), out.data());
+    const bool conv_batch = eq.didConverge();
+    write_bin(dir + "/eq_out.bin", out);
+    const std::vector<double> one = eq.calculate_species_concentrations(std::vector<double>(in.begin(), in.begin() + 4));
+    write_bin(dir + "/eq_one.bin", one);
+    if (eq.getTotalSolves() != n + 1 || eq.getNumSpecies() != 4) throw std::runtime_error("solve bookkeeping");
+
+    // ---- lattice: flow, then transport + kinetics on the same pack
+    const int nx = 32, ny = 16, nz = 16;
+    const long long N = (long long)nx * ny * nz;
+    std::vector<uint8_t> mask((size_t)N);
+    if (clb_gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.45, 0, 2, 0, 1, mask.data()) != CLB_OK)
+      throw std::runtime_error("sphere pack");
+    std::array<int, 256> kind;
+    std::array<double, 256> om;
+    kind.fill(CLB_KIND_FLUID);
+    kind[0] = CLB_KIND_NONE;
+    kind[1] = CLB_KIND_BB;
+    om.fill(1.0 / 0.8);
+    complab_gpu::Lattice lat(nx, ny, nz, CLB_PREC_FP64);
+    lat.setDynamics(kind, om);
+    lat.uploadMask(mask);
+    lat.flowInit(1.0, 0.999);
+    std::vector<double> energy;
+    lat.flowSteps(200, &energy);
+    write_bin(dir + "/u.bin", lat.velocity());
+    const std::array<double, 4> st = lat.flowStats();
+
+    std::vector<double> b((size_t)N);
+    if (clb_gen_biomass_seed(N, mask.data(), 2, 0.02, 10.0, 100.0, 1, 2, b.data()) != CLB_OK)
+      throw std::runtime_error("biomass seed");
+    lat.setSolutes(4, std::vector<double>(4 * 256, 1.0 / 0.8), {0, 1, 0, 1, 1, 1, 1, 1},
+                   {1.0, 0, 0.5, 0, 0, 0, 0, 0});
+    lat.coupleVelocity();
+    lat.setBiomass(1, b);
+    lat.setKinetics(CLB_KINETICS_PLUGIN, 1.0, 0, 1);
+    lat.solutesInit({0.2, 0.1, 0.0, 0.0});
+    lat.step(30);
+    write_bin(dir + "/conc.bin", lat.concentrations());
+    write_bin(dir + "/bmass.bin", lat.biomass());
+    std::printf("ok %d %lld %.0f %.6e\n", conv_batch ? 1 : 0, (long long)eq.getLastIterations(), st[3],
+                energy.back());
+    return 0;
+  } catch (const complab_gpu::Error& e) {
+    std::fprintf(stderr, "complab_gpu error %d: %s\n", e.code(), e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "error: %s\n", e.what());
+    return 1;
+  }
+}

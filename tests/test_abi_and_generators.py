@@ -1,6 +1,7 @@
 """The C ABI library loads and exports every entry point include/clb.h declares; the
 host-side input generators (no GPU needed) are deterministic and meet their specs."""
 import ctypes
+import os
 
 import numpy as np
 
@@ -63,3 +64,14 @@ def test_biomass_seed(clb):
     assert k == round(0.01 * npore)
     assert np.all(m[b > 0] == 2)
     assert b[b > 0].min() >= 10 and b.max() <= 100
+
+
+def test_cpp_host_header_compiles():
+    # the C++ host interface builds against the C ABI alone (no HIP headers)
+    import subprocess
+    from conftest import PKG, ROOT
+    src = "#include \"complab_gpu.hh\"\nint main(){ complab_gpu::EquilibriumChemistry<double>* e = nullptr; (void)e; return 0; }\n"
+    r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-x", "c++", "-",
+                        "-I" + os.path.join(PKG, "host"), "-I" + os.path.join(ROOT, "include")],
+                       input=src, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
