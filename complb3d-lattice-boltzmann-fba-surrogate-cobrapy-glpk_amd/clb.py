@@ -24,6 +24,7 @@ KIND_NONE, KIND_BB, KIND_FLUID = 0, 1, 2
 BC_DIRICHLET, BC_NEUMANN = 0, 1
 KINETICS_NONE, KINETICS_PLUGIN = 0, 1
 WARM_REF, WARM_USER, WARM_MLP = 0, 1, 2
+SPEC_FUSED, SPEC_SEPARATE = 0, 1
 
 
 class ClbError(RuntimeError):
@@ -96,6 +97,7 @@ def load():
         "clb_set_phase": [vp, vp],
         "clb_download_phase": [vp, vp],
         "clb_node_conversion": [vp, c_int, c_int, c_int, c_int, c_double, c_double, vp],
+        "clb_set_speciation_mode": [vp, c_int],
         "clb_flow_stats": [vp, vp],
         "clb_solutes_reset": [vp, vp],
         "clb_set_graph": [vp, c_int, c_int],
@@ -339,6 +341,9 @@ class Lattice:
 
     def set_graph(self, enable, steps_per_graph=100):
         self._chk(self.lib.clb_set_graph(self.h, int(enable), steps_per_graph), "set_graph")
+
+    def set_speciation_mode(self, mode):
+        self._chk(self.lib.clb_set_speciation_mode(self.h, int(mode)), "set_speciation_mode")
 
     def speciation_stats(self):
         s = np.zeros(3, dtype=np.int64)

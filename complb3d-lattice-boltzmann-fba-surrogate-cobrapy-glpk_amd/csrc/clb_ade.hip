@@ -1,208 +1,16 @@
 // clb_ade.hip -- D3Q7 advection-diffusion for every solute, fused with the per-voxel
-// biotic and abiotic kinetics, in one pass over the populations.
-//
-// Replaces, per step of the loop at src/complab.cpp:1152-1860:
-//   * Palabos AdvectionDiffusionBGKdynamics collide (:1252) and stream (:1849) of every
-//     mobile solute, with Dirichlet / Neumann x faces (soluteDomainSetup,
-//     src/complab_functions.hh:296-340) and bounce-back / no-dynamics nodes; immobile
-//     solutes are neither collided nor streamed (src/complab.cpp:248-266);
-//   * dC = dC0 resets and run_kinetics + update_rxnLattices
-//     (src/complab3d_processors_part1.hh:72-127, 345-415), which call the compiled
-//     defineRxnKinetics hook per pore voxel;
-//   * the abiotic block: run_abiotic_kinetics / surfaceAbioticKinetics3D +
-//     update_abiotic_rxnLattices (src/complab3d_processors_part1.hh:158-336,
-//     src/precipitationVOP.hh:53-145) with the compiled defineAbioticRxnKinetics hook;
-//   * the biomass threshold reclassification of updateLocalMaskNtotalLattices3D
-//     (src/complab3d_processors_part2.hh:99-210, single pore type).
-// All solutes of a node are read once, collided, reacted and written once (AA pattern).
-// Populations are stored as g_i = f_i - t_i * c_ref[s]. An immobile solute keeps its
-// whole concentration in the rest population, whose AA slot is the node's own in both
-// step parities, so it costs one read and one write per node and never moves.
+// biotic and abiotic kinetics, in one pass over the populations (the per-node update is in
+// clb_ade_node.h), plus the initialisation, concentration, node-conversion and reset
+// kernels of the transport state.
 //
 // HBM traffic per fluid node and step: 7 populations read + written per mobile solute
 // (1 per immobile one), 3 velocity components, 1 mask byte (+ biomass read/write and the
-// post-kinetics concentrations when kinetics / speciation are on).
-#include "clb_kinetics_device.h"
-#include "clb_lattice.h"
+// post-kinetics concentrations when the separate speciation kernel follows).
+#include "clb_ade_node.h"
 
 namespace clb {
 
 namespace {
-
-template <typename R, int S, bool ODD>
-__device__ __forceinline__ void load7(const AdeArgs<R>& a, const Nbr& nb, int n, R* p) {
-  const long long N = a.geo.N;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (a.immobile[s]) {
-      p[s * 7] = a.g[(long long)(s * 7) * N + n];
-#pragma unroll
-      for (int i = 1; i < 7; ++i) p[s * 7 + i] = R(0);
-      continue;
-    }
-#pragma unroll
-    for (int i = 0; i < 7; ++i) {
-      if (!ODD) {
-        p[s * 7 + i] = a.g[(long long)(s * 7 + i) * N + n];
-      } else {
-        if (nb.xok(-Q7::cx[i]))
-          p[s * 7 + i] = a.g[(long long)(s * 7 + opp_dir(i)) * N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])];
-        else
-          p[s * 7 + i] = R(0);
-      }
-    }
-  }
-}
-
-template <typename R, int S, bool ODD>
-__device__ __forceinline__ void store7(const AdeArgs<R>& a, const Nbr& nb, int n, const R* p) {
-  const long long N = a.geo.N;
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (a.immobile[s]) {
-      a.g[(long long)(s * 7) * N + n] = p[s * 7];
-      continue;
-    }
-#pragma unroll
-    for (int i = 0; i < 7; ++i) {
-      if (!ODD) {
-        a.g[(long long)(s * 7 + opp_dir(i)) * N + n] = p[s * 7 + i];
-      } else {
-        if (nb.xok(Q7::cx[i])) a.g[(long long)(s * 7 + i) * N + nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])] = p[s * 7 + i];
-      }
-    }
-  }
-}
-
-template <typename R>
-__device__ __forceinline__ R pop_sum(const R* q) {
-  R r = 0;
-#pragma unroll
-  for (int i = 0; i < 7; ++i) r += q[i];
-  return r;
-}
-
-// add dC to solute s of the node (dC * t_i per population; all of it in the rest slot
-// for an immobile solute)
-template <typename R>
-__device__ __forceinline__ void inject(const AdeArgs<R>& a, int s, R* p, double dC) {
-  if (a.immobile[s]) {
-    p[s * 7] += (R)dC;
-  } else {
-#pragma unroll
-    for (int i = 0; i < 7; ++i) p[s * 7 + i] += (R)(dC * Q7::w(i));
-  }
-}
-
-// Collide (with boundary completion), biotic and abiotic kinetics, optional
-// post-kinetics concentration output and biomass threshold rule for one fluid node.
-// rho_bc[s] is the Neumann density (interior neighbour's pre-collision density) for
-// boundary nodes.
-template <typename R, int S, bool KIN, bool CBUF>
-__device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int y, int z, int n, uint8_t code, R* p,
-                                         const R* rho_bc) {
-  const Geom& g = a.geo;
-  if (x == 0 || x == g.nx - 1) {
-    const int side = (x == 0) ? 0 : 1;
-    const int miss = (x == 0) ? 1 : 2;  // population entering the domain
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      if (a.immobile[s]) continue;
-      const R cb = (a.bc_type[2 * s + side] == CLB_BC_DIRICHLET) ? a.bc_value[2 * s + side] : rho_bc[s];
-      R others = 0;
-#pragma unroll
-      for (int i = 0; i < 7; ++i)
-        if (i != miss) others += p[s * 7 + i];
-      p[s * 7 + miss] = (cb - a.c_ref[s]) - others;
-    }
-  }
-  const R ux = a.u[n], uy = a.u[g.N + n], uz = a.u[2 * g.N + n];
-#pragma unroll
-  for (int s = 0; s < S; ++s) {
-    if (a.immobile[s]) continue;
-    const R rs = pop_sum(&p[s * 7]);
-    const R rho = rs + a.c_ref[s];
-    const R om = a.omega[s * 256 + code];
-#pragma unroll
-    for (int i = 0; i < 7; ++i) {
-      const R cu = R(Q7::cx[i]) * ux + R(Q7::cy[i]) * uy + R(Q7::cz[i]) * uz;
-      const R eq = R(Q7::w(i)) * rs + R(4.0 * Q7::w(i)) * rho * cu;
-      p[s * 7 + i] += om * (eq - p[s * 7 + i]);
-    }
-  }
-  const bool react = (x > 0 && x < g.nx - 1 && code != a.solid_code && code != a.bb_code);
-  if (KIN && react && a.kinetics) {
-    KVec C(S, 0.0), Bv(a.nb, 0.0), sr(S, 0.0), br(a.nb, 0.0);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      double c0 = (double)pop_sum(&p[s * 7]) + (double)a.c_ref[s];
-      if (c0 < 1e-12) c0 = 0;
-      C[s] = c0;
-    }
-    for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
-      if (b < a.nb) {
-        double b0 = (double)a.bmass[(long long)b * g.N + n];
-        if (b0 < 1e-12) b0 = 0;
-        Bv[b] = b0;
-      }
-    defineRxnKinetics(Bv, C, sr, br, (plb::plint)code);
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      const double dC = sr[s] * a.dt;
-      if (dC > 1e-12 || dC < -1e-12) inject(a, s, p, dC);
-    }
-    for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
-      if (b < a.nb) {
-        const double dB = br[b] * a.dt;
-        if (dB > 1e-12 || dB < -1e-12) {
-          R* bp = &a.bmass[(long long)b * g.N + n];
-          *bp = (R)((double)*bp + dB);
-        }
-      }
-  }
-  if (KIN && react && a.abiotic) {
-    bool gate = true;
-    if (a.abiotic == 2) {
-      // surfaceAbioticKinetics3D: only pore voxels with a wall among the 6 face neighbours
-      const Nbr nb(x, y, z, g);
-      gate = false;
-#pragma unroll
-      for (int i = 1; i < 7; ++i) {
-        const uint8_t m = a.mask[nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])];
-        if (m == a.solid_code || m == a.bb_code) gate = true;
-      }
-    }
-    if (gate) {
-      KVec C(S, 0.0), sr(S, 0.0);
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        double c0 = (double)pop_sum(&p[s * 7]) + (double)a.c_ref[s];
-        if (c0 < 1e-12) c0 = 0;
-        C[s] = c0;
-      }
-      defineAbioticRxnKinetics(C, sr, (plb::plint)code);
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        const double dC = sr[s] * a.dt;
-        if (dC > 1e-12 || dC < -1e-12) inject(a, s, p, dC);
-      }
-    }
-  }
-  if (react && a.mask_update && a.nb > 0) {
-    double btot = 0;
-    for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
-      if (b < a.nb) {
-        const double bv = (double)a.bmass[(long long)b * g.N + n];
-        if (bv > 1e-14) btot += bv;
-      }
-    if (code == a.pore_code && btot >= a.bthreshold) a.mask[n] = (uint8_t)a.biofilm_code;
-    else if (code == a.biofilm_code && btot < a.bthreshold) a.mask[n] = (uint8_t)a.pore_code;
-  }
-  if (CBUF && react) {
-#pragma unroll
-    for (int s = 0; s < S; ++s) a.cbuf[(long long)s * g.N + n] = (double)pop_sum(&p[s * 7]) + (double)a.c_ref[s];
-  }
-}
 
 // One thread per (x, y, z) with 1 <= x <= nx-2. The thread of x = 1 also updates the
 // inlet node x = 0 and the thread of x = nx-2 the outlet node x = nx-1, after it has taken
@@ -241,7 +49,8 @@ __global__ void __launch_bounds__(256) ade_step_kernel(AdeArgs<R> a) {
 #pragma unroll
         for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&q[s * 7]) + a.c_ref[s];
       }
-      ade_node<R, S, false, false>(a, xb, y, z, nbn, cb, q, rho_in);
+      if (xb == 0) ade_node<R, S, false, false, 1>(a, xb, y, z, nbn, cb, q, rho_in);
+      else ade_node<R, S, false, false, 2>(a, xb, y, z, nbn, cb, q, rho_in);
       store7<R, S, ODD>(a, nb, nbn, q);
     }
   }

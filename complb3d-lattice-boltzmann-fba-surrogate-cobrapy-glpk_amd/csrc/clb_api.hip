@@ -87,9 +87,15 @@ struct clb_ctx {
   int* d_list = nullptr;
   long long n_reactive = 0;
   bool list_dirty = true;
-  unsigned int* d_counter = nullptr;
+  unsigned int* d_counter = nullptr;  // [0] work queue, [1] node conversion, [2] stragglers
   unsigned long long* d_stats = nullptr;
   int spec_grid = 0;
+  // speciation fused into the transport kernel (CLB_SPEC_FUSED) or after it (SEPARATE)
+  int spec_mode = CLB_SPEC_FUSED;
+  int* d_slist = nullptr;                   // voxels left to the persistent kernel
+  unsigned long long* d_blk_stats = nullptr;  // per transport block: solves, evaluations
+  long long blk_stats_n = 0;
+  bool fluid_dirty = false;
 
   // batch speciation
   long long bn = 0;
@@ -298,6 +304,30 @@ __global__ void reactive_list_kernel(const uint8_t* mask, const uint8_t* kind, G
   if (r) list[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int)n;
 }
 
+// Number of fluid voxels of the current mask (grid-stride, one atomic per block).
+__global__ void fluid_count_kernel(const uint8_t* mask, const uint8_t* kind, long long N, unsigned int* out) {
+  __shared__ unsigned int part[4];
+  unsigned int cnt = 0;
+  for (long long n = blockIdx.x * 256LL + threadIdx.x; n < N; n += (long long)gridDim.x * 256)
+    cnt += kind[mask[n]] == CLB_KIND_FLUID ? 1u : 0u;
+  for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
+int count_fluid(clb_ctx* c) {
+  launch_fill_u32(c->d_counter + 3, 1, 0u, c->stream);
+  hipLaunchKernelGGL(fluid_count_kernel, dim3(1024), dim3(256), 0, c->stream, c->d_mask, c->d_kind, c->geo.N,
+                     c->d_counter + 3);
+  unsigned int h = 0;
+  CLB_CHECK_HIP(c, hipMemcpyAsync(&h, c->d_counter + 3, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  c->perf.fluid_nodes = h;
+  c->fluid_dirty = false;
+  return CLB_OK;
+}
+
 int build_reactive_list(clb_ctx* c) {
   int rc = dalloc(c, &c->d_list, (size_t)c->geo.N);
   if (rc) return rc;
@@ -313,11 +343,59 @@ int build_reactive_list(clb_ctx* c) {
   return CLB_OK;
 }
 
+// Whether the coupled step runs the speciation inside the transport kernel: a compiled
+// shape exists for (S solutes = species, nc <= 4 components) and the mode asks for it.
+bool fused_spec(const clb_ctx* c) {
+  if (!c->have_tableau || c->spec_mode != CLB_SPEC_FUSED) return false;
+  const int nc = c->tb.nc, S = c->S;
+  return c->tb.ns == S && nc >= 1 && nc <= 4 && (nc < 2 ? 2 : nc) <= S;
+}
+
+// Buffers of the fused step (allocated outside any capture).
+int prepare_fused(clb_ctx* c) {
+  int rc = dalloc(c, &c->d_slist, (size_t)c->geo.N);
+  if (rc) return rc;
+  const long long nb = fused_blocks(c->geo, c->ade_block);
+  if (c->blk_stats_n != nb) {
+    dfree(c->d_blk_stats);
+    c->d_blk_stats = nullptr;
+    rc = dalloc(c, &c->d_blk_stats, (size_t)(2 * nb));
+    if (rc) return rc;
+    launch_fill_u64(c->d_blk_stats, 2 * nb, 0ull, c->stream);
+    c->blk_stats_n = nb;
+  }
+  return CLB_OK;
+}
+
 // One coupled step on the stream, no host synchronisation (so it can be captured).
 template <typename R>
 void enqueue_step(clb_ctx* c) {
   const bool odd = (c->ade_t & 1) != 0;
   AdeArgs<R> a = ade_args<R>(c);
+  if (fused_spec(c)) {
+    launch_fill_u32(c->d_counter + 2, 1, 0u, c->stream);
+    const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats};
+    {
+      Timed tm(c, 1);
+      launch_ade_spec_step<R>(a, odd, c->ade_block, c->tb, io, c->stream);
+    }
+    count_launch(c, 1);
+    // voxels whose fused solve did not converge: the persistent kernel, same step
+    launch_fill_u32(c->d_counter, 1, 0u, c->stream);
+    SpecArgs sa{};
+    sa.n = 0;
+    sa.n_dev = c->d_counter + 2;
+    sa.iters = c->d_slist;
+    sa.counter = c->d_counter;
+    sa.stats = c->d_stats;
+    {
+      Timed tm(c, 2);
+      launch_spec_coupled(a, odd, c->tb, sa, c->spec_grid, c->stream);
+    }
+    count_launch(c, 2);
+    c->ade_t++;
+    return;
+  }
   {
     Timed tm(c, 1);
     launch_ade_step<R>(a, odd, c->ade_block, c->stream);
@@ -349,11 +427,18 @@ void drop_graph(clb_ctx* c) {
 
 template <typename R>
 int step_t(clb_ctx* c, int nsteps) {
-  if (c->have_tableau && c->list_dirty) {
+  const bool fused = fused_spec(c);
+  if (fused) {
+    int rc = prepare_fused(c);
+    if (rc) return rc;
+  } else if (c->have_tableau && c->list_dirty) {
+    // the separate speciation kernel takes the reactive list and its length from the
+    // host, so a captured graph holds them: rebuild both
     int rc = build_reactive_list(c);
     if (rc) return rc;
     drop_graph(c);
   }
+  const bool spec_launch = c->have_tableau && (fused || c->n_reactive > 0);
   const int seg = 2 * c->graph_pairs;
   if (c->use_graph && !c->timing && nsteps >= seg) {
     // Replay a captured graph of `seg` steps (it starts on an even step, as the AA
@@ -373,13 +458,13 @@ int step_t(clb_ctx* c, int nsteps) {
       c->graph_valid = true;
       c->ade_t = t0;  // capture recorded the steps; they run at replay
       c->perf.ade_launches -= seg;
-      if (c->have_tableau && c->n_reactive > 0) c->perf.spec_launches -= seg;
+      if (spec_launch) c->perf.spec_launches -= seg;
     }
     while (nsteps >= seg) {
       CLB_CHECK_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
       c->ade_t += seg;
       c->perf.ade_launches += seg;
-      if (c->have_tableau && c->n_reactive > 0) c->perf.spec_launches += seg;
+      if (spec_launch) c->perf.spec_launches += seg;
       nsteps -= seg;
     }
   }
@@ -567,7 +652,7 @@ int clb_destroy(clb_ctx* c) {
   void* ptrs[] = {c->d_mask, c->d_kind, c->d_flow_omega, c->d_f, c->d_energy, c->d_ade_omega, c->d_cref, c->d_g,
                   c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
-                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats};
+                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -899,7 +984,26 @@ int clb_speciation_stats(clb_ctx* c, long long* stats) {
   unsigned long long h[3];
   CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   CLB_CHECK_HIP(c, hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+  // solves finished inside the transport kernel (all of them converged)
+  if (c->d_blk_stats && c->blk_stats_n > 0) {
+    std::vector<unsigned long long> b((size_t)(2 * c->blk_stats_n));
+    CLB_CHECK_HIP(c, hipMemcpy(b.data(), c->d_blk_stats, sizeof(unsigned long long) * b.size(),
+                               hipMemcpyDeviceToHost));
+    for (long long k = 0; k < c->blk_stats_n; ++k) {
+      h[0] += b[2 * k];
+      h[1] += b[2 * k];
+      h[2] += b[2 * k + 1];
+    }
+  }
   for (int i = 0; i < 3; ++i) stats[i] = (long long)h[i];
+  return CLB_OK;
+}
+
+int clb_set_speciation_mode(clb_ctx* c, int mode) {
+  if (!c || (mode != CLB_SPEC_FUSED && mode != CLB_SPEC_SEPARATE)) return CLB_ERR_ARG;
+  if (mode != c->spec_mode) drop_graph(c);
+  c->spec_mode = mode;
+  c->list_dirty = true;
   return CLB_OK;
 }
 
@@ -1220,13 +1324,10 @@ int clb_node_conversion(clb_ctx* c, int mineral_solute, int pore_code, int preci
   CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   if (changed) *changed = h;
   if (h > 0) {
+    // the captured step reads the mask on the device; only the separate speciation
+    // path's host-side reactive list has to be rebuilt (step_t does it)
     c->list_dirty = true;
-    drop_graph(c);
-    std::vector<uint8_t> m((size_t)c->geo.N);
-    CLB_CHECK_HIP(c, hipMemcpy(m.data(), c->d_mask, (size_t)c->geo.N, hipMemcpyDeviceToHost));
-    long long fluid = 0;
-    for (long long n = 0; n < c->geo.N; ++n) fluid += (c->h_kind[m[n]] == CLB_KIND_FLUID);
-    c->perf.fluid_nodes = fluid;
+    c->fluid_dirty = true;
   }
   return CLB_OK;
 }
@@ -1288,6 +1389,10 @@ int clb_get_perf(clb_ctx* c, clb_perf* p) {
   if (!c || !p) return CLB_ERR_ARG;
   int rc = fold_timing(c);
   if (rc) return rc;
+  if (c->fluid_dirty) {
+    rc = count_fluid(c);
+    if (rc) return rc;
+  }
   *p = c->perf;
   return CLB_OK;
 }
@@ -1300,6 +1405,7 @@ int clb_reset_perf(clb_ctx* c) {
   c->perf.fluid_nodes = f;
   c->perf.reactive_nodes = r;
   launch_fill_u64(c->d_stats, 4, 0ull, c->stream);
+  if (c->d_blk_stats) launch_fill_u64(c->d_blk_stats, 2 * c->blk_stats_n, 0ull, c->stream);
   return CLB_OK;
 }
 

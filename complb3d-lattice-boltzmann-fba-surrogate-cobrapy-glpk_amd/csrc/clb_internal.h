@@ -88,6 +88,14 @@ struct SpecArgs {
   uint8_t* conv;             // n
   unsigned int* counter;     // work-queue head
   unsigned long long* stats; // solves, converged, iterations
+  const unsigned int* n_dev; // item count in device memory (overrides n when set)
+};
+
+// Transport step with the speciation fused in (clb_ade_spec.h).
+struct FusedSpecIO {
+  int* slist;                     // voxels left to the persistent kernel (capacity N)
+  unsigned int* scount;           // their number (cleared before the step)
+  unsigned long long* blk_stats;  // per block of the launch: solves, evaluations
 };
 
 // ------------------------------------------------------------------ launchers
@@ -119,6 +127,13 @@ void launch_spec_coupled(const AdeArgs<float>& a, bool odd_done, const Tableau& 
                          hipStream_t s);
 void launch_spec_coupled(const AdeArgs<double>& a, bool odd_done, const Tableau& tb, SpecArgs sa, int grid,
                          hipStream_t s);
+
+// Returns false when no fused shape is compiled for (S, nc): the caller then uses the
+// transport kernel + spec_kernel over the reactive list.
+template <typename R>
+bool launch_ade_spec_step(const AdeArgs<R>& a, bool odd, dim3 block, const Tableau& tb, const FusedSpecIO& io,
+                          hipStream_t s);
+long long fused_blocks(const Geom& g, dim3 block);
 
 void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, hipStream_t s);
 int spec_occupancy_grid(int nc, int device);

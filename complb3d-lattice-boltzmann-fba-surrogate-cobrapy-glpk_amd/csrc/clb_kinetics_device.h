@@ -14,10 +14,13 @@
 
 #include <hip/hip_runtime.h>
 
+// every standard header a plugin may include, so that its own #include lines are no-ops
+// under the std remap below
 #include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <initializer_list>
+#include <iostream>
 #include <vector>
 
 #include "../../include/clb.h"
@@ -85,12 +88,15 @@ __host__ __device__ inline bool isfinite(double x) { return __builtin_isfinite(x
 #error "CLB_ABIOTIC_HEADER must name the defineAbioticKinetics.hh plugin to compile"
 #endif
 
+// Internal linkage: several translation units compile the hooks into their kernels.
+namespace {
 #define std clb_kstd
 #pragma clang force_cuda_host_device begin
 #include CLB_KINETICS_HEADER
 #include CLB_ABIOTIC_HEADER
 #pragma clang force_cuda_host_device end
 #undef std
+}  // namespace
 
 namespace clb {
 using KVec = clb_kstd::vector<double>;

@@ -32,6 +32,7 @@
 // serves every nc <= NC.
 #pragma once
 
+#include "clb_fastmath.h"
 #include "clb_lattice.h"
 
 namespace clb {
@@ -83,7 +84,7 @@ __device__ __forceinline__ void evaluate(const Tableau& tb, const int* vsp, int 
       double lc = lk;
 #pragma unroll
       for (int j = 0; j < NC; ++j) lc += tb.S[i * CLB_MAX_COMPONENTS + j] * x[j];
-      const double cr = exp10(fmin(lc, 250.0));
+      const double cr = fast_exp10(fmin(lc, 250.0));
       const double cc = (lc > kMaxLog) ? kMaxConc : ((lc < kMinLog) ? kMinConc : cr);
       craw[i] = cr;
       gsum += cr;
@@ -108,7 +109,7 @@ __device__ __forceinline__ void evaluate(const Tableau& tb, const int* vsp, int 
     double SR, SP;
     if (T[j] >= 0.0) { SR = spr[j]; SP = T[j] + snr[j]; }
     else { SR = -T[j] + spr[j]; SP = snr[j]; }
-    double fj = tb.inv_mu0[j] * (log10(fmax(SP, kMinConc)) - log10(fmax(SR, kMinConc)));
+    double fj = tb.inv_mu0[j] * fast_log10(fmax(SP, kMinConc) / fmax(SR, kMinConc));
     fj = clampd(fj, -10.0, 10.0);
     if (!isfinite(fj) || !live) fj = 0.0;
     judge2 += fj * fj;
@@ -118,7 +119,7 @@ __device__ __forceinline__ void evaluate(const Tableau& tb, const int* vsp, int 
     else { SRs = -T[j] + sps[j]; SPs = sns[j]; }
     SRs = fmax(SRs, 1e-300);
     SPs = fmax(SPs, 1e-300);
-    const double fsj = live ? tb.inv_mu0[j] * (log10(SPs) - log10(SRs)) : 0.0;
+    const double fsj = live ? tb.inv_mu0[j] * (fast_log10(SPs) - fast_log10(SRs)) : 0.0;
     fs[j] = fsj;
     fsn2 += fsj * fsj;
     rP[j] = live ? tb.inv_mu0[j] / SPs : 0.0;
@@ -235,6 +236,179 @@ __device__ __forceinline__ double quad_form(const double* A, const double* d) {
   return q;
 }
 
+// ---- near-equilibrium solve inside the transport kernel (fused coupled step)
+//
+// In the coupled step a voxel's start is the previous step's equilibrium moved by one
+// transport + kinetics update, so Newton converges in a few evaluations. This variant is
+// Newton on the convex potential G (Hessian ln10 S^T diag(c) S, gradient S^T c - T) with
+// backtracking: a trial point is taken when G satisfies the Armijo condition or when the
+// reference judge decreases (near the solution G changes below its rounding while the
+// judge still converges quadratically). At most kFusedEvals evaluations; a voxel that has
+// not met the judge by then is handed to the persistent kernel above.
+constexpr int kFusedEvals = 6;
+
+// Species concentration of calc_species (:442-496) at log activities x.
+template <int NC>
+__device__ __forceinline__ double species_conc(const Tableau& tb, int i, const double* x) {
+  double lc = tb.logK[i];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) lc += tb.S[i * CLB_MAX_COMPONENTS + j] * x[j];
+  lc = clampd(lc, kMinLog, kMaxLog);
+  return clampd(fast_exp10(lc), kMinConc, kMaxConc);
+}
+
+// Lean evaluation for the fused solve: G(x) = sum_i c_i / ln10 - T.x, the reference judge
+// |f|^2 (clamped species, floors, +-10 clamp, zero-total guard, as in evaluate above), the
+// packed Hessian ln10 S^T diag(c) S and rhs = -(S^T c - T); a component under the
+// zero-total guard gets an identity row and a zero rhs, so it keeps its start.
+template <int NC, int NSM, bool JUDGE>
+__device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const double* T, double& G,
+                                       double& judge2, double* A, double* rhs) {
+  constexpr int NP = NC * (NC + 1) / 2;
+  double pos[NC], neg[NC], raw[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) { pos[j] = 0.0; neg[j] = 0.0; raw[j] = 0.0; }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) A[q] = 0.0;
+  double gsum = 0.0;
+#pragma unroll
+  for (int i = 0; i < NSM; ++i) {
+    if (i >= tb.ns || !tb.is_eq[i]) continue;
+    double lc = tb.logK[i];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) lc += tb.S[i * CLB_MAX_COMPONENTS + j] * x[j];
+    const double cr = fast_exp10(fmin(lc, 250.0));
+    const double cc = (lc > kMaxLog) ? kMaxConc : ((lc < kMinLog) ? kMinConc : cr);
+    gsum += cr;
+    const double lcr = kLn10 * cr;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      const double s = tb.S[i * CLB_MAX_COMPONENTS + j];
+      if (JUDGE) {
+        if (s > 1e-15) pos[j] += s * cc;
+        else if (s < -1e-15) neg[j] -= s * cc;
+      }
+      raw[j] += s * cr;
+      const double w = lcr * s;
+#pragma unroll
+      for (int k = j; k < NC; ++k) A[up<NC>(j, k)] += w * tb.S[i * CLB_MAX_COMPONENTS + k];
+    }
+  }
+  double jg = 0.0, gtx = 0.0;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const bool live = T[j] > 1e-18;
+    if (JUDGE) {
+      const double SR = (T[j] >= 0.0) ? pos[j] : -T[j] + pos[j];
+      const double SP = (T[j] >= 0.0) ? T[j] + neg[j] : neg[j];
+      double fj = tb.inv_mu0[j] * fast_log10(fmax(SP, kMinConc) / fmax(SR, kMinConc));
+      fj = clampd(fj, -10.0, 10.0);
+      if (!isfinite(fj) || !live) fj = 0.0;
+      jg += fj * fj;
+    }
+    gtx += T[j] * x[j];
+    rhs[j] = live ? -(raw[j] - T[j]) : 0.0;
+#pragma unroll
+    for (int k = j; k < NC; ++k)
+      if (!(live && T[k] > 1e-18)) A[up<NC>(j, k)] = (j == k) ? 1.0 : 0.0;
+  }
+  judge2 = jg;
+  G = gsum / kLn10 - gtx;
+}
+
+// c: concentrations of the NSM species (tb.ns of them used), already floored at 1e-30 as
+// run_equilibrium_biotic reads them (:1373-1400). Returns true when the judge is met; x
+// then holds the log activities and evals the number of evaluations.
+template <int NC, int NSM>
+__device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, double* x, int& evals) {
+  constexpr int NP = NC * (NC + 1) / 2;
+  double T[NC], d[NC];
+  // calc_component_totals (:520-538) and the component-slot guess (:977-994)
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NSM; ++i)
+      if (i < tb.ns && tb.is_eq[i]) s += tb.S[i * CLB_MAX_COMPONENTS + j] * clampd(c[i], kMinConc, kMaxConc);
+    T[j] = (j < tb.nc) ? fmax(s, kMinConc) : 0.0;
+    double cc = 1e-7;
+    const int cs = (j < tb.nc) ? tb.comp_species[j] : -1;
+#pragma unroll
+    for (int i = 0; i < NSM; ++i)
+      if (i == cs) cc = c[i];
+    x[j] = (j < tb.nc) ? fast_log10(clampd(cc, kMinConc, kMaxConc)) : 0.0;
+    d[j] = 0.0;
+  }
+  const double tol2 = tb.tol * tb.tol;
+  // start: gradient and Hessian only. The solve always takes at least one step, and a
+  // start that already met the judge stays within it after that (tiny) step, so the
+  // judge is first evaluated at the first trial point.
+  double G = 0.0, judge2 = 0.0, slope = 0.0, alpha = 1.0;
+  {
+    double A[NP], rhs[NC], jdummy;
+    eval_g<NC, NSM, false>(tb, x, T, G, jdummy, A, rhs);
+    evals = 1;
+    if (!chol_solve<NC>(A, 0.0, rhs, d)) return false;
+    double m = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) m = fmax(m, fabs(d[j]));
+    if (m > kMaxStep) {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) d[j] *= kMaxStep / m;
+    }
+#pragma unroll
+    for (int j = 0; j < NC; ++j) slope -= rhs[j] * d[j];
+  }
+  // a trial point is taken on the Armijo condition for G, or when the judge improves on
+  // the current point's (the first trial: when it already meets the tolerance)
+  bool have_judge = false, conv = false;
+#pragma unroll 1
+  for (int it = 1; it < kFusedEvals; ++it) {
+    double p[NC], A[NP], rhs[NC], Gp, jp;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) p[j] = clampd(x[j] + alpha * d[j], kMinLog, kMaxLog);
+    eval_g<NC, NSM, true>(tb, p, T, Gp, jp, A, rhs);
+    ++evals;
+    if (!(Gp <= G + 1e-4 * alpha * slope || jp < (have_judge ? judge2 : tol2))) {
+      alpha *= 0.5;
+      continue;
+    }
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x[j] = p[j];
+    G = Gp;
+    judge2 = jp;
+    have_judge = true;
+    if (judge2 < tol2) { conv = true; break; }
+    if (!chol_solve<NC>(A, 0.0, rhs, d)) break;
+    double m = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) m = fmax(m, fabs(d[j]));
+    if (m > kMaxStep) {
+#pragma unroll
+      for (int j = 0; j < NC; ++j) d[j] *= kMaxStep / m;
+    }
+    slope = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) slope -= rhs[j] * d[j];
+    alpha = 1.0;
+  }
+  return conv;
+}
+
+// Limited increment of species i after a converged solve at x (calc_species, NaN guard
+// and clamp of :1187-1214, then the 10 % limiter and 1e-20 floor of :1434-1447).
+template <int NC>
+__device__ __forceinline__ double spec_increment(const Tableau& tb, int i, const double* x, double c) {
+  double eq = tb.is_eq[i] ? species_conc<NC>(tb, i, x) : ((c > kMinConc) ? fmin(c, kMaxConc) : kMinConc);
+  if (!isfinite(eq)) eq = (c > kMinConc) ? fmin(c, kMaxConc) : kMinConc;
+  eq = clampd(eq, kMinConc, kMaxConc);
+  double dd = eq - c;
+  const double mx = fmax(1e-9, 0.1 * fabs(c));
+  if (fabs(dd) > mx) dd = (dd > 0.0) ? mx : -mx;
+  if (c + dd < 1e-20) dd = 1e-20 - c;
+  return dd;
+}
+
 struct CoupledIO {
   void* g;              // populations (R*)
   const uint8_t* mask;
@@ -277,7 +451,7 @@ template <typename R, int NC, int NSM, int MODE>
 __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int warm, CoupledIO io) {
   constexpr int NP = NC * (NC + 1) / 2;
   const int lane = threadIdx.x & 63;
-  const long long n_items = sa.n;
+  const long long n_items = sa.n_dev ? (long long)*sa.n_dev : sa.n;
   int vsp[4];
 #pragma unroll
   for (int v = 0; v < 4; ++v) vsp[v] = sa.var_species[v];
@@ -334,7 +508,7 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
             for (int j = 0; j < NC; ++j) {
               if (j >= tb.nc) x[j] = 0.0;
               else if (user) x[j] = clampd(sa.logc[v * tb.nc + j], kMinLog, kMaxLog);
-              else x[j] = log10(clampd(T[j], kMinConc, kMaxConc));
+              else x[j] = fast_log10(clampd(T[j], kMinConc, kMaxConc));
             }
           } else {
             node = io.list[v];
@@ -358,7 +532,7 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
 #pragma unroll
               for (int i = 0; i < NSM; ++i)
                 if (i == cs) cc = conc[i];
-              x[j] = (j < tb.nc) ? log10(clampd(cc, kMinConc, kMaxConc)) : 0.0;
+              x[j] = (j < tb.nc) ? fast_log10(clampd(cc, kMinConc, kMaxConc)) : 0.0;
             }
           }
         } else {

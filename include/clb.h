@@ -46,6 +46,9 @@ extern "C" {
 #define CLB_WARM_USER 1      /* caller-provided log activities */
 #define CLB_WARM_MLP 2       /* MLP prediction (BF16 MFMA), reference rule outside the training box */
 
+#define CLB_SPEC_FUSED 0     /* speciation inside the transport kernel (clb_set_speciation_mode) */
+#define CLB_SPEC_SEPARATE 1  /* speciation in its own kernel after transport */
+
 #define CLB_MAX_SOLUTES 8
 #define CLB_MAX_BIOMASS 4
 #define CLB_MAX_COMPONENTS 8
@@ -118,6 +121,13 @@ int clb_set_tableau(clb_ctx* ctx, int nc, int ns, const double* S, const double*
 int clb_set_mask_update(clb_ctx* ctx, int enable, double threshold, int pore_code, int biofilm_code);
 int clb_step(clb_ctx* ctx, int nsteps);                       /* collide -> kinetics -> speciation -> stream */
 int clb_speciation_stats(clb_ctx* ctx, long long* stats /*3: solves, converged, iterations*/);
+/* Where the coupled step solves the speciation (no reference counterpart: an execution
+ * choice). CLB_SPEC_FUSED (default): inside the transport kernel, from the concentrations
+ * it already holds, with the persistent kernel finishing the few voxels that need more
+ * iterations; used when the solutes are the tableau's species and nc <= 4, otherwise the
+ * separate path runs. CLB_SPEC_SEPARATE: transport kernel, then the persistent kernel over
+ * every reactive voxel. Both meet the same judge. */
+int clb_set_speciation_mode(clb_ctx* ctx, int mode);
 
 /* Immobile solutes (vec_immobile, src/complab.cpp:248-266): neither collided nor streamed,
  * changed only by reactions. immobile: S flags; set before clb_solutes_init. */

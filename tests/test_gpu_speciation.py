@@ -99,8 +99,9 @@ def coupled_setup(clb, oracle, c_in, c_0):
     return nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val
 
 
-def coupled_gpu(clb, prec, nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val, c_0, steps):
+def coupled_gpu(clb, prec, nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val, c_0, steps, mode=0):
     lat = clb.Lattice(nx, ny, nz, precision=prec)
+    lat.set_speciation_mode(mode)
     lat.set_dynamics(kind, om)
     lat.upload_mask(pack)
     lat.set_solutes(omg, bc_type, bc_val)
@@ -115,14 +116,15 @@ def coupled_gpu(clb, prec, nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val, 
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("mode", [0, 1], ids=["fused", "separate"])
 @pytest.mark.parametrize("prec,tol", [(0, 2e-6), (1, 1e-5)])
-def test_coupled_step_with_speciation(clb, oracle, prec, tol):
+def test_coupled_step_with_speciation(clb, oracle, prec, tol, mode):
     """Carbonate species as four solutes; a more concentrated, more acidic water enters from
     the west into resident water, both at equilibrium, so the reference's PCF + Anderson
     converges in every voxel solve and the two answers must agree. Both solvers stop at
     |f| < 1e-10; [H+] (~5e-8) is a small difference of totals ~1e-4, so that stopping
     tolerance allows ~1e-10 * 1e-4 / 5e-8 ~ 2e-7 relative on it: the FP64 bound is 2e-6."""
-    print(f"case: coupled transport + speciation prec={prec}", flush=True)
+    print(f"case: coupled transport + speciation prec={prec} mode={mode}", flush=True)
     c_in, c_0 = carbonate_exact(2e-3, 4e-4), carbonate_exact(1e-3, 1e-4)
     nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val = coupled_setup(clb, oracle, c_in, c_0)
     steps = 60
@@ -131,25 +133,26 @@ def test_coupled_step_with_speciation(clb, oracle, prec, tol):
     run.run(steps)
     ref = run.conc()
     assert run.eq_stats[0] == run.eq_stats[1] > 0          # the reference converged everywhere
-    c, stats = coupled_gpu(clb, prec, nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val, c_0, steps)
+    c, stats = coupled_gpu(clb, prec, nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val, c_0, steps, mode)
     assert stats[0] == stats[1] == run.eq_stats[0]
     for s in range(4):
         assert rel_l2(c[s], ref[s]) < tol
 
 
 @pytest.mark.gpu
-def test_coupled_far_from_equilibrium_converges(clb, oracle):
+@pytest.mark.parametrize("mode", [0, 1], ids=["fused", "separate"])
+def test_coupled_far_from_equilibrium_converges(clb, oracle, mode):
     """Resident and inlet water far from equilibrium: the reference's PCF + Anderson leaves
     a share of voxel solves unconverged after 200 iterations (and applies the last
     iterate); every GPU solve meets the 1e-10 judge."""
-    print("case: coupled far from equilibrium", flush=True)
+    print(f"case: coupled far from equilibrium mode={mode}", flush=True)
     c_in, c_0 = [2e-3, 1e-4, 1e-9, 1e-3], [1e-3, 1e-7, 1e-5, 1e-5]
     nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val = coupled_setup(clb, oracle, c_in, c_0)
     run = oracle.AdeRun(nx, ny, nz, pack, kind, omg, bc_type, bc_val, u, c_0, tableau=(CARB_S, CARB_LOGK, CARB_COMP),
                         eq_tol=TOL, eq_max_iter=200)
     run.run(20)
     assert run.eq_stats[1] < run.eq_stats[0]
-    c, stats = coupled_gpu(clb, 0, nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val, c_0, 20)
+    c, stats = coupled_gpu(clb, 0, nx, ny, nz, pack, kind, om, u, omg, bc_type, bc_val, c_0, 20, mode)
     assert stats[0] == stats[1] == run.eq_stats[0]
     # every fluid voxel keeps positive concentrations (wall voxels report 0)
     assert np.all(np.isfinite(c)) and c[:, pack == 2].min() > 0
