@@ -933,10 +933,12 @@ int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* l
     tb.logK[i] = logK[i];
     int eq = 0;
     for (int j = 0; j < nc; ++j) {
-      tb.S[i * CLB_MAX_COMPONENTS + j] = S[i * nc + j];
+      // coefficients below 1e-15 in magnitude count as zero (the kernels' sign tests)
+      tb.S[i * CLB_MAX_COMPONENTS + j] = std::fabs(S[i * nc + j]) > 1e-15 ? S[i * nc + j] : 0.0;
       if (std::fabs(S[i * nc + j]) > 1e-10) eq = 1;
     }
     tb.is_eq[i] = eq;
+    tb.eqw[i] = eq ? 1.0 : 0.0;
   }
   for (int j = 0; j < CLB_MAX_COMPONENTS; ++j) {
     double mu0 = 1.0;

@@ -70,6 +70,7 @@ struct Tableau {
   double logK[CLB_MAX_SPECIES];
   double inv_mu0[CLB_MAX_COMPONENTS];  // 1 / (smallest positive coefficient <= 1)
   int is_eq[CLB_MAX_SPECIES];          // row has a nonzero entry
+  double eqw[CLB_MAX_SPECIES];         // 1.0 for an equilibrium species (i < ns), else 0.0
   int comp_species[CLB_MAX_COMPONENTS];
   double tol;
   int max_iter;

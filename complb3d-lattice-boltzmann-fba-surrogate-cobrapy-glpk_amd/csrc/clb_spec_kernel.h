@@ -109,7 +109,7 @@ __device__ __forceinline__ void evaluate(const Tableau& tb, const int* vsp, int 
     double SR, SP;
     if (T[j] >= 0.0) { SR = spr[j]; SP = T[j] + snr[j]; }
     else { SR = -T[j] + spr[j]; SP = snr[j]; }
-    double fj = tb.inv_mu0[j] * fast_log10(fmax(SP, kMinConc) / fmax(SR, kMinConc));
+    double fj = tb.inv_mu0[j] * fast_log10(fmax(SP, kMinConc) * fast_rcp(fmax(SR, kMinConc)));
     fj = clampd(fj, -10.0, 10.0);
     if (!isfinite(fj) || !live) fj = 0.0;
     judge2 += fj * fj;
@@ -186,10 +186,12 @@ __device__ __forceinline__ void evaluate(const Tableau& tb, const int* vsp, int 
 }
 
 // Solve (A + lam I) d = b by Cholesky (A symmetric, packed upper). Returns false when a
-// pivot is not positive.
+// pivot is not positive. The factor is kept with reciprocal pivots, so the solve costs one
+// reciprocal square root per column and no divisions.
 template <int NC>
 __device__ __forceinline__ bool chol_solve(const double* A, double lam, const double* b, double* d) {
   double L[NC * (NC + 1) / 2];  // packed lower, row-major: L(j,k) k <= j at j(j+1)/2 + k
+  double inv[NC];               // 1 / L(j,j)
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
 #pragma unroll
@@ -198,10 +200,11 @@ __device__ __forceinline__ bool chol_solve(const double* A, double lam, const do
 #pragma unroll
       for (int m = 0; m < k; ++m) s -= L[j * (j + 1) / 2 + m] * L[k * (k + 1) / 2 + m];
       if (j == k) {
-        if (!(s > 0.0)) return false;
-        L[j * (j + 1) / 2 + j] = sqrt(s);
+        if (!(s > 0.0) || !isfinite(s)) return false;
+        inv[j] = fast_rsqrt(s);
+        L[j * (j + 1) / 2 + j] = s * inv[j];
       } else {
-        L[j * (j + 1) / 2 + k] = s / L[k * (k + 1) / 2 + k];
+        L[j * (j + 1) / 2 + k] = s * inv[k];
       }
     }
   }
@@ -211,14 +214,14 @@ __device__ __forceinline__ bool chol_solve(const double* A, double lam, const do
     double s = b[j];
 #pragma unroll
     for (int m = 0; m < j; ++m) s -= L[j * (j + 1) / 2 + m] * y[m];
-    y[j] = s / L[j * (j + 1) / 2 + j];
+    y[j] = s * inv[j];
   }
 #pragma unroll
   for (int j = NC - 1; j >= 0; --j) {
     double s = y[j];
 #pragma unroll
     for (int m = j + 1; m < NC; ++m) s -= L[m * (m + 1) / 2 + j] * d[m];
-    d[j] = s / L[j * (j + 1) / 2 + j];
+    d[j] = s * inv[j];
     if (!isfinite(d[j])) return false;
   }
   return true;
@@ -246,6 +249,7 @@ __device__ __forceinline__ double quad_form(const double* A, const double* d) {
 // judge still converges quadratically). At most kFusedEvals evaluations; a voxel that has
 // not met the judge by then is handed to the persistent kernel above.
 constexpr int kFusedEvals = 6;
+constexpr double kJudgeStep = 3e-5;
 
 // Species concentration of calc_species (:442-496) at log activities x.
 template <int NC>
@@ -261,37 +265,42 @@ __device__ __forceinline__ double species_conc(const Tableau& tb, int i, const d
 // |f|^2 (clamped species, floors, +-10 clamp, zero-total guard, as in evaluate above), the
 // packed Hessian ln10 S^T diag(c) S and rhs = -(S^T c - T); a component under the
 // zero-total guard gets an identity row and a zero rhs, so it keeps its start.
-template <int NC, int NSM, bool JUDGE>
-__device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const double* T, double& G,
+template <int NC, int NSM>
+__device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const double* T, bool JUDGE, double& G,
                                        double& judge2, double* A, double* rhs) {
   constexpr int NP = NC * (NC + 1) / 2;
+  // Branch-free: the sign split of each coefficient is fmax(+-s, 0) and a species outside
+  // the equilibrium set has weight 0 (its row is zero as well), so every species and
+  // component runs the same multiply-adds (uniform branches on tableau entries compile to
+  // exec-mask juggling that costs more than the arithmetic).
   double pos[NC], neg[NC], raw[NC];
 #pragma unroll
   for (int j = 0; j < NC; ++j) { pos[j] = 0.0; neg[j] = 0.0; raw[j] = 0.0; }
 #pragma unroll
   for (int q = 0; q < NP; ++q) A[q] = 0.0;
   double gsum = 0.0;
-#pragma unroll
+  // one species at a time: the index is uniform, so the tableau entries come from scalar
+  // loads and only one species' temporaries are live (the solve shares the kernel's
+  // register budget with the transport phase)
+#pragma unroll 1
   for (int i = 0; i < NSM; ++i) {
-    if (i >= tb.ns || !tb.is_eq[i]) continue;
+    const double eqw = tb.eqw[i];
     double lc = tb.logK[i];
 #pragma unroll
-    for (int j = 0; j < NC; ++j) lc += tb.S[i * CLB_MAX_COMPONENTS + j] * x[j];
-    const double cr = fast_exp10(fmin(lc, 250.0));
-    const double cc = (lc > kMaxLog) ? kMaxConc : ((lc < kMinLog) ? kMinConc : cr);
+    for (int j = 0; j < NC; ++j) lc = fma(tb.S[i * CLB_MAX_COMPONENTS + j], x[j], lc);
+    const double cr = eqw * fast_exp10(fmin(lc, 250.0));
+    const double cc = eqw * ((lc > kMaxLog) ? kMaxConc : ((lc < kMinLog) ? kMinConc : cr));
     gsum += cr;
     const double lcr = kLn10 * cr;
 #pragma unroll
     for (int j = 0; j < NC; ++j) {
       const double s = tb.S[i * CLB_MAX_COMPONENTS + j];
-      if (JUDGE) {
-        if (s > 1e-15) pos[j] += s * cc;
-        else if (s < -1e-15) neg[j] -= s * cc;
-      }
-      raw[j] += s * cr;
+      pos[j] = fma(fmax(s, 0.0), cc, pos[j]);
+      neg[j] = fma(fmax(-s, 0.0), cc, neg[j]);
+      raw[j] = fma(s, cr, raw[j]);
       const double w = lcr * s;
 #pragma unroll
-      for (int k = j; k < NC; ++k) A[up<NC>(j, k)] += w * tb.S[i * CLB_MAX_COMPONENTS + k];
+      for (int k = j; k < NC; ++k) A[up<NC>(j, k)] = fma(w, tb.S[i * CLB_MAX_COMPONENTS + k], A[up<NC>(j, k)]);
     }
   }
   double jg = 0.0, gtx = 0.0;
@@ -301,12 +310,12 @@ __device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const
     if (JUDGE) {
       const double SR = (T[j] >= 0.0) ? pos[j] : -T[j] + pos[j];
       const double SP = (T[j] >= 0.0) ? T[j] + neg[j] : neg[j];
-      double fj = tb.inv_mu0[j] * fast_log10(fmax(SP, kMinConc) / fmax(SR, kMinConc));
+      double fj = tb.inv_mu0[j] * fast_log10(fmax(SP, kMinConc) * fast_rcp(fmax(SR, kMinConc)));
       fj = clampd(fj, -10.0, 10.0);
       if (!isfinite(fj) || !live) fj = 0.0;
       jg += fj * fj;
     }
-    gtx += T[j] * x[j];
+    gtx = fma(T[j], x[j], gtx);
     rhs[j] = live ? -(raw[j] - T[j]) : 0.0;
 #pragma unroll
     for (int k = j; k < NC; ++k)
@@ -329,7 +338,7 @@ __device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, d
     double s = 0.0;
 #pragma unroll
     for (int i = 0; i < NSM; ++i)
-      if (i < tb.ns && tb.is_eq[i]) s += tb.S[i * CLB_MAX_COMPONENTS + j] * clampd(c[i], kMinConc, kMaxConc);
+      s = fma(tb.eqw[i] * tb.S[i * CLB_MAX_COMPONENTS + j], clampd(c[i], kMinConc, kMaxConc), s);
     T[j] = (j < tb.nc) ? fmax(s, kMinConc) : 0.0;
     double cc = 1e-7;
     const int cs = (j < tb.nc) ? tb.comp_species[j] : -1;
@@ -346,7 +355,7 @@ __device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, d
   double G = 0.0, judge2 = 0.0, slope = 0.0, alpha = 1.0;
   {
     double A[NP], rhs[NC], jdummy;
-    eval_g<NC, NSM, false>(tb, x, T, G, jdummy, A, rhs);
+    eval_g<NC, NSM>(tb, x, T, false, G, jdummy, A, rhs);
     evals = 1;
     if (!chol_solve<NC>(A, 0.0, rhs, d)) return false;
     double m = 0.0;
@@ -359,17 +368,26 @@ __device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, d
 #pragma unroll
     for (int j = 0; j < NC; ++j) slope -= rhs[j] * d[j];
   }
-  // a trial point is taken on the Armijo condition for G, or when the judge improves on
-  // the current point's (the first trial: when it already meets the tolerance)
+  // A trial point is taken on the Armijo condition for G, or when the judge improves on the
+  // current point's (the first judged point: when it meets the tolerance). The judge is only
+  // evaluated at points reached by a short step (|d| < kJudgeStep): after a longer Newton
+  // step the residual is still of order |d|^2 and cannot meet 1e-10, so its logs are skipped
+  // and the next step is taken; a voxel is declared converged only where the judge was
+  // evaluated and met.
   bool have_judge = false, conv = false;
+  double dmax = 0.0;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) dmax = fmax(dmax, fabs(d[j]));
 #pragma unroll 1
   for (int it = 1; it < kFusedEvals; ++it) {
-    double p[NC], A[NP], rhs[NC], Gp, jp;
+    double p[NC], A[NP], rhs[NC], Gp, jp = 0.0;
 #pragma unroll
     for (int j = 0; j < NC; ++j) p[j] = clampd(x[j] + alpha * d[j], kMinLog, kMaxLog);
-    eval_g<NC, NSM, true>(tb, p, T, Gp, jp, A, rhs);
+    const bool judged = alpha * dmax < kJudgeStep;
+    eval_g<NC, NSM>(tb, p, T, judged, Gp, jp, A, rhs);
     ++evals;
-    if (!(Gp <= G + 1e-4 * alpha * slope || jp < (have_judge ? judge2 : tol2))) {
+    const bool armijo = Gp <= G + 1e-4 * alpha * slope;
+    if (!(armijo || (judged && jp < (have_judge ? judge2 : tol2)))) {
       alpha *= 0.5;
       continue;
     }
@@ -377,15 +395,16 @@ __device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, d
     for (int j = 0; j < NC; ++j) x[j] = p[j];
     G = Gp;
     judge2 = jp;
-    have_judge = true;
-    if (judge2 < tol2) { conv = true; break; }
+    have_judge = judged;
+    if (judged && judge2 < tol2) { conv = true; break; }
     if (!chol_solve<NC>(A, 0.0, rhs, d)) break;
-    double m = 0.0;
+    dmax = 0.0;
 #pragma unroll
-    for (int j = 0; j < NC; ++j) m = fmax(m, fabs(d[j]));
-    if (m > kMaxStep) {
+    for (int j = 0; j < NC; ++j) dmax = fmax(dmax, fabs(d[j]));
+    if (dmax > kMaxStep) {
 #pragma unroll
-      for (int j = 0; j < NC; ++j) d[j] *= kMaxStep / m;
+      for (int j = 0; j < NC; ++j) d[j] *= kMaxStep / dmax;
+      dmax = kMaxStep;
     }
     slope = 0.0;
 #pragma unroll
