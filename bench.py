@@ -1,17 +1,20 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X path for CompLB3D's per-timestep hot path.
 
-Headline ("value"): coupled MLUPS of the kinetics + biomass case -- 256 x 128 x 128 sphere
-pack (radius 3-5, seed 0, porosity 0.45), 4 D3Q7 solutes (tau 0.8) with dual-Monod
-kinetics (Ks 1e-2, k_max dt 1e-3, yield 0.4) and biomass seeded in 1 % of pore voxels
-(B ~ U[10, 100]), FP32 population store, advected by the D3Q19 flow solved on the same
-pack. One step = collide + kinetics + biomass threshold rule + stream for every solute.
+Headline ("value"): coupled MLUPS of the full coupled case -- 384 x 192 x 192 sphere pack
+(radius 3-5, seed 0, porosity 0.35), D3Q19 flow solved on it with the FP32 deviation store,
+6 D3Q7 solutes (tau 0.8) carrying dual-Monod kinetics on biomass seeded in 1 % of the pore
+voxels and per-voxel FP64 equilibrium speciation on a 3-component / 6-species tableau
+(seed 11) every step, precipitation / dissolution node conversion every 100 steps followed
+by a flow re-solve, steps replayed from captured HIP graphs. One step = collide + kinetics
++ biomass threshold rule + speciation + stream for every solute; the node conversions and
+flow re-solves of the run are inside the timed region.
 
-Beside it, on the same GPU: D3Q19 flow MLUPS, the FP64 store, and the speciation legs --
-plain FP64 Newton solves/s on the 128^3 Gaussian-smoothed totals field of the 8 / 24
-tableau (seed 3), and MLP-warm vs plain Newton solves/s at the same 1e-10 residual on
-fresh samples of the 12 -> 128 -> 128 -> 8 warm-start setting (log10 totals + 4
-jittered logK). The CPU checker is timed on a bounded sample on the host cores.
+Beside it, on the same GPU: the kinetics + biomass case at 256 x 128 x 128 (4 solutes, FP32
+and FP64 stores), D3Q19 flow MLUPS, plain FP64 Newton solves/s on the 128^3 Gaussian-
+smoothed totals field of the 8 / 24 tableau (seed 3), and MLP-warm vs plain Newton solves/s
+at the same 1e-10 residual on fresh samples of the 12 -> 128 -> 128 -> 8 warm-start setting.
+The CPU checker is timed on a bounded slab of the headline workload on the host cores.
 """
 import argparse
 import json
@@ -28,6 +31,19 @@ sys.path.insert(0, PKG)
 METRIC = "coupled MLUPS + speciation solves/s, MLP-warm vs plain Newton @1e-10 residual"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+# algorithmic bytes per voxel-step (FP32 store): 7 populations read + written per solute
+# (56 B), 3 velocity components (12 B), the mask byte
+BYTES_D3Q19 = 2 * 19 * 4 + 1
+BYTES_COUPLED6 = 6 * 56 + 12 + 1
+BYTES_KIN4 = 4 * 56 + 12 + 1 + 8  # + biomass read and write
+# HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the default
+# workloads, scaled by a copy kernel of known size (profiles/hbm_traffic_fetch_write.md);
+# mean of the odd and even steps. Reported only when the workload is the default one.
+MEASURED_TRAFFIC = {
+    "full_coupled_384x192x192": (7.428e9 + 7.112e9) / 2,
+    "flow_384x192x192": (1.625e9 + 1.542e9) / 2,
+    "kinetics_256x128x128": 1.037e9,
+}
 
 
 def log(*a):
@@ -40,7 +56,97 @@ def kind_omega(tau=0.8):
     return kind, np.full(256, 1.0 / tau)
 
 
-def coupled_case(clb, nx, ny, nz, prec, flow_steps):
+def equilibrium(clb, S, lk, comp, totals):
+    """Species concentrations of the equilibrium with the given component totals."""
+    lat = clb.Lattice(4, 1, 1)
+    lat.set_tableau(S, lk, comp, tol=1e-12, max_iter=200)
+    lat.batch_load(np.atleast_2d(totals))
+    lat.batch_solve(clb.WARM_REF)
+    x, _, _, cv = lat.batch_fetch()
+    lat.close()
+    if not cv.all():
+        raise RuntimeError("equilibrium of the boundary water did not converge")
+    return 10.0 ** (lk + S @ x[0])
+
+
+def full_case_inputs(clb, nx, ny, nz):
+    pack = clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.35, seed=0)
+    b = clb.gen_biomass_seed(pack, pore=2, fraction=0.01, bmin=10.0, bmax=100.0, seed_sites=1, seed_density=2)
+    mask = pack.copy()
+    mask[b >= 10.0] = 3
+    S, lk = clb.gen_tableau(3, 6, seed=11, log_t_min=-4.0, log_t_max=-2.0)
+    c0 = equilibrium(clb, S, lk, [0, 1, 2], [1e-3, 5e-4, 1e-4])
+    cin = equilibrium(clb, S, lk, [0, 1, 2], [4e-3, 2e-3, 2e-4])
+    return pack, mask, b, S, lk, c0, cin
+
+
+def full_coupled(clb, args):
+    nx, ny, nz = args.nx, args.ny, args.nz
+    N = nx * ny * nz
+    pack, mask, b, S, lk, c0, cin = full_case_inputs(clb, nx, ny, nz)
+    kind, om = kind_omega(0.8)
+    lat = clb.Lattice(nx, ny, nz, precision=clb.PREC_FP32)
+    lat.set_dynamics(kind, om)
+    lat.upload_mask(mask)
+    lat.flow_init(1.0, 0.999)
+    lat.flow_steps(args.flow_steps)
+    lat.set_solutes(np.full((6, 256), 1.0 / 0.8), [0, 1] * 6, sum([[cin[s], 0.0] for s in range(6)], []))
+    lat.couple_velocity()
+    lat.set_biomass(b)
+    lat.set_kinetics(clb.KINETICS_PLUGIN, 1.0, 0, 1)
+    lat.set_mask_update(True, 10.0, 2, 3)
+    lat.set_tableau(S, lk, [0, 1, 2], tol=1e-10, max_iter=200)
+    lat.solutes_init(c0)
+    lat.set_phase(np.zeros(N, dtype=np.uint8))
+    # mineral threshold: the 0.01 % upper tail of species 5 after the start-up transient
+    lat.step(args.warmup)
+    c5 = lat.conc()[5][mask >= 2]
+    thr = float(np.quantile(c5, 0.9999))
+    lat.set_graph(True, args.interval)
+    lat.step(args.interval)  # capture the graph
+    lat.sync()
+
+    def block_run(nblocks):
+        conv = res = 0
+        for _ in range(nblocks):
+            lat.step(args.interval)
+            ch = lat.node_conversion(5, 2, 1, 9, thr, 0.5 * thr)
+            conv += ch
+            if ch:
+                lat.flow_steps(args.resolve_steps)
+                lat.couple_velocity()
+                res += args.resolve_steps
+        return conv, res
+
+    nblocks = max(args.steps // args.interval, 1)
+    steps = nblocks * args.interval
+    t0 = time.perf_counter()
+    conversions, resolves = block_run(nblocks)
+    lat.sync()
+    wall = time.perf_counter() - t0
+    # per-kernel durations on the library's stream (HIP events), uncaptured steps
+    lat.reset_perf()
+    s0 = lat.speciation_stats()
+    lat.set_timing(True)
+    lat.step(20)
+    lat.flow_steps(20)
+    lat.sync()
+    p = lat.perf()
+    lat.set_timing(False)
+    s1 = lat.speciation_stats() - s0
+    lat.close()
+    ade_ms = p.ade_ms / max(p.ade_launches, 1)
+    spec_ms = p.spec_ms / max(p.spec_launches, 1)
+    flow_ms = p.flow_ms / max(p.flow_launches, 1)
+    solves_step = s1[0] / 20.0
+    return dict(N=N, steps=steps, wall=wall, ms_per_step=wall * 1e3 / steps, mlups=N * steps / wall / 1e6,
+                ade_ms=ade_ms, straggler_ms=spec_ms, flow_ms=flow_ms, fluid=int(p.fluid_nodes),
+                conversions=int(conversions), resolve_steps=int(resolves), solves_per_step=solves_step,
+                converged=int(s1[1]), solves=int(s1[0]), mean_evals=float(s1[2] / max(s1[0], 1)),
+                coupled_solves_per_s=solves_step / (wall / steps))
+
+
+def kinetics_case(clb, nx, ny, nz, prec, flow_steps):
     pack = clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.45, seed=0)
     b = clb.gen_biomass_seed(pack, pore=2, fraction=0.01, bmin=10.0, bmax=100.0, seed_sites=1, seed_density=2)
     mask = pack.copy()
@@ -50,34 +156,27 @@ def coupled_case(clb, nx, ny, nz, prec, flow_steps):
     lat.set_dynamics(kind, om)
     lat.upload_mask(mask)
     lat.flow_init(1.0, 0.999)
-    lat.set_timing(True)
-    t0 = time.perf_counter()
     lat.flow_steps(flow_steps)
-    lat.sync()
-    flow_wall = time.perf_counter() - t0
-    pf = lat.perf()
-    lat.set_timing(False)
-    omg = np.full((4, 256), 1.0 / 0.8)
-    lat.set_solutes(omg, [0, 1, 0, 1, 1, 1, 1, 1], [1.0, 0, 0.5, 0, 0, 0, 0, 0])
+    lat.set_solutes(np.full((4, 256), 1.0 / 0.8), [0, 1, 0, 1, 1, 1, 1, 1], [1.0, 0, 0.5, 0, 0, 0, 0, 0])
     lat.couple_velocity()
     lat.set_biomass(b)
     lat.set_kinetics(clb.KINETICS_PLUGIN, 1.0, 0, 1)
     lat.set_mask_update(True, 10.0, 2, 3)
     lat.solutes_init([0.2, 0.1, 0.0, 0.0])
-    return lat, mask, b, dict(flow_wall=flow_wall, flow_ms=pf.flow_ms, flow_launches=pf.flow_launches,
-                              fluid=pf.fluid_nodes)
+    return lat
 
 
 def timed_steps(lat, steps, warmup):
     lat.step(warmup)
     lat.sync()
-    lat.reset_perf()
-    lat.set_timing(True)
-    lat.sync()
     t0 = time.perf_counter()
     lat.step(steps)
     lat.sync()
     wall = time.perf_counter() - t0
+    lat.reset_perf()
+    lat.set_timing(True)
+    lat.step(20)
+    lat.sync()
     p = lat.perf()
     lat.set_timing(False)
     return wall, p
@@ -88,10 +187,9 @@ def speciation_legs(clb, n_side, n_train, epochs):
     out = {}
     lat = clb.Lattice(4, 1, 1)
     lat.set_tableau(S, lk, list(range(8)), tol=1e-10, max_iter=200)
-    # plain Newton on the smoothed field
     tot = clb.gen_totals_field(n_side, n_side, n_side, 8, sigma=4.0, seed=4)
     lat.batch_load(tot)
-    lat.batch_solve(clb.WARM_REF)            # warm-up (also compiles / loads code objects)
+    lat.batch_solve(clb.WARM_REF)            # warm-up (also loads the code objects)
     lat.sync()
     lat.reset_perf()
     lat.set_timing(True)
@@ -106,7 +204,6 @@ def speciation_legs(clb, n_side, n_train, epochs):
     out["field"] = dict(n=n, wall_s=wall, kernel_ms=p.spec_ms, solves_per_s=n / (p.spec_ms * 1e-3),
                         converged=float(cv.mean()), max_resid=float(r.max()), mean_iters=float(it.mean()),
                         max_iters=int(it.max()), sum_iters=int(it.sum()))
-    # MLP trained on plain-Newton solves, then warm vs plain on fresh samples
     var = [8, 9, 10, 11]
 
     def samples(m, seed):
@@ -149,41 +246,67 @@ def speciation_legs(clb, n_side, n_train, epochs):
 
 
 def cpu_baseline(oracle_mod, clb, nx, ny, nz, steps):
-    """The FP64 checker on the same coupled workload (full domain, a few steps)."""
-    pack = clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.45, seed=0)
-    b = clb.gen_biomass_seed(pack, pore=2, fraction=0.01, bmin=10.0, bmax=100.0, seed_sites=1, seed_density=2)
-    mask = pack.copy()
-    mask[b >= 10.0] = 3
+    """The FP64 checker (the reference's algorithms, PCF + Anderson speciation) on a slab of
+    the headline workload: same generator and porosity, same 6 solutes, kinetics, biomass and
+    tableau; a uniform pore velocity in place of the flow solve."""
+    pack, mask, b, S, lk, c0, cin = full_case_inputs(clb, nx, ny, nz)
     kind, _ = kind_omega()
     u = np.zeros((pack.size, 3))
-    u[pack == 2, 0] = 1e-3
-    run = oracle_mod.AdeRun(nx, ny, nz, mask, kind, np.full((4, 256), 1.25), [0, 1, 0, 1, 1, 1, 1, 1],
-                            [1.0, 0, 0.5, 0, 0, 0, 0, 0], u, [0.2, 0.1, 0.0, 0.0], kinetics=True, biomass=b,
-                            mask_update=(10.0, 2, 3))
+    u[mask >= 2, 0] = 1e-3
+    run = oracle_mod.AdeRun(nx, ny, nz, mask, kind, np.full((6, 256), 1.25), [0, 1] * 6,
+                            sum([[cin[s], 0.0] for s in range(6)], []), u, c0, kinetics=True, biomass=b,
+                            tableau=(S, lk, [0, 1, 2]), eq_tol=1e-10, eq_max_iter=200, mask_update=(10.0, 2, 3))
     run.run(1)
     t0 = time.perf_counter()
     run.run(steps)
     dt = time.perf_counter() - t0
-    return dict(value=nx * ny * nz * steps / dt / 1e6, unit="MLUPS", cores=oracle_mod.num_threads(),
-                kind="port", sample=f"FP64 checker, full {nx}x{ny}x{nz} coupled case, {steps} steps ({dt:.1f} s)")
+    return dict(value=nx * ny * nz * steps / dt / 1e6, unit="MLUPS", cores=oracle_mod.num_threads(), kind="port",
+                sample=f"FP64 checker, the coupled step (6 solutes, kinetics, PCF+Anderson speciation) on a "
+                       f"{nx}x{ny}x{nz} slab of the same workload, {steps} steps ({dt:.1f} s)")
+
+
+def copy_calibration():
+    """Device-to-device copies of known size (the HIP runtime's copy kernel), for scaling the
+    HBM traffic counters: each copy reads and writes 1 GiB."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    n = ctypes.c_size_t(1 << 30)
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(src), n) or hip.hipMalloc(ctypes.byref(dst), n):
+        raise RuntimeError("hipMalloc failed for the copy calibration")
+    hip.hipMemset(src, 1, n)
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, n, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed for the copy calibration")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"copy calibration: 3 copies of {n.value} bytes")
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--nx", type=int, default=256)
-    ap.add_argument("--ny", type=int, default=128)
-    ap.add_argument("--nz", type=int, default=128)
-    ap.add_argument("--flow-steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--nx", type=int, default=384)
+    ap.add_argument("--ny", type=int, default=192)
+    ap.add_argument("--nz", type=int, default=192)
+    ap.add_argument("--interval", type=int, default=100, help="steps between node conversions")
+    ap.add_argument("--resolve-steps", type=int, default=100, help="flow steps after a geometry change")
+    ap.add_argument("--flow-steps", type=int, default=1000)
+    ap.add_argument("--kin-steps", type=int, default=200)
     ap.add_argument("--spec-side", type=int, default=128)
     ap.add_argument("--train-solves", type=int, default=2000000)
     ap.add_argument("--epochs", type=int, default=10)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-nz", type=int, default=192)
+    ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--skip-speciation", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-kinetics", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the multi-GPU slab "
@@ -193,35 +316,39 @@ def main():
     import clb
 
     log("device:", clb.device_name())
-    nx, ny, nz = args.nx, args.ny, args.nz
-    N = nx * ny * nz
-    # ---- headline: FP32 coupled step
-    lat, mask, b, fl = coupled_case(clb, nx, ny, nz, clb.PREC_FP32, args.flow_steps)
-    log(f"flow: {args.flow_steps} steps, {fl['flow_ms'] / args.flow_steps:.3f} ms/step")
-    wall, p = timed_steps(lat, args.steps, args.warmup)
-    fluid = p.fluid_nodes
-    S, nb, rs = 4, 1, 4
-    ms_step = wall * 1e3 / args.steps
-    mlups = N * args.steps / wall / 1e6
-    ade_ms = p.ade_ms / max(p.ade_launches, 1)
-    bytes_launch = fluid * (2 * 7 * S * rs + 3 * rs + nb * rs) + N * 1
-    achieved = bytes_launch / (ade_ms * 1e-3) / 1e9
-    lat.close()
-    log(f"coupled fp32: {ms_step:.3f} ms/step wall, kernel {ade_ms:.3f} ms, {mlups:.1f} MLUPS, "
-        f"{achieved:.0f} GB/s")
-    flow_ms = fl["flow_ms"] / max(fl["flow_launches"], 1)
-    flow_bytes = fluid * 2 * 19 * 4 + N
-    extra = dict(flow_fp32=dict(ms_per_step=flow_ms, mlups=N / (flow_ms * 1e-3) / 1e6,
-                                mflups=fluid / (flow_ms * 1e-3) / 1e6,
-                                gbs=flow_bytes / (flow_ms * 1e-3) / 1e9),
-                 fluid_nodes=int(fluid), nodes=N)
-    # ---- FP64 store for comparison
-    if not args.skip_fp64:
-        lat64, *_ = coupled_case(clb, nx, ny, nz, clb.PREC_FP64, 2)
-        w64, p64 = timed_steps(lat64, max(args.steps // 4, 10), 5)
-        lat64.close()
-        extra["coupled_fp64"] = dict(ms_per_step=w64 * 1e3 / max(args.steps // 4, 10),
-                                     mlups=N * max(args.steps // 4, 10) / w64 / 1e6)
+    if args.copy_calibration:
+        copy_calibration()
+    # ---- headline: the full coupled case
+    fc = full_coupled(clb, args)
+    N = fc["N"]
+    achieved = BYTES_COUPLED6 * N / (fc["ade_ms"] * 1e-3) / 1e9
+    log(f"full coupled: {fc['ms_per_step']:.3f} ms/step, {fc['mlups']:.1f} MLUPS; transport+speciation kernel "
+        f"{fc['ade_ms']:.3f} ms ({achieved:.0f} GB/s algorithmic), stragglers {fc['straggler_ms']:.3f} ms, "
+        f"{fc['solves_per_step']:.0f} solves/step, {fc['mean_evals']:.2f} evaluations/solve")
+    extra = dict(full_coupled=fc)
+    extra["flow_fp32"] = dict(ms_per_step=fc["flow_ms"], mlups=N / (fc["flow_ms"] * 1e-3) / 1e6,
+                              gbs=BYTES_D3Q19 * N / (fc["flow_ms"] * 1e-3) / 1e9,
+                              frac=BYTES_D3Q19 * N / (fc["flow_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    # ---- kinetics + biomass, 256 x 128 x 128, 4 solutes
+    if not args.skip_kinetics:
+        kn = (256, 128, 128)
+        Nk = kn[0] * kn[1] * kn[2]
+        lat = kinetics_case(clb, *kn, clb.PREC_FP32, 400)
+        wall, p = timed_steps(lat, args.kin_steps, 20)
+        lat.close()
+        kms = p.ade_ms / max(p.ade_launches, 1)
+        extra["kinetics_fp32"] = dict(workload="kinetics + biomass 256x128x128, 4 solutes, FP32 store",
+                                      ms_per_step=wall * 1e3 / args.kin_steps, mlups=Nk * args.kin_steps / wall / 1e6,
+                                      kernel_ms=kms, gbs=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9,
+                                      frac=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        log(f"kinetics fp32: {extra['kinetics_fp32']['mlups']:.1f} MLUPS, kernel {kms:.3f} ms, "
+            f"{extra['kinetics_fp32']['frac']:.3f} of HBM peak")
+        if not args.skip_fp64:
+            lat = kinetics_case(clb, *kn, clb.PREC_FP64, 2)
+            w64, _ = timed_steps(lat, max(args.kin_steps // 4, 10), 5)
+            lat.close()
+            extra["kinetics_fp64"] = dict(ms_per_step=w64 * 1e3 / max(args.kin_steps // 4, 10),
+                                          mlups=Nk * max(args.kin_steps // 4, 10) / w64 / 1e6)
     # ---- speciation legs
     if not args.skip_speciation:
         sp = speciation_legs(clb, args.spec_side, args.train_solves, args.epochs)
@@ -238,28 +365,33 @@ def main():
     if not args.skip_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as orc
-        cpu = cpu_baseline(orc, clb, nx, ny, nz, args.cpu_steps)
+        cpu = cpu_baseline(orc, clb, args.nx, args.ny, args.cpu_nz, args.cpu_steps)
         log(f"cpu baseline: {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
     line = {
         "metric": METRIC,
-        "value": round(mlups, 2),
+        "value": round(fc["mlups"], 2),
         "unit": "MLUPS",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": fc["steps"],
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(fc["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
-        "config": {"workload": f"coupled kinetics + biomass, {nx}x{ny}x{nz} sphere pack phi 0.45, "
-                               f"4 D3Q7 solutes dual-Monod, FP32 store", "nodes": N, "fluid_nodes": int(fluid),
-                   "solutes": S, "parallelism": "single GPU"},
+        "config": {"workload": f"full coupled case {args.nx}x{args.ny}x{args.nz}, sphere pack phi 0.35, D3Q19 + "
+                               f"6 D3Q7 solutes (FP32 deviation store), dual-Monod kinetics + biomass, FP64 "
+                               f"speciation 3/6 tableau every step, node conversion every {args.interval} steps "
+                               f"+ {args.resolve_steps}-step flow re-solve, captured graphs",
+                   "nodes": N, "fluid_nodes": fc["fluid"], "solutes": 6, "speciation": "fp64",
+                   "speciation_solves_per_s": round(fc["coupled_solves_per_s"], 1), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "ade_step_kernel<float,4>", "kernel_ms": round(ade_ms, 4),
-                     "bytes_per_launch": int(bytes_launch)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (MEASURED_TRAFFIC["full_coupled_384x192x192"]
+                                 if (args.nx, args.ny, args.nz) == (384, 192, 192) else None),
+                     "kernel": "ade_spec_step_kernel<float,6,*,3>", "kernel_ms": round(fc["ade_ms"], 4),
+                     "bytes_per_launch": int(BYTES_COUPLED6 * N)},
         "cpu_baseline": cpu,
         "extra": extra,
     }
