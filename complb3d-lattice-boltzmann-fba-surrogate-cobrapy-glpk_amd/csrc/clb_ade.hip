@@ -195,6 +195,54 @@ __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int
   if ((threadIdx.x & 63) == 0 && bal) atomicAdd(changed, (unsigned int)__popcll(bal));
 }
 
+// Fold the deferred speciation increments into the stored populations (before anything
+// other than the next step reads them). For a next step of parity ODD, every fluid node
+// adds to the slots load7 will read the contributions add_pending would add (pend_r: the
+// previous step's increments, pend_w: the step before), and to each population it has sent
+// towards a wall (read back one step later, so add_pending never sees it) t_j dC of its own
+// previous-step increment. Each slot belongs to exactly one node, so there are no races; the
+// caller clears both buffers afterwards.
+template <typename R, bool ODD>
+__global__ void pending_flush_kernel(AdeArgs<R> a) {
+  const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const Geom& g = a.geo;
+  if (n64 >= g.N) return;
+  const int n = (int)n64;
+  if (a.kind[a.mask[n]] != CLB_KIND_FLUID) return;
+  const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+  const Nbr nb(x, y, z, g);
+  for (int s = 0; s < a.S; ++s) {
+    const R* P1 = a.pend_r + (long long)s * g.N;
+    const R* P2 = a.pend_w + (long long)s * g.N;
+    if (a.immobile[s]) {
+      a.g[(long long)(s * 7) * g.N + n] += P1[n];
+      continue;
+    }
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      // the population i that node n reads in the next step
+      if (nb.xok(-Q7::cx[i])) {
+        R v = P1[n];
+        if (i > 0) {
+          const int m = nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i]);
+          v = (a.kind[a.mask[m]] == CLB_KIND_FLUID) ? P1[m] : P2[n];
+        }
+        R* q = ODD ? &a.g[(long long)(s * 7 + opp_dir(i)) * g.N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])]
+                   : &a.g[(long long)(s * 7 + i) * g.N + n];
+        *q += R(Q7::w(i)) * v;
+      }
+      // the population i that node n sent towards a wall in the previous step
+      if (i > 0 && nb.xok(Q7::cx[i])) {
+        const int w = nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i]);
+        if (a.kind[a.mask[w]] != CLB_KIND_FLUID) {
+          R* q = ODD ? &a.g[(long long)(s * 7 + opp_dir(i)) * g.N + n] : &a.g[(long long)(s * 7 + i) * g.N + w];
+          *q += R(Q7::w(i)) * P1[n];
+        }
+      }
+    }
+  }
+}
+
 // stabilizeADElattice3D (src/complab3d_processors_part3.hh:261-310): every fluid (pore /
 // biofilm) node back to the equilibrium of c0 at rest; wall nodes keep their populations.
 template <typename R>
@@ -277,6 +325,16 @@ void launch_ade_reset(const AdeArgs<R>& a, bool next_odd, const double* c0, hipS
   hipLaunchKernelGGL(ade_reset_kernel<R>, dim3((unsigned)((a.geo.N + bs - 1) / bs)), dim3(bs), 0, s, a, next_odd, c0);
 }
 
+template <typename R>
+void launch_pending_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s) {
+  const int bs = 256;
+  const dim3 grid((unsigned)((a.geo.N + bs - 1) / bs));
+  if (odd_next) hipLaunchKernelGGL((pending_flush_kernel<R, true>), grid, dim3(bs), 0, s, a);
+  else hipLaunchKernelGGL((pending_flush_kernel<R, false>), grid, dim3(bs), 0, s, a);
+}
+
+template void launch_pending_flush<float>(const AdeArgs<float>&, bool, hipStream_t);
+template void launch_pending_flush<double>(const AdeArgs<double>&, bool, hipStream_t);
 template void launch_ade_reset<float>(const AdeArgs<float>&, bool, const double*, hipStream_t);
 template void launch_ade_reset<double>(const AdeArgs<double>&, bool, const double*, hipStream_t);
 template void launch_ade_init<float>(const Geom&, int, float*, const double*, const float*, const int*, hipStream_t);

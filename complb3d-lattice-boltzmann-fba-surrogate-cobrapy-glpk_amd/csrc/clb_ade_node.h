@@ -73,6 +73,51 @@ __device__ __forceinline__ void store7(const AdeArgs<R>& a, const Nbr& nb, int n
   }
 }
 
+// Deferred speciation increments (fused coupled step): the voxel that solved writes its
+// limited dC per solute once instead of adding dC t_i to the seven slots it had stored.
+// Streaming is a permutation and the increment is additive, so the same state results when
+// the loader adds to population i of node n the t_i dC its source received:
+//  * the rest population and a population from a fluid neighbour n - c_i left its source in
+//    the previous step: dC of the source from pend_r;
+//  * a population from a wall is n's own, reflected: with bounce-back it left n two steps
+//    ago, so it takes n's dC of that step, which is still in pend_w (the buffer this step
+//    writes; n's entry is read here, before the solve phase rewrites it);
+//  * a population entering through an x face is rebuilt by the boundary completion.
+// Face nodes and walls never hold an increment. An immobile solute takes its whole dC in
+// the rest population. Branch-free, like inject.
+template <typename R, int S>
+__device__ __forceinline__ void add_pending(const AdeArgs<R>& a, const Nbr& nb, int n, R* p) {
+  const long long N = a.geo.N;
+  int src[7];
+  bool wall[7];
+  src[0] = n;
+  wall[0] = false;
+#pragma unroll
+  for (int i = 1; i < 7; ++i) {
+    wall[i] = false;
+    if (nb.xok(-Q7::cx[i])) {
+      const int m = nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i]);
+      wall[i] = a.kind[a.mask[m]] != CLB_KIND_FLUID;
+      src[i] = wall[i] ? n : m;
+    } else {
+      src[i] = -1;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < S; ++s) {
+    const R* P1 = a.pend_r + (long long)s * N;  // previous step
+    const R* P2 = a.pend_w + (long long)s * N;  // the step before
+    const bool imm = a.immobile[s] != 0;
+    const R own = P1[n];
+    p[s * 7] += imm ? own : R(Q7::w(0)) * own;
+#pragma unroll
+    for (int i = 1; i < 7; ++i) {
+      const R v = (src[i] < 0) ? R(0) : (wall[i] ? P2[n] : P1[src[i]]);
+      p[s * 7 + i] += imm ? R(0) : R(Q7::w(i)) * v;
+    }
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ R pop_sum(const R* q) {
   R r = 0;

@@ -4,12 +4,13 @@
 // abiotic -> speciation -> stream).
 //
 // The transport kernel hands each reactive voxel's post-kinetics concentrations to the
-// near-equilibrium Newton solve (spec_fused) through LDS, inside the same block, and adds
-// the limited increments to the populations it has just stored, so speciation adds no HBM
-// traffic beyond cache re-reads. A voxel whose solve has not met the judge within the fused
-// budget writes its concentrations to cbuf and is appended to a straggler list; the
-// persistent work-queue kernel (spec_kernel, MODE 1) then solves those voxels from cbuf and
-// adds their increments to the stored populations, in the same step.
+// near-equilibrium Newton solve (spec_fused) through LDS, inside the same block. The limited
+// increments are not written into the seven population slots (a read-modify-write of data
+// the kernel has just stored): each voxel writes its dC per solute once, and the next step
+// adds t_i dC to the populations as it loads them (add_pending). A voxel whose solve has not
+// met the judge within the fused budget writes its concentrations to cbuf and is appended to
+// a straggler list; the persistent work-queue kernel (spec_kernel, MODE 1) solves those
+// voxels from cbuf and writes their increments the same way, in the same step.
 #pragma once
 
 #include "clb_ade_node.h"
@@ -20,39 +21,20 @@ namespace clb {
 
 namespace {
 
-// Add dC * t_i of solute s to the slots node n wrote in this step (the AA slot map of
-// store7: even step -> (opp(i), n), odd step -> (i, n + c_i); an immobile solute's whole
-// increment goes to its rest slot).
-template <typename R, bool ODD>
-__device__ __forceinline__ void add_stored(const AdeArgs<R>& a, const Nbr& nb, int s, int n, double dC) {
-  const long long N = a.geo.N;
-  if (a.immobile[s]) {
-    R* q = &a.g[(long long)(s * 7) * N + n];
-    *q = (R)((double)*q + dC);
-    return;
-  }
-#pragma unroll
-  for (int i = 0; i < 7; ++i) {
-    R* q = ODD ? &a.g[(long long)(s * 7 + i) * N + nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])]
-               : &a.g[(long long)(s * 7 + opp_dir(i)) * N + n];
-    *q = (R)((double)*q + dC * Q7::w(i));
-  }
-}
-
 // Block of at most 256 threads, x fastest; every thread reaches the barriers.
-//  1. transport: load, collide, kinetics, store (as ade_step_kernel); each reactive voxel
-//     leaves its post-kinetics concentrations in an LDS slot, slots compacted over the block;
-//  2. solve: the first `count` threads of the block each solve one slot, so every lane of
-//     a solving wavefront has work whatever the pore fraction;
-//  3. increments: each reactive voxel adds its limited increments to the slots it stored
-//     in phase 1 (just written, so they come from cache); a voxel whose solve did not
-//     converge writes cbuf and joins the straggler list instead.
+//  1. transport: load (plus the previous step's deferred increments), collide, kinetics,
+//     store, as ade_step_kernel; each reactive voxel leaves its post-kinetics
+//     concentrations and its index in an LDS slot, slots compacted over the block;
+//  2. solve: the first `count` threads of the block each solve one slot, so every lane of a
+//     solving wavefront has work whatever the pore fraction, and write the limited
+//     increments to pend_w (added when the next step loads the populations). A voxel whose
+//     solve did not converge writes zeros there, its concentrations to cbuf, and joins the
+//     straggler list for the persistent kernel.
 template <typename R, int S, bool ODD, int NC>
 __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
-  __shared__ double sc[S][256];        // concentrations in, increments out
-  __shared__ unsigned char ok[256];    // solve converged
+  __shared__ double sc[S][256];        // post-kinetics concentrations of the block's reactive voxels
+  __shared__ int snode[256];           // their node indices
   __shared__ unsigned int woff[5];     // per-wavefront slot offsets, [4] = block total
-  __shared__ unsigned long long red[2][4];
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
@@ -88,12 +70,14 @@ __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tablea
     const Nbr nb(x, y, z, g);
     R p[S * 7];
     load7<R, S, ODD>(a, nb, n, p);
+    add_pending<R, S>(a, nb, n, p);
 #pragma unroll
     for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&p[s * 7]) + a.c_ref[s];
     ade_node<R, S, true, false>(a, x, y, z, n, code, p, nullptr);
     if (react) {
 #pragma unroll
       for (int s = 0; s < S; ++s) sc[s][slot] = fmax((double)pop_sum(&p[s * 7]) + (double)a.c_ref[s], kMinConc);
+      snode[slot] = n;
     }
     store7<R, S, ODD>(a, nb, n, p);
   }
@@ -105,6 +89,7 @@ __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tablea
       const Nbr nb(xb, y, z, g);
       R q[S * 7];
       load7<R, S, ODD>(a, nb, nbn, q);
+      add_pending<R, S>(a, nb, nbn, q);
       if (!fluid) {
         // interior neighbour is a wall: the Neumann face keeps its own density
 #pragma unroll
@@ -118,35 +103,25 @@ __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tablea
   __syncthreads();
 
   // ---- 2. solve the block's reactive voxels on its first `count` threads
-  bool solved = false;
-  int evals = 0;
+  bool solved = false, straggler = false;
+  int evals = 0, node = 0;
   if (lane < count) {
     double c[S], xs[NC];
 #pragma unroll
     for (int s = 0; s < S; ++s) c[s] = sc[s][lane];
+    node = snode[lane];
     solved = spec_fused<NC, S>(tb, c, xs, evals);
-    if (solved) {
+    straggler = !solved;
 #pragma unroll
-      for (int s = 0; s < S; ++s) sc[s][lane] = spec_increment<NC>(tb, s, xs, c[s]);
-    }
-    ok[lane] = solved ? 1 : 0;
-  }
-  __syncthreads();
-
-  // ---- 3. increments, or hand the voxel to the persistent kernel
-  bool straggler = false;
-  if (react) {
-    const Nbr nb(x, y, z, g);
-    if (ok[slot]) {
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-        const double dC = sc[s][slot];
-        if (dC > 1e-12 || dC < -1e-12) add_stored<R, ODD>(a, nb, s, n, dC);
+    for (int s = 0; s < S; ++s) {
+      double d = 0.0;
+      if (solved) {
+        d = spec_increment<NC>(tb, s, xs, sc[s][lane]);
+        if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
+      } else {
+        a.cbuf[(long long)s * g.N + node] = sc[s][lane];
       }
-    } else {
-      straggler = true;
-#pragma unroll
-      for (int s = 0; s < S; ++s) a.cbuf[(long long)s * g.N + n] = sc[s][slot];
+      a.pend_w[(long long)s * g.N + node] = (R)d;
     }
   }
   const unsigned long long sb = __ballot(straggler);
@@ -154,24 +129,18 @@ __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tablea
     unsigned int base = 0;
     if (wlane == 0) base = atomicAdd(io.scount, (unsigned int)__popcll(sb));
     base = __shfl(base, 0);
-    if (straggler) io.slist[base + __popcll(sb & ((1ull << wlane) - 1ull))] = n;
+    if (straggler) io.slist[base + __popcll(sb & ((1ull << wlane) - 1ull))] = node;
   }
-  // ---- statistics: per-block accumulators (solves, evaluations), no atomics
-  unsigned long long it = solved ? (unsigned long long)evals : 0ull;
-  for (int off = 32; off > 0; off >>= 1) it += __shfl_xor(it, off);
-  const unsigned long long nsol = __popcll(__ballot(solved));
-  if (wlane == 0) {
-    red[0][wave] = nsol;
-    red[1][wave] = it;
-  }
-  __syncthreads();
-  if (lane == 0) {
-    const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-    unsigned long long s0 = 0, s1 = 0;
-    for (int w = 0; w < nwaves; ++w) { s0 += red[0][w]; s1 += red[1][w]; }
-    if (s0) {
-      io.blk_stats[2 * bid] += s0;
-      io.blk_stats[2 * bid + 1] += s1;
+  // ---- statistics: per-wavefront accumulators (solves, evaluations), no atomics
+  if (lane < ((count + 63) & ~63)) {
+    unsigned long long it = solved ? (unsigned long long)evals : 0ull;
+    for (int off = 32; off > 0; off >>= 1) it += __shfl_xor(it, off);
+    const unsigned long long nsol = __popcll(__ballot(solved));
+    if (wlane == 0 && nsol) {
+      const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+      unsigned long long* st = io.blk_stats + 2 * (bid * 4 + wave);
+      st[0] += nsol;
+      st[1] += it;
     }
   }
 }

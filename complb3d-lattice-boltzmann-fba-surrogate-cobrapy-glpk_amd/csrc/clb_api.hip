@@ -93,8 +93,13 @@ struct clb_ctx {
   // speciation fused into the transport kernel (CLB_SPEC_FUSED) or after it (SEPARATE)
   int spec_mode = CLB_SPEC_FUSED;
   int* d_slist = nullptr;                   // voxels left to the persistent kernel
-  unsigned long long* d_blk_stats = nullptr;  // per transport block: solves, evaluations
-  long long blk_stats_n = 0;
+  unsigned long long* d_blk_stats = nullptr;  // per transport wavefront: solves, evaluations
+  long long blk_stats_n = 0;                  // wavefronts covered
+  // deferred speciation increments of the fused step: two S*N buffers alternating by step
+  // parity (the step of parity p reads buffer p and writes buffer p^1)
+  uint8_t* d_pend = nullptr;
+  size_t pend_bytes = 0;
+  bool pend_dirty = false;
   bool fluid_dirty = false;
 
   // batch speciation
@@ -223,6 +228,8 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   a.u = (const R*)c->d_u;
   a.bmass = (R*)c->d_b;
   a.cbuf = c->have_tableau ? c->d_cbuf : nullptr;
+  a.pend_r = nullptr;
+  a.pend_w = nullptr;
   a.geo = c->geo;
   a.S = c->S;
   a.nb = c->nb;
@@ -353,17 +360,60 @@ bool fused_spec(const clb_ctx* c) {
 
 // Buffers of the fused step (allocated outside any capture).
 int prepare_fused(clb_ctx* c) {
-  int rc = dalloc(c, &c->d_slist, (size_t)c->geo.N);
+  int rc = CLB_OK;
+  if (!c->d_slist) rc = dalloc(c, &c->d_slist, (size_t)c->geo.N);
   if (rc) return rc;
-  const long long nb = fused_blocks(c->geo, c->ade_block);
-  if (c->blk_stats_n != nb) {
-    dfree(c->d_blk_stats);
-    c->d_blk_stats = nullptr;
-    rc = dalloc(c, &c->d_blk_stats, (size_t)(2 * nb));
+  const long long nw = 4 * fused_blocks(c->geo, c->ade_block);
+  if (c->blk_stats_n != nw) {
+    rc = dalloc(c, &c->d_blk_stats, (size_t)(2 * nw));
     if (rc) return rc;
-    launch_fill_u64(c->d_blk_stats, 2 * nb, 0ull, c->stream);
-    c->blk_stats_n = nb;
+    launch_fill_u64(c->d_blk_stats, 2 * nw, 0ull, c->stream);
+    c->blk_stats_n = nw;
   }
+  const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+  const size_t pb = 2 * (size_t)c->S * (size_t)c->geo.N * rs;
+  if (c->pend_bytes != pb) {
+    rc = dalloc(c, &c->d_pend, pb);
+    if (rc) return rc;
+    launch_fill_u32((unsigned int*)c->d_pend, (long long)(pb / 4), 0u, c->stream);
+    c->pend_bytes = pb;
+    c->pend_dirty = false;
+  }
+  return CLB_OK;
+}
+
+// The pending buffer the step of parity `odd` reads (w = false) or writes (w = true).
+template <typename R>
+R* pend_buf(clb_ctx* c, bool odd, bool w) {
+  const bool which = w ? !odd : odd;
+  return (R*)c->d_pend + (which ? (size_t)c->S * (size_t)c->geo.N : 0);
+}
+
+// Fold the deferred increments into the populations and clear them, so that anything
+// other than the next fused step (downloads, node conversion, resets, mode changes) sees
+// the complete state.
+template <typename R>
+int flush_pending_t(clb_ctx* c) {
+  const bool odd_next = (c->ade_t & 1) != 0;
+  AdeArgs<R> a = ade_args<R>(c);
+  a.pend_r = pend_buf<R>(c, odd_next, false);
+  a.pend_w = pend_buf<R>(c, odd_next, true);
+  launch_pending_flush<R>(a, odd_next, c->stream);
+  launch_fill_u32((unsigned int*)c->d_pend, (long long)(c->pend_bytes / 4), 0u, c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->pend_dirty = false;
+  return CLB_OK;
+}
+
+int flush_pending(clb_ctx* c) {
+  if (!c->pend_dirty || !c->d_pend || !c->ade_ready) return CLB_OK;
+  return c->fp32 ? flush_pending_t<float>(c) : flush_pending_t<double>(c);
+}
+
+// Drop the deferred increments (the populations are being overwritten).
+int clear_pending(clb_ctx* c) {
+  if (c->d_pend) launch_fill_u32((unsigned int*)c->d_pend, (long long)(c->pend_bytes / 4), 0u, c->stream);
+  c->pend_dirty = false;
   return CLB_OK;
 }
 
@@ -373,6 +423,9 @@ void enqueue_step(clb_ctx* c) {
   const bool odd = (c->ade_t & 1) != 0;
   AdeArgs<R> a = ade_args<R>(c);
   if (fused_spec(c)) {
+    a.pend_r = pend_buf<R>(c, odd, false);
+    a.pend_w = pend_buf<R>(c, odd, true);
+    c->pend_dirty = true;
     launch_fill_u32(c->d_counter + 2, 1, 0u, c->stream);
     const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats};
     {
@@ -652,7 +705,7 @@ int clb_destroy(clb_ctx* c) {
   void* ptrs[] = {c->d_mask, c->d_kind, c->d_flow_omega, c->d_f, c->d_energy, c->d_ade_omega, c->d_cref, c->d_g,
                   c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
-                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats};
+                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -756,6 +809,10 @@ int clb_flow_download_velocity(clb_ctx* c, double* u, double* rho) {
 int clb_set_solutes(clb_ctx* c, int S, const double* omega, const int* bc_type, const double* bc_value) {
   if (c) drop_graph(c);
   if (!c || S < 1 || S > CLB_MAX_SOLUTES || !omega || !bc_type || !bc_value) return CLB_ERR_ARG;
+  {
+    const int rc0 = flush_pending(c);
+    if (rc0) return rc0;
+  }
   for (int s = 0; s < S; ++s)
     for (int i = 0; i < 256; ++i)
       if (c->h_kind[i] == CLB_KIND_FLUID && !(omega[s * 256 + i] > 0.0 && omega[s * 256 + i] < 2.0))
@@ -796,7 +853,9 @@ int clb_solutes_init(clb_ctx* c, const double* c0) {
   if (!c || !c0) return CLB_ERR_ARG;
   if (!c->d_g) return fail(c, CLB_ERR_STATE, "clb_set_solutes first");
   double* dc0 = nullptr;
-  int rc = dalloc(c, &dc0, (size_t)c->S);
+  int rc = clear_pending(c);  // every population is overwritten
+  if (rc) return rc;
+  rc = dalloc(c, &dc0, (size_t)c->S);
   if (rc) return rc;
   const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
   if (!c->d_cref) {
@@ -877,7 +936,9 @@ int clb_download_conc(clb_ctx* c, double* out) {
   if (!c || !out) return CLB_ERR_ARG;
   if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "solutes not initialised");
   double* dc = nullptr;
-  int rc = dalloc(c, &dc, (size_t)c->S * c->geo.N);
+  int rc = flush_pending(c);
+  if (rc) return rc;
+  rc = dalloc(c, &dc, (size_t)c->S * c->geo.N);
   if (rc) return rc;
   const bool odd_next = (c->ade_t & 1) != 0;
   if (c->fp32) launch_ade_conc<float>(ade_args<float>(c), odd_next, dc, c->stream);
@@ -919,6 +980,10 @@ int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* l
                     double tol, int max_iter) {
   if (c) drop_graph(c);
   if (!c) return CLB_ERR_ARG;
+  {
+    const int rc0 = flush_pending(c);
+    if (rc0) return rc0;
+  }
   if (nc == 0) {
     c->have_tableau = false;
     return CLB_OK;
@@ -988,7 +1053,7 @@ int clb_speciation_stats(clb_ctx* c, long long* stats) {
   CLB_CHECK_HIP(c, hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
   // solves finished inside the transport kernel (all of them converged)
   if (c->d_blk_stats && c->blk_stats_n > 0) {
-    std::vector<unsigned long long> b((size_t)(2 * c->blk_stats_n));
+    std::vector<unsigned long long> b((size_t)(2 * c->blk_stats_n));  // per wavefront
     CLB_CHECK_HIP(c, hipMemcpy(b.data(), c->d_blk_stats, sizeof(unsigned long long) * b.size(),
                                hipMemcpyDeviceToHost));
     for (long long k = 0; k < c->blk_stats_n; ++k) {
@@ -1003,6 +1068,8 @@ int clb_speciation_stats(clb_ctx* c, long long* stats) {
 
 int clb_set_speciation_mode(clb_ctx* c, int mode) {
   if (!c || (mode != CLB_SPEC_FUSED && mode != CLB_SPEC_SEPARATE)) return CLB_ERR_ARG;
+  const int rc0 = flush_pending(c);
+  if (rc0) return rc0;
   if (mode != c->spec_mode) drop_graph(c);
   c->spec_mode = mode;
   c->list_dirty = true;
@@ -1269,6 +1336,8 @@ int clb_mlp_train_from_batch(clb_ctx* c, int epochs, int batch, double lr, uint6
 int clb_set_immobile(clb_ctx* c, const int* immobile) {
   if (!c || !immobile) return CLB_ERR_ARG;
   if (c->S < 1) return fail(c, CLB_ERR_STATE, "clb_set_solutes first");
+  const int rc0 = flush_pending(c);
+  if (rc0) return rc0;
   for (int s = 0; s < CLB_MAX_SOLUTES; ++s) c->h_immobile[s] = (s < c->S) ? (immobile[s] ? 1 : 0) : 0;
   drop_graph(c);
   return CLB_OK;
@@ -1304,6 +1373,10 @@ int clb_node_conversion(clb_ctx* c, int mineral_solute, int pore_code, int preci
   if (!c || mineral_solute < 0 || mineral_solute >= c->S || pore_code < 0 || pore_code > 255 || precip_code < 0 ||
       precip_code > 255 || phase_id < 0 || phase_id > 255)
     return CLB_ERR_ARG;
+  {
+    const int rc0 = flush_pending(c);
+    if (rc0) return rc0;
+  }
   if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "solutes not initialised");
   if (!c->d_phase) {
     int rc = dalloc(c, &c->d_phase, (size_t)c->geo.N);
@@ -1359,7 +1432,9 @@ int clb_solutes_reset(clb_ctx* c, const double* c0) {
   if (!c || !c0) return CLB_ERR_ARG;
   if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "solutes not initialised");
   double* dc0 = nullptr;
-  int rc = dalloc(c, &dc0, (size_t)c->S);
+  int rc = flush_pending(c);  // wall nodes keep their populations, increments included
+  if (rc) return rc;
+  rc = dalloc(c, &dc0, (size_t)c->S);
   if (rc) return rc;
   CLB_CHECK_HIP(c, hipMemcpy(dc0, c0, sizeof(double) * c->S, hipMemcpyHostToDevice));
   const bool odd_next = (c->ade_t & 1) != 0;

@@ -49,6 +49,10 @@ struct AdeArgs {
   const R* u;             // 3*N (x, y, z planes)
   R* bmass;               // nb*N
   double* cbuf;           // S*N post-kinetics concentrations for the speciation kernel (or null)
+  // Speciation increments deferred to the next load (fused speciation only, else null):
+  // pend_r holds the previous step's dC per solute and voxel, pend_w receives this step's.
+  const R* pend_r;
+  R* pend_w;
   Geom geo;
   int S, nb;
   int bc_type[2 * CLB_MAX_SOLUTES];
@@ -96,7 +100,7 @@ struct SpecArgs {
 struct FusedSpecIO {
   int* slist;                     // voxels left to the persistent kernel (capacity N)
   unsigned int* scount;           // their number (cleared before the step)
-  unsigned long long* blk_stats;  // per block of the launch: solves, evaluations
+  unsigned long long* blk_stats;  // per wavefront of the launch: solves, evaluations
 };
 
 // ------------------------------------------------------------------ launchers
@@ -135,6 +139,10 @@ template <typename R>
 bool launch_ade_spec_step(const AdeArgs<R>& a, bool odd, dim3 block, const Tableau& tb, const FusedSpecIO& io,
                           hipStream_t s);
 long long fused_blocks(const Geom& g, dim3 block);
+// Add the deferred speciation increments (pend_r of a) to the stored populations, laid out
+// for a next step of parity odd_next.
+template <typename R>
+void launch_pending_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s);
 
 void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, hipStream_t s);
 int spec_occupancy_grid(int nc, int device);

@@ -14,6 +14,7 @@ void launch_spec_coupled(const AdeArgs<double>& a, bool odd_done, const Tableau&
   io.geo = a.geo;
   io.odd_done = odd_done ? 1 : 0;
   for (int s = 0; s < CLB_MAX_SOLUTES; ++s) io.immobile[s] = a.immobile[s];
+  io.pend_w = (void*)a.pend_w;
   sa.iters = nullptr;
   dispatch_spec<double, 1>(tb, sa, CLB_WARM_REF, io, grid, s);
 }

@@ -436,12 +436,18 @@ struct CoupledIO {
   Geom geo;
   int odd_done;         // parity of the step whose post-collision values are in g
   int immobile[CLB_MAX_SOLUTES];
+  void* pend_w;         // (R*) deferred increments: dC is written here instead of into g
 };
 
 template <typename R>
 __device__ __forceinline__ void add_increment(const CoupledIO& io, int s, int n, double dC) {
   R* g = (R*)io.g;
   const Geom& geo = io.geo;
+  if (io.pend_w) {
+    // fused step: the increment is added when the next step loads the populations
+    ((R*)io.pend_w)[(long long)s * geo.N + n] = (R)dC;
+    return;
+  }
   if (io.immobile[s]) {
     // immobile solute: the whole concentration sits in the node's rest slot
     R* q = &g[(long long)(s * 7) * geo.N + n];
