@@ -484,6 +484,8 @@ int step_t(clb_ctx* c, int nsteps) {
   if (fused) {
     int rc = prepare_fused(c);
     if (rc) return rc;
+    // every fused step, eager or replayed from a graph, leaves deferred increments
+    if (nsteps > 0) c->pend_dirty = true;
   } else if (c->have_tableau && c->list_dirty) {
     // the separate speciation kernel takes the reactive list and its length from the
     // host, so a captured graph holds them: rebuild both
