@@ -40,7 +40,7 @@ BYTES_KIN4 = 4 * 56 + 12 + 1 + 8  # + biomass read and write
 # workloads, scaled by a copy kernel of known size (profiles/hbm_traffic_fetch_write.md);
 # mean of the odd and even steps. Reported only when the workload is the default one.
 MEASURED_TRAFFIC = {
-    "full_coupled_384x192x192": (7.428e9 + 7.112e9) / 2,
+    "full_coupled_384x192x192": (6.414e9 + 6.321e9) / 2,
     "flow_384x192x192": (1.625e9 + 1.542e9) / 2,
     "kinetics_256x128x128": 1.037e9,
 }
@@ -390,7 +390,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (MEASURED_TRAFFIC["full_coupled_384x192x192"]
                                  if (args.nx, args.ny, args.nz) == (384, 192, 192) else None),
-                     "kernel": "ade_spec_step_kernel<float,6,*,3>", "kernel_ms": round(fc["ade_ms"], 4),
+                     "kernel": "ade_spec_step_kernel<float,6,*,3,512>", "kernel_ms": round(fc["ade_ms"], 4),
                      "bytes_per_launch": int(BYTES_COUPLED6 * N)},
         "cpu_baseline": cpu,
         "extra": extra,

@@ -25,6 +25,7 @@ BC_DIRICHLET, BC_NEUMANN = 0, 1
 KINETICS_NONE, KINETICS_PLUGIN = 0, 1
 WARM_REF, WARM_USER, WARM_MLP = 0, 1, 2
 SPEC_FUSED, SPEC_SEPARATE = 0, 1
+KERNEL_FLOW, KERNEL_ADE, KERNEL_FUSED = 0, 1, 2
 
 
 class ClbError(RuntimeError):
@@ -98,6 +99,8 @@ def load():
         "clb_download_phase": [vp, vp],
         "clb_node_conversion": [vp, c_int, c_int, c_int, c_int, c_double, c_double, vp],
         "clb_set_speciation_mode": [vp, c_int],
+        "clb_set_fused_budget": [vp, c_int],
+        "clb_set_block_shape": [vp, c_int, c_int, c_int],
         "clb_flow_stats": [vp, vp],
         "clb_solutes_reset": [vp, vp],
         "clb_set_graph": [vp, c_int, c_int],
@@ -344,6 +347,12 @@ class Lattice:
 
     def set_speciation_mode(self, mode):
         self._chk(self.lib.clb_set_speciation_mode(self.h, int(mode)), "set_speciation_mode")
+
+    def set_fused_budget(self, evals):
+        self._chk(self.lib.clb_set_fused_budget(self.h, int(evals)), "set_fused_budget")
+
+    def set_block_shape(self, kernel, bx, by):
+        self._chk(self.lib.clb_set_block_shape(self.h, int(kernel), int(bx), int(by)), "set_block_shape")
 
     def speciation_stats(self):
         s = np.zeros(3, dtype=np.int64)

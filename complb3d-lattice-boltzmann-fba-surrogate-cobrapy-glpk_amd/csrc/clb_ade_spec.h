@@ -21,20 +21,33 @@ namespace clb {
 
 namespace {
 
-// Block of at most 256 threads, x fastest; every thread reaches the barriers.
+// Block of at most BT threads (BT = 256 or 512), x fastest; every thread reaches the barriers.
 //  1. transport: load (plus the previous step's deferred increments), collide, kinetics,
 //     store, as ade_step_kernel; each reactive voxel leaves its post-kinetics
 //     concentrations and its index in an LDS slot, slots compacted over the block;
 //  2. solve: the first `count` threads of the block each solve one slot, so every lane of a
 //     solving wavefront has work whatever the pore fraction, and write the limited
 //     increments to pend_w (added when the next step loads the populations). A voxel whose
-//     solve did not converge writes zeros there, its concentrations to cbuf, and joins the
-//     straggler list for the persistent kernel.
-template <typename R, int S, bool ODD, int NC>
-__global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
-  __shared__ double sc[S][256];        // post-kinetics concentrations of the block's reactive voxels
-  __shared__ int snode[256];           // their node indices
-  __shared__ unsigned int woff[5];     // per-wavefront slot offsets, [4] = block total
+//     solve did not converge within io.evals evaluations writes zeros there, its
+//     concentrations to cbuf and its next trial point to io.sx, and joins the straggler
+//     list for the persistent kernel.
+//
+// With FP32 populations, up to six solutes and three components, the kernel is held to four
+// wavefronts per SIMD (128 VGPRs; the compiler spills about one register): the transport
+// phase is latency-bound on HBM and the solve on FP64 dependences, so occupancy matters
+// more than that spill. Other shapes would spill heavily at 128 and keep the compiler's
+// choice.
+template <typename R, int S, int NC>
+constexpr int fused_waves_per_eu() { return (sizeof(R) == 4 && S <= 6 && NC <= 3) ? 4 : 1; }
+
+template <typename R, int S, bool ODD, int NC, int BT>
+__global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(fused_waves_per_eu<R, S, NC>(), 8)))
+ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
+  constexpr int NW = BT / 64;
+  __shared__ double sc[S][BT];         // post-kinetics concentrations of the block's reactive voxels
+  __shared__ double scr[S][BT];        // species concentrations of the solve's latest evaluation
+  __shared__ int snode[BT];            // node indices of the slots
+  __shared__ unsigned int woff[NW + 1];  // per-wavefront slot offsets, [NW] = block total
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
@@ -58,11 +71,11 @@ __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tablea
       woff[w] = acc;
       acc += c;
     }
-    woff[4] = acc;
+    woff[NW] = acc;
   }
   __syncthreads();
   const int slot = (int)woff[wave] + __popcll(rb & ((1ull << wlane) - 1ull));
-  const int count = (int)woff[4];
+  const int count = (int)woff[NW];
 
   // ---- 1. transport
   R rho_in[S];
@@ -105,18 +118,23 @@ __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tablea
   // ---- 2. solve the block's reactive voxels on its first `count` threads
   bool solved = false, straggler = false;
   int evals = 0, node = 0;
+  double xs[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) xs[j] = 0.0;
   if (lane < count) {
-    double c[S], xs[NC];
+    double c[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) c[s] = sc[s][lane];
     node = snode[lane];
-    solved = spec_fused<NC, S>(tb, c, xs, evals);
+    solved = spec_fused<NC, S, BT>(tb, c, xs, evals, io.evals, &scr[0][lane]);
     straggler = !solved;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       double d = 0.0;
       if (solved) {
-        d = spec_increment<NC>(tb, s, xs, sc[s][lane]);
+        // the judged evaluation that converged was at xs: its species concentrations are
+        // still in scr (clamping them equals calc_species' clamp of log c to [-30, 1])
+        d = spec_increment_eq(tb.is_eq[s], scr[s][lane], sc[s][lane]);
         if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
       } else {
         a.cbuf[(long long)s * g.N + node] = sc[s][lane];
@@ -129,7 +147,12 @@ __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tablea
     unsigned int base = 0;
     if (wlane == 0) base = atomicAdd(io.scount, (unsigned int)__popcll(sb));
     base = __shfl(base, 0);
-    if (straggler) io.slist[base + __popcll(sb & ((1ull << wlane) - 1ull))] = node;
+    if (straggler) {
+      const unsigned int k = base + __popcll(sb & ((1ull << wlane) - 1ull));
+      io.slist[k] = node;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) io.sx[(long long)k * 4 + j] = (j < NC) ? xs[j] : 0.0;
+    }
   }
   // ---- statistics: per-wavefront accumulators (solves, evaluations), no atomics
   if (lane < ((count + 63) & ~63)) {
@@ -138,7 +161,7 @@ __global__ void __launch_bounds__(256) ade_spec_step_kernel(AdeArgs<R> a, Tablea
     const unsigned long long nsol = __popcll(__ballot(solved));
     if (wlane == 0 && nsol) {
       const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-      unsigned long long* st = io.blk_stats + 2 * (bid * 4 + wave);
+      unsigned long long* st = io.blk_stats + 2 * (bid * NW + wave);
       st[0] += nsol;
       st[1] += it;
     }
@@ -149,8 +172,13 @@ template <typename R, int S, int NC>
 void launch_fused_shape(const AdeArgs<R>& a, bool odd, dim3 block, const Tableau& tb, const FusedSpecIO& io,
                         hipStream_t st) {
   dim3 grid((a.geo.nx + block.x - 1) / block.x, (a.geo.ny + block.y - 1) / block.y, a.geo.nz);
-  if (odd) hipLaunchKernelGGL((ade_spec_step_kernel<R, S, true, NC>), grid, block, 0, st, a, tb, io);
-  else hipLaunchKernelGGL((ade_spec_step_kernel<R, S, false, NC>), grid, block, 0, st, a, tb, io);
+  if (block.x * block.y <= 256) {
+    if (odd) hipLaunchKernelGGL((ade_spec_step_kernel<R, S, true, NC, 256>), grid, block, 0, st, a, tb, io);
+    else hipLaunchKernelGGL((ade_spec_step_kernel<R, S, false, NC, 256>), grid, block, 0, st, a, tb, io);
+  } else {
+    if (odd) hipLaunchKernelGGL((ade_spec_step_kernel<R, S, true, NC, 512>), grid, block, 0, st, a, tb, io);
+    else hipLaunchKernelGGL((ade_spec_step_kernel<R, S, false, NC, 512>), grid, block, 0, st, a, tb, io);
+  }
 }
 
 }  // namespace
@@ -166,6 +194,7 @@ bool dispatch_fused(const AdeArgs<R>& a, bool odd, dim3 block, const Tableau& tb
                     hipStream_t st) {
   const int S = a.S, nc = tb.nc;
   if (tb.ns != S || nc < 1 || nc > 4) return false;
+  if (block.x * block.y > 512 || block.z != 1 || io.evals < 2) return false;
   const int NC = (nc <= 2) ? 2 : nc;
   if (NC > S) return false;
 #define CLB_FUSED(S_, NC_) \

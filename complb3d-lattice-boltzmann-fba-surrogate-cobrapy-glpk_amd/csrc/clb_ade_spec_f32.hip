@@ -9,8 +9,9 @@ bool launch_ade_spec_step<float>(const AdeArgs<float>& a, bool odd, dim3 block, 
   return dispatch_fused<float>(a, odd, block, tb, io, s);
 }
 
-long long fused_blocks(const Geom& g, dim3 block) {
-  return (long long)((g.nx + block.x - 1) / block.x) * ((g.ny + block.y - 1) / block.y) * g.nz;
+long long fused_waves(const Geom& g, dim3 block) {
+  const long long blocks = (long long)((g.nx + block.x - 1) / block.x) * ((g.ny + block.y - 1) / block.y) * g.nz;
+  return blocks * (block.x * block.y <= 256 ? 4 : 8);
 }
 
 }  // namespace clb

@@ -59,6 +59,7 @@ struct clb_ctx {
   int nb = 0;
   void* d_b = nullptr;
   dim3 ade_block{64, 4, 1};
+  dim3 spec_block{64, 8, 1};  // fused transport + speciation kernel (up to 512 threads)
 
   int h_immobile[CLB_MAX_SOLUTES] = {0};
   int* d_immobile = nullptr;
@@ -93,6 +94,8 @@ struct clb_ctx {
   // speciation fused into the transport kernel (CLB_SPEC_FUSED) or after it (SEPARATE)
   int spec_mode = CLB_SPEC_FUSED;
   int* d_slist = nullptr;                   // voxels left to the persistent kernel
+  double* d_sx = nullptr;                   // their next trial points (4 per straggler)
+  int fused_evals = 3;                      // evaluation budget of the fused solve
   unsigned long long* d_blk_stats = nullptr;  // per transport wavefront: solves, evaluations
   long long blk_stats_n = 0;                  // wavefronts covered
   // deferred speciation increments of the fused step: two S*N buffers alternating by step
@@ -362,8 +365,9 @@ bool fused_spec(const clb_ctx* c) {
 int prepare_fused(clb_ctx* c) {
   int rc = CLB_OK;
   if (!c->d_slist) rc = dalloc(c, &c->d_slist, (size_t)c->geo.N);
+  if (!rc && !c->d_sx) rc = dalloc(c, &c->d_sx, (size_t)4 * (size_t)c->geo.N);
   if (rc) return rc;
-  const long long nw = 4 * fused_blocks(c->geo, c->ade_block);
+  const long long nw = fused_waves(c->geo, c->spec_block);
   if (c->blk_stats_n != nw) {
     rc = dalloc(c, &c->d_blk_stats, (size_t)(2 * nw));
     if (rc) return rc;
@@ -427,10 +431,10 @@ void enqueue_step(clb_ctx* c) {
     a.pend_w = pend_buf<R>(c, odd, true);
     c->pend_dirty = true;
     launch_fill_u32(c->d_counter + 2, 1, 0u, c->stream);
-    const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats};
+    const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats, c->d_sx, c->fused_evals};
     {
       Timed tm(c, 1);
-      launch_ade_spec_step<R>(a, odd, c->ade_block, c->tb, io, c->stream);
+      launch_ade_spec_step<R>(a, odd, c->spec_block, c->tb, io, c->stream);
     }
     count_launch(c, 1);
     // voxels whose fused solve did not converge: the persistent kernel, same step
@@ -439,6 +443,7 @@ void enqueue_step(clb_ctx* c) {
     sa.n = 0;
     sa.n_dev = c->d_counter + 2;
     sa.iters = c->d_slist;
+    sa.logc = c->d_sx;  // each continues from where its fused solve stopped
     sa.counter = c->d_counter;
     sa.stats = c->d_stats;
     {
@@ -707,7 +712,7 @@ int clb_destroy(clb_ctx* c) {
   void* ptrs[] = {c->d_mask, c->d_kind, c->d_flow_omega, c->d_f, c->d_energy, c->d_ade_omega, c->d_cref, c->d_g,
                   c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
-                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend};
+                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1075,6 +1080,24 @@ int clb_set_speciation_mode(clb_ctx* c, int mode) {
   if (mode != c->spec_mode) drop_graph(c);
   c->spec_mode = mode;
   c->list_dirty = true;
+  return CLB_OK;
+}
+
+int clb_set_block_shape(clb_ctx* c, int kernel, int bx, int by) {
+  if (!c || bx < 1 || by < 1 || (bx * by) % 64 != 0) return CLB_ERR_ARG;
+  const int cap = (kernel == CLB_KERNEL_FUSED) ? 512 : 256;
+  if (bx * by > cap || (kernel != CLB_KERNEL_FLOW && kernel != CLB_KERNEL_ADE && kernel != CLB_KERNEL_FUSED))
+    return CLB_ERR_ARG;
+  dim3& b = (kernel == CLB_KERNEL_FLOW) ? c->flow_block : (kernel == CLB_KERNEL_ADE) ? c->ade_block : c->spec_block;
+  if ((int)b.x != bx || (int)b.y != by) drop_graph(c);  // captured launches hold the old grid
+  b = dim3((unsigned)bx, (unsigned)by, 1);
+  return CLB_OK;
+}
+
+int clb_set_fused_budget(clb_ctx* c, int evals) {
+  if (!c || evals < 2 || evals > 64) return CLB_ERR_ARG;
+  if (evals != c->fused_evals) drop_graph(c);  // captured launches hold the old budget
+  c->fused_evals = evals;
   return CLB_OK;
 }
 

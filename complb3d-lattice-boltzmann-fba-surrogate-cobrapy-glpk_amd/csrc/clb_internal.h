@@ -101,6 +101,8 @@ struct FusedSpecIO {
   int* slist;                     // voxels left to the persistent kernel (capacity N)
   unsigned int* scount;           // their number (cleared before the step)
   unsigned long long* blk_stats;  // per wavefront of the launch: solves, evaluations
+  double* sx;                     // per straggler: log activities its solve continues from (stride 4)
+  int evals;                      // evaluation budget of the fused solve (>= 2)
 };
 
 // ------------------------------------------------------------------ launchers
@@ -138,7 +140,9 @@ void launch_spec_coupled(const AdeArgs<double>& a, bool odd_done, const Tableau&
 template <typename R>
 bool launch_ade_spec_step(const AdeArgs<R>& a, bool odd, dim3 block, const Tableau& tb, const FusedSpecIO& io,
                           hipStream_t s);
-long long fused_blocks(const Geom& g, dim3 block);
+// Wavefront slots of a fused launch (the per-wavefront statistics it writes): blocks of at
+// most 256 threads hold 4, larger blocks (at most 512 threads) 8.
+long long fused_waves(const Geom& g, dim3 block);
 // Add the deferred speciation increments (pend_r of a) to the stored populations, laid out
 // for a next step of parity odd_next.
 template <typename R>

@@ -185,13 +185,11 @@ __device__ __forceinline__ void evaluate(const Tableau& tb, const int* vsp, int 
   }
 }
 
-// Solve (A + lam I) d = b by Cholesky (A symmetric, packed upper). Returns false when a
-// pivot is not positive. The factor is kept with reciprocal pivots, so the solve costs one
-// reciprocal square root per column and no divisions.
+// Cholesky factor of A + lam I (A symmetric, packed upper): L packed lower, row-major
+// (L(j,k), k <= j, at j(j+1)/2 + k), with reciprocal pivots inv, so a solve costs no
+// division. Returns false when a pivot is not positive.
 template <int NC>
-__device__ __forceinline__ bool chol_solve(const double* A, double lam, const double* b, double* d) {
-  double L[NC * (NC + 1) / 2];  // packed lower, row-major: L(j,k) k <= j at j(j+1)/2 + k
-  double inv[NC];               // 1 / L(j,j)
+__device__ __forceinline__ bool chol_factor(const double* A, double lam, double* L, double* inv) {
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
 #pragma unroll
@@ -208,6 +206,12 @@ __device__ __forceinline__ bool chol_solve(const double* A, double lam, const do
       }
     }
   }
+  return true;
+}
+
+// d = (L L^T)^-1 b; false when the result is not finite.
+template <int NC>
+__device__ __forceinline__ bool chol_apply(const double* L, const double* inv, const double* b, double* d) {
   double y[NC];
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
@@ -216,15 +220,23 @@ __device__ __forceinline__ bool chol_solve(const double* A, double lam, const do
     for (int m = 0; m < j; ++m) s -= L[j * (j + 1) / 2 + m] * y[m];
     y[j] = s * inv[j];
   }
+  bool ok = true;
 #pragma unroll
   for (int j = NC - 1; j >= 0; --j) {
     double s = y[j];
 #pragma unroll
     for (int m = j + 1; m < NC; ++m) s -= L[m * (m + 1) / 2 + j] * d[m];
     d[j] = s * inv[j];
-    if (!isfinite(d[j])) return false;
+    ok = ok && isfinite(d[j]);
   }
-  return true;
+  return ok;
+}
+
+// Solve (A + lam I) d = b by Cholesky.
+template <int NC>
+__device__ __forceinline__ bool chol_solve(const double* A, double lam, const double* b, double* d) {
+  double L[NC * (NC + 1) / 2], inv[NC];
+  return chol_factor<NC>(A, lam, L, inv) && chol_apply<NC>(L, inv, b, d);
 }
 
 template <int NC>
@@ -246,10 +258,16 @@ __device__ __forceinline__ double quad_form(const double* A, const double* d) {
 // Newton on the convex potential G (Hessian ln10 S^T diag(c) S, gradient S^T c - T) with
 // backtracking: a trial point is taken when G satisfies the Armijo condition or when the
 // reference judge decreases (near the solution G changes below its rounding while the
-// judge still converges quadratically). At most kFusedEvals evaluations; a voxel that has
-// not met the judge by then is handed to the persistent kernel above.
-constexpr int kFusedEvals = 6;
-constexpr double kJudgeStep = 3e-5;
+// judge still converges quadratically). A short Newton step gets the third-order
+// (Chebyshev) correction: with c_i(x + d) = c_i(x) 10^(S_i.d) exactly, the gradient at
+// x + d is g + H d + (ln10^2 / 2) sum_i c_i S_i (S_i.d)^2 + O(d^3), so
+// d = d0 - H^-1 (ln10^2 / 2) sum_i c_i S_i (S_i.d0)^2 with d0 = -H^-1 g leaves an error of
+// order |d0|^3 instead of |d0|^2, and a start one transport step away from equilibrium
+// usually meets the judge at the first trial point. At most `max_evals` evaluations; a
+// voxel that has not met the judge by then is handed, with its next trial point, to the
+// persistent kernel above.
+constexpr double kJudgeStep = 3e-4;   // judge a trial point reached by a step shorter than this
+constexpr double kCubicStep = 0.05;   // correct Newton steps shorter than this (log10 units)
 
 // Species concentration of calc_species (:442-496) at log activities x.
 template <int NC>
@@ -265,9 +283,12 @@ __device__ __forceinline__ double species_conc(const Tableau& tb, int i, const d
 // |f|^2 (clamped species, floors, +-10 clamp, zero-total guard, as in evaluate above), the
 // packed Hessian ln10 S^T diag(c) S and rhs = -(S^T c - T); a component under the
 // zero-total guard gets an identity row and a zero rhs, so it keeps its start.
-template <int NC, int NSM>
+//
+// The weighted species concentrations are left in crs[i * BT] (LDS), for the correction
+// of the Newton step.
+template <int NC, int NSM, int BT>
 __device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const double* T, bool JUDGE, double& G,
-                                       double& judge2, double* A, double* rhs) {
+                                       double& judge2, double* A, double* rhs, double* crs) {
   constexpr int NP = NC * (NC + 1) / 2;
   // Branch-free: the sign split of each coefficient is fmax(+-s, 0) and a species outside
   // the equilibrium set has weight 0 (its row is zero as well), so every species and
@@ -290,6 +311,7 @@ __device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const
     for (int j = 0; j < NC; ++j) lc = fma(tb.S[i * CLB_MAX_COMPONENTS + j], x[j], lc);
     const double cr = eqw * fast_exp10(fmin(lc, 250.0));
     const double cc = eqw * ((lc > kMaxLog) ? kMaxConc : ((lc < kMinLog) ? kMinConc : cr));
+    crs[i * BT] = cr;
     gsum += cr;
     const double lcr = kLn10 * cr;
 #pragma unroll
@@ -325,11 +347,59 @@ __device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const
   G = gsum / kLn10 - gtx;
 }
 
+// Newton direction at a point whose Hessian A, right-hand side rhs (= -gradient) and
+// species concentrations crs (LDS) eval_g has just produced: d = A^-1 rhs, with the
+// third-order correction when the step is short, limited to kMaxStep. A component under
+// the zero-total guard (identity row, zero rhs) keeps d_j = 0. Returns false when A is
+// not positive definite.
+template <int NC, int NSM, int BT>
+__device__ __forceinline__ bool newton_dir(const Tableau& tb, const double* A, const double* rhs, const double* T,
+                                           const double* crs, double* d, double& dmax) {
+  double L[NC * (NC + 1) / 2], inv[NC];
+  if (!chol_factor<NC>(A, 0.0, L, inv) || !chol_apply<NC>(L, inv, rhs, d)) return false;
+  dmax = 0.0;
+#pragma unroll
+  for (int j = 0; j < NC; ++j) dmax = fmax(dmax, fabs(d[j]));
+  if (dmax < kCubicStep) {
+    double t3[NC], e[NC];
+#pragma unroll
+    for (int j = 0; j < NC; ++j) t3[j] = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < NSM; ++i) {
+      double u = 0.0;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) u = fma(tb.S[i * CLB_MAX_COMPONENTS + j], d[j], u);
+      const double w = crs[i * BT] * u * u;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) t3[j] = fma(w, tb.S[i * CLB_MAX_COMPONENTS + j], t3[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < NC; ++j) t3[j] = (T[j] > 1e-18) ? (-0.5 * kLn10 * kLn10) * t3[j] : 0.0;
+    if (chol_apply<NC>(L, inv, t3, e)) {
+      dmax = 0.0;
+#pragma unroll
+      for (int j = 0; j < NC; ++j) {
+        d[j] += e[j];
+        dmax = fmax(dmax, fabs(d[j]));
+      }
+    }
+  }
+  if (dmax > kMaxStep) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) d[j] *= kMaxStep / dmax;
+    dmax = kMaxStep;
+  }
+  return true;
+}
+
 // c: concentrations of the NSM species (tb.ns of them used), already floored at 1e-30 as
-// run_equilibrium_biotic reads them (:1373-1400). Returns true when the judge is met; x
-// then holds the log activities and evals the number of evaluations.
-template <int NC, int NSM>
-__device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, double* x, int& evals) {
+// run_equilibrium_biotic reads them (:1373-1400); crs: this thread's LDS column (stride
+// BT) for the species concentrations. Returns true when the judge is met, with x the log
+// activities; otherwise x is the next trial point, where a continuing solve starts.
+// evals: the number of evaluations.
+template <int NC, int NSM, int BT>
+__device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, double* x, int& evals, int max_evals,
+                                           double* crs) {
   constexpr int NP = NC * (NC + 1) / 2;
   double T[NC], d[NC];
   // calc_component_totals (:520-538) and the component-slot guess (:977-994)
@@ -352,39 +422,28 @@ __device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, d
   // start: gradient and Hessian only. The solve always takes at least one step, and a
   // start that already met the judge stays within it after that (tiny) step, so the
   // judge is first evaluated at the first trial point.
-  double G = 0.0, judge2 = 0.0, slope = 0.0, alpha = 1.0;
+  double G = 0.0, judge2 = 0.0, slope = 0.0, alpha = 1.0, dmax = 0.0;
   {
     double A[NP], rhs[NC], jdummy;
-    eval_g<NC, NSM>(tb, x, T, false, G, jdummy, A, rhs);
+    eval_g<NC, NSM, BT>(tb, x, T, false, G, jdummy, A, rhs, crs);
     evals = 1;
-    if (!chol_solve<NC>(A, 0.0, rhs, d)) return false;
-    double m = 0.0;
-#pragma unroll
-    for (int j = 0; j < NC; ++j) m = fmax(m, fabs(d[j]));
-    if (m > kMaxStep) {
-#pragma unroll
-      for (int j = 0; j < NC; ++j) d[j] *= kMaxStep / m;
-    }
+    if (!newton_dir<NC, NSM, BT>(tb, A, rhs, T, crs, d, dmax)) return false;
 #pragma unroll
     for (int j = 0; j < NC; ++j) slope -= rhs[j] * d[j];
   }
   // A trial point is taken on the Armijo condition for G, or when the judge improves on the
-  // current point's (the first judged point: when it meets the tolerance). The judge is only
-  // evaluated at points reached by a short step (|d| < kJudgeStep): after a longer Newton
-  // step the residual is still of order |d|^2 and cannot meet 1e-10, so its logs are skipped
-  // and the next step is taken; a voxel is declared converged only where the judge was
-  // evaluated and met.
+  // current point's (the first judged point: when it meets the tolerance). The judge is
+  // evaluated at points reached by a short step (|d| < kJudgeStep) and at the last point the
+  // budget allows; after a longer step the residual cannot meet 1e-10 yet, so its logs are
+  // skipped. A voxel is declared converged only where the judge was evaluated and met.
   bool have_judge = false, conv = false;
-  double dmax = 0.0;
-#pragma unroll
-  for (int j = 0; j < NC; ++j) dmax = fmax(dmax, fabs(d[j]));
 #pragma unroll 1
-  for (int it = 1; it < kFusedEvals; ++it) {
+  for (int it = 1; it < max_evals; ++it) {
     double p[NC], A[NP], rhs[NC], Gp, jp = 0.0;
 #pragma unroll
     for (int j = 0; j < NC; ++j) p[j] = clampd(x[j] + alpha * d[j], kMinLog, kMaxLog);
-    const bool judged = alpha * dmax < kJudgeStep;
-    eval_g<NC, NSM>(tb, p, T, judged, Gp, jp, A, rhs);
+    const bool judged = alpha * dmax < kJudgeStep || it + 1 == max_evals;
+    eval_g<NC, NSM, BT>(tb, p, T, judged, Gp, jp, A, rhs, crs);
     ++evals;
     const bool armijo = Gp <= G + 1e-4 * alpha * slope;
     if (!(armijo || (judged && jp < (have_judge ? judge2 : tol2)))) {
@@ -397,28 +456,24 @@ __device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, d
     judge2 = jp;
     have_judge = judged;
     if (judged && judge2 < tol2) { conv = true; break; }
-    if (!chol_solve<NC>(A, 0.0, rhs, d)) break;
-    dmax = 0.0;
-#pragma unroll
-    for (int j = 0; j < NC; ++j) dmax = fmax(dmax, fabs(d[j]));
-    if (dmax > kMaxStep) {
-#pragma unroll
-      for (int j = 0; j < NC; ++j) d[j] *= kMaxStep / dmax;
-      dmax = kMaxStep;
-    }
+    if (!newton_dir<NC, NSM, BT>(tb, A, rhs, T, crs, d, dmax)) { alpha = 0.0; break; }
     slope = 0.0;
 #pragma unroll
     for (int j = 0; j < NC; ++j) slope -= rhs[j] * d[j];
     alpha = 1.0;
   }
+  if (!conv) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x[j] = clampd(x[j] + alpha * d[j], kMinLog, kMaxLog);
+  }
   return conv;
 }
 
-// Limited increment of species i after a converged solve at x (calc_species, NaN guard
-// and clamp of :1187-1214, then the 10 % limiter and 1e-20 floor of :1434-1447).
-template <int NC>
-__device__ __forceinline__ double spec_increment(const Tableau& tb, int i, const double* x, double c) {
-  double eq = tb.is_eq[i] ? species_conc<NC>(tb, i, x) : ((c > kMinConc) ? fmin(c, kMaxConc) : kMinConc);
+// Limited increment of a species whose equilibrium concentration is ceq (NaN guard and
+// clamp of :1187-1214, then the 10 % limiter and 1e-20 floor of :1434-1447); c is its
+// concentration before the solve. A species outside the equilibrium set keeps c.
+__device__ __forceinline__ double spec_increment_eq(bool is_eq, double ceq, double c) {
+  double eq = is_eq ? ceq : ((c > kMinConc) ? fmin(c, kMaxConc) : kMinConc);
   if (!isfinite(eq)) eq = (c > kMinConc) ? fmin(c, kMaxConc) : kMinConc;
   eq = clampd(eq, kMinConc, kMaxConc);
   double dd = eq - c;
@@ -426,6 +481,12 @@ __device__ __forceinline__ double spec_increment(const Tableau& tb, int i, const
   if (fabs(dd) > mx) dd = (dd > 0.0) ? mx : -mx;
   if (c + dd < 1e-20) dd = 1e-20 - c;
   return dd;
+}
+
+// Limited increment of species i after a converged solve at x (calc_species, then as above).
+template <int NC>
+__device__ __forceinline__ double spec_increment(const Tableau& tb, int i, const double* x, double c) {
+  return spec_increment_eq(tb.is_eq[i], tb.is_eq[i] ? species_conc<NC>(tb, i, x) : 0.0, c);
 }
 
 struct CoupledIO {
@@ -484,7 +545,7 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
   const double tol2 = tb.tol * tb.tol;
 
   long long item = -1;
-  bool have = false, exhausted = false;
+  bool have = false, exhausted = false, first = true;
   double x[NC], T[NC], d[NC], lkv[4];
   double A[NP], rhs[NC];
   double merit = 0.0, lam = -1.0, nu = 2.0, alpha = 1.0, best = 0.0, judge2 = 0.0;
@@ -501,7 +562,13 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
     if (want) {
       const int leader = __ffsll((long long)want) - 1;
       unsigned int base = 0;
-      if (lane == leader) base = atomicAdd(sa.counter, (unsigned int)__popcll(want));
+      if (lane == leader) {
+        // at a wavefront's first grab a drained queue is seen with a plain load, so a grid
+        // launched for few items does not queue up atomics on the one counter
+        base = first ? __hip_atomic_load(sa.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        if ((long long)base < n_items) base = atomicAdd(sa.counter, (unsigned int)__popcll(want));
+      }
+      first = false;
       base = __shfl(base, leader);
       if (need) {
         const unsigned long long below = want & ((1ull << lane) - 1ull);
@@ -549,7 +616,13 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
                 if (i < tb.ns && tb.is_eq[i]) s += tb.S[i * CLB_MAX_COMPONENTS + j] * clampd(conc[i], kMinConc, kMaxConc);
               T[j] = (j < tb.nc) ? fmax(s, kMinConc) : 0.0;
             }
-            // initial guess: log10 of the component species' concentration (:977-994)
+            // initial guess: log10 of the component species' concentration (:977-994), or,
+            // for a voxel handed over by the fused step, the point its solve reached
+            // (sa.logc, stride 4)
+            bool cont = sa.logc != nullptr && tb.nc <= 4;
+#pragma unroll
+            for (int j = 0; j < NC; ++j)
+              if (j < tb.nc && j < 4 && cont && !isfinite(sa.logc[v * 4 + j])) cont = false;
 #pragma unroll
             for (int j = 0; j < NC; ++j) {
               double cc = 1e-7;
@@ -558,6 +631,7 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
               for (int i = 0; i < NSM; ++i)
                 if (i == cs) cc = conc[i];
               x[j] = (j < tb.nc) ? fast_log10(clampd(cc, kMinConc, kMaxConc)) : 0.0;
+              if (cont && j < tb.nc && j < 4) x[j] = clampd(sa.logc[v * 4 + j], kMinLog, kMaxLog);
             }
           }
         } else {
@@ -724,7 +798,7 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
     my_conv += __shfl_xor(my_conv, off);
     my_iters += __shfl_xor(my_iters, off);
   }
-  if (lane == 0 && sa.stats) {
+  if (lane == 0 && sa.stats && my_solves) {
     atomicAdd(&sa.stats[0], my_solves);
     atomicAdd(&sa.stats[1], my_conv);
     atomicAdd(&sa.stats[2], my_iters);

@@ -128,6 +128,17 @@ int clb_speciation_stats(clb_ctx* ctx, long long* stats /*3: solves, converged, 
  * separate path runs. CLB_SPEC_SEPARATE: transport kernel, then the persistent kernel over
  * every reactive voxel. Both meet the same judge. */
 int clb_set_speciation_mode(clb_ctx* ctx, int mode);
+/* Evaluation budget of the fused solve, 2..64 (no reference counterpart: a tuning choice,
+ * default 3). A voxel that has not met the judge within it is finished by the persistent
+ * kernel in the same step, from the point its fused solve reached; a small budget keeps the
+ * transport kernel's wavefronts from waiting on their slowest lane. */
+int clb_set_fused_budget(clb_ctx* ctx, int evals);
+#define CLB_KERNEL_FLOW 0    /* D3Q19 collide + stream (at most 256 threads) */
+#define CLB_KERNEL_ADE 1     /* D3Q7 transport without fused speciation (at most 256 threads) */
+#define CLB_KERNEL_FUSED 2   /* D3Q7 transport with the fused speciation (at most 512 threads) */
+/* Thread-block shape (bx along x, by along y; bx*by a multiple of 64) of one of the step
+ * kernels (no reference counterpart: a launch choice; default 64 x 4, fused 64 x 8). */
+int clb_set_block_shape(clb_ctx* ctx, int kernel, int bx, int by);
 
 /* Immobile solutes (vec_immobile, src/complab.cpp:248-266): neither collided nor streamed,
  * changed only by reactions. immobile: S flags; set before clb_solutes_init. */

@@ -6,7 +6,8 @@ replayed from captured graphs, and separate, eager. Node conversions between the
 turn pore voxels into walls and back, so the deferred increments are folded into the
 populations (including those in flight towards walls) with the geometry changing under
 them. Both modes solve every voxel to the same 1e-10 judge, so the fields agree to the
-solver tolerance and the masks exactly."""
+solver tolerance and the masks exactly, whatever the fused evaluation budget (voxels beyond
+it continue in the persistent kernel from where the fused solve stopped) and block shape."""
 import numpy as np
 import pytest
 
@@ -14,7 +15,7 @@ from conftest import d3q19_kind_omega, rel_l2
 from test_oracle_speciation import CARB_COMP, CARB_LOGK, CARB_S, carbonate_exact
 
 
-def run(clb, oracle, mode, graph, nseg=4, seg=20):
+def run(clb, oracle, mode, graph, nseg=4, seg=20, budget=None, block=None):
     nx, ny, nz = 32, 16, 16
     pack = clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.5, seed=0)
     kind, om = d3q19_kind_omega(0.8)
@@ -24,6 +25,10 @@ def run(clb, oracle, mode, graph, nseg=4, seg=20):
     c_in, c_0 = carbonate_exact(2e-3, 4e-4), carbonate_exact(1e-3, 1e-4)
     lat = clb.Lattice(nx, ny, nz, precision=0)
     lat.set_speciation_mode(mode)
+    if budget is not None:
+        lat.set_fused_budget(budget)
+    if block is not None:
+        lat.set_block_shape(clb.KERNEL_FUSED, *block)
     lat.set_dynamics(kind, om)
     lat.upload_mask(pack)
     lat.set_solutes(np.full((4, 256), 1.0 / 0.8), [0, 1] * 4, [c_in[0], 0, c_in[1], 0, c_in[2], 0, c_in[3], 0])
@@ -48,13 +53,34 @@ def run(clb, oracle, mode, graph, nseg=4, seg=20):
     return c, m, st, changes
 
 
+_separate = {}
+
+
 @pytest.mark.gpu
-def test_fused_graph_matches_separate_with_node_conversion(clb, oracle):
-    print("case: fused (graph) vs separate (eager), node conversions between segments", flush=True)
-    cf, mf, sf, chf = run(clb, oracle, 0, True)
-    cs, ms, ss, chs = run(clb, oracle, 1, False)
+@pytest.mark.parametrize("budget,block", [(3, (64, 8)), (2, (64, 4)), (8, (32, 8)), (4, (128, 4))])
+def test_fused_graph_matches_separate_with_node_conversion(clb, oracle, budget, block):
+    print(f"case: fused (graph, budget {budget}, block {block}) vs separate (eager), node conversions", flush=True)
+    cf, mf, sf, chf = run(clb, oracle, 0, True, budget=budget, block=block)
+    if not _separate:
+        _separate["r"] = run(clb, oracle, 1, False)
+    cs, ms, ss, chs = _separate["r"]
     assert sum(chf) > 0 and chf == chs
     assert np.array_equal(mf, ms)
     assert sf[0] == sf[1] and ss[0] == ss[1] and sf[0] == ss[0]
     for s in range(4):
         assert rel_l2(cf[s], cs[s]) < 2e-6
+
+
+@pytest.mark.gpu
+def test_launch_settings_rejected(clb):
+    print("case: budget / block shape argument checks", flush=True)
+    lat = clb.Lattice(32, 16, 8, precision=0)
+    for bad in (1, 0, 65):
+        with pytest.raises(clb.ClbError):
+            lat.set_fused_budget(bad)
+    for kern, bx, by in ((clb.KERNEL_FUSED, 64, 16), (clb.KERNEL_ADE, 64, 8), (clb.KERNEL_FLOW, 32, 3), (7, 64, 4)):
+        with pytest.raises(clb.ClbError):
+            lat.set_block_shape(kern, bx, by)
+    lat.set_block_shape(clb.KERNEL_FLOW, 128, 2)
+    lat.set_fused_budget(5)
+    lat.close()
