@@ -102,6 +102,8 @@ def full_coupled(clb, args):
     lat.step(args.warmup)
     c5 = lat.conc()[5][mask >= 2]
     thr = float(np.quantile(c5, 0.9999))
+    # launch shapes and fused budget picked on this state (set-up: the state is restored)
+    tune = lat.autotune(args.tune_steps) if args.tune_steps > 0 else None
     lat.set_graph(True, args.interval)
     lat.step(args.interval)  # capture the graph
     lat.sync()
@@ -143,7 +145,7 @@ def full_coupled(clb, args):
                 ade_ms=ade_ms, straggler_ms=spec_ms, flow_ms=flow_ms, fluid=int(p.fluid_nodes),
                 conversions=int(conversions), resolve_steps=int(resolves), solves_per_step=solves_step,
                 converged=int(s1[1]), solves=int(s1[0]), mean_evals=float(s1[2] / max(s1[0], 1)),
-                coupled_solves_per_s=solves_step / (wall / steps))
+                coupled_solves_per_s=solves_step / (wall / steps), autotune=tune)
 
 
 def kinetics_case(clb, nx, ny, nz, prec, flow_steps):
@@ -296,6 +298,7 @@ def main():
     ap.add_argument("--interval", type=int, default=100, help="steps between node conversions")
     ap.add_argument("--resolve-steps", type=int, default=100, help="flow steps after a geometry change")
     ap.add_argument("--flow-steps", type=int, default=1000)
+    ap.add_argument("--tune-steps", type=int, default=10, help="steps per autotuner trial (0: default shapes)")
     ap.add_argument("--kin-steps", type=int, default=200)
     ap.add_argument("--spec-side", type=int, default=128)
     ap.add_argument("--train-solves", type=int, default=2000000)
@@ -322,6 +325,12 @@ def main():
     fc = full_coupled(clb, args)
     N = fc["N"]
     achieved = BYTES_COUPLED6 * N / (fc["ade_ms"] * 1e-3) / 1e9
+    tune = fc["autotune"]
+    fused_bt = 512 if tune is None or tune["fused_bx"] * tune["fused_by"] > 256 else 256
+    if tune:
+        log(f"autotune ({tune['trials']} trials): flow {tune['flow_bx']}x{tune['flow_by']} "
+            f"{tune['flow_ms']:.3f} ms, fused {tune['fused_bx']}x{tune['fused_by']} budget {tune['fused_budget']} "
+            f"{tune['fused_ms']:.3f} ms")
     log(f"full coupled: {fc['ms_per_step']:.3f} ms/step, {fc['mlups']:.1f} MLUPS; transport+speciation kernel "
         f"{fc['ade_ms']:.3f} ms ({achieved:.0f} GB/s algorithmic), stragglers {fc['straggler_ms']:.3f} ms, "
         f"{fc['solves_per_step']:.0f} solves/step, {fc['mean_evals']:.2f} evaluations/solve")
@@ -334,13 +343,14 @@ def main():
         kn = (256, 128, 128)
         Nk = kn[0] * kn[1] * kn[2]
         lat = kinetics_case(clb, *kn, clb.PREC_FP32, 400)
+        ktune = lat.autotune(args.tune_steps) if args.tune_steps > 0 else None
         wall, p = timed_steps(lat, args.kin_steps, 20)
         lat.close()
         kms = p.ade_ms / max(p.ade_launches, 1)
         extra["kinetics_fp32"] = dict(workload="kinetics + biomass 256x128x128, 4 solutes, FP32 store",
                                       ms_per_step=wall * 1e3 / args.kin_steps, mlups=Nk * args.kin_steps / wall / 1e6,
                                       kernel_ms=kms, gbs=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9,
-                                      frac=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                                      frac=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, autotune=ktune)
         log(f"kinetics fp32: {extra['kinetics_fp32']['mlups']:.1f} MLUPS, kernel {kms:.3f} ms, "
             f"{extra['kinetics_fp32']['frac']:.3f} of HBM peak")
         if not args.skip_fp64:
@@ -390,7 +400,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (MEASURED_TRAFFIC["full_coupled_384x192x192"]
                                  if (args.nx, args.ny, args.nz) == (384, 192, 192) else None),
-                     "kernel": "ade_spec_step_kernel<float,6,*,3,512>", "kernel_ms": round(fc["ade_ms"], 4),
+                     "kernel": f"ade_spec_step_kernel<float,6,*,3,{fused_bt}>", "kernel_ms": round(fc["ade_ms"], 4),
                      "bytes_per_launch": int(BYTES_COUPLED6 * N)},
         "cpu_baseline": cpu,
         "extra": extra,

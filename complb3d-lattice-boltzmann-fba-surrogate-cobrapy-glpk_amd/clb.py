@@ -32,6 +32,13 @@ class ClbError(RuntimeError):
     pass
 
 
+class TuneReport(ctypes.Structure):
+    _fields_ = [("flow_bx", ctypes.c_int), ("flow_by", ctypes.c_int), ("flow_ms", ctypes.c_double),
+                ("ade_bx", ctypes.c_int), ("ade_by", ctypes.c_int), ("ade_ms", ctypes.c_double),
+                ("fused_bx", ctypes.c_int), ("fused_by", ctypes.c_int), ("fused_budget", ctypes.c_int),
+                ("fused_ms", ctypes.c_double), ("trials", ctypes.c_int)]
+
+
 class _Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int), ("nx", ctypes.c_int),
                 ("ny", ctypes.c_int), ("nz", ctypes.c_int)]
@@ -101,6 +108,7 @@ def load():
         "clb_set_speciation_mode": [vp, c_int],
         "clb_set_fused_budget": [vp, c_int],
         "clb_set_block_shape": [vp, c_int, c_int, c_int],
+        "clb_autotune": [vp, c_int, vp],
         "clb_flow_stats": [vp, vp],
         "clb_solutes_reset": [vp, vp],
         "clb_set_graph": [vp, c_int, c_int],
@@ -353,6 +361,13 @@ class Lattice:
 
     def set_block_shape(self, kernel, bx, by):
         self._chk(self.lib.clb_set_block_shape(self.h, int(kernel), int(bx), int(by)), "set_block_shape")
+
+    def autotune(self, steps=10):
+        """Pick the fastest block shapes (and fused budget) on the current state; the state
+        is left unchanged. Returns the report as a dict."""
+        r = TuneReport()
+        self._chk(self.lib.clb_autotune(self.h, int(steps), ctypes.addressof(r)), "autotune")
+        return {k: getattr(r, k) for k, _ in TuneReport._fields_}
 
     def speciation_stats(self):
         s = np.zeros(3, dtype=np.int64)

@@ -1512,3 +1512,207 @@ int clb_reset_perf(clb_ctx* c) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- launch-shape autotuner
+namespace {
+
+// Device copies of everything a step changes, so each trial starts from the same state and
+// the caller's state is back, bit for bit, when tuning ends.
+struct TuneSnapshot {
+  std::vector<std::pair<void*, size_t>> src;  // live buffer, bytes
+  std::vector<void*> copy;
+  unsigned long long stats[4] = {0, 0, 0, 0};
+  long long flow_t = 0, ade_t = 0;
+  bool pend_dirty = false;
+  clb_perf perf{};
+};
+
+void snap_free(TuneSnapshot& s) {
+  for (void* p : s.copy) dfree(p);
+  s.copy.clear();
+}
+
+int snap_take(clb_ctx* c, TuneSnapshot& s) {
+  const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+  const size_t N = (size_t)c->geo.N;
+  if (c->d_f) s.src.push_back({c->d_f, 19 * N * rs});
+  if (c->d_g) s.src.push_back({c->d_g, (size_t)c->S * 7 * N * rs});
+  if (c->d_b && c->nb > 0) s.src.push_back({c->d_b, (size_t)c->nb * N * rs});
+  if (c->d_mask) s.src.push_back({c->d_mask, N});
+  if (c->d_pend && c->pend_bytes) s.src.push_back({c->d_pend, c->pend_bytes});
+  for (auto& e : s.src) {
+    void* p = nullptr;
+    if (hipMalloc(&p, e.second) != hipSuccess) {
+      snap_free(s);
+      return fail(c, CLB_ERR_HIP, "autotune: no device memory for the state copy");
+    }
+    s.copy.push_back(p);
+    CLB_CHECK_HIP(c, hipMemcpyAsync(p, e.first, e.second, hipMemcpyDeviceToDevice, c->stream));
+  }
+  long long st[3] = {0, 0, 0};
+  int rc = clb_speciation_stats(c, st);  // synchronizes
+  if (rc) return rc;
+  CLB_CHECK_HIP(c, hipMemcpy(s.stats, c->d_stats, sizeof(s.stats), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 3; ++i) s.stats[i] = (unsigned long long)st[i];
+  s.flow_t = c->flow_t;
+  s.ade_t = c->ade_t;
+  s.pend_dirty = c->pend_dirty;
+  s.perf = c->perf;
+  return CLB_OK;
+}
+
+int snap_restore(clb_ctx* c, const TuneSnapshot& s) {
+  for (size_t i = 0; i < s.src.size(); ++i)
+    CLB_CHECK_HIP(c, hipMemcpyAsync(s.src[i].first, s.copy[i], s.src[i].second, hipMemcpyDeviceToDevice, c->stream));
+  c->flow_t = s.flow_t;
+  c->ade_t = s.ade_t;
+  c->pend_dirty = s.pend_dirty;
+  return CLB_OK;
+}
+
+// Device time of `steps` steps of one kind (warm-up steps first), state restored before.
+template <typename R>
+int tune_trial(clb_ctx* c, const TuneSnapshot& s, bool flow, int steps, hipEvent_t e0, hipEvent_t e1, double& ms) {
+  int rc = snap_restore(c, s);
+  if (rc) return rc;
+  if (!flow && fused_spec(c)) {
+    rc = prepare_fused(c);
+    if (rc) return rc;
+  }
+  auto run = [&](int n) {
+    for (int t = 0; t < n; ++t) {
+      if (flow) {
+        launch_flow_step<R>(flow_args<R>(c, nullptr), (c->flow_t & 1) != 0, c->flow_block, c->stream);
+        c->flow_t++;
+      } else {
+        enqueue_step<R>(c);
+      }
+    }
+  };
+  run(2);
+  CLB_CHECK_HIP(c, hipEventRecord(e0, c->stream));
+  run(steps);
+  CLB_CHECK_HIP(c, hipEventRecord(e1, c->stream));
+  CLB_CHECK_HIP(c, hipEventSynchronize(e1));
+  CLB_CHECK_HIP(c, hipGetLastError());
+  float t = 0.f;
+  CLB_CHECK_HIP(c, hipEventElapsedTime(&t, e0, e1));
+  ms = (double)t / steps;
+  return CLB_OK;
+}
+
+template <typename R>
+int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
+  static const int shapes256[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}, {64, 2}, {128, 1}};
+  static const int shapes512[][2] = {{64, 8}, {64, 4}, {128, 4}, {32, 8}, {128, 2}, {256, 2}};
+  static const int budgets[] = {2, 3, 4};
+  TuneSnapshot s;
+  int rc = snap_take(c, s);
+  if (rc) return rc;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    snap_free(s);
+    return fail(c, CLB_ERR_HIP, "autotune: hipEventCreate failed");
+  }
+  const bool timing = c->timing;
+  c->timing = false;
+  *rep = clb_tune_report{};
+  auto body = [&]() -> int {
+    if (c->flow_ready) {
+      double best = 1e300;
+      dim3 pick = c->flow_block;
+      for (auto& sh : shapes256) {
+        c->flow_block = dim3(sh[0], sh[1], 1);
+        double ms = 0;
+        int r = tune_trial<R>(c, s, true, steps, e0, e1, ms);
+        if (r) return r;
+        rep->trials++;
+        if (ms < best) { best = ms; pick = c->flow_block; }
+      }
+      c->flow_block = pick;
+      rep->flow_bx = (int)pick.x;
+      rep->flow_by = (int)pick.y;
+      rep->flow_ms = best;
+    }
+    if (c->ade_ready) {
+      const bool fused = fused_spec(c);
+      if (!fused && c->have_tableau && c->list_dirty) {
+        int r = build_reactive_list(c);
+        if (r) return r;
+      }
+      double best = 1e300;
+      if (fused) {
+        dim3 pick = c->spec_block;
+        int pick_b = c->fused_evals;
+        for (int b : budgets)
+          for (auto& sh : shapes512) {
+            c->spec_block = dim3(sh[0], sh[1], 1);
+            c->fused_evals = b;
+            double ms = 0;
+            int r = tune_trial<R>(c, s, false, steps, e0, e1, ms);
+            if (r) return r;
+            rep->trials++;
+            if (ms < best) { best = ms; pick = c->spec_block; pick_b = b; }
+          }
+        c->spec_block = pick;
+        c->fused_evals = pick_b;
+        rep->fused_bx = (int)pick.x;
+        rep->fused_by = (int)pick.y;
+        rep->fused_budget = pick_b;
+        rep->fused_ms = best;
+      } else {
+        dim3 pick = c->ade_block;
+        for (auto& sh : shapes256) {
+          c->ade_block = dim3(sh[0], sh[1], 1);
+          double ms = 0;
+          int r = tune_trial<R>(c, s, false, steps, e0, e1, ms);
+          if (r) return r;
+          rep->trials++;
+          if (ms < best) { best = ms; pick = c->ade_block; }
+        }
+        c->ade_block = pick;
+        rep->ade_bx = (int)pick.x;
+        rep->ade_by = (int)pick.y;
+        rep->ade_ms = best;
+      }
+    }
+    return CLB_OK;
+  };
+  rc = body();
+  // the caller's state, statistics and counters, whatever the outcome
+  int rc2 = snap_restore(c, s);
+  if (!rc2 && c->ade_ready && fused_spec(c)) rc2 = prepare_fused(c);
+  if (!rc2) {
+    if (c->d_blk_stats) launch_fill_u64(c->d_blk_stats, 2 * c->blk_stats_n, 0ull, c->stream);
+    if (hipMemcpyAsync(c->d_stats, s.stats, sizeof(s.stats), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc2 = fail(c, CLB_ERR_HIP, "autotune: restoring the statistics failed");
+  }
+  c->perf = s.perf;
+  c->timing = timing;
+  c->list_dirty = true;   // the mask rule may have run during the trials
+  c->fluid_dirty = true;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  snap_free(s);
+  return rc ? rc : rc2;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clb_autotune(clb_ctx* c, int steps, clb_tune_report* report) {
+  if (!c || !report || steps < 2 || (steps & 1)) return CLB_ERR_ARG;
+  if (!c->flow_ready && !c->ade_ready) return fail(c, CLB_ERR_STATE, "nothing to tune: no flow or solutes set up");
+  if (c->ade_ready) {  // the coupled step must be runnable, as clb_step requires
+    if (c->kin_kind == CLB_KINETICS_PLUGIN && c->nb > 0 && !c->d_b) return fail(c, CLB_ERR_STATE, "no biomass");
+    if (c->have_tableau && c->tb.ns != c->S) return fail(c, CLB_ERR_ARG, "tableau species count != solute count");
+  }
+  int rc = flush_pending(c);
+  if (rc) return rc;
+  drop_graph(c);  // captured launches hold the old shapes
+  return c->fp32 ? autotune_t<float>(c, steps, report) : autotune_t<double>(c, steps, report);
+}
+
+}  // extern "C"

@@ -151,6 +151,12 @@ class Lattice {
   }
   void setGraph(bool enable, int stepsPerGraph) { chk(clb_set_graph(c_, enable ? 1 : 0, stepsPerGraph), "clb_set_graph"); }
   void step(int n) { chk(clb_step(c_, n), "clb_step"); }
+  // Fastest launch shapes (and fused budget) for the current state; the state is kept.
+  clb_tune_report autotune(int steps = 10) {
+    clb_tune_report r{};
+    chk(clb_autotune(c_, steps, &r), "clb_autotune");
+    return r;
+  }
   std::vector<double> concentrations() const {
     std::vector<double> c((size_t)(S_ * nodes()));
     chk(clb_download_conc(c_, c.data()), "clb_download_conc");

@@ -140,6 +140,24 @@ int clb_set_fused_budget(clb_ctx* ctx, int evals);
  * kernels (no reference counterpart: a launch choice; default 64 x 4, fused 64 x 8). */
 int clb_set_block_shape(clb_ctx* ctx, int kernel, int bx, int by);
 
+/* Launch-shape autotuner (no reference counterpart; the replacement for Palabos' static
+ * block decomposition). Times `steps` (even, >= 2) steps of each configured kernel for every
+ * candidate block shape -- and, for the fused speciation, every evaluation budget 2..4 -- on
+ * the context's current state, keeps the fastest, and reports it. Every trial starts from
+ * the same state, and the state, statistics and counters are restored bit for bit
+ * afterwards. Needs device memory for one copy of the state while it runs. */
+typedef struct clb_tune_report {
+  int flow_bx, flow_by;          /* D3Q19 step (0 when no flow is set up) */
+  double flow_ms;                /* its device time per step */
+  int ade_bx, ade_by;            /* D3Q7 step without fused speciation (0 when not run) */
+  double ade_ms;
+  int fused_bx, fused_by;        /* D3Q7 step with fused speciation (0 when not run) */
+  int fused_budget;
+  double fused_ms;               /* per step, persistent-kernel finish included */
+  int trials;
+} clb_tune_report;
+int clb_autotune(clb_ctx* ctx, int steps, clb_tune_report* report);
+
 /* Immobile solutes (vec_immobile, src/complab.cpp:248-266): neither collided nor streamed,
  * changed only by reactions. immobile: S flags; set before clb_solutes_init. */
 int clb_set_immobile(clb_ctx* ctx, const int* immobile);

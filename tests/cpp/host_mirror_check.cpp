@@ -89,7 +89,11 @@ int main(int argc, char** argv) {
     lat.setBiomass(1, b);
     lat.setKinetics(CLB_KINETICS_PLUGIN, 1.0, 0, 1);
     lat.solutesInit({0.2, 0.1, 0.0, 0.0});
-    lat.step(30);
+    lat.step(10);
+    // the autotuner keeps the state: the 30 steps still match the oracle's
+    const clb_tune_report tr = lat.autotune(4);
+    if (tr.trials != 12 || tr.ade_bx * tr.ade_by < 64) throw std::runtime_error("autotune report");
+    lat.step(20);
     write_bin(dir + "/conc.bin", lat.concentrations());
     write_bin(dir + "/bmass.bin", lat.biomass());
     std::printf("ok %d %lld %.0f %.6e\n", conv_batch ? 1 : 0, (long long)eq.getLastIterations(), st[3],
