@@ -25,6 +25,7 @@ BC_DIRICHLET, BC_NEUMANN = 0, 1
 KINETICS_NONE, KINETICS_PLUGIN = 0, 1
 WARM_REF, WARM_USER, WARM_MLP = 0, 1, 2
 SPEC_FUSED, SPEC_SEPARATE = 0, 1
+CA_DONE, CA_STALLED, CA_SWEEP_CAP, CA_AGE_CAP, CA_NONFINITE = 0, 1, 2, 3, 4
 KERNEL_FLOW, KERNEL_ADE, KERNEL_FUSED = 0, 1, 2
 
 
@@ -37,6 +38,12 @@ class TuneReport(ctypes.Structure):
                 ("ade_bx", ctypes.c_int), ("ade_by", ctypes.c_int), ("ade_ms", ctypes.c_double),
                 ("fused_bx", ctypes.c_int), ("fused_by", ctypes.c_int), ("fused_budget", ctypes.c_int),
                 ("fused_ms", ctypes.c_double), ("trials", ctypes.c_int)]
+
+
+class CaReport(ctypes.Structure):
+    _fields_ = [("sweeps", ctypes.c_int), ("age_passes", ctypes.c_int), ("status", ctypes.c_int),
+                ("mask_changes", ctypes.c_longlong), ("bmax_before", ctypes.c_double),
+                ("bmax_after", ctypes.c_double)]
 
 
 class _Config(ctypes.Structure):
@@ -109,6 +116,11 @@ def load():
         "clb_set_fused_budget": [vp, c_int],
         "clb_set_block_shape": [vp, c_int, c_int, c_int],
         "clb_autotune": [vp, c_int, vp],
+        "clb_ca_init": [vp],
+        "clb_ca_redistribute": [vp, c_double, c_int, ctypes.c_ulonglong, vp],
+        "clb_ca_update_age": [vp, c_double, c_int, vp],
+        "clb_ca_download_age": [vp, vp],
+        "clb_ca_set_age": [vp, vp],
         "clb_flow_stats": [vp, vp],
         "clb_solutes_reset": [vp, vp],
         "clb_set_graph": [vp, c_int, c_int],
@@ -361,6 +373,33 @@ class Lattice:
 
     def set_block_shape(self, kernel, bx, by):
         self._chk(self.lib.clb_set_block_shape(self.h, int(kernel), int(bx), int(by)), "set_block_shape")
+
+    # CA biofilm: biomass redistribution
+    def ca_init(self):
+        self._chk(self.lib.clb_ca_init(self.h), "ca_init")
+
+    def ca_redistribute(self, bmax, half=0, seed=1):
+        """Sweep until the max total biomass is within 1e-12 of bmax; returns the report as a dict."""
+        r = CaReport()
+        self._chk(self.lib.clb_ca_redistribute(self.h, float(bmax), int(half), int(seed), ctypes.addressof(r)),
+                  "ca_redistribute")
+        return {k: getattr(r, k) for k, _ in CaReport._fields_}
+
+    def ca_update_age(self, bmax, max_passes=1000):
+        p = ctypes.c_int(0)
+        self._chk(self.lib.clb_ca_update_age(self.h, float(bmax), int(max_passes), ctypes.addressof(p)),
+                  "ca_update_age")
+        return p.value
+
+    def ca_age(self):
+        a = np.zeros(self.N, dtype=np.int32)
+        self._chk(self.lib.clb_ca_download_age(self.h, _ptr(a)), "ca_download_age")
+        return a
+
+    def ca_set_age(self, age):
+        a = np.ascontiguousarray(age, dtype=np.int32).ravel()
+        assert a.size == self.N
+        self._chk(self.lib.clb_ca_set_age(self.h, _ptr(a)), "ca_set_age")
 
     def autotune(self, steps=10):
         """Pick the fastest block shapes (and fused budget) on the current state; the state

@@ -7,6 +7,7 @@
 // kinetics -> speciation -> mask update -> stream, done as one fused transport/kinetics
 // kernel followed by the speciation kernel.
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -104,6 +105,18 @@ struct clb_ctx {
   size_t pend_bytes = 0;
   bool pend_dirty = false;
   bool fluid_dirty = false;
+
+  // CA biomass redistribution
+  bool ca_ready = false;
+  int* d_age = nullptr;               // age (wavefront) lattice, two buffers for Jacobi passes
+  int* d_age2 = nullptr;
+  uint8_t* d_dpos = nullptr;          // sign of the distance field (1: not a wall at set-up)
+  double* d_btot = nullptr;           // total biomass lattice
+  double* d_caout = nullptr;          // 6 N: what each voxel sends across each face this sweep
+  void* d_bold = nullptr;             // biomass at the start of the sweep
+  size_t bold_bytes = 0;
+  unsigned long long* d_cared = nullptr;  // [0] max total (bits), [1] mask changes, [2] age changes
+  unsigned long long ca_sweep = 0;    // sweeps run so far (keys the random draws)
 
   // batch speciation
   long long bn = 0;
@@ -712,7 +725,8 @@ int clb_destroy(clb_ctx* c) {
   void* ptrs[] = {c->d_mask, c->d_kind, c->d_flow_omega, c->d_f, c->d_energy, c->d_ade_omega, c->d_cref, c->d_g,
                   c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
-                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx};
+                  c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
+                  c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1713,6 +1727,176 @@ int clb_autotune(clb_ctx* c, int steps, clb_tune_report* report) {
   if (rc) return rc;
   drop_graph(c);  // captured launches hold the old shapes
   return c->fp32 ? autotune_t<float>(c, steps, report) : autotune_t<double>(c, steps, report);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- CA biomass redistribution
+namespace {
+
+// updateLocalMaskNtotalLattices3D over the lattice; returns the max total and adds the mask
+// changes to *changes.
+template <typename R>
+int ca_mask_total(clb_ctx* c, bool fresh, double* mx, long long* changes) {
+  launch_fill_u64(c->d_cared, 2, 0ull, c->stream);
+  launch_ca_mask_total<R>((const R*)c->d_b, c->nb, c->d_mask, c->d_btot, c->geo, c->solid_code, c->bb_code,
+                          c->pore_code, c->biofilm_code, c->bthr, fresh, c->d_cared, c->stream);
+  unsigned long long h[2];
+  CLB_CHECK_HIP(c, hipMemcpyAsync(h, c->d_cared, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipGetLastError());
+  double v;
+  std::memcpy(&v, &h[0], sizeof(v));
+  *mx = v;
+  *changes += (long long)h[1];
+  return CLB_OK;
+}
+
+// updateAgeDistance3D repeated until no age changes (src/complab.cpp:1699-1708), at most
+// `cap` passes; returns the passes run, or -1 when the cap was reached with ages still moving.
+int ca_age_relax(clb_ctx* c, double bmax, int cap, int* passes) {
+  *passes = 0;
+  while (true) {
+    launch_fill_u64(c->d_cared + 2, 1, 0ull, c->stream);
+    launch_ca_age(c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->geo, bmax, c->d_cared + 2, c->stream);
+    unsigned long long ch = 0;
+    CLB_CHECK_HIP(c, hipMemcpyAsync(&ch, c->d_cared + 2, sizeof(ch), hipMemcpyDeviceToHost, c->stream));
+    CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    CLB_CHECK_HIP(c, hipGetLastError());
+    std::swap(c->d_age, c->d_age2);
+    ++*passes;
+    if (ch == 0) return CLB_OK;
+    if (*passes >= cap) {
+      *passes = -*passes;
+      return CLB_OK;
+    }
+  }
+}
+
+template <typename R>
+int ca_redistribute_t(clb_ctx* c, double bmax, int half, unsigned long long seed, clb_ca_report* rep) {
+  const size_t bbytes = (size_t)c->nb * (size_t)c->geo.N * (c->fp32 ? sizeof(float) : sizeof(double));
+  int rc = CLB_OK;
+  if (!c->d_caout) rc = dalloc(c, &c->d_caout, (size_t)6 * (size_t)c->geo.N);
+  if (!rc && c->bold_bytes != bbytes) {
+    rc = dalloc(c, (uint8_t**)&c->d_bold, bbytes);
+    c->bold_bytes = rc ? 0 : bbytes;
+  }
+  if (rc) return rc;
+  *rep = clb_ca_report{};
+  double mx = 0.0;
+  rc = ca_mask_total<R>(c, true, &mx, &rep->mask_changes);
+  if (rc) return rc;
+  rep->bmax_before = mx;
+  rep->bmax_after = mx;
+  if (!std::isfinite(mx)) {
+    rep->status = CLB_CA_NONFINITE;
+    return CLB_OK;
+  }
+  // the reference's 1e-12 (FP64); FP32 biomass resolves a total near bmax only to its last
+  // bits, so there the cap is met within 8 FP32 ulps of bmax
+  const double thrd = c->fp32 ? std::max(1e-12, 8.0 * (double)FLT_EPSILON * bmax) : 1e-12;
+  if (!(mx - bmax > thrd)) return CLB_OK;
+  double prev = mx;
+  int stall = 0;
+  for (int whilecount = 0; mx - bmax > thrd; ++whilecount) {
+    CLB_CHECK_HIP(c, hipMemcpyAsync(c->d_bold, c->d_b, bbytes, hipMemcpyDeviceToDevice, c->stream));
+    launch_ca_push(c->d_mask, c->d_btot, c->d_age, c->d_caout, c->geo, c->solid_code, c->bb_code, c->pore_code, bmax,
+                   half, seed, c->ca_sweep, c->stream);
+    launch_ca_gather<R>((const R*)c->d_bold, (R*)c->d_b, c->nb, c->d_mask, c->d_btot, c->d_caout, c->geo,
+                        c->solid_code, c->bb_code, c->stream);
+    ++c->ca_sweep;
+    ++rep->sweeps;
+    rc = ca_mask_total<R>(c, false, &mx, &rep->mask_changes);
+    if (rc) return rc;
+    rep->bmax_after = mx;
+    // no progress for 100 sweeps: the excess is boxed in (src/complab.cpp:1693-1695)
+    if (mx > prev - thrd) {
+      if (++stall >= 100) {
+        rep->status = CLB_CA_STALLED;
+        break;
+      }
+    } else {
+      stall = 0;
+    }
+    prev = mx;
+    if (whilecount % 50 == 0) {
+      int passes = 0;
+      rc = ca_age_relax(c, bmax, 1000, &passes);
+      if (rc) return rc;
+      rep->age_passes += passes < 0 ? -passes : passes;
+      if (passes < 0) {
+        rep->status = CLB_CA_AGE_CAP;
+        break;
+      }
+    }
+    if (whilecount > 1000) {  // the reference's cap (src/complab.cpp:1710)
+      rep->status = CLB_CA_SWEEP_CAP;
+      break;
+    }
+  }
+  return CLB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clb_ca_init(clb_ctx* c) {
+  if (!c) return CLB_ERR_ARG;
+  const size_t N = (size_t)c->geo.N;
+  int rc = dalloc(c, &c->d_age, N);
+  if (!rc) rc = dalloc(c, &c->d_age2, N);
+  if (!rc) rc = dalloc(c, &c->d_dpos, N);
+  if (!rc) rc = dalloc(c, &c->d_btot, N);
+  if (!rc && !c->d_cared) rc = dalloc(c, &c->d_cared, (size_t)4);
+  if (rc) return rc;
+  launch_ca_init(c->d_mask, c->d_age, c->d_dpos, c->geo, c->solid_code, c->bb_code, c->pore_code, c->stream);
+  launch_fill_u64((unsigned long long*)c->d_btot, (long long)N, 0ull, c->stream);
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->ca_ready = true;
+  return CLB_OK;
+}
+
+int clb_ca_redistribute(clb_ctx* c, double bmax, int half, unsigned long long seed, clb_ca_report* report) {
+  if (!c || !report || !(bmax > 0.0) || (half != 0 && half != 1)) return CLB_ERR_ARG;
+  if (!c->ca_ready) return fail(c, CLB_ERR_STATE, "clb_ca_init first");
+  if (c->nb < 1 || !c->d_b) return fail(c, CLB_ERR_STATE, "no biomass");
+  int rc = flush_pending(c);  // the mask may change under deferred increments
+  if (rc) return rc;
+  rc = c->fp32 ? ca_redistribute_t<float>(c, bmax, half, seed, report)
+               : ca_redistribute_t<double>(c, bmax, half, seed, report);
+  if (report->mask_changes > 0) {
+    c->list_dirty = true;
+    c->fluid_dirty = true;
+  }
+  return rc;
+}
+
+int clb_ca_update_age(clb_ctx* c, double bmax, int max_passes, int* passes) {
+  if (!c || max_passes < 1 || !(bmax > 0.0)) return CLB_ERR_ARG;
+  if (!c->ca_ready) return fail(c, CLB_ERR_STATE, "clb_ca_init first");
+  int p = 0;
+  int rc = ca_age_relax(c, bmax, max_passes, &p);
+  if (passes) *passes = p;
+  return rc;
+}
+
+int clb_ca_download_age(clb_ctx* c, int* age) {
+  if (!c || !age) return CLB_ERR_ARG;
+  if (!c->ca_ready) return fail(c, CLB_ERR_STATE, "clb_ca_init first");
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipMemcpy(age, c->d_age, sizeof(int) * (size_t)c->geo.N, hipMemcpyDeviceToHost));
+  return CLB_OK;
+}
+
+int clb_ca_set_age(clb_ctx* c, const int* age) {
+  if (!c || !age) return CLB_ERR_ARG;
+  if (!c->ca_ready) return fail(c, CLB_ERR_STATE, "clb_ca_init first");
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_age, age, sizeof(int) * (size_t)c->geo.N, hipMemcpyHostToDevice));
+  return CLB_OK;
 }
 
 }  // extern "C"

@@ -1,0 +1,330 @@
+// clb_ca.hip -- cellular-automaton biomass redistribution (the CA biofilm solver):
+//   pushExcessBiomass3D / halfPushExcessBiomass3D  src/complab3d_processors_part1.hh:432-894
+//   updateLocalMaskNtotalLattices3D                src/complab3d_processors_part2.hh:99-210
+//   updateAgeDistance3D                            src/complab3d_processors_part2.hh:718-842
+//   createAgeDomain3D                              src/complab3d_processors_part3.hh:347-367
+//   the sweep loop                                 src/complab.cpp:1453-1731
+//
+// The reference sweeps voxels in place, in loop order, with std::random_shuffle and
+// std::rand: its result depends on the visiting order and on the clock-seeded generator.
+// Here every sweep has two phases over a frozen state, so it does not depend on thread
+// order. The rule is the reference's, restated with the frozen state:
+//  1. push decision (ca_push_kernel): each overfull voxel splits its excess over its
+//     non-wall face neighbours. A random order of the neighbours comes from a counter-based
+//     generator keyed by (seed, sweep, voxel, draw). Neighbours are filled up to Bmax in
+//     that order, judged on the sweep's starting totals, as the reference's stale total
+//     lattice does. The rest goes to one neighbour chosen by the age rule. Only the six
+//     amounts are written;
+//  2. gather (ca_gather_kernel): each voxel subtracts what it sends and adds what its
+//     neighbours send it, species by species in proportion to the sender's starting
+//     biomass. Sender and receiver evaluate the same products, so mass is conserved to
+//     the last bit.
+// FP contraction is off in this file: the CPU restatement reproduces every operation, so
+// with FP64 storage the two agree bit for bit.
+#include "clb_internal.h"
+
+namespace clb {
+namespace {
+
+#pragma clang fp contract(off)
+
+constexpr double kCaThrd = 1e-12;     // thrd (src/complab3d_processors.hh:60)
+constexpr double kCaThrdMask = 1e-14; // COMPLAB_THRD (src/complab_functions.hh:64)
+
+__device__ __forceinline__ unsigned long long ca_mix(unsigned long long z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Draw k (0..7) of voxel n in sweep `sweep`.
+__device__ __forceinline__ unsigned long long ca_rand(unsigned long long seed, unsigned long long sweep, long long n,
+                                                      int k) {
+  return ca_mix(seed ^ ca_mix(sweep ^ ca_mix((unsigned long long)n * 8ull + (unsigned long long)k)));
+}
+
+// Face neighbour d (0 +x, 1 -x, 2 +y, 3 -y, 4 +z, 5 -z: the reference's order) of (x,y,z),
+// inside [x0, x1] along x and the whole box along y and z (no wrap); -1 outside.
+__device__ __forceinline__ long long ca_nbr(const Geom& g, int x, int y, int z, int d, int x0, int x1) {
+  const int dx = (d == 0) - (d == 1), dy = (d == 2) - (d == 3), dz = (d == 4) - (d == 5);
+  const int xx = x + dx, yy = y + dy, zz = z + dz;
+  if (xx < x0 || xx > x1 || yy < 0 || yy >= g.ny || zz < 0 || zz >= g.nz) return -1;
+  return xx + (long long)g.nx * (yy + (long long)g.ny * zz);
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+  // non-negative doubles (and NaN, whose pattern is larger still) order as their bit patterns
+  atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// Total biomass and mask (updateLocalMaskNtotalLattices3D): bmass = sum of species above
+// 1e-14; the stored total follows it when they differ by more than 1e-14 (always at
+// `fresh`). Pore -> biofilm at bmass >= bthr, biofilm -> pore below it, on the interior
+// planes 1 <= x <= nx-2 (as the transport kernel's mask rule). red[0]: max total over those
+// planes (bits; the inlet / outlet planes take no part in the CA), red[1]: mask changes.
+template <typename R>
+__global__ void ca_mask_total_kernel(const R* b, int nb, uint8_t* mask, double* btot, Geom g, int solid, int bb,
+                                     int pore, int biofilm, double bthr, bool fresh, unsigned long long* red) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  double t = 0.0;
+  unsigned int changed = 0;
+  bool inner = false;
+  if (n < g.N) {
+    const int m = mask[n];
+    const int xn = (int)(n % g.nx);
+    inner = xn >= 1 && xn <= g.nx - 2;
+    if (m != solid && m != bb) {
+      double bm = 0.0;
+      for (int i = 0; i < nb; ++i) {
+        const double v = (double)b[(long long)i * g.N + n];
+        if (v > kCaThrdMask) bm += v;
+      }
+      t = btot[n];
+      if (fresh || fabs(bm - t) > kCaThrdMask) {
+        t = bm;
+        btot[n] = bm;
+      }
+      const int x = (int)(n % g.nx);
+      if (x >= 1 && x <= g.nx - 2) {
+        if (m == pore && bm >= bthr) {
+          mask[n] = (uint8_t)biofilm;
+          changed = 1;
+        } else if (m == biofilm && bm < bthr) {
+          mask[n] = (uint8_t)pore;
+          changed = 1;
+        }
+      }
+    } else {
+      t = btot[n];
+    }
+  }
+  // wavefront reductions, one atomic each per wavefront
+  double mx = (inner && (t > 0.0 || t != t)) ? t : 0.0;
+  unsigned long long bits = (unsigned long long)__double_as_longlong(mx);
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(bits, off);
+    bits = o > bits ? o : bits;
+    changed += __shfl_xor(changed, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomic_max_nonneg(red, __longlong_as_double((long long)bits));
+    if (changed) atomicAdd(&red[1], (unsigned long long)changed);
+  }
+}
+
+// Phase 1: out[d * N + n] = total biomass voxel n sends to its neighbour d this sweep.
+// half: halfPushExcessBiomass3D (biofilm voxels send half their biomass, the age rule
+// accepts neighbours of equal age) instead of pushExcessBiomass3D (the excess over Bmax).
+__global__ void ca_push_kernel(const uint8_t* mask, const double* btot, const int* age, double* out, Geom g,
+                               int solid, int bb, int pore, double bmax, int half, unsigned long long seed,
+                               unsigned long long sweep) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  double o[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
+  const int m = mask[n];
+  const double T = btot[n];
+  bool push = x >= 1 && x <= g.nx - 2 && m != solid && m != bb && T > bmax;
+  if (half) push = push && m != pore;
+  double bMd = half ? T * 0.5 : T - bmax;
+  if (!half) push = push && bMd > kCaThrd;
+  if (push) {
+    int kept[6];
+    long long nn[6];
+    int nk = 0;
+    for (int d = 0; d < 6; ++d) {
+      const long long q = ca_nbr(g, x, y, z, d, 1, g.nx - 2);
+      if (q >= 0 && mask[q] != solid && mask[q] != bb) {
+        kept[nk] = d;
+        nn[nk] = q;
+        ++nk;
+      }
+    }
+    if (nk > 0) {
+      int idx[6] = {0, 1, 2, 3, 4, 5};
+      for (int i = nk - 1; i >= 1; --i) {  // Fisher-Yates over the kept neighbours
+        const int j = (int)(ca_rand(seed, sweep, n, i) % (unsigned long long)(i + 1));
+        const int t = idx[i];
+        idx[i] = idx[j];
+        idx[j] = t;
+      }
+      bool done = false;
+      for (int t = 0; t < nk && !done; ++t) {
+        const int k = idx[t];
+        const long long q = nn[k];
+        const double nt = (mask[q] == pore) ? 0.0 : btot[q];
+        if (nt < bmax) {
+          const double hold = bmax - nt;
+          double part;
+          if (bMd > hold) {
+            part = hold;
+            bMd -= hold;
+          } else {
+            part = bMd;
+            bMd = 0.0;
+            done = true;
+          }
+          o[kept[k]] += part;
+        }
+      }
+      if (!done) {
+        // the rest to a neighbour nearer the biofilm surface (lower age), else any
+        const int a0 = age[n];
+        int cand[6], nc = 0;
+        for (int k = 0; k < nk; ++k) {
+          const int a1 = age[nn[k]];
+          if (half ? (a0 >= a1) : (a0 > a1)) cand[nc++] = kept[k];
+        }
+        int d;
+        if (nc > 1) d = cand[(int)(ca_rand(seed, sweep, n, 6) % (unsigned long long)nc)];
+        else if (nc == 1) d = cand[0];
+        else d = kept[nk > 1 ? (int)(ca_rand(seed, sweep, n, 7) % (unsigned long long)nk) : 0];
+        o[d] += bMd;
+      }
+    }
+  }
+  for (int d = 0; d < 6; ++d) out[(long long)d * g.N + n] = o[d];
+}
+
+// Phase 2: species i of voxel n loses b_i / T * out for each of its sends and gains the
+// same expression, evaluated with the sender's values, for each send it receives. An
+// amount at or below 1e-12 is not moved (the reference's shove threshold).
+template <typename R>
+__global__ void ca_gather_kernel(const R* b_old, R* b, int nb, const uint8_t* mask, const double* btot,
+                                 const double* out, Geom g, int solid, int bb) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  const int m = mask[n];
+  if (m == solid || m == bb) return;
+  const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
+  double os[6];
+  long long src[6];
+  bool any = false;
+  for (int d = 0; d < 6; ++d) {
+    os[d] = out[(long long)d * g.N + n];
+    any = any || os[d] != 0.0;
+    // the neighbour across face d sends towards n through its face opp(d)
+    const long long q = ca_nbr(g, x, y, z, d, 0, g.nx - 1);
+    src[d] = (q >= 0 && out[(long long)(d ^ 1) * g.N + q] != 0.0) ? q : -1;
+    any = any || src[d] >= 0;
+  }
+  if (!any) return;
+  const double T = btot[n];
+  for (int i = 0; i < nb; ++i) {
+    const double bi = (double)b_old[(long long)i * g.N + n];
+    double v = bi;
+    for (int d = 0; d < 6; ++d) {
+      if (os[d] != 0.0) {
+        const double s = bi / T * os[d];
+        if (s > kCaThrd) v -= s;
+      }
+    }
+    for (int d = 0; d < 6; ++d) {
+      const long long q = src[d];
+      if (q >= 0) {
+        const double s = (double)b_old[(long long)i * g.N + q] / btot[q] * out[(long long)(d ^ 1) * g.N + q];
+        if (s > kCaThrd) v += s;
+      }
+    }
+    b[(long long)i * g.N + n] = (R)v;
+  }
+}
+
+// One Jacobi pass of updateAgeDistance3D over voxels holding biomass (total > 1e-14);
+// neighbours count when their distance flag is set (not wall in the distance field).
+__global__ void ca_age_kernel(const int* age_in, int* age_out, const uint8_t* dpos, const double* btot, Geom g,
+                              double bmax, unsigned long long* changed) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  unsigned int ch = 0;
+  if (n < g.N) {
+    const int age = age_in[n];
+    int na = age;
+    const double B = btot[n];
+    if (B > kCaThrdMask) {
+      const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
+      long long nn[6];
+      int nk = 0;
+      for (int d = 0; d < 6; ++d) {
+        const long long q = ca_nbr(g, x, y, z, d, 0, g.nx - 1);
+        if (q >= 0 && dpos[q]) nn[nk++] = q;
+      }
+      if (age == 0) {
+        na = 1;
+      } else if (age == 1 && (B - bmax) > -kCaThrdMask) {
+        bool ok = true;
+        for (int k = 0; k < nk; ++k)
+          if (age_in[nn[k]] == 0) ok = false;
+        if (ok) na = 2;
+      } else {
+        int cnt = 0;
+        for (int k = 0; k < nk; ++k)
+          if (age_in[nn[k]] >= age && (btot[nn[k]] - bmax) > -kCaThrdMask) ++cnt;
+        if (cnt == nk) na = age + 1;
+      }
+    }
+    age_out[n] = na;
+    ch = na != age ? 1u : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) ch += __shfl_xor(ch, off);
+  if ((threadIdx.x & 63) == 0 && ch) atomicAdd(changed, (unsigned long long)ch);
+}
+
+// createAgeDomain3D (-1 wall, 0 pore, 1 other) and the distance field's sign (the
+// distance is -1 on solid, 0 on bounce-back, positive elsewhere: calculateDistanceFromSolid,
+// src/complab_functions.hh:169-230).
+__global__ void ca_init_kernel(const uint8_t* mask, int* age, uint8_t* dpos, Geom g, int solid, int bb, int pore) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  const int m = mask[n];
+  const bool wall = m == solid || m == bb;
+  age[n] = wall ? -1 : (m == pore ? 0 : 1);
+  dpos[n] = wall ? 0 : 1;
+}
+
+inline unsigned blocks_of(long long N) { return (unsigned)((N + 255) / 256); }
+
+}  // namespace
+
+void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, int pore,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(ca_init_kernel, dim3(blocks_of(g.N)), dim3(256), 0, s, mask, age, dpos, g, solid, bb, pore);
+}
+
+template <typename R>
+void launch_ca_mask_total(const R* b, int nb, uint8_t* mask, double* btot, const Geom& g, int solid, int bb, int pore,
+                          int biofilm, double bthr, bool fresh, unsigned long long* red, hipStream_t s) {
+  hipLaunchKernelGGL((ca_mask_total_kernel<R>), dim3(blocks_of(g.N)), dim3(256), 0, s, b, nb, mask, btot, g, solid,
+                     bb, pore, biofilm, bthr, fresh, red);
+}
+
+void launch_ca_push(const uint8_t* mask, const double* btot, const int* age, double* out, const Geom& g, int solid,
+                    int bb, int pore, double bmax, int half, unsigned long long seed, unsigned long long sweep,
+                    hipStream_t s) {
+  hipLaunchKernelGGL(ca_push_kernel, dim3(blocks_of(g.N)), dim3(256), 0, s, mask, btot, age, out, g, solid, bb, pore,
+                     bmax, half, seed, sweep);
+}
+
+template <typename R>
+void launch_ca_gather(const R* b_old, R* b, int nb, const uint8_t* mask, const double* btot, const double* out,
+                      const Geom& g, int solid, int bb, hipStream_t s) {
+  hipLaunchKernelGGL((ca_gather_kernel<R>), dim3(blocks_of(g.N)), dim3(256), 0, s, b_old, b, nb, mask, btot, out, g,
+                     solid, bb);
+}
+
+void launch_ca_age(const int* age_in, int* age_out, const uint8_t* dpos, const double* btot, const Geom& g,
+                   double bmax, unsigned long long* changed, hipStream_t s) {
+  hipLaunchKernelGGL(ca_age_kernel, dim3(blocks_of(g.N)), dim3(256), 0, s, age_in, age_out, dpos, btot, g, bmax,
+                     changed);
+}
+
+template void launch_ca_mask_total<float>(const float*, int, uint8_t*, double*, const Geom&, int, int, int, int,
+                                          double, bool, unsigned long long*, hipStream_t);
+template void launch_ca_mask_total<double>(const double*, int, uint8_t*, double*, const Geom&, int, int, int, int,
+                                           double, bool, unsigned long long*, hipStream_t);
+template void launch_ca_gather<float>(const float*, float*, int, const uint8_t*, const double*, const double*,
+                                      const Geom&, int, int, hipStream_t);
+template void launch_ca_gather<double>(const double*, double*, int, const uint8_t*, const double*, const double*,
+                                       const Geom&, int, int, hipStream_t);
+
+}  // namespace clb

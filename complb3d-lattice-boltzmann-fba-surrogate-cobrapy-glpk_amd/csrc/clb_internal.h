@@ -151,6 +151,21 @@ void launch_pending_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s);
 void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, hipStream_t s);
 int spec_occupancy_grid(int nc, int device);
 
+// CA biomass redistribution (clb_ca.hip)
+void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, int pore,
+                    hipStream_t s);
+template <typename R>
+void launch_ca_mask_total(const R* b, int nb, uint8_t* mask, double* btot, const Geom& g, int solid, int bb, int pore,
+                          int biofilm, double bthr, bool fresh, unsigned long long* red, hipStream_t s);
+void launch_ca_push(const uint8_t* mask, const double* btot, const int* age, double* out, const Geom& g, int solid,
+                    int bb, int pore, double bmax, int half, unsigned long long seed, unsigned long long sweep,
+                    hipStream_t s);
+template <typename R>
+void launch_ca_gather(const R* b_old, R* b, int nb, const uint8_t* mask, const double* btot, const double* out,
+                      const Geom& g, int solid, int bb, hipStream_t s);
+void launch_ca_age(const int* age_in, int* age_out, const uint8_t* dpos, const double* btot, const Geom& g,
+                   double bmax, unsigned long long* changed, hipStream_t s);
+
 // MLP kernels (FP32 training, BF16 MFMA inference)
 struct MlpDims {
   int din, H, dout;

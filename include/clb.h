@@ -188,6 +188,45 @@ int clb_solutes_reset(clb_ctx* ctx, const double* c0 /*S*/);
 /* Replay the coupled step from captured HIP graphs of steps_per_graph steps (even). */
 int clb_set_graph(clb_ctx* ctx, int enable, int steps_per_graph);
 
+/* ---- CA biofilm: biomass redistribution (the sweep loop of src/complab.cpp:1453-1731)
+ * pushExcessBiomass3D / halfPushExcessBiomass3D (src/complab3d_processors_part1.hh:432-894),
+ * updateLocalMaskNtotalLattices3D (src/complab3d_processors_part2.hh:99-210) after every
+ * sweep, updateAgeDistance3D (src/complab3d_processors_part2.hh:718-842) until the ages
+ * settle after the first sweep and every 50th. The reference's sweep depends on its loop
+ * order and on std::rand; this one is deterministic: each sweep decides every push from
+ * the state at its start, with random neighbour orders drawn from (seed, sweep, voxel),
+ * then applies them. The pull step (pullExcessBiomass3D) only carries pushes across MPI
+ * subdomain faces and has no counterpart on one GPU. Codes and the pore <-> biofilm
+ * threshold are those of clb_set_mask_update; walls are clb_set_kinetics' solid_code and
+ * bb_code. Biomass is moved on the planes 1 <= x <= nx-2, with no wrap in y or z. */
+#define CLB_CA_DONE 0        /* max total biomass <= bmax + 1e-12 */
+#define CLB_CA_STALLED 1     /* 100 sweeps without progress: the excess is boxed in */
+#define CLB_CA_SWEEP_CAP 2   /* more than 1000 sweeps */
+#define CLB_CA_AGE_CAP 3     /* the age relaxation did not settle in 1000 passes */
+#define CLB_CA_NONFINITE 4   /* non-finite biomass: nothing was moved */
+typedef struct clb_ca_report {
+  int sweeps;              /* push sweeps run */
+  int age_passes;          /* age passes run */
+  int status;              /* CLB_CA_* */
+  long long mask_changes;  /* pore <-> biofilm flips during the call */
+  double bmax_before;      /* max total biomass before and after */
+  double bmax_after;
+} clb_ca_report;
+/* Age lattice from the current mask (createAgeDomain3D, src/complab3d_processors_part3.hh:
+ * 347-367: -1 wall, 0 pore code, 1 other) and the sign of the distance field
+ * (calculateDistanceFromSolid, src/complab_functions.hh:169-230), which stays fixed. */
+int clb_ca_init(clb_ctx* ctx);
+/* Sweep until the max total biomass is within 1e-12 of bmax (max_bMassRho; with FP32
+ * biomass, within 8 FP32 ulps of it), or a stop condition of the reference's loop;
+ * half = 1: halfPushExcessBiomass3D. Mask flips are
+ * counted in the report (nonzero => the caller updates the flow, as after node conversion). */
+int clb_ca_redistribute(clb_ctx* ctx, double bmax, int half, unsigned long long seed, clb_ca_report* report);
+/* Age passes alone, until no age changes or max_passes (src/complab.cpp:1739 runs one);
+ * passes: passes run, negative when max_passes was reached with ages still changing. */
+int clb_ca_update_age(clb_ctx* ctx, double bmax, int max_passes, int* passes);
+int clb_ca_download_age(clb_ctx* ctx, int* age /*N*/);
+int clb_ca_set_age(clb_ctx* ctx, const int* age /*N*/);
+
 /* ---- batched speciation (the per-voxel solve of EquilibriumChemistry, standalone)
  * totals: n*nc component totals; var_species[nvar] + logk_var[n*nvar]: per-voxel logK of
  * nvar species (NULL / 0 when the tableau's logK holds everywhere); logc: n*nc log10

@@ -36,6 +36,10 @@ def load():
         "orc_ade_density": ([cl, ci, vp, vp], None),
         "orc_ade_run": ([vp, vp, vp, vp, vp, ci, vp], None),
         "orc_node_conversion": ([cl, vp, vp, vp, ci, ci, ci, ci, ci, cd, cd], cl),
+        "orc_ca_init": ([cl, vp, ci, ci, ci, vp, vp], None),
+        "orc_ca_redistribute": ([ci, ci, ci, ci, vp, vp, vp, vp, vp, ci, ci, ci, ci, cd, cd, ci,
+                                 ctypes.c_uint64, ctypes.c_uint64, vp, vp], None),
+        "orc_ca_update_age": ([ci, ci, ci, vp, vp, vp, cd, ci], ci),
         "orc_classify_solid": ([ci, ci, ci, vp, vp, ci, ci], None),
         "orc_mlp_forward": ([ci, ci, ci, vp, cl, vp, vp], None),
         "orc_mlp_grad": ([ci, ci, ci, vp, cl, vp, vp, vp], cd),
@@ -201,6 +205,47 @@ def node_conversion(mask, phase, mineral, pore_code, solid_code, bb_code, precip
     n = lib.orc_node_conversion(m.size, _p(m), _p(ph), _p(mi), pore_code, solid_code, bb_code, precip_code, phase_id,
                                 precip_max, reopen)
     return m, ph, n
+
+
+def ca_init(mask, solid=0, bb=1, pore=2):
+    """createAgeDomain3D and the distance field's sign: (age int32, dpos uint8)."""
+    lib = load()
+    m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
+    age = np.zeros(m.size, dtype=np.int32)
+    dpos = np.zeros(m.size, dtype=np.uint8)
+    lib.orc_ca_init(m.size, _p(m), solid, bb, pore, _p(age), _p(dpos))
+    return age, dpos
+
+
+class CaState:
+    """The CA's lattices for the checker: biomass (nb, N) FP64, mask, age, distance sign and
+    total biomass, advanced in place by redistribute()."""
+
+    def __init__(self, nx, ny, nz, b, mask, solid=0, bb=1, pore=2, biofilm=3, bthr=10.0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.b = np.ascontiguousarray(np.atleast_2d(b), dtype=np.float64).copy()
+        self.mask = np.ascontiguousarray(mask, dtype=np.uint8).ravel().copy()
+        self.age, self.dpos = ca_init(self.mask, solid, bb, pore)
+        self.btot = np.zeros(self.mask.size)
+        self.codes = (solid, bb, pore, biofilm)
+        self.bthr = bthr
+        self.sweep = 0
+
+    def redistribute(self, bmax, half=0, seed=1):
+        """The sweep loop; returns (sweeps, age passes, status, mask flips, max before, max after)."""
+        lib = load()
+        rep = np.zeros(4, dtype=np.int64)
+        mx = np.zeros(2)
+        s, bb, p, bf = self.codes
+        lib.orc_ca_redistribute(self.nx, self.ny, self.nz, self.b.shape[0], _p(self.b), _p(self.mask), _p(self.age),
+                                _p(self.dpos), _p(self.btot), s, bb, p, bf, self.bthr, bmax, half, seed, self.sweep,
+                                _p(rep), _p(mx))
+        self.sweep += int(rep[0])
+        return int(rep[0]), int(rep[1]), int(rep[2]), int(rep[3]), float(mx[0]), float(mx[1])
+
+    def update_age(self, bmax, max_passes=1000):
+        return load().orc_ca_update_age(self.nx, self.ny, self.nz, _p(self.age), _p(self.dpos), _p(self.btot), bmax,
+                                        max_passes)
 
 
 class AdeRun:
