@@ -36,6 +36,10 @@ FP64_PEAK_TFLOPS = 78.6
 BYTES_D3Q19 = 2 * 19 * 4 + 1
 BYTES_COUPLED6 = 6 * 56 + 12 + 1
 BYTES_KIN4 = 4 * 56 + 12 + 1 + 8  # + biomass read and write
+# CA sweep, FP32 biomass, one species, per voxel (the dense part: the pushes themselves touch
+# few voxels): total/mask update 4 b + 1 mask + 16 total r/w; push decision 1 mask + 8 total
+# + 4 age + 1 flags; gather 1 mask + 1 flags; the sweep's biomass copy 8
+BYTES_CA_SWEEP = 21 + 14 + 2 + 8
 # HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the default
 # workloads, scaled by a copy kernel of known size (profiles/hbm_traffic_fetch_write.md);
 # mean of the odd and even steps. Reported only when the workload is the default one.
@@ -166,6 +170,48 @@ def kinetics_case(clb, nx, ny, nz, prec, flow_steps):
     lat.set_mask_update(True, 10.0, 2, 3)
     lat.solutes_init([0.2, 0.1, 0.0, 0.0])
     return lat
+
+
+def outside_main_pore_space(mask, nx, ny, nz, walls=(0, 1)):
+    """Voxels outside the largest face-connected open region (no wrap in y or z, as the CA's
+    neighbourhood). A pocket sealed by walls that biomass fills to capacity has no empty voxel
+    to hold its ages back, and the reference's age rule then raises them on every pass (an
+    isolated voxel too: its rule holds for an empty neighbour set), so the age relaxation
+    never settles and the reference's run ends there."""
+    from scipy import ndimage
+    open_ = ~np.isin(mask.reshape(nz, ny, nx), walls)
+    lab, n = ndimage.label(open_)
+    if n == 0:
+        return np.ones(mask.size, dtype=bool)
+    sizes = np.bincount(lab.ravel())
+    sizes[0] = 0
+    return (lab != int(np.argmax(sizes))).ravel()
+
+
+def ca_leg(lat, N, bmax=100.0, grow=2.5):
+    """CA biomass redistribution on the kinetics case: the biomass (1 % seeds up to Bmax,
+    after the timed steps) is scaled by `grow`, so seeds hold up to 2.5 Bmax, then swept
+    until every voxel is back under Bmax. Seeds outside the main pore space are removed first
+    (outside_main_pore_space)."""
+    b = lat.biomass()
+    b[:, outside_main_pore_space(lat.download_mask(), lat.nx, lat.ny, lat.nz)] = 0.0
+    lat.ca_init()
+    lat.ca_redistribute(1e30)  # set-up: allocates the sweep buffers; nothing is over that cap
+    lat.set_biomass(b * grow)
+    lat.sync()
+    t0 = time.perf_counter()
+    rep = lat.ca_redistribute(bmax, half=0, seed=1)
+    lat.sync()
+    wall = time.perf_counter() - t0
+    sweeps = max(rep["sweeps"], 1)
+    ms = wall * 1e3 / sweeps
+    log(f"CA: {rep['sweeps']} sweeps + {rep['age_passes']} age passes in {wall * 1e3:.2f} ms ({ms:.3f} ms/sweep), "
+        f"max {rep['bmax_before']:.1f} -> {rep['bmax_after']:.4f}, status {rep['status']}, "
+        f"{rep['mask_changes']} mask flips")
+    return dict(workload="CA redistribution, kinetics case 256x128x128, biomass x2.5", sweeps=rep["sweeps"],
+                age_passes=rep["age_passes"], status=rep["status"], wall_ms=wall * 1e3, ms_per_sweep=ms,
+                voxel_sweeps_per_s=N * sweeps / wall, gbs=BYTES_CA_SWEEP * N / (ms * 1e-3) / 1e9,
+                bmax_before=rep["bmax_before"], bmax_after=rep["bmax_after"], mask_changes=rep["mask_changes"])
 
 
 def timed_steps(lat, steps, warmup):
@@ -309,6 +355,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-kinetics", action="store_true")
+    ap.add_argument("--skip-ca", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -345,6 +392,8 @@ def main():
         lat = kinetics_case(clb, *kn, clb.PREC_FP32, 400)
         ktune = lat.autotune(args.tune_steps) if args.tune_steps > 0 else None
         wall, p = timed_steps(lat, args.kin_steps, 20)
+        if not args.skip_ca:
+            extra["ca"] = ca_leg(lat, Nk)
         lat.close()
         kms = p.ade_ms / max(p.ade_launches, 1)
         extra["kinetics_fp32"] = dict(workload="kinetics + biomass 256x128x128, 4 solutes, FP32 store",

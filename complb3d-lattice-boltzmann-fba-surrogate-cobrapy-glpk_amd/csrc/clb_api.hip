@@ -113,6 +113,7 @@ struct clb_ctx {
   uint8_t* d_dpos = nullptr;          // sign of the distance field (1: not a wall at set-up)
   double* d_btot = nullptr;           // total biomass lattice
   double* d_caout = nullptr;          // 6 N: what each voxel sends across each face this sweep
+  uint8_t* d_caflag = nullptr;        // N: which of those six are not zero
   void* d_bold = nullptr;             // biomass at the start of the sweep
   size_t bold_bytes = 0;
   unsigned long long* d_cared = nullptr;  // [0] max total (bits), [1] mask changes, [2] age changes
@@ -726,7 +727,8 @@ int clb_destroy(clb_ctx* c) {
                   c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
                   c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
-                  c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared};
+                  c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared,
+                  c->d_caflag};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1778,6 +1780,7 @@ int ca_redistribute_t(clb_ctx* c, double bmax, int half, unsigned long long seed
   const size_t bbytes = (size_t)c->nb * (size_t)c->geo.N * (c->fp32 ? sizeof(float) : sizeof(double));
   int rc = CLB_OK;
   if (!c->d_caout) rc = dalloc(c, &c->d_caout, (size_t)6 * (size_t)c->geo.N);
+  if (!rc && !c->d_caflag) rc = dalloc(c, &c->d_caflag, (size_t)c->geo.N);
   if (!rc && c->bold_bytes != bbytes) {
     rc = dalloc(c, (uint8_t**)&c->d_bold, bbytes);
     c->bold_bytes = rc ? 0 : bbytes;
@@ -1801,10 +1804,10 @@ int ca_redistribute_t(clb_ctx* c, double bmax, int half, unsigned long long seed
   int stall = 0;
   for (int whilecount = 0; mx - bmax > thrd; ++whilecount) {
     CLB_CHECK_HIP(c, hipMemcpyAsync(c->d_bold, c->d_b, bbytes, hipMemcpyDeviceToDevice, c->stream));
-    launch_ca_push(c->d_mask, c->d_btot, c->d_age, c->d_caout, c->geo, c->solid_code, c->bb_code, c->pore_code, bmax,
-                   half, seed, c->ca_sweep, c->stream);
-    launch_ca_gather<R>((const R*)c->d_bold, (R*)c->d_b, c->nb, c->d_mask, c->d_btot, c->d_caout, c->geo,
-                        c->solid_code, c->bb_code, c->stream);
+    launch_ca_push(c->d_mask, c->d_btot, c->d_age, c->d_caout, c->d_caflag, c->geo, c->solid_code, c->bb_code,
+                   c->pore_code, bmax, half, seed, c->ca_sweep, c->stream);
+    launch_ca_gather<R>((const R*)c->d_bold, (R*)c->d_b, c->nb, c->d_mask, c->d_btot, c->d_caout, c->d_caflag,
+                        c->geo, c->solid_code, c->bb_code, c->stream);
     ++c->ca_sweep;
     ++rep->sweeps;
     rc = ca_mask_total<R>(c, false, &mx, &rep->mask_changes);

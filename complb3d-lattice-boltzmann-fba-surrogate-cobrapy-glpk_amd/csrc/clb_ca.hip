@@ -113,11 +113,13 @@ __global__ void ca_mask_total_kernel(const R* b, int nb, uint8_t* mask, double* 
   }
 }
 
-// Phase 1: out[d * N + n] = total biomass voxel n sends to its neighbour d this sweep.
+// Phase 1: out[d * N + n] = total biomass voxel n sends to its neighbour d this sweep,
+// written only where it is not zero; bit d of flags[n] says which (flags are written for
+// every voxel, so the amounts need no clearing).
 // half: halfPushExcessBiomass3D (biofilm voxels send half their biomass, the age rule
 // accepts neighbours of equal age) instead of pushExcessBiomass3D (the excess over Bmax).
-__global__ void ca_push_kernel(const uint8_t* mask, const double* btot, const int* age, double* out, Geom g,
-                               int solid, int bb, int pore, double bmax, int half, unsigned long long seed,
+__global__ void ca_push_kernel(const uint8_t* mask, const double* btot, const int* age, double* out, uint8_t* flags,
+                               Geom g, int solid, int bb, int pore, double bmax, int half, unsigned long long seed,
                                unsigned long long sweep) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= g.N) return;
@@ -184,7 +186,14 @@ __global__ void ca_push_kernel(const uint8_t* mask, const double* btot, const in
       }
     }
   }
-  for (int d = 0; d < 6; ++d) out[(long long)d * g.N + n] = o[d];
+  unsigned int fl = 0;
+  for (int d = 0; d < 6; ++d) {
+    if (o[d] != 0.0) {
+      fl |= 1u << d;
+      out[(long long)d * g.N + n] = o[d];
+    }
+  }
+  flags[n] = (uint8_t)fl;
 }
 
 // Phase 2: species i of voxel n loses b_i / T * out for each of its sends and gains the
@@ -192,7 +201,7 @@ __global__ void ca_push_kernel(const uint8_t* mask, const double* btot, const in
 // amount at or below 1e-12 is not moved (the reference's shove threshold).
 template <typename R>
 __global__ void ca_gather_kernel(const R* b_old, R* b, int nb, const uint8_t* mask, const double* btot,
-                                 const double* out, Geom g, int solid, int bb) {
+                                 const double* out, const uint8_t* flags, Geom g, int solid, int bb) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= g.N) return;
   const int m = mask[n];
@@ -200,13 +209,13 @@ __global__ void ca_gather_kernel(const R* b_old, R* b, int nb, const uint8_t* ma
   const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
   double os[6];
   long long src[6];
-  bool any = false;
+  const unsigned int fl = flags[n];
+  bool any = fl != 0;
   for (int d = 0; d < 6; ++d) {
-    os[d] = out[(long long)d * g.N + n];
-    any = any || os[d] != 0.0;
+    os[d] = ((fl >> d) & 1u) ? out[(long long)d * g.N + n] : 0.0;
     // the neighbour across face d sends towards n through its face opp(d)
     const long long q = ca_nbr(g, x, y, z, d, 0, g.nx - 1);
-    src[d] = (q >= 0 && out[(long long)(d ^ 1) * g.N + q] != 0.0) ? q : -1;
+    src[d] = (q >= 0 && ((flags[q] >> (d ^ 1)) & 1u)) ? q : -1;
     any = any || src[d] >= 0;
   }
   if (!any) return;
@@ -298,18 +307,18 @@ void launch_ca_mask_total(const R* b, int nb, uint8_t* mask, double* btot, const
                      bb, pore, biofilm, bthr, fresh, red);
 }
 
-void launch_ca_push(const uint8_t* mask, const double* btot, const int* age, double* out, const Geom& g, int solid,
-                    int bb, int pore, double bmax, int half, unsigned long long seed, unsigned long long sweep,
-                    hipStream_t s) {
-  hipLaunchKernelGGL(ca_push_kernel, dim3(blocks_of(g.N)), dim3(256), 0, s, mask, btot, age, out, g, solid, bb, pore,
-                     bmax, half, seed, sweep);
+void launch_ca_push(const uint8_t* mask, const double* btot, const int* age, double* out, uint8_t* flags,
+                    const Geom& g, int solid, int bb, int pore, double bmax, int half, unsigned long long seed,
+                    unsigned long long sweep, hipStream_t s) {
+  hipLaunchKernelGGL(ca_push_kernel, dim3(blocks_of(g.N)), dim3(256), 0, s, mask, btot, age, out, flags, g, solid, bb,
+                     pore, bmax, half, seed, sweep);
 }
 
 template <typename R>
 void launch_ca_gather(const R* b_old, R* b, int nb, const uint8_t* mask, const double* btot, const double* out,
-                      const Geom& g, int solid, int bb, hipStream_t s) {
-  hipLaunchKernelGGL((ca_gather_kernel<R>), dim3(blocks_of(g.N)), dim3(256), 0, s, b_old, b, nb, mask, btot, out, g,
-                     solid, bb);
+                      const uint8_t* flags, const Geom& g, int solid, int bb, hipStream_t s) {
+  hipLaunchKernelGGL((ca_gather_kernel<R>), dim3(blocks_of(g.N)), dim3(256), 0, s, b_old, b, nb, mask, btot, out,
+                     flags, g, solid, bb);
 }
 
 void launch_ca_age(const int* age_in, int* age_out, const uint8_t* dpos, const double* btot, const Geom& g,
@@ -323,8 +332,8 @@ template void launch_ca_mask_total<float>(const float*, int, uint8_t*, double*, 
 template void launch_ca_mask_total<double>(const double*, int, uint8_t*, double*, const Geom&, int, int, int, int,
                                            double, bool, unsigned long long*, hipStream_t);
 template void launch_ca_gather<float>(const float*, float*, int, const uint8_t*, const double*, const double*,
-                                      const Geom&, int, int, hipStream_t);
+                                      const uint8_t*, const Geom&, int, int, hipStream_t);
 template void launch_ca_gather<double>(const double*, double*, int, const uint8_t*, const double*, const double*,
-                                       const Geom&, int, int, hipStream_t);
+                                       const uint8_t*, const Geom&, int, int, hipStream_t);
 
 }  // namespace clb

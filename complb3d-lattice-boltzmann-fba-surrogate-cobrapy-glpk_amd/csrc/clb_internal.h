@@ -157,12 +157,12 @@ void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g,
 template <typename R>
 void launch_ca_mask_total(const R* b, int nb, uint8_t* mask, double* btot, const Geom& g, int solid, int bb, int pore,
                           int biofilm, double bthr, bool fresh, unsigned long long* red, hipStream_t s);
-void launch_ca_push(const uint8_t* mask, const double* btot, const int* age, double* out, const Geom& g, int solid,
-                    int bb, int pore, double bmax, int half, unsigned long long seed, unsigned long long sweep,
-                    hipStream_t s);
+void launch_ca_push(const uint8_t* mask, const double* btot, const int* age, double* out, uint8_t* flags,
+                    const Geom& g, int solid, int bb, int pore, double bmax, int half, unsigned long long seed,
+                    unsigned long long sweep, hipStream_t s);
 template <typename R>
 void launch_ca_gather(const R* b_old, R* b, int nb, const uint8_t* mask, const double* btot, const double* out,
-                      const Geom& g, int solid, int bb, hipStream_t s);
+                      const uint8_t* flags, const Geom& g, int solid, int bb, hipStream_t s);
 void launch_ca_age(const int* age_in, int* age_out, const uint8_t* dpos, const double* btot, const Geom& g,
                    double bmax, unsigned long long* changed, hipStream_t s);
 
