@@ -117,6 +117,8 @@ def load():
         "clb_set_block_shape": [vp, c_int, c_int, c_int],
         "clb_autotune": [vp, c_int, vp],
         "clb_ca_init": [vp],
+        "clb_set_dissolution": [vp, c_int, c_int, vp, vp, c_int],
+        "clb_set_solute_field": [vp, c_int, vp],
         "clb_ca_redistribute": [vp, c_double, c_int, ctypes.c_ulonglong, vp],
         "clb_ca_update_age": [vp, c_double, c_int, vp],
         "clb_ca_download_age": [vp, vp],
@@ -373,6 +375,18 @@ class Lattice:
 
     def set_block_shape(self, kernel, bx, by):
         self._chk(self.lib.clb_set_block_shape(self.h, int(kernel), int(bx), int(by)), "set_block_shape")
+
+    # mineral dissolution
+    def set_dissolution(self, enable, phase_ids=(), substrates=(), surface_only=True):
+        ids = np.ascontiguousarray(phase_ids, dtype=np.int32)
+        sub = np.ascontiguousarray(substrates, dtype=np.int32)
+        self._chk(self.lib.clb_set_dissolution(self.h, int(enable), ids.size, _ptr(ids), _ptr(sub), int(surface_only)),
+                  "set_dissolution")
+
+    def set_solute_field(self, solute, c):
+        cc = _f64(c).ravel()
+        assert cc.size == self.N
+        self._chk(self.lib.clb_set_solute_field(self.h, int(solute), _ptr(cc)), "set_solute_field")
 
     # CA biofilm: biomass redistribution
     def ca_init(self):

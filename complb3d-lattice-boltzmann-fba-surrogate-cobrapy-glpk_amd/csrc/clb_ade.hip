@@ -256,6 +256,32 @@ __global__ void ade_reset_kernel(AdeArgs<R> a, bool next_odd, const double* c0) 
   write_rest_state(a, next_odd, x, y, z, n, c0);
 }
 
+// One solute's field at rest: an immobile solute's amount in every voxel's rest slot; a
+// mobile solute's at-rest equilibrium of c[n] in every fluid voxel's populations, in the
+// layout the next step (parity next_odd) reads.
+template <typename R>
+__global__ void solute_field_kernel(AdeArgs<R> a, bool next_odd, int s, const double* c) {
+  const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const Geom& g = a.geo;
+  if (n64 >= g.N) return;
+  const int n = (int)n64;
+  const double v = c[n] - (double)a.c_ref[s];
+  if (a.immobile[s]) {
+    a.g[(long long)(s * 7) * g.N + n] = (R)v;
+    return;
+  }
+  if (a.kind[a.mask[n]] != CLB_KIND_FLUID) return;
+  const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
+  const Nbr nb(x, y, z, g);
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    const R val = (R)(Q7::w(i) * v);
+    if (!next_odd) a.g[(long long)(s * 7 + i) * g.N + n] = val;
+    else if (nb.xok(-Q7::cx[i]))
+      a.g[(long long)(s * 7 + opp_dir(i)) * g.N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])] = val;
+  }
+}
+
 template <typename R, int S>
 void launch_ade_step_s(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t st) {
   dim3 grid((a.geo.nx + block.x - 1) / block.x, (a.geo.ny + block.y - 1) / block.y, a.geo.nz);
@@ -326,6 +352,12 @@ void launch_ade_reset(const AdeArgs<R>& a, bool next_odd, const double* c0, hipS
 }
 
 template <typename R>
+void launch_solute_field(const AdeArgs<R>& a, bool next_odd, int solute, const double* c, hipStream_t s) {
+  hipLaunchKernelGGL(solute_field_kernel<R>, dim3((unsigned)((a.geo.N + 255) / 256)), dim3(256), 0, s, a, next_odd,
+                     solute, c);
+}
+
+template <typename R>
 void launch_pending_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s) {
   const int bs = 256;
   const dim3 grid((unsigned)((a.geo.N + bs - 1) / bs));
@@ -337,6 +369,8 @@ template void launch_pending_flush<float>(const AdeArgs<float>&, bool, hipStream
 template void launch_pending_flush<double>(const AdeArgs<double>&, bool, hipStream_t);
 template void launch_ade_reset<float>(const AdeArgs<float>&, bool, const double*, hipStream_t);
 template void launch_ade_reset<double>(const AdeArgs<double>&, bool, const double*, hipStream_t);
+template void launch_solute_field<float>(const AdeArgs<float>&, bool, int, const double*, hipStream_t);
+template void launch_solute_field<double>(const AdeArgs<double>&, bool, int, const double*, hipStream_t);
 template void launch_ade_init<float>(const Geom&, int, float*, const double*, const float*, const int*, hipStream_t);
 template void launch_ade_init<double>(const Geom&, int, double*, const double*, const double*, const int*,
                                       hipStream_t);

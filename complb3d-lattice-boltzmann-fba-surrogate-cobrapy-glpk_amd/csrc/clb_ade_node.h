@@ -226,6 +226,12 @@ __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int y, int 
       default: biotic_kinetics<R, S, CLB_MAX_BIOMASS>(a, n, code, p); break;
     }
   }
+  if (react && a.water) {
+    // the water a neighbouring mineral sees: after the biotic kinetics, before the abiotic
+    // block (surfaceDissolutionKinetics3D shares the abiotic block's starting state)
+#pragma unroll
+    for (int s = 0; s < S; ++s) a.water[(long long)s * g.N + n] = (R)(pop_sum(&p[s * 7]) + a.c_ref[s]);
+  }
   if (KIN && react && a.abiotic) {
     bool gate = true;
     if (a.abiotic == 2) {

@@ -63,6 +63,12 @@ struct clb_ctx {
   dim3 spec_block{64, 8, 1};  // fused transport + speciation kernel (up to 512 threads)
 
   int h_immobile[CLB_MAX_SOLUTES] = {0};
+  // mineral dissolution (surfaceDissolutionKinetics3D)
+  bool dissol_on = false;
+  int phase_sub[CLB_MAX_PHASES];       // immobile solute of each phase id, -1: none
+  int dissol_surface = 1;
+  void* d_water = nullptr;             // S*N post-biotic concentrations (R)
+  size_t water_bytes = 0;
   int* d_immobile = nullptr;
   int abiotic = 0;
   uint8_t* d_phase = nullptr;
@@ -247,6 +253,7 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   a.cbuf = c->have_tableau ? c->d_cbuf : nullptr;
   a.pend_r = nullptr;
   a.pend_w = nullptr;
+  a.water = (c->dissol_on && c->d_water) ? (R*)c->d_water : nullptr;
   a.geo = c->geo;
   a.S = c->S;
   a.nb = c->nb;
@@ -435,6 +442,14 @@ int clear_pending(clb_ctx* c) {
   return CLB_OK;
 }
 
+DissolArgs dissol_args(const clb_ctx* c) {
+  DissolArgs d;
+  d.phase = c->d_phase;
+  for (int k = 0; k < CLB_MAX_PHASES; ++k) d.phase_sub[k] = c->phase_sub[k];
+  d.surface_only = c->dissol_surface;
+  return d;
+}
+
 // One coupled step on the stream, no host synchronisation (so it can be captured).
 template <typename R>
 void enqueue_step(clb_ctx* c) {
@@ -465,6 +480,7 @@ void enqueue_step(clb_ctx* c) {
       launch_spec_coupled(a, odd, c->tb, sa, c->spec_grid, c->stream);
     }
     count_launch(c, 2);
+    if (c->dissol_on) launch_dissolution<R>(a, dissol_args(c), odd, c->stream);
     c->ade_t++;
     return;
   }
@@ -486,6 +502,7 @@ void enqueue_step(clb_ctx* c) {
     }
     count_launch(c, 2);
   }
+  if (c->dissol_on) launch_dissolution<R>(a, dissol_args(c), odd, c->stream);
   c->ade_t++;
 }
 
@@ -499,6 +516,15 @@ void drop_graph(clb_ctx* c) {
 
 template <typename R>
 int step_t(clb_ctx* c, int nsteps) {
+  if (c->dissol_on) {
+    const size_t wb = (size_t)c->S * (size_t)c->geo.N * sizeof(R);
+    if (c->water_bytes != wb) {
+      int rc = dalloc(c, (uint8_t**)&c->d_water, wb);
+      c->water_bytes = rc ? 0 : wb;
+      if (rc) return rc;
+      drop_graph(c);  // captured launches hold the old buffer
+    }
+  }
   const bool fused = fused_spec(c);
   if (fused) {
     int rc = prepare_fused(c);
@@ -684,6 +710,7 @@ int clb_create(const clb_config* cfg, clb_ctx** out) {
   if (cfg->device < 0 || cfg->device >= ndev) return CLB_ERR_ARG;
   if (hipSetDevice(cfg->device) != hipSuccess) return CLB_ERR_HIP;
   clb_ctx* c = new clb_ctx();
+  for (int k = 0; k < CLB_MAX_PHASES; ++k) c->phase_sub[k] = -1;
   c->cfg = *cfg;
   c->geo = Geom{cfg->nx, cfg->ny, cfg->nz, N};
   c->fp32 = cfg->precision == CLB_PREC_FP32;
@@ -728,7 +755,7 @@ int clb_destroy(clb_ctx* c) {
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
                   c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
                   c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared,
-                  c->d_caflag};
+                  c->d_caflag, c->d_water};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1096,6 +1123,47 @@ int clb_set_speciation_mode(clb_ctx* c, int mode) {
   if (mode != c->spec_mode) drop_graph(c);
   c->spec_mode = mode;
   c->list_dirty = true;
+  return CLB_OK;
+}
+
+int clb_set_dissolution(clb_ctx* c, int enable, int nphase, const int* phase_ids, const int* substrates,
+                        int surface_only) {
+  if (!c || nphase < 0 || nphase >= CLB_MAX_PHASES || (nphase > 0 && (!phase_ids || !substrates)))
+    return CLB_ERR_ARG;
+  if (enable) {
+    if (!c->d_phase) return fail(c, CLB_ERR_STATE, "clb_set_phase first");
+    for (int k = 0; k < nphase; ++k) {
+      if (phase_ids[k] < 1 || phase_ids[k] >= CLB_MAX_PHASES) return fail(c, CLB_ERR_ARG, "phase id out of range");
+      if (substrates[k] < 0 || substrates[k] >= c->S || !c->h_immobile[substrates[k]])
+        return fail(c, CLB_ERR_ARG, "a phase's inventory must be an immobile solute");
+    }
+  }
+  int rc = flush_pending(c);
+  if (rc) return rc;
+  drop_graph(c);
+  for (int k = 0; k < CLB_MAX_PHASES; ++k) c->phase_sub[k] = -1;
+  for (int k = 0; k < nphase; ++k) c->phase_sub[phase_ids[k]] = substrates[k];
+  c->dissol_on = enable != 0 && nphase > 0;
+  c->dissol_surface = surface_only ? 1 : 0;
+  return CLB_OK;
+}
+
+int clb_set_solute_field(clb_ctx* c, int solute, const double* field) {
+  if (!c || !field || solute < 0) return CLB_ERR_ARG;
+  if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "clb_solutes_init first");
+  if (solute >= c->S) return CLB_ERR_ARG;
+  int rc = flush_pending(c);
+  if (rc) return rc;
+  double* d = nullptr;
+  rc = dalloc(c, &d, (size_t)c->geo.N);
+  if (rc) return rc;
+  CLB_CHECK_HIP(c, hipMemcpy(d, field, sizeof(double) * (size_t)c->geo.N, hipMemcpyHostToDevice));
+  const bool odd_next = (c->ade_t & 1) != 0;
+  if (c->fp32) launch_solute_field<float>(ade_args<float>(c), odd_next, solute, d, c->stream);
+  else launch_solute_field<double>(ade_args<double>(c), odd_next, solute, d, c->stream);
+  hipError_t e = hipStreamSynchronize(c->stream);
+  dfree(d);
+  if (e != hipSuccess) return fail(c, CLB_ERR_HIP, hipGetErrorString(e));
   return CLB_OK;
 }
 

@@ -53,6 +53,9 @@ struct AdeArgs {
   // pend_r holds the previous step's dC per solute and voxel, pend_w receives this step's.
   const R* pend_r;
   R* pend_w;
+  // Mineral dissolution (else null): concentrations after the biotic kinetics of every
+  // reactive voxel (the water a neighbouring mineral sees), S*N.
+  R* water;
   Geom geo;
   int S, nb;
   int bc_type[2 * CLB_MAX_SOLUTES];
@@ -123,6 +126,8 @@ void launch_node_conversion(const AdeArgs<R>& a, bool odd_next, uint8_t* phase, 
 template <typename R>
 void launch_ade_reset(const AdeArgs<R>& a, bool next_odd, const double* c0, hipStream_t s);
 template <typename R>
+void launch_solute_field(const AdeArgs<R>& a, bool next_odd, int solute, const double* c, hipStream_t s);
+template <typename R>
 void launch_flow_stats(const FlowArgs<R>& a, bool odd_next, double* acc, hipStream_t s);
 template <typename R>
 void launch_ade_step(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t s);
@@ -150,6 +155,18 @@ void launch_pending_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s);
 
 void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, hipStream_t s);
 int spec_occupancy_grid(int nc, int device);
+
+// Mineral dissolution (clb_dissol.hip): surfaceDissolutionKinetics3D with the compiled
+// defineDissolutionRate hook. phase_sub[id] = the immobile solute holding phase id's
+// inventory (-1: not dissolvable), ids 1..CLB_MAX_PHASES-1.
+#define CLB_MAX_PHASES 16
+struct DissolArgs {
+  const uint8_t* phase;       // N phase ids (0: none)
+  int phase_sub[CLB_MAX_PHASES];
+  int surface_only;           // 1: a phase voxel without an open face neighbour does not react
+};
+template <typename R>
+void launch_dissolution(const AdeArgs<R>& a, const DissolArgs& d, bool odd_done, hipStream_t s);
 
 // CA biomass redistribution (clb_ca.hip)
 void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, int pore,

@@ -26,6 +26,11 @@ namespace SecondOrderAbiotic {
     const int    ip  = 3;         // product
     const double dt  = 1.0;       // s, the transport time step
     const double MAX_RATE_FRACTION = 0.5;
+    // dissolution of solid phase 1: first order in the attacking solute and in the mineral
+    // left, releasing one mole of `irel` per mole dissolved (the worked example's form)
+    const double k_diss = 2.0e-2;  // L/mol/s
+    const int    iatt   = 1;       // the solute that attacks the mineral (consumed)
+    const int    irel   = 0;       // the solute the mineral releases
 }
 
 void defineAbioticRxnKinetics(std::vector<double> C, std::vector<double>& subsR, plb::plint mask)
@@ -45,6 +50,26 @@ void defineAbioticRxnKinetics(std::vector<double> C, std::vector<double>& subsR,
     subsR[ib] -= R;
     subsR[ip] += R;
 
+    for (std::size_t i = 0; i < subsR.size(); ++i)
+        if (std::isnan(subsR[i]) || std::isinf(subsR[i])) subsR[i] = 0.0;
+}
+
+/* Mineral dissolution hook (the reference's defineAbioticKinetics.hh:292 interface): given
+ * the water in contact with the mineral, the phase id and the inventory left, return the
+ * mineral's rate (negative: dissolving) and the released / consumed substrates (mol/L/s). */
+inline void defineDissolutionRate(plb::plint phaseId, const std::vector<double>& C, double mineral,
+                                  std::vector<double>& subsR, double& mineralR, plb::plint mask)
+{
+    using namespace SecondOrderAbiotic;
+    (void) mask;
+    mineralR = 0.0;
+    for (std::size_t i = 0; i < subsR.size(); ++i) subsR[i] = 0.0;
+    if (phaseId != 1 || C.size() < 2 || subsR.size() < 2) return;
+    const double rate = k_diss * std::max(C[iatt], 0.0) * std::max(mineral, 0.0);
+    mineralR = -rate;
+    subsR[irel] += rate;
+    subsR[iatt] -= rate;
+    if (std::isnan(mineralR) || std::isinf(mineralR)) mineralR = 0.0;
     for (std::size_t i = 0; i < subsR.size(); ++i)
         if (std::isnan(subsR[i]) || std::isinf(subsR[i])) subsR[i] = 0.0;
 }

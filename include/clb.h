@@ -179,6 +179,22 @@ int clb_download_phase(clb_ctx* ctx, uint8_t* phase);
  * mask code changed (nonzero => the caller re-solves the flow). */
 int clb_node_conversion(clb_ctx* ctx, int mineral_solute, int pore_code, int precip_code, int phase_id,
                         double precip_max, double reopen, long long* changed);
+/* Mineral dissolution at the wetted surface of declared solid phases
+ * (surfaceDissolutionKinetics3D, src/dissolutionVOP.hh:229-459) with the compiled
+ * defineDissolutionRate hook of the abiotic plugin (defineAbioticKinetics.hh:292): every
+ * step, each voxel of the phase lattice (clb_set_phase) whose phase id is declared here
+ * dissolves against the averaged water of its open face neighbours; its inventory (the
+ * immobile solute substrates[k]) shrinks by at most half per step, and the products go to
+ * the open neighbours, split evenly, in the same step. The water is taken after the biotic
+ * kinetics; the products join the step's increments before streaming. surface_only = 0
+ * lets enclosed phase voxels react too (their products are lost, as in the reference).
+ * Reopening is clb_node_conversion's reopen threshold. */
+int clb_set_dissolution(clb_ctx* ctx, int enable, int nphase, const int* phase_ids, const int* substrates,
+                        int surface_only);
+/* One solute's concentration field, at rest (initial_fill of a solid phase, a restart):
+ * an immobile solute's amount in every voxel, a mobile solute's at-rest equilibrium in
+ * every fluid voxel. */
+int clb_set_solute_field(clb_ctx* ctx, int solute, const double* c /*N*/);
 /* Velocity statistics of the flow solve (src/complab.cpp:625-671): stats[0] mean |u| over
  * fluid nodes, [1] max |u|, [2] mean u_x over the plane x = nx-2, [3] fluid nodes counted. */
 int clb_flow_stats(clb_ctx* ctx, double* stats /*4*/);

@@ -27,3 +27,14 @@ extern "C" __attribute__((visibility("default"))) void orc_abiotic_plugin(const 
   defineAbioticRxnKinetics(c, sr, (plb::plint)mask);
   for (int i = 0; i < ns; ++i) subsR[i] = sr[i];
 }
+
+// defineDissolutionRate (defineAbioticKinetics.hh:292 of the reference) for the checker.
+extern "C" __attribute__((visibility("default"))) void orc_dissolution_plugin(long long phase_id, const double* C,
+                                                                               int ns, double mineral, double* subsR,
+                                                                               double* mineralR, long long mask) {
+  std::vector<double> c(C, C + ns), sr(ns, 0.0);
+  double mr = 0.0;
+  defineDissolutionRate((plb::plint)phase_id, c, mineral, sr, mr, (plb::plint)mask);
+  for (int i = 0; i < ns; ++i) subsR[i] = sr[i];
+  *mineralR = mr;
+}

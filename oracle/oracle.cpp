@@ -610,6 +610,8 @@ typedef void (*orc_kinetics_fn)(const double* B, int nb, const double* C, int ns
                                 long long mask);
 // Abiotic hook (defineAbioticKinetics.hh:130-134): C[ns] in, subsR[ns] out, mask code.
 typedef void (*orc_abiotic_fn)(const double* C, int ns, double* subsR, long long mask);
+typedef void (*orc_dissol_fn)(long long phase_id, const double* C, int ns, double mineral, double* subsR,
+                              double* mineralR, long long mask);
 
 struct orc_ade_params {
   int nx, ny, nz, S;       // grid and number of solutes
@@ -639,7 +641,73 @@ struct orc_ade_params {
   // abiotic kinetics (run_abiotic_kinetics / surfaceAbioticKinetics3D + update)
   orc_abiotic_fn abiotic;    // null: none
   int abiotic_surface;       // 1: only voxels with a solid / wall face neighbour
+  // mineral dissolution (surfaceDissolutionKinetics3D), null: none
+  orc_dissol_fn dissol;
+  const uint8_t* phase;      // [N] phase ids
+  const int* phase_sub;      // [16] immobile solute of each phase id, -1: none
+  int dissol_surface;        // 1: enclosed phase voxels do not react
 };
+
+// surfaceDissolutionKinetics3D (src/dissolutionVOP.hh:229-459) over the post-collision
+// populations `post` (deviation form, density = sum + 1): phase voxels on the planes
+// 1 <= x <= nx-2 dissolve against the averaged water of their open face neighbours (no
+// wrap in y or z); at most half the inventory per step, products scaled alike and split
+// evenly among the open neighbours, added as dC t_i; the inventory falls by the same dC t_i.
+static void dissolution_pass(const orc_ade_params* P, const Grid& gr, const uint8_t* mask, double* post,
+                             const double* water) {
+  const int S = P->S;
+  const long N = gr.N;
+  static const int D[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  std::vector<double> w(S), sr(S);
+  for (int z = 0; z < gr.nz; ++z)
+    for (int y = 0; y < gr.ny; ++y)
+      for (int x = 1; x <= gr.nx - 2; ++x) {
+        const long m = gr.idx(x, y, z);
+        const int pid = P->phase[m];
+        if (pid <= 0 || pid >= 16) continue;
+        const int sm = P->phase_sub[pid];
+        if (sm < 0 || sm >= S) continue;
+        double mineral = 1.0;
+        for (int i = 0; i < 7; ++i) mineral += post[((long)sm * 7 + i) * N + m];
+        if (!(mineral > 1e-12)) continue;
+        long open[6];
+        int no = 0;
+        for (int d = 0; d < 6; ++d) {
+          const int xx = x + D[d][0], yy = y + D[d][1], zz = z + D[d][2];
+          if (xx < 1 || xx > gr.nx - 2 || yy < 0 || yy >= gr.ny || zz < 0 || zz >= gr.nz) continue;
+          const long q = gr.idx(xx, yy, zz);
+          const int mq = mask[q];
+          if (mq == P->solid_code || mq == P->bb_code || P->kind[mq] != KIND_FLUID) continue;
+          open[no++] = q;
+        }
+        if (no == 0 && P->dissol_surface) continue;
+        for (int s = 0; s < S; ++s) {
+          w[s] = 0.0;
+          for (int k = 0; k < no; ++k) w[s] += water[(long)s * N + open[k]];
+          if (no > 0) w[s] /= (double)no;
+          if (w[s] < 0.0) w[s] = 0.0;
+          sr[s] = 0.0;
+        }
+        double mr = 0.0;
+        P->dissol(pid, w.data(), S, mineral, sr.data(), &mr, (long long)mask[m]);
+        if (!(mr < 0.0)) continue;
+        double dm = mr * P->dt, scale = 1.0;
+        const double max_remove = 0.5 * mineral;
+        if (-dm > max_remove) scale = max_remove / (-dm);
+        dm *= scale;
+        if (!(-dm > 1e-12)) continue;
+        for (int i = 0; i < 7; ++i) post[((long)sm * 7 + i) * N + m] += dm * W7[i];
+        if (no == 0) continue;
+        for (int s = 0; s < S; ++s) {
+          if (s == sm) continue;
+          const double rel = sr[s] * P->dt * scale;
+          if (std::fabs(rel) <= 1e-12) continue;
+          const double per_face = rel / (double)no;
+          for (int k = 0; k < no; ++k)
+            for (int i = 0; i < 7; ++i) post[((long)s * 7 + i) * N + open[k]] += per_face * W7[i];
+        }
+      }
+}
 
 ORC_API void orc_ade_init(long N, int S, const double* c0, double* g) {
   for (int s = 0; s < S; ++s)
@@ -670,6 +738,22 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
   const long plane = (long)ny * nz;
   std::vector<double> gtmp((size_t)S * 7 * N);
   Tableau tb{P->nc, S, P->eqS, P->eqLogK, P->eqCompSpecies};
+  // with dissolution the step has three passes: update every voxel, dissolve, stream
+  const bool dissol = P->dissol != nullptr;
+  std::vector<double> post(dissol ? (size_t)S * 7 * N : 0), water(dissol ? (size_t)S * N : 0);
+  auto stream_node = [&](long n, int x, int y, int z, const double* p) {
+    for (int s = 0; s < S; ++s)
+      for (int i = 0; i < 7; ++i) {
+        if (P->immobile && P->immobile[s]) {
+          gtmp[((long)s * 7 + i) * N + n] = p[s * 7 + i];
+          continue;
+        }
+        int xx = x + C7[i][0];
+        if (xx < 0 || xx >= nx) continue;
+        int yy = gr.wy(y + C7[i][1]), zz = gr.wz(z + C7[i][2]);
+        gtmp[((long)s * 7 + i) * N + gr.idx(xx, yy, zz)] = p[s * 7 + i];
+      }
+  };
   long solves = 0, conv = 0, itsum = 0;
   for (int t = 0; t < nsteps; ++t) {
     std::memcpy(gtmp.data(), g, sizeof(double) * S * 7 * N);
@@ -734,6 +818,15 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
               for (int b = 0; b < P->nb; ++b) {
                 double dB = br[b] * P->dt;
                 if (dB > 1e-12 || dB < -1e-12) bmass[(long)b * N + n] += dB;
+              }
+            }
+            // ---- the water a neighbouring mineral sees (dissolution shares the abiotic
+            //      block's starting state)
+            if (react && dissol) {
+              for (int s = 0; s < S; ++s) {
+                double r = 0;
+                for (int i = 0; i < 7; ++i) r += p[s * 7 + i];
+                water[(long)s * N + n] = r + 1.0;
               }
             }
             // ---- abiotic kinetics (after the biotic update, src/complab.cpp:1318-1353)
@@ -806,18 +899,23 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
             }
           }
           // ---- stream (src/complab.cpp:1849); immobile solutes stay in place
-          for (int s = 0; s < S; ++s)
-            for (int i = 0; i < 7; ++i) {
-              if (P->immobile && P->immobile[s]) {
-                gtmp[((long)s * 7 + i) * N + n] = p[s * 7 + i];
-                continue;
-              }
-              int xx = x + C7[i][0];
-              if (xx < 0 || xx >= nx) continue;
-              int yy = gr.wy(y + C7[i][1]), zz = gr.wz(z + C7[i][2]);
-              gtmp[((long)s * 7 + i) * N + gr.idx(xx, yy, zz)] = p[s * 7 + i];
-            }
+          if (!dissol) {
+            stream_node(n, x, y, z, p.data());
+          } else {
+            for (int q = 0; q < S * 7; ++q) post[(long)q * N + n] = p[q];
+          }
         }
+    if (dissol) {
+      dissolution_pass(P, gr, mask, post.data(), water.data());
+      std::vector<double> p((size_t)S * 7);
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 0; x < nx; ++x) {
+            const long n = gr.idx(x, y, z);
+            for (int q = 0; q < S * 7; ++q) p[q] = post[(long)q * N + n];
+            stream_node(n, x, y, z, p.data());
+          }
+    }
     std::memcpy(g, gtmp.data(), sizeof(double) * S * 7 * N);
     // ---- Neumann faces: density of the adjacent interior plane after stream
     for (int s = 0; s < S; ++s)

@@ -75,6 +75,8 @@ class AdeParams(ctypes.Structure):
         ("mask_update", ctypes.c_int), ("bmass_threshold", ctypes.c_double),
         ("pore_code", ctypes.c_int), ("biofilm_code", ctypes.c_int),
         ("immobile", ctypes.c_void_p), ("abiotic", ctypes.c_void_p), ("abiotic_surface", ctypes.c_int),
+        ("dissol", ctypes.c_void_p), ("phase", ctypes.c_void_p), ("phase_sub", ctypes.c_void_p),
+        ("dissol_surface", ctypes.c_int),
     ]
 
 
@@ -176,6 +178,11 @@ def abiotic_plugin_ptr():
     return ctypes.cast(lib.orc_abiotic_plugin, ctypes.c_void_p).value
 
 
+def dissolution_plugin_ptr():
+    lib = load()
+    return ctypes.cast(lib.orc_dissolution_plugin, ctypes.c_void_p).value
+
+
 def abiotic_plugin(C, mask_code):
     lib = load()
     C = _f(C)
@@ -253,7 +260,7 @@ class AdeRun:
 
     def __init__(self, nx, ny, nz, mask, kind, omega, bc_type, bc_value, u, c0, solid_code=0, bb_code=1,
                  kinetics=False, dt=1.0, biomass=None, tableau=None, eq_tol=1e-10, eq_max_iter=200,
-                 mask_update=None, immobile=None, abiotic=0):
+                 mask_update=None, immobile=None, abiotic=0, dissolution=None):
         self.lib = load()
         self.nx, self.ny, self.nz = nx, ny, nz
         self.N = N = nx * ny * nz
@@ -296,8 +303,28 @@ class AdeRun:
         if abiotic:
             P.abiotic = abiotic_plugin_ptr()
             P.abiotic_surface = 1 if abiotic == 2 else 0
+        if dissolution is not None:
+            # (phase ids per voxel, {phase id: immobile solute}, surface_only)
+            ph, subs, surf = dissolution
+            self.phase = np.ascontiguousarray(ph, dtype=np.uint8).ravel().copy()
+            self.phase_sub = np.full(16, -1, dtype=np.int32)
+            for pid, sub in subs.items():
+                self.phase_sub[pid] = sub
+            P.dissol = dissolution_plugin_ptr()
+            P.phase, P.phase_sub = _p(self.phase).value, _p(self.phase_sub).value
+            P.dissol_surface = 1 if surf else 0
         self.P = P
         self.eq_stats = np.zeros(3, dtype=np.int64)
+
+    def set_field(self, solute, c):
+        """One solute's field at rest: an immobile solute in every voxel, a mobile one in the
+        fluid voxels (f_i - t_i = t_i (c - 1))."""
+        c = _f(c).ravel()
+        w = np.array([0.25] + [0.125] * 6)
+        sel = np.ones(self.N, dtype=bool) if self.immobile[solute] else (self.kind[self.mask] == KIND_FLUID)
+        for i in range(7):
+            blk = self.g[(solute * 7 + i) * self.N:(solute * 7 + i + 1) * self.N]
+            blk[sel] = w[i] * (c[sel] - 1.0)
 
     def run(self, nsteps):
         self.lib.orc_ade_run(ctypes.byref(self.P), _p(self.mask), _p(self.g), _p(self.bmass), _p(self.bc_rho),
