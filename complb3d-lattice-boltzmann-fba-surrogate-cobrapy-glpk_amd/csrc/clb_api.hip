@@ -43,7 +43,8 @@ struct clb_ctx {
   long long flow_t = 0;
   double rho_in = 1.0, rho_out = 1.0;
   bool flow_ready = false;
-  double* d_energy = nullptr;
+  double* d_energy = nullptr;   // energy sums + per-block partials of the flow step
+  long long energy_n = 0;
   dim3 flow_block{64, 4, 1};
 
   // solutes
@@ -293,12 +294,17 @@ int upload_tables(clb_ctx* c) {
 
 template <typename R>
 int flow_steps_t(clb_ctx* c, int nsteps, double* energy) {
+  if (energy) {
+    const long long need = 2 + 2 * flow_blocks(c->geo, c->flow_block);
+    if (c->energy_n != need) {
+      int rc = dalloc(c, &c->d_energy, (size_t)need);
+      c->energy_n = rc ? 0 : need;
+      if (rc) return rc;
+    }
+  }
   for (int t = 0; t < nsteps; ++t) {
     double* e = nullptr;
-    if (energy) {
-      launch_fill_f64(c->d_energy, 2, 0.0, c->stream);
-      e = c->d_energy;
-    }
+    if (energy) e = c->d_energy;  // every block writes its partial; the reduction writes [0..1]
     {
       Timed tm(c, 0);
       launch_flow_step<R>(flow_args<R>(c, e), (c->flow_t & 1) != 0, c->flow_block, c->stream);
@@ -724,6 +730,7 @@ int clb_create(const clb_config* cfg, clb_ctx** out) {
   if (!rc) rc = dalloc(c, &c->d_kind, 256);
   if (!rc) rc = dalloc(c, (uint8_t**)&c->d_flow_omega, 256 * rs);
   if (!rc) rc = dalloc(c, &c->d_energy, 2);
+  if (!rc) c->energy_n = 2;
   if (!rc) rc = dalloc(c, &c->d_counter, 4);
   if (!rc) rc = dalloc(c, &c->d_stats, 4);
   if (rc) {

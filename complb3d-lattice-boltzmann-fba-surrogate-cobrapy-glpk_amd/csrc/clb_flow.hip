@@ -121,7 +121,37 @@ __global__ void __launch_bounds__(256) flow_step_kernel(FlowArgs<R> a) {
       if (t < s) { se[t] += se[t + s]; sc[t] += sc[t + s]; }
       __syncthreads();
     }
-    if (t == 0) { atomicAdd(&a.energy[0], se[0]); atomicAdd(&a.energy[1], sc[0]); }
+    if (t == 0) {
+      // one partial per block (a deterministic sum, and no atomics on one address)
+      const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+      a.energy[2 + 2 * b] = se[0];
+      a.energy[3 + 2 * b] = sc[0];
+    }
+  }
+}
+
+// energy[0..1] = the sums of the per-block partials energy[2 + 2b], energy[3 + 2b], in a
+// fixed order (one block).
+__global__ void __launch_bounds__(256) energy_reduce_kernel(double* energy, long long nblocks) {
+  __shared__ double se[256], sc[256];
+  double e = 0.0, c = 0.0;
+  for (long long b = threadIdx.x; b < nblocks; b += 256) {
+    e += energy[2 + 2 * b];
+    c += energy[3 + 2 * b];
+  }
+  se[threadIdx.x] = e;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      se[threadIdx.x] += se[threadIdx.x + s];
+      sc[threadIdx.x] += sc[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    energy[0] = se[0];
+    energy[1] = sc[0];
   }
 }
 
@@ -245,6 +275,8 @@ void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s)
   if (a.energy) {
     if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((flow_step_kernel<R, false, true>), grid, block, 0, s, a);
+    hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(256), 0, s, a.energy,
+                       (long long)grid.x * grid.y * grid.z);
   } else {
     if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, false>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((flow_step_kernel<R, false, false>), grid, block, 0, s, a);
@@ -261,6 +293,10 @@ void launch_flow_velocity(const FlowArgs<R>& a, bool odd_next, double* u, double
 
 template void launch_flow_init<float>(const Geom&, float*, double, double, hipStream_t);
 template void launch_flow_init<double>(const Geom&, double*, double, double, hipStream_t);
+long long flow_blocks(const Geom& g, dim3 block) {
+  return (long long)((g.nx + block.x - 1) / block.x) * ((g.ny + block.y - 1) / block.y) * g.nz;
+}
+
 template void launch_flow_step<float>(const FlowArgs<float>&, bool, dim3, hipStream_t);
 template void launch_flow_step<double>(const FlowArgs<double>&, bool, dim3, hipStream_t);
 template void launch_flow_velocity<float>(const FlowArgs<float>&, bool, double*, double*, float*, hipStream_t);

@@ -37,7 +37,7 @@ struct FlowArgs {
   const R* omega;         // 256
   Geom g;
   R rhobar_in, rhobar_out;
-  double* energy;         // accumulator (sum |u|^2/2, count) or null
+  double* energy;         // [0..1] sum |u|^2/2 and count; [2 + 2b..] per-block partials; or null
 };
 
 template <typename R>
@@ -111,6 +111,9 @@ struct FusedSpecIO {
 // ------------------------------------------------------------------ launchers
 template <typename R>
 void launch_flow_init(const Geom& g, R* f, double rho_in, double rho_out, hipStream_t s);
+// Per-block energy partials a launch of this block shape writes (the energy buffer holds
+// 2 + 2 * flow_blocks doubles).
+long long flow_blocks(const Geom& g, dim3 block);
 template <typename R>
 void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s);
 template <typename R>
