@@ -117,6 +117,8 @@ def load():
         "clb_set_block_shape": [vp, c_int, c_int, c_int],
         "clb_autotune": [vp, c_int, vp],
         "clb_ca_init": [vp],
+        "clb_save_state": [vp, ctypes.c_char_p],
+        "clb_load_state": [vp, ctypes.c_char_p],
         "clb_set_dissolution": [vp, c_int, c_int, vp, vp, c_int],
         "clb_set_solute_field": [vp, c_int, vp],
         "clb_ca_redistribute": [vp, c_double, c_int, ctypes.c_ulonglong, vp],
@@ -162,6 +164,67 @@ def _f64(a):
 
 
 # ------------------------------------------------------------------ generators (host)
+# ------------------------------------------------------------------ files (host only)
+GEOM_X_FASTEST = "x_fastest"  # one integer per line, x fastest, then y, then z (makeGeometry.py)
+GEOM_X_SLICES = "x_slices"    # x slowest, each x slice z fastest (readGeometry's slice reads)
+
+
+def read_geometry(path, nx, ny, nz, order=GEOM_X_FASTEST):
+    """readGeometry (src/complab_functions.hh:127-167): the nx*ny*nz material numbers of a
+    geometry file, returned as the (nx + 2)*ny*nz mask of the lattice (x fastest), whose
+    planes x = 0 and x = nx + 1 copy x = 1 and x = nx, as the reference's reader does.
+
+    The reference's reader takes the file slice by slice along x (each slice in Palabos'
+    field order, z fastest: GEOM_X_SLICES), while its makeGeometry.py writes x fastest
+    (GEOM_X_FASTEST); `order` says which the file follows."""
+    v = np.loadtxt(path, dtype=np.int64, ndmin=1).ravel()
+    if v.size != nx * ny * nz:
+        raise ClbError(f"{path}: {v.size} values, expected {nx}*{ny}*{nz}")
+    if order == GEOM_X_FASTEST:
+        g = v.reshape(nz, ny, nx)
+    elif order == GEOM_X_SLICES:
+        g = v.reshape(nx, ny, nz).transpose(2, 1, 0)
+    else:
+        raise ClbError(f"unknown geometry order {order!r}")
+    if g.min() < 0 or g.max() > 255:
+        raise ClbError(f"{path}: material numbers must be 0..255")
+    out = np.empty((nz, ny, nx + 2), dtype=np.uint8)
+    out[:, :, 1:-1] = g
+    out[:, :, 0] = g[:, :, 0]
+    out[:, :, -1] = g[:, :, -1]
+    return out.ravel()
+
+
+def write_vti(path, nx, ny, nz, fields, crop_x=True):
+    """VTK XML ImageData of lattice fields (each (N,) or (N, 3), x fastest), base64 binary,
+    over the planes 1 <= x <= nx-2 when crop_x (the reference's writers, writeAdvVTI /
+    writeNsVTI / saveGeometry, write Box3D(1, nx-2, 0, ny-1, 0, nz-1)). float32 / float64 /
+    integer arrays keep their type."""
+    import base64
+    x0, x1 = (1, nx - 2) if crop_x else (0, nx - 1)
+    types = {np.dtype(np.float32): "Float32", np.dtype(np.float64): "Float64", np.dtype(np.int32): "Int32",
+             np.dtype(np.uint8): "UInt8", np.dtype(np.int64): "Int64"}
+    parts = []
+    for name, a in fields.items():
+        a = np.asarray(a)
+        comps = 1 if a.ndim == 1 else a.shape[1]
+        a = a.reshape(nz, ny, nx, comps)[:, :, x0:x1 + 1, :]
+        if a.dtype not in types:
+            a = a.astype(np.float64)
+        raw = np.ascontiguousarray(a).astype(a.dtype.newbyteorder("<")).tobytes()
+        blob = base64.b64encode(np.uint32(len(raw)).astype("<u4").tobytes() + raw).decode("ascii")
+        parts.append(f'        <DataArray type="{types[a.dtype]}" Name="{name}" NumberOfComponents="{comps}" '
+                     f'format="binary">\n{blob}\n        </DataArray>\n')
+    ext = f"0 {x1 - x0} 0 {ny - 1} 0 {nz - 1}"
+    with open(path, "w") as f:
+        f.write('<?xml version="1.0"?>\n')
+        f.write('<VTKFile type="ImageData" version="0.1" byte_order="LittleEndian" header_type="UInt32">\n')
+        f.write(f'  <ImageData WholeExtent="{ext}" Origin="0 0 0" Spacing="1 1 1">\n')
+        f.write(f'    <Piece Extent="{ext}">\n      <PointData>\n')
+        f.writelines(parts)
+        f.write("      </PointData>\n    </Piece>\n  </ImageData>\n</VTKFile>\n")
+
+
 def gen_sphere_pack(nx, ny, nz, rmin=3.0, rmax=5.0, porosity=0.45, seed=0, pore=2, solid=0, bb=1):
     lib = load()
     m = np.zeros(nx * ny * nz, dtype=np.uint8)
@@ -375,6 +438,13 @@ class Lattice:
 
     def set_block_shape(self, kernel, bx, by):
         self._chk(self.lib.clb_set_block_shape(self.h, int(kernel), int(bx), int(by)), "set_block_shape")
+
+    # checkpoint
+    def save_state(self, path):
+        self._chk(self.lib.clb_save_state(self.h, os.fsencode(path)), "save_state")
+
+    def load_state(self, path):
+        self._chk(self.lib.clb_load_state(self.h, os.fsencode(path)), "load_state")
 
     # mineral dissolution
     def set_dissolution(self, enable, phase_ids=(), substrates=(), surface_only=True):

@@ -1978,3 +1978,138 @@ int clb_ca_set_age(clb_ctx* c, const int* age) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- checkpoint
+namespace {
+
+constexpr char kChkMagic[8] = {'C', 'L', 'B', 'S', 'T', 'A', 'T', 'E'};
+constexpr int kChkVersion = 1;
+
+struct ChkHeader {
+  char magic[8];
+  int version, nx, ny, nz, S, nb, fp32;
+  int has_flow, has_ade, has_phase, has_ca;
+  long long flow_t, ade_t;
+  unsigned long long ca_sweep;
+};
+
+struct ChkBlock {
+  void* dev;
+  size_t bytes;
+};
+
+// The evolving state, in file order: mask, phase, flow populations, ADE velocity and
+// populations, biomass, CA age and total lattices.
+std::vector<ChkBlock> chk_blocks(clb_ctx* c, const ChkHeader& h) {
+  const size_t N = (size_t)c->geo.N, rs = h.fp32 ? sizeof(float) : sizeof(double);
+  std::vector<ChkBlock> v;
+  v.push_back({c->d_mask, N});
+  if (h.has_phase) v.push_back({c->d_phase, N});
+  if (h.has_flow) v.push_back({c->d_f, 19 * N * rs});
+  if (h.has_ade) {
+    v.push_back({c->d_u, 3 * N * rs});
+    v.push_back({c->d_g, (size_t)h.S * 7 * N * rs});
+  }
+  if (h.nb > 0) v.push_back({c->d_b, (size_t)h.nb * N * rs});
+  if (h.has_ca) {
+    v.push_back({c->d_age, N * sizeof(int)});
+    v.push_back({c->d_btot, N * sizeof(double)});
+  }
+  return v;
+}
+
+ChkHeader chk_header_of(const clb_ctx* c) {
+  ChkHeader h{};
+  std::memcpy(h.magic, kChkMagic, sizeof(kChkMagic));
+  h.version = kChkVersion;
+  h.nx = c->geo.nx;
+  h.ny = c->geo.ny;
+  h.nz = c->geo.nz;
+  h.S = c->ade_ready ? c->S : 0;
+  h.nb = (c->d_b && c->nb > 0) ? c->nb : 0;
+  h.fp32 = c->fp32 ? 1 : 0;
+  h.has_flow = c->flow_ready ? 1 : 0;
+  h.has_ade = c->ade_ready ? 1 : 0;
+  h.has_phase = c->d_phase ? 1 : 0;
+  h.has_ca = c->ca_ready ? 1 : 0;
+  h.flow_t = c->flow_t;
+  h.ade_t = c->ade_t;
+  h.ca_sweep = c->ca_sweep;
+  return h;
+}
+
+}  // namespace
+
+extern "C" {
+
+int clb_save_state(clb_ctx* c, const char* path) {
+  if (!c || !path) return CLB_ERR_ARG;
+  int rc = flush_pending(c);  // the deferred increments belong in the populations
+  if (rc) return rc;
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  const ChkHeader h = chk_header_of(c);
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(c, CLB_ERR_ARG, std::string("cannot write ") + path);
+  bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1;
+  std::vector<uint8_t> host;
+  for (const ChkBlock& b : chk_blocks(c, h)) {
+    if (!ok) break;
+    host.resize(b.bytes);
+    if (hipMemcpy(host.data(), b.dev, b.bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+      std::fclose(f);
+      return fail(c, CLB_ERR_HIP, "checkpoint: device read failed");
+    }
+    ok = std::fwrite(host.data(), 1, b.bytes, f) == b.bytes;
+  }
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? CLB_OK : fail(c, CLB_ERR_ARG, std::string("short write to ") + path);
+}
+
+int clb_load_state(clb_ctx* c, const char* path) {
+  if (!c || !path) return CLB_ERR_ARG;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(c, CLB_ERR_ARG, std::string("cannot read ") + path);
+  ChkHeader h{};
+  if (std::fread(&h, sizeof(h), 1, f) != 1 || std::memcmp(h.magic, kChkMagic, sizeof(kChkMagic)) != 0 ||
+      h.version != kChkVersion) {
+    std::fclose(f);
+    return fail(c, CLB_ERR_ARG, "not a checkpoint of this library");
+  }
+  // the context must have been set up the same way (the checkpoint holds the evolving state)
+  const ChkHeader mine = chk_header_of(c);
+  if (h.nx != mine.nx || h.ny != mine.ny || h.nz != mine.nz || h.S != mine.S || h.nb != mine.nb ||
+      h.fp32 != mine.fp32 || h.has_flow != mine.has_flow || h.has_ade != mine.has_ade ||
+      h.has_phase != mine.has_phase || h.has_ca != mine.has_ca) {
+    std::fclose(f);
+    return fail(c, CLB_ERR_STATE, "checkpoint does not match the context's set-up (sizes, solutes, biomass, "
+                                  "precision, or which of flow / solutes / phase / CA are set up)");
+  }
+  int rc = clear_pending(c);  // the populations are about to be overwritten
+  if (rc) {
+    std::fclose(f);
+    return rc;
+  }
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<uint8_t> host;
+  for (const ChkBlock& b : chk_blocks(c, h)) {
+    host.resize(b.bytes);
+    if (std::fread(host.data(), 1, b.bytes, f) != b.bytes) {
+      std::fclose(f);
+      return fail(c, CLB_ERR_ARG, "checkpoint is truncated");
+    }
+    if (hipMemcpy(b.dev, host.data(), b.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+      std::fclose(f);
+      return fail(c, CLB_ERR_HIP, "checkpoint: device write failed");
+    }
+  }
+  std::fclose(f);
+  c->flow_t = h.flow_t;
+  c->ade_t = h.ade_t;
+  c->ca_sweep = h.ca_sweep;
+  c->list_dirty = true;
+  c->fluid_dirty = true;
+  drop_graph(c);  // a captured segment starts on the parity it was captured for
+  return CLB_OK;
+}
+
+}  // extern "C"

@@ -19,6 +19,9 @@
 //    592-1860) with calls on the device-resident state. Its methods throw complab_gpu::Error
 //    on any CLB_ERR_* return, carrying clb_last_error().
 //
+//  * complab_gpu::readGeometry / writeVTI: the geometry file the driver reads and the VTK
+//    ImageData files it writes (host only; they need no GPU).
+//
 // Header-only; link with -lclb. Requires a GPU at construction: there is no CPU fallback.
 #pragma once
 
@@ -26,6 +29,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -151,6 +155,8 @@ class Lattice {
   }
   void setGraph(bool enable, int stepsPerGraph) { chk(clb_set_graph(c_, enable ? 1 : 0, stepsPerGraph), "clb_set_graph"); }
   void step(int n) { chk(clb_step(c_, n), "clb_step"); }
+  void saveState(const std::string& path) { chk(clb_save_state(c_, path.c_str()), "clb_save_state"); }
+  void loadState(const std::string& path) { chk(clb_load_state(c_, path.c_str()), "clb_load_state"); }
   // Fastest launch shapes (and fused budget) for the current state; the state is kept.
   clb_tune_report autotune(int steps = 10) {
     clb_tune_report r{};
@@ -399,5 +405,105 @@ class EquilibriumChemistry {
   plint totalSolves_ = 0, totalConverged_ = 0, totalDiverged_ = 0;
   std::string lastError_;
 };
+
+
+// ------------------------------------------------------------------ files (host only)
+enum class GeometryOrder {
+  XFastest,  // one integer per line, x fastest, then y, then z (the reference's makeGeometry.py)
+  XSlices    // x slowest, each x slice z fastest (readGeometry's slice-by-slice reads)
+};
+
+// readGeometry (src/complab_functions.hh:127-167): nx*ny*nz material numbers from a text
+// file, returned as the (nx + 2)*ny*nz lattice mask (x fastest) whose planes x = 0 and
+// x = nx + 1 copy x = 1 and x = nx. The reference's reader consumes the file in slices
+// along x while its makeGeometry.py writes x fastest, so the caller says which order the
+// file has. Throws Error(CLB_ERR_ARG, ...) on a missing file, a wrong count or a code
+// outside 0..255.
+inline std::vector<uint8_t> readGeometry(const std::string& path, int nx, int ny, int nz,
+                                         GeometryOrder order = GeometryOrder::XFastest) {
+  std::ifstream in(path);
+  if (!in) throw Error(CLB_ERR_ARG, "cannot open geometry file " + path);
+  const long long n = (long long)nx * ny * nz, NX = nx + 2;
+  std::vector<uint8_t> m((size_t)(NX * ny * nz));
+  long long k = 0, v = 0;
+  while (k < n && (in >> v)) {
+    if (v < 0 || v > 255) throw Error(CLB_ERR_ARG, path + ": material number outside 0..255");
+    long long x, y, z;
+    if (order == GeometryOrder::XFastest) {
+      x = k % nx;
+      y = (k / nx) % ny;
+      z = k / ((long long)nx * ny);
+    } else {
+      z = k % nz;
+      y = (k / nz) % ny;
+      x = k / ((long long)nz * ny);
+    }
+    m[(size_t)(x + 1 + NX * (y + (long long)ny * z))] = (uint8_t)v;
+    ++k;
+  }
+  if (k != n || (in >> v)) throw Error(CLB_ERR_ARG, path + ": expected " + std::to_string(n) + " values");
+  for (long long z = 0; z < nz; ++z)
+    for (long long y = 0; y < ny; ++y) {
+      const long long row = NX * (y + (long long)ny * z);
+      m[(size_t)row] = m[(size_t)(row + 1)];
+      m[(size_t)(row + NX - 1)] = m[(size_t)(row + NX - 2)];
+    }
+  return m;
+}
+
+namespace detail {
+inline std::string base64(const std::vector<uint8_t>& b) {
+  static const char* t = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+  std::string o;
+  o.reserve((b.size() + 2) / 3 * 4);
+  for (size_t i = 0; i < b.size(); i += 3) {
+    const uint32_t v = (uint32_t)b[i] << 16 | (i + 1 < b.size() ? (uint32_t)b[i + 1] << 8 : 0u) |
+                       (i + 2 < b.size() ? (uint32_t)b[i + 2] : 0u);
+    o += t[(v >> 18) & 63];
+    o += t[(v >> 12) & 63];
+    o += (i + 1 < b.size()) ? t[(v >> 6) & 63] : '=';
+    o += (i + 2 < b.size()) ? t[v & 63] : '=';
+  }
+  return o;
+}
+}  // namespace detail
+
+// One field of a VTK file: `comps` values per voxel, x fastest.
+struct VtiField {
+  std::string name;
+  const double* data;
+  int comps;
+};
+
+// VTK XML ImageData (Float64, base64 binary, little endian) of lattice fields over the
+// planes 1 <= x <= nx-2, the box the reference's writers output (writeAdvVTI / writeNsVTI /
+// saveGeometry, src/complab_functions.hh:86-125). Throws Error(CLB_ERR_ARG, ...) when the
+// file cannot be written.
+inline void writeVTI(const std::string& path, int nx, int ny, int nz, const std::vector<VtiField>& fields) {
+  std::ofstream out(path);
+  if (!out) throw Error(CLB_ERR_ARG, "cannot write " + path);
+  const std::string ext = "0 " + std::to_string(nx - 3) + " 0 " + std::to_string(ny - 1) + " 0 " + std::to_string(nz - 1);
+  out << "<?xml version=\"1.0\"?>\n"
+      << "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\" header_type=\"UInt32\">\n"
+      << "  <ImageData WholeExtent=\"" << ext << "\" Origin=\"0 0 0\" Spacing=\"1 1 1\">\n"
+      << "    <Piece Extent=\"" << ext << "\">\n      <PointData>\n";
+  for (const VtiField& f : fields) {
+    std::vector<uint8_t> raw(4);
+    for (int z = 0; z < nz; ++z)
+      for (int y = 0; y < ny; ++y)
+        for (int x = 1; x <= nx - 2; ++x)
+          for (int c = 0; c < f.comps; ++c) {
+            const double v = f.data[((size_t)x + (size_t)nx * (y + (size_t)ny * z)) * f.comps + c];
+            const uint8_t* p = reinterpret_cast<const uint8_t*>(&v);
+            raw.insert(raw.end(), p, p + sizeof(double));
+          }
+    const uint32_t nbytes = (uint32_t)(raw.size() - 4);
+    for (int i = 0; i < 4; ++i) raw[i] = (uint8_t)(nbytes >> (8 * i));
+    out << "        <DataArray type=\"Float64\" Name=\"" << f.name << "\" NumberOfComponents=\"" << f.comps
+        << "\" format=\"binary\">\n" << detail::base64(raw) << "\n        </DataArray>\n";
+  }
+  out << "      </PointData>\n    </Piece>\n  </ImageData>\n</VTKFile>\n";
+  if (!out) throw Error(CLB_ERR_ARG, "short write to " + path);
+}
 
 }  // namespace complab_gpu

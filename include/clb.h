@@ -195,6 +195,14 @@ int clb_set_dissolution(clb_ctx* ctx, int enable, int nphase, const int* phase_i
  * an immobile solute's amount in every voxel, a mobile solute's at-rest equilibrium in
  * every fluid voxel. */
 int clb_set_solute_field(clb_ctx* ctx, int solute, const double* c /*N*/);
+/* Checkpoint (the reference's saveBinaryBlock / loadBinaryBlock checkpoints,
+ * src/complab.cpp:1122-1132, 1238-1247, in one file of this library's own format): the
+ * evolving state -- mask, phase lattice, flow and solute populations (deferred increments
+ * folded in), ADE velocity, biomass, CA age and total lattices -- and the step counters.
+ * clb_load_state needs a context set up the same way (sizes, solutes, biomass, precision,
+ * and which of flow / solutes / phase lattice / CA exist) and continues bit for bit. */
+int clb_save_state(clb_ctx* ctx, const char* path);
+int clb_load_state(clb_ctx* ctx, const char* path);
 /* Velocity statistics of the flow solve (src/complab.cpp:625-671): stats[0] mean |u| over
  * fluid nodes, [1] max |u|, [2] mean u_x over the plane x = nx-2, [3] fluid nodes counted. */
 int clb_flow_stats(clb_ctx* ctx, double* stats /*4*/);
