@@ -121,6 +121,9 @@ def load():
         "clb_load_state": [vp, ctypes.c_char_p],
         "clb_set_dissolution": [vp, c_int, c_int, vp, vp, c_int],
         "clb_set_solute_field": [vp, c_int, vp],
+        "clb_set_surrogate": [vp, c_int, vp, vp, vp, vp, vp, vp, vp, vp, c_int],
+        "clb_set_surrogate_net": [vp, c_int, c_int, vp, c_int, vp, vp, vp, vp, c_double, c_double, c_double,
+                                  c_double, c_int, c_int, vp],
         "clb_ca_redistribute": [vp, c_double, c_int, ctypes.c_ulonglong, vp],
         "clb_ca_update_age": [vp, c_double, c_int, vp],
         "clb_ca_download_age": [vp, vp],
@@ -223,6 +226,119 @@ def write_vti(path, nx, ny, nz, fields, crop_x=True):
         f.write(f'    <Piece Extent="{ext}">\n      <PointData>\n')
         f.writelines(parts)
         f.write("      </PointData>\n    </Piece>\n  </ImageData>\n</VTKFile>\n")
+
+
+def _strip_comments(text):
+    import re
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    return re.sub(r"//[^\n]*", " ", text)
+
+
+def _braced(text, start):
+    """The text between the brace at `start` and its match."""
+    depth = 0
+    for i in range(start, len(text)):
+        if text[i] == "{":
+            depth += 1
+        elif text[i] == "}":
+            depth -= 1
+            if depth == 0:
+                return text[start + 1:i]
+    raise ClbError("unbalanced braces in surrogate header")
+
+
+def _parse_network(text):
+    import re
+    num = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?"
+
+    def values(name):
+        m = re.search(r"\b" + name + r"\s*(?:\[[^\]]*\]\s*)*=\s*(\{|" + num + r")", text)
+        if not m:
+            return None
+        if m.group(1) != "{":
+            return [[float(m.group(1))]]
+        body = _braced(text, m.start(1))
+        rows = re.findall(r"\{([^{}]*)\}", body)
+        if not rows:
+            rows = [body]
+        return [[float(v) for v in re.findall(num, r)] for r in rows]
+
+    def scalar(name, default=None):
+        v = values(name)
+        if v is None:
+            if default is None:
+                raise ClbError(f"surrogate header: {name} not found")
+            return default
+        return v[0][0]
+
+    layers = []
+    W = values("IW1_1")
+    if W is None:
+        raise ClbError("surrogate header: no IW1_1 (input weights) found")
+    k = 1
+    while W is not None:
+        b = values(f"b{k}")
+        if b is None:
+            raise ClbError(f"surrogate header: b{k} not found")
+        b = [v for row in b for v in row]
+        if len(W) != len(b) or len({len(r) for r in W}) != 1:
+            raise ClbError(f"surrogate header: layer {k} shapes do not match")
+        layers.append((np.array(W, dtype=np.float64), np.array(b, dtype=np.float64)))
+        W = values(f"LW{k + 1}_{k}")
+        k += 1
+    n_in = layers[0][0].shape[1]
+    for (Wa, _), (Wb, _) in zip(layers, layers[1:]):
+        if Wb.shape[1] != Wa.shape[0]:
+            raise ClbError("surrogate header: consecutive layers do not chain")
+    if layers[-1][0].shape[0] != 1:
+        raise ClbError("surrogate header: the output layer must have one neuron")
+    xo = values("x_offset")
+    if xo is not None:  # the trainer's generated header: arrays, inputs in substrate order
+        x_offset = [v for r in xo for v in r]
+        x_gain = [v for r in values("x_gain") for v in r]
+        in_sub = list(range(n_in))
+    else:  # the model file: one scalar per input, input k taken from Fin[iM][j]
+        x_offset = [scalar(f"x_offset{j}") for j in range(n_in)]
+        x_gain = [scalar(f"x_gain{j}") for j in range(n_in)]
+        in_sub = list(range(n_in))
+        for kk, j in re.findall(r"Xp0\s*\[\s*(\d+)\s*\]\s*=\s*\(\s*Fin\s*\[[^\]]*\]\s*\[\s*(\d+)\s*\]", text):
+            in_sub[int(kk)] = int(j)
+    m = re.search(r"\blogOutput\s*=\s*(true|false)", text)
+    return {
+        "n_in": n_in,
+        "in_sub": in_sub,
+        "widths": [Wl.shape[0] for Wl, _ in layers],
+        "params": np.concatenate([np.concatenate([Wl.ravel(), bl]) for Wl, bl in layers]),
+        "x_offset": x_offset,
+        "x_gain": x_gain,
+        "x_ymin": scalar("x_ymin", -1.0),
+        "y_ymin": scalar("y_ymin", -1.0),
+        "y_gain": scalar("y_gain"),
+        "y_offset": scalar("y_offset", 0.0),
+        "log_output": bool(m and m.group(1) == "true"),
+    }
+
+
+def read_surrogate_header(path):
+    """The networks of a surrogate model header, as {microbe id: network} for
+    Lattice.set_surrogate_net. Reads both forms the reference produces: the model file
+    (surrogateModel.hh:252-500: std::vector weight tables IW1_1, LW2_1, ..., b1, ...,
+    scalar x_offsetK / x_gainK, one block per `microbeId == k` branch, inputs named by
+    its Fin[iM][j] lines) and the trainer's generated header
+    (surrogate_training/trainSurrogate.py:245-395: array weights, x_offset[] / x_gain[],
+    logOutput, inputs in substrate order, one network, returned as id 0)."""
+    import re
+    with open(path) as fh:
+        text = _strip_comments(fh.read())
+    marks = [(m.start(), int(m.group(1))) for m in re.finditer(r"microbeId\s*==\s*(\d+)", text)]
+    if not marks:
+        return {0: _parse_network(text)}
+    nets = {}
+    for i, (pos, mid) in enumerate(marks):
+        end = marks[i + 1][0] if i + 1 < len(marks) else len(text)
+        if re.search(r"\bIW1_1\s*(?:\[[^\]]*\]\s*)*=", text[pos:end]):
+            nets[mid] = _parse_network(text[pos:end])
+    return nets
 
 
 def gen_sphere_pack(nx, ny, nz, rmin=3.0, rmax=5.0, porosity=0.45, seed=0, pore=2, solid=0, bb=1):
@@ -452,6 +568,44 @@ class Lattice:
         sub = np.ascontiguousarray(substrates, dtype=np.int32)
         self._chk(self.lib.clb_set_dissolution(self.h, int(enable), ids.size, _ptr(ids), _ptr(sub), int(surface_only)),
                   "set_dissolution")
+
+    def set_surrogate(self, ids, Kc, decay, vmax=None, max_uptake=None, molar_mass=None, fix_conc=None,
+                      fix_lb=None, use_totals=False):
+        """Microbes grown by the surrogate processor (clb_set_surrogate); ids = their biomass
+        fields; Kc, vmax, max_uptake: (n, S); decay, molar_mass: (n,); fix_*: (S,)."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32).ravel()
+        n = ids.size
+        S = self.S
+
+        def mat(a):
+            return None if a is None else _f64(np.reshape(a, (n, S)))
+
+        def flags(a):
+            return None if a is None else np.ascontiguousarray(a, dtype=np.int32).ravel()
+        kc, vm, mu = mat(Kc), mat(vmax), mat(max_uptake)
+        dec = _f64(np.reshape(decay, (n,))) if n else None
+        mw = None if molar_mass is None else _f64(np.reshape(molar_mass, (n,)))
+        fc, fl = flags(fix_conc), flags(fix_lb)
+        self._chk(self.lib.clb_set_surrogate(self.h, n, _ptr(ids), _ptr(kc), _ptr(dec), _ptr(vm), _ptr(mu), _ptr(mw),
+                                             _ptr(fc), _ptr(fl), int(bool(use_totals))), "set_surrogate")
+
+    def set_surrogate_net(self, microbe, net, closure=0, yields=None):
+        """Microbe `microbe`'s network (a dict as read_surrogate_header returns), or None to
+        remove it; closure 0 / 1 / 2 with yields (S,) in gDW per mmol."""
+        if net is None:
+            self._chk(self.lib.clb_set_surrogate_net(self.h, int(microbe), 0, None, 0, None, None, None, None, 0.0,
+                                                     0.0, 1.0, 0.0, 0, 0, None), "set_surrogate_net")
+            return
+        ins = np.ascontiguousarray(net["in_sub"], dtype=np.int32)
+        wid = np.ascontiguousarray(net["widths"], dtype=np.int32)
+        prm = _f64(net["params"]).ravel()
+        xo, xg = _f64(net["x_offset"]).ravel(), _f64(net["x_gain"]).ravel()
+        y = None if yields is None else _f64(yields).ravel()
+        self._chk(self.lib.clb_set_surrogate_net(self.h, int(microbe), int(net["n_in"]), _ptr(ins), wid.size, _ptr(wid),
+                                                 _ptr(prm), _ptr(xo), _ptr(xg), float(net["x_ymin"]),
+                                                 float(net["y_ymin"]), float(net["y_gain"]), float(net["y_offset"]),
+                                                 int(bool(net.get("log_output"))), int(closure), _ptr(y)),
+                  "set_surrogate_net")
 
     def set_solute_field(self, solute, c):
         cc = _f64(c).ravel()

@@ -285,7 +285,7 @@ __global__ void solute_field_kernel(AdeArgs<R> a, bool next_odd, int s, const do
 template <typename R, int S>
 void launch_ade_step_s(const AdeArgs<R>& a, bool odd, dim3 block, hipStream_t st) {
   dim3 grid((a.geo.nx + block.x - 1) / block.x, (a.geo.ny + block.y - 1) / block.y, a.geo.nz);
-  const bool kin = (a.kinetics != 0) || (a.abiotic != 0);
+  const bool kin = (a.kinetics != 0) || (a.abiotic != 0) || (a.srg_inc != nullptr);
   const bool cb = a.cbuf != nullptr;
 #define CLB_ADE_LAUNCH(O, K, C) hipLaunchKernelGGL((ade_step_kernel<R, S, O, K, C>), grid, block, 0, st, a)
   if (odd) {

@@ -141,7 +141,8 @@ __device__ __forceinline__ void inject(const AdeArgs<R>& a, int s, R* p, double 
   for (int i = 1; i < 7; ++i) p[s * 7 + i] += (R)(imm ? 0.0 : d * Q7::w(i));
 }
 
-// The compiled biotic hook for NB biomass pools: the vectors the plugin receives have
+// The compiled biotic hook for NB biomass pools (when set), plus the surrogate processor's
+// increments (when on): the vectors the plugin receives have
 // compile-time sizes, so its loops over .size() unroll and every element stays in a
 // register (a run-time size would force the vectors, and with them the populations, into
 // scratch memory).
@@ -161,12 +162,22 @@ __device__ __forceinline__ void biotic_kinetics(const AdeArgs<R>& a, int n, uint
     if (b0 < 1e-12) b0 = 0;
     Bv[b] = b0;
   }
-  defineRxnKinetics(Bv, C, sr, br, (plb::plint)code);
+  if (a.kinetics) defineRxnKinetics(Bv, C, sr, br, (plb::plint)code);
+  // the surrogate processor's increments join the kinetics' in the same dC / dB lattices,
+  // which update_rxnLattices applies when their sum exceeds 1e-12
+  // (src/complab3d_processors_surrogate.hh:262-283, src/complab.cpp:1294-1313)
 #pragma unroll
-  for (int s = 0; s < S; ++s) inject(a, s, p, sr[s] * a.dt);
+  for (int s = 0; s < S; ++s) {
+    double d = sr[s] * a.dt;
+    if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
+    if (a.srg_inc) d += (double)a.srg_inc[(long long)s * g.N + n];
+    inject(a, s, p, d);
+  }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
-    const double dB = br[b] * a.dt;
+    double dB = br[b] * a.dt;
+    if (!(dB > 1e-12 || dB < -1e-12)) dB = 0.0;
+    if (a.srg_inc) dB += (double)a.srg_inc[(long long)(S + b) * g.N + n];
     if (dB > 1e-12 || dB < -1e-12) {
       R* bp = &a.bmass[(long long)b * g.N + n];
       *bp = (R)((double)*bp + dB);
@@ -217,7 +228,7 @@ __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int y, int 
     }
   }
   const bool react = (FACE == 0 && code != a.solid_code && code != a.bb_code);
-  if (KIN && react && a.kinetics) {
+  if (KIN && react && (a.kinetics || a.srg_inc)) {
     switch (a.nb) {
       case 0: biotic_kinetics<R, S, 0>(a, n, code, p); break;
       case 1: biotic_kinetics<R, S, 1>(a, n, code, p); break;

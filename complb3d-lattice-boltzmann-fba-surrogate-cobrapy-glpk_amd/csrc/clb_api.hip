@@ -70,6 +70,13 @@ struct clb_ctx {
   int dissol_surface = 1;
   void* d_water = nullptr;             // S*N post-biotic concentrations (R)
   size_t water_bytes = 0;
+  // surrogate metabolic model (run_surrogate3D)
+  bool srg_on = false;
+  bool srg_dirty = false;             // host tables not yet on the device
+  SrgTables h_srg{};
+  SrgTables* d_srg = nullptr;
+  void* d_srg_inc = nullptr;          // (S + nb) * N increments of the step (R)
+  size_t srg_inc_bytes = 0;
   int* d_immobile = nullptr;
   int abiotic = 0;
   uint8_t* d_phase = nullptr;
@@ -255,6 +262,7 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   a.pend_r = nullptr;
   a.pend_w = nullptr;
   a.water = (c->dissol_on && c->d_water) ? (R*)c->d_water : nullptr;
+  a.srg_inc = nullptr;  // set by the step that ran the surrogate kernel
   a.geo = c->geo;
   a.S = c->S;
   a.nb = c->nb;
@@ -461,9 +469,18 @@ template <typename R>
 void enqueue_step(clb_ctx* c) {
   const bool odd = (c->ade_t & 1) != 0;
   AdeArgs<R> a = ade_args<R>(c);
-  if (fused_spec(c)) {
+  const bool fused = fused_spec(c);
+  if (fused) {
     a.pend_r = pend_buf<R>(c, odd, false);
     a.pend_w = pend_buf<R>(c, odd, true);
+  }
+  if (c->srg_on) {
+    // the surrogate processor reads the state the transport kernel is about to load
+    Timed tm(c, 1);
+    launch_surrogate<R>(a, c->d_srg, (R*)c->d_srg_inc, odd, c->stream);
+    a.srg_inc = (const R*)c->d_srg_inc;
+  }
+  if (fused) {
     c->pend_dirty = true;
     launch_fill_u32(c->d_counter + 2, 1, 0u, c->stream);
     const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats, c->d_sx, c->fused_evals};
@@ -520,8 +537,62 @@ void drop_graph(clb_ctx* c) {
   c->graph_valid = false;
 }
 
+// Surrogate tables and increment buffer for the next steps (outside any capture). The
+// component totals come from the tableau as it is now (MetabolicTotals::build,
+// src/complab3d_metabolic.hh:233-262: component j is carried by the substrate named like it,
+// comp_species[j], and held by every species with a coefficient above 1e-12).
+int srg_prepare(clb_ctx* c) {
+  const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+  const size_t ib = (size_t)(c->S + c->nb) * (size_t)c->geo.N * rs;
+  if (c->srg_inc_bytes != ib) {
+    int rc = dalloc(c, (uint8_t**)&c->d_srg_inc, ib);
+    c->srg_inc_bytes = rc ? 0 : ib;
+    if (rc) return rc;
+    drop_graph(c);  // captured launches hold the old buffer
+  }
+  for (int k = 0; k < c->h_srg.nm; ++k)
+    if (c->h_srg.id[k] >= c->nb) return fail(c, CLB_ERR_STATE, "a surrogate microbe's biomass field is not set");
+  if (!c->d_srg) {
+    int rc = dalloc(c, (uint8_t**)&c->d_srg, sizeof(SrgTables));
+    if (rc) return rc;
+    c->srg_dirty = true;
+    drop_graph(c);
+  }
+  if (c->srg_dirty) {
+    SrgTables& t = c->h_srg;
+    t.totals_active = 0;
+    for (int s = 0; s < CLB_MAX_SOLUTES; ++s) {
+      t.has_contrib[s] = 0;
+      for (int i = 0; i < CLB_MAX_SOLUTES; ++i) t.contrib[s][i] = 0.0;
+    }
+    if (t.use_totals && c->have_tableau) {
+      const Tableau& tb = c->tb;
+      for (int j = 0; j < tb.nc; ++j) {
+        const int sidx = tb.comp_species[j];
+        if (sidx < 0 || sidx >= c->S) continue;
+        for (int i = 0; i < tb.ns && i < c->S; ++i) {
+          const double cf = tb.S[i * CLB_MAX_COMPONENTS + j];
+          if (cf > 1e-12 || cf < -1e-12) {
+            t.contrib[sidx][i] += cf;
+            t.has_contrib[sidx] = 1;
+          }
+        }
+        if (t.has_contrib[sidx]) t.totals_active = 1;
+      }
+    }
+    CLB_CHECK_HIP(c, hipMemcpyAsync(c->d_srg, &t, sizeof(SrgTables), hipMemcpyHostToDevice, c->stream));
+    CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    c->srg_dirty = false;
+  }
+  return CLB_OK;
+}
+
 template <typename R>
 int step_t(clb_ctx* c, int nsteps) {
+  if (c->srg_on) {
+    int rc = srg_prepare(c);
+    if (rc) return rc;
+  }
   if (c->dissol_on) {
     const size_t wb = (size_t)c->S * (size_t)c->geo.N * sizeof(R);
     if (c->water_bytes != wb) {
@@ -762,7 +833,7 @@ int clb_destroy(clb_ctx* c) {
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
                   c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
                   c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared,
-                  c->d_caflag, c->d_water};
+                  c->d_caflag, c->d_water, c->d_srg, c->d_srg_inc};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1037,6 +1108,7 @@ int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* l
                     double tol, int max_iter) {
   if (c) drop_graph(c);
   if (!c) return CLB_ERR_ARG;
+  c->srg_dirty = true;  // the surrogate's component totals follow the tableau
   {
     const int rc0 = flush_pending(c);
     if (rc0) return rc0;
@@ -1152,6 +1224,104 @@ int clb_set_dissolution(clb_ctx* c, int enable, int nphase, const int* phase_ids
   for (int k = 0; k < nphase; ++k) c->phase_sub[phase_ids[k]] = substrates[k];
   c->dissol_on = enable != 0 && nphase > 0;
   c->dissol_surface = surface_only ? 1 : 0;
+  return CLB_OK;
+}
+
+int clb_set_surrogate(clb_ctx* c, int n, const int* ids, const double* Kc, const double* decay, const double* vmax,
+                      const double* max_uptake, const double* molar_mass, const int* fix_conc, const int* fix_lb,
+                      int use_totals) {
+  if (!c || n < 0 || n > CLB_MAX_BIOMASS) return CLB_ERR_ARG;
+  const int S = c->S;
+  if (n > 0) {
+    if (!ids || !Kc || !decay) return fail(c, CLB_ERR_ARG, "ids, Kc and decay are required");
+    if (S < 1) return fail(c, CLB_ERR_STATE, "clb_set_solutes first");
+    for (int k = 0; k < n; ++k) {
+      if (ids[k] < 0 || ids[k] >= CLB_MAX_BIOMASS) return fail(c, CLB_ERR_ARG, "microbe id out of range");
+      for (int j = 0; j < k; ++j)
+        if (ids[j] == ids[k]) return fail(c, CLB_ERR_ARG, "a microbe is listed twice");
+      if (molar_mass && !(molar_mass[k] > 0.0 && std::isfinite(molar_mass[k])))
+        return fail(c, CLB_ERR_ARG, "biomass molar mass must be positive");
+      if (!std::isfinite(decay[k])) return fail(c, CLB_ERR_ARG, "decay must be finite");
+    }
+  }
+  drop_graph(c);  // the step's launch sequence changes
+  SrgTables& t = c->h_srg;
+  t = SrgTables{};
+  t.nm = n;
+  for (int k = 0; k < n; ++k) {
+    t.id[k] = ids[k];
+    t.decay[k] = decay[k];
+    t.mw[k] = molar_mass ? molar_mass[k] : 24.6;  // the default <biomass_molar_mass>
+    for (int s = 0; s < S; ++s) {
+      t.Kc[k][s] = Kc[k * S + s];
+      t.vmax[k][s] = vmax ? std::fabs(vmax[k * S + s]) : 0.0;
+      t.maxup[k][s] = max_uptake ? max_uptake[k * S + s] : -1e30;
+    }
+  }
+  for (int s = 0; s < S; ++s) {
+    t.fixc[s] = fix_conc ? (fix_conc[s] != 0) : 0;
+    t.fixlb[s] = fix_lb ? (fix_lb[s] != 0) : 0;
+  }
+  t.use_totals = use_totals ? 1 : 0;
+  c->srg_on = n > 0;
+  c->srg_dirty = true;
+  return CLB_OK;
+}
+
+int clb_set_surrogate_net(clb_ctx* c, int id, int n_in, const int* in_sub, int n_layers, const int* width,
+                          const double* params, const double* x_offset, const double* x_gain, double x_ymin,
+                          double y_ymin, double y_gain, double y_offset, int log_output, int closure,
+                          const double* yield) {
+  if (!c) return CLB_ERR_ARG;
+  SrgTables& t = c->h_srg;
+  int k = -1;
+  for (int j = 0; j < t.nm; ++j)
+    if (t.id[j] == id) k = j;
+  if (k < 0) return fail(c, CLB_ERR_ARG, "not a surrogate microbe (clb_set_surrogate first)");
+  if (n_layers == 0) {  // no network: the microbe grows at 0 and decays
+    t.net[k] = SrgNet{};
+    c->srg_dirty = true;
+    return CLB_OK;
+  }
+  if (n_in < 1 || n_in > CLB_MAX_SOLUTES || !in_sub || n_layers < 1 || n_layers > CLB_SRG_MAX_LAYERS || !width ||
+      !params || !x_offset || !x_gain || closure < 0 || closure > 2 || (closure > 0 && !yield))
+    return fail(c, CLB_ERR_ARG, "bad surrogate network description");
+  for (int j = 0; j < n_in; ++j)
+    if (in_sub[j] < 0 || in_sub[j] >= c->S) return fail(c, CLB_ERR_ARG, "network input names no substrate");
+  for (int l = 0; l < n_layers; ++l)
+    if (width[l] < 1 || width[l] > CLB_SRG_MAX_WIDTH)
+      return fail(c, CLB_ERR_ARG, "layer width out of range (at most " + std::to_string(CLB_SRG_MAX_WIDTH) + ")");
+  if (width[n_layers - 1] != 1) return fail(c, CLB_ERR_ARG, "the output layer has one neuron");
+  if (!(y_gain != 0.0) || !std::isfinite(y_gain)) return fail(c, CLB_ERR_ARG, "y_gain must be finite and nonzero");
+  SrgNet& nt = t.net[k];
+  nt = SrgNet{};
+  nt.n_in = n_in;
+  nt.n_layers = n_layers;
+  nt.log_output = log_output ? 1 : 0;
+  nt.closure = closure;
+  for (int j = 0; j < n_in; ++j) {
+    nt.in_sub[j] = in_sub[j];
+    nt.x_offset[j] = x_offset[j];
+    nt.x_gain[j] = x_gain[j];
+  }
+  nt.x_ymin = x_ymin;
+  nt.y_ymin = y_ymin;
+  nt.y_gain = y_gain;
+  nt.y_offset = y_offset;
+  for (int s = 0; s < c->S; ++s) nt.yield[s] = yield ? yield[s] : 0.0;
+  size_t off = 0;
+  int prev = n_in;
+  for (int l = 0; l < n_layers; ++l) {
+    nt.width[l] = width[l];
+    for (int r = 0; r < width[l]; ++r)
+      for (int q = 0; q < prev; ++q) nt.W[l][r][q] = params[off + (size_t)r * prev + q];
+    off += (size_t)width[l] * prev;
+    for (int r = 0; r < width[l]; ++r) nt.b[l][r] = params[off + r];
+    off += width[l];
+    prev = width[l];
+  }
+  nt.on = 1;
+  c->srg_dirty = true;
   return CLB_OK;
 }
 

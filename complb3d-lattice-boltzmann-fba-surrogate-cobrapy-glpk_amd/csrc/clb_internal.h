@@ -56,6 +56,9 @@ struct AdeArgs {
   // Mineral dissolution (else null): concentrations after the biotic kinetics of every
   // reactive voxel (the water a neighbouring mineral sees), S*N.
   R* water;
+  // Surrogate metabolic model (else null): the processor's increments of this step,
+  // S planes of dC then nb planes of dB, written for every reactive voxel before transport.
+  const R* srg_inc;
   Geom geo;
   int S, nb;
   int bc_type[2 * CLB_MAX_SOLUTES];
@@ -170,6 +173,39 @@ struct DissolArgs {
 };
 template <typename R>
 void launch_dissolution(const AdeArgs<R>& a, const DissolArgs& d, bool odd_done, hipStream_t s);
+
+// Surrogate metabolic model (clb_surrogate.hip): run_surrogate3D with a MATLAB-exported
+// feed-forward network per microbe. Tables live in device memory (uniform reads).
+#define CLB_SRG_MAX_WIDTH 16
+#define CLB_SRG_MAX_LAYERS 6
+struct SrgNet {
+  int on;                       // 0: the microbe has no network (grows at 0, decays)
+  int n_in, n_layers, log_output, closure;
+  int width[CLB_SRG_MAX_LAYERS];
+  int in_sub[CLB_MAX_SOLUTES];  // input k reads Fin[microbe][in_sub[k]]
+  double x_offset[CLB_MAX_SOLUTES], x_gain[CLB_MAX_SOLUTES];
+  double x_ymin, y_ymin, y_gain, y_offset;
+  double yield[CLB_MAX_SOLUTES];  // gDW per mmol (closure 1, 2); <= 0: not tied to growth
+  double W[CLB_SRG_MAX_LAYERS][CLB_SRG_MAX_WIDTH][CLB_SRG_MAX_WIDTH];  // row = neuron, zero padded
+  double b[CLB_SRG_MAX_LAYERS][CLB_SRG_MAX_WIDTH];
+};
+struct SrgTables {
+  int nm;                                   // surrogate microbes
+  int id[CLB_MAX_BIOMASS];                  // their biomass fields
+  double Kc[CLB_MAX_BIOMASS][CLB_MAX_SOLUTES];
+  double vmax[CLB_MAX_BIOMASS][CLB_MAX_SOLUTES];   // 0: unset
+  double maxup[CLB_MAX_BIOMASS][CLB_MAX_SOLUTES];  // floor (<= 0); <= -1e29: unset
+  double decay[CLB_MAX_BIOMASS];            // 1/s
+  double mw[CLB_MAX_BIOMASS];               // gDW per mol
+  int fixc[CLB_MAX_SOLUTES], fixlb[CLB_MAX_SOLUTES];
+  int use_totals;                           // uptake limited by component totals
+  int totals_active;                        // the tableau maps at least one component
+  int has_contrib[CLB_MAX_SOLUTES];
+  double contrib[CLB_MAX_SOLUTES][CLB_MAX_SOLUTES];  // [substrate][species] coefficient, 0: none
+  SrgNet net[CLB_MAX_BIOMASS];
+};
+template <typename R>
+void launch_surrogate(const AdeArgs<R>& a, const SrgTables* t, R* inc, bool odd, hipStream_t s);
 
 // CA biomass redistribution (clb_ca.hip)
 void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, int pore,

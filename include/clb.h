@@ -195,6 +195,43 @@ int clb_set_dissolution(clb_ctx* ctx, int enable, int nphase, const int* phase_i
  * an immobile solute's amount in every voxel, a mobile solute's at-rest equilibrium in
  * every fluid voxel. */
 int clb_set_solute_field(clb_ctx* ctx, int solute, const double* c /*N*/);
+/* Surrogate metabolic model (run_surrogate3D, src/complab3d_processors_surrogate.hh:74-295,
+ * called at src/complab.cpp:1294-1301): every step, with the biotic kinetics, at every
+ * reactive voxel (1 <= x <= nx-2, fluid, not the solid / bounce-back codes of
+ * clb_set_kinetics, which also gives dt), each listed microbe holding more than 1e-12 takes
+ * up substrates at its Michaelis-Menten estimate and grows at the rate its network predicts
+ * (or decays); its increments join the kinetics' before they apply.
+ *   n: microbes grown by the surrogate (0 switches it off, at most CLB_MAX_BIOMASS);
+ *   ids[n]: each one's biomass field (clb_set_biomass), also the id its network is set by;
+ *   Kc[n*S]: half-saturation constants (<half_saturation_constants>);
+ *   decay[n]: first-order decay, 1/s (<decay_coefficient>);
+ *   vmax[n*S] or NULL: <maximum_uptake_flux>, mmol/gDW/h (0: supply-limited);
+ *   max_uptake[n*S] or NULL: <substrate_lower_bounds>, a floor <= 0 (<= -1e29: none);
+ *   molar_mass[n] or NULL: <biomass_molar_mass>, gDW/mol (default 24.6);
+ *   fix_conc[S], fix_lb[S] or NULL: <fix_concentration>, <fix_lower_bounds>;
+ *   use_totals: <fba_concentration_basis> total -- uptake limited by the component totals of
+ *   the tableau (clb_set_tableau) and drawn from every species carrying the component.
+ * Fields of the metabolic config of src/complab3d_metabolic.hh:394-520. Clears the networks. */
+int clb_set_surrogate(clb_ctx* ctx, int n, const int* ids, const double* Kc, const double* decay,
+                      const double* vmax, const double* max_uptake, const double* molar_mass,
+                      const int* fix_conc, const int* fix_lb, int use_totals);
+/* The network of surrogate microbe `id`, in place of the compiled defineSurrogateModel
+ * (surrogateModel.hh:252-500): a MATLAB-exported feed-forward net -- inputs mapped by
+ * mapminmax, xs = (x - x_offset) * x_gain + x_ymin; tansig hidden layers; one linear output
+ * neuron; growth = (y - y_ymin) / y_gain + y_offset (10^that with log_output); growth below
+ * 1e-8 or not finite is 0.
+ *   n_in inputs, input k = Fin[id][in_sub[k]] (the uptake estimates, mmol/gDW/h);
+ *   n_layers weight layers (at most 6) of width[l] neurons (at most 16; the last is 1);
+ *   params: per layer W (width[l] x width[l-1], row per neuron; width[-1] = n_in), then b;
+ *   closure 0: consumption stays at the estimate (the shipped model); 1: consumption =
+ *   growth / yield[s], at most the estimate; 2: growth limited by the scarcest substrate,
+ *   min over s of estimate * yield[s], and consumption = growth / yield[s] (the two closures
+ *   the reference's model file describes); yield[S] in gDW/mmol, <= 0: not tied to growth.
+ * n_layers = 0 removes the network: the microbe then grows at 0 and decays. */
+int clb_set_surrogate_net(clb_ctx* ctx, int id, int n_in, const int* in_sub, int n_layers, const int* width,
+                          const double* params, const double* x_offset, const double* x_gain, double x_ymin,
+                          double y_ymin, double y_gain, double y_offset, int log_output, int closure,
+                          const double* yield);
 /* Checkpoint (the reference's saveBinaryBlock / loadBinaryBlock checkpoints,
  * src/complab.cpp:1122-1132, 1238-1247, in one file of this library's own format): the
  * evolving state -- mask, phase lattice, flow and solute populations (deferred increments

@@ -26,6 +26,8 @@
 //   * Biomass threshold mask reclassification (src/complab3d_processors_part2.hh:99-210,
 //     deterministic single-pore-type form) and precipitation / dissolution node
 //     conversion (src/precipitationVOP.hh:176-204, src/dissolutionVOP.hh:469-511).
+//   * The surrogate metabolic processor run_surrogate3D (surrogate.cpp), joined to the
+//     kinetics increments before update_rxnLattices applies them.
 //   * MLP forward / backward (tanh hidden layers, linear output, mean-squared error),
 //     standard back-propagation as in surrogate_training/trainSurrogate.py:132-166.
 //
@@ -612,6 +614,10 @@ typedef void (*orc_kinetics_fn)(const double* B, int nb, const double* C, int ns
 typedef void (*orc_abiotic_fn)(const double* C, int ns, double* subsR, long long mask);
 typedef void (*orc_dissol_fn)(long long phase_id, const double* C, int ns, double mineral, double* subsR,
                               double* mineralR, long long mask);
+// The surrogate processor (surrogate.cpp): one voxel's deposited dC[S] / dB[nb].
+struct orc_srg;
+extern "C" void orc_srg_voxel(const orc_srg* P, int S, int nb, const double* conc, const double* bmass, double dt,
+                              double* dC, double* dB);
 
 struct orc_ade_params {
   int nx, ny, nz, S;       // grid and number of solutes
@@ -646,6 +652,8 @@ struct orc_ade_params {
   const uint8_t* phase;      // [N] phase ids
   const int* phase_sub;      // [16] immobile solute of each phase id, -1: none
   int dissol_surface;        // 1: enclosed phase voxels do not react
+  // surrogate metabolic processor (run_surrogate3D), null: none
+  const orc_srg* srg;
 };
 
 // surfaceDissolutionKinetics3D (src/dissolutionVOP.hh:229-459) over the post-collision
@@ -794,8 +802,10 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
               }
             }
             bool react = (x > 0 && x < nx - 1 && code != P->solid_code && code != P->bb_code);
-            // ---- kinetics (run_kinetics + update_rxnLattices)
-            if (react && P->kinetics) {
+            // ---- kinetics (run_kinetics, run_surrogate3D + update_rxnLattices): both add
+            //      into the same dC / dB lattices, applied once when the sum exceeds 1e-12
+            //      (src/complab.cpp:1264-1313)
+            if (react && (P->kinetics || P->srg)) {
               std::vector<double> C(S), B(P->nb > 0 ? P->nb : 1), sr(S, 0.0), br(P->nb > 0 ? P->nb : 1, 0.0);
               for (int s = 0; s < S; ++s) {
                 double r = 0;
@@ -809,14 +819,29 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
                 if (b0 < 1e-12) b0 = 0;
                 B[b] = b0;
               }
-              P->kinetics(B.data(), P->nb, C.data(), S, sr.data(), br.data(), (long long)code);
+              if (P->kinetics) P->kinetics(B.data(), P->nb, C.data(), S, sr.data(), br.data(), (long long)code);
+              std::vector<double> sdC(S, 0.0), sdB(P->nb > 0 ? P->nb : 1, 0.0);
+              if (P->srg) {
+                std::vector<double> craw(S), braw(P->nb > 0 ? P->nb : 1, 0.0);
+                for (int s = 0; s < S; ++s) {
+                  double r = 0;
+                  for (int i = 0; i < 7; ++i) r += p[s * 7 + i];
+                  craw[s] = r + 1.0;
+                }
+                for (int b = 0; b < P->nb; ++b) braw[b] = bmass[(long)b * N + n];
+                orc_srg_voxel(P->srg, S, P->nb, craw.data(), braw.data(), P->dt, sdC.data(), sdB.data());
+              }
               for (int s = 0; s < S; ++s) {
                 double dC = sr[s] * P->dt;
+                if (!(dC > 1e-12 || dC < -1e-12)) dC = 0.0;
+                dC += sdC[s];
                 if (dC > 1e-12 || dC < -1e-12)
                   for (int i = 0; i < 7; ++i) p[s * 7 + i] += dC * W7[i];
               }
               for (int b = 0; b < P->nb; ++b) {
                 double dB = br[b] * P->dt;
+                if (!(dB > 1e-12 || dB < -1e-12)) dB = 0.0;
+                dB += sdB[b];
                 if (dB > 1e-12 || dB < -1e-12) bmass[(long)b * N + n] += dB;
               }
             }

@@ -46,6 +46,8 @@ def load():
         "orc_kinetics_plugin": ([vp, ci, vp, ci, vp, vp, ctypes.c_longlong], None),
         "orc_abiotic_plugin": ([vp, ci, vp, ctypes.c_longlong], None),
         "orc_num_threads": ([], ci),
+        "orc_srg_net_eval": ([vp, vp, ci], cd),
+        "orc_srg_voxel": ([vp, ci, ci, vp, vp, cd, vp, vp], None),
     }
     for name, (args, res) in sigs.items():
         fn = getattr(lib, name)
@@ -77,6 +79,27 @@ class AdeParams(ctypes.Structure):
         ("immobile", ctypes.c_void_p), ("abiotic", ctypes.c_void_p), ("abiotic_surface", ctypes.c_int),
         ("dissol", ctypes.c_void_p), ("phase", ctypes.c_void_p), ("phase_sub", ctypes.c_void_p),
         ("dissol_surface", ctypes.c_int),
+        ("srg", ctypes.c_void_p),
+    ]
+
+
+class SrgNetParams(ctypes.Structure):
+    _fields_ = [
+        ("n_in", ctypes.c_int), ("in_sub", ctypes.c_void_p), ("n_layers", ctypes.c_int),
+        ("width", ctypes.c_void_p), ("params", ctypes.c_void_p), ("x_offset", ctypes.c_void_p),
+        ("x_gain", ctypes.c_void_p), ("x_ymin", ctypes.c_double), ("y_ymin", ctypes.c_double),
+        ("y_gain", ctypes.c_double), ("y_offset", ctypes.c_double), ("log_output", ctypes.c_int),
+        ("closure", ctypes.c_int), ("yield_", ctypes.c_void_p),
+    ]
+
+
+class SrgParams(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int), ("id", ctypes.c_void_p), ("Kc", ctypes.c_void_p), ("decay", ctypes.c_void_p),
+        ("vmax", ctypes.c_void_p), ("maxup", ctypes.c_void_p), ("mw", ctypes.c_void_p),
+        ("fixc", ctypes.c_void_p), ("fixlb", ctypes.c_void_p), ("use_totals", ctypes.c_int),
+        ("nc", ctypes.c_int), ("eqS", ctypes.c_void_p), ("eqCompSpecies", ctypes.c_void_p),
+        ("nets", ctypes.c_void_p),
     ]
 
 
@@ -260,7 +283,7 @@ class AdeRun:
 
     def __init__(self, nx, ny, nz, mask, kind, omega, bc_type, bc_value, u, c0, solid_code=0, bb_code=1,
                  kinetics=False, dt=1.0, biomass=None, tableau=None, eq_tol=1e-10, eq_max_iter=200,
-                 mask_update=None, immobile=None, abiotic=0, dissolution=None):
+                 mask_update=None, immobile=None, abiotic=0, dissolution=None, surrogate=None):
         self.lib = load()
         self.nx, self.ny, self.nz = nx, ny, nz
         self.N = N = nx * ny * nz
@@ -313,6 +336,9 @@ class AdeRun:
             P.dissol = dissolution_plugin_ptr()
             P.phase, P.phase_sub = _p(self.phase).value, _p(self.phase_sub).value
             P.dissol_surface = 1 if surf else 0
+        if surrogate is not None:
+            self.surrogate = surrogate  # keeps its arrays alive
+            P.srg = ctypes.addressof(surrogate.P)
         self.P = P
         self.eq_stats = np.zeros(3, dtype=np.int64)
 
@@ -351,6 +377,73 @@ class AdeRun:
                 val = self.bc_value[2 * s + side] if self.bc_type[2 * s + side] == 0 else rho[s, side]
                 c[s, :, :, xb] = np.where(fluid[:, :, xb], val, c[s, :, :, xb])
         return c.reshape(self.S, self.N)
+
+
+# ------------------------------------------------------------------ surrogate processor
+class Surrogate:
+    """The surrogate processor's settings for the checker (surrogate.cpp). nets: one entry per
+    microbe, a dict (n_in, in_sub, widths, params, x_offset, x_gain, x_ymin, y_ymin, y_gain,
+    y_offset, log_output; optional closure, yield) or None for no network."""
+
+    def __init__(self, S, ids, Kc, decay, nets, vmax=None, max_uptake=None, molar_mass=None, fix_conc=None,
+                 fix_lb=None, use_totals=False, tableau=None):
+        n = len(ids)
+        self._keep = []
+        def arr(a, dt=np.float64):
+            a = np.ascontiguousarray(a, dtype=dt).ravel()
+            self._keep.append(a)
+            return a.ctypes.data
+        P = SrgParams()
+        P.n = n
+        P.id = arr(ids, np.int32)
+        P.Kc = arr(np.reshape(Kc, (n, S)))
+        P.decay = arr(decay)
+        P.vmax = arr(np.zeros((n, S)) if vmax is None else np.abs(np.reshape(vmax, (n, S))))
+        P.maxup = arr(np.full((n, S), -1e30) if max_uptake is None else np.reshape(max_uptake, (n, S)))
+        P.mw = arr(np.full(n, 24.6) if molar_mass is None else molar_mass)
+        P.fixc = arr(np.zeros(S) if fix_conc is None else fix_conc, np.int32)
+        P.fixlb = arr(np.zeros(S) if fix_lb is None else fix_lb, np.int32)
+        P.use_totals = 1 if use_totals else 0
+        if tableau is not None:
+            Sm, cs = tableau
+            Sm = np.asarray(Sm, dtype=np.float64)
+            P.nc = Sm.shape[1]
+            P.eqS = arr(Sm)
+            P.eqCompSpecies = arr(cs, np.int32)
+        self.net_structs = (SrgNetParams * max(n, 1))()
+        for k, net in enumerate(nets):
+            q = self.net_structs[k]
+            if net is None:
+                q.n_layers = 0
+                continue
+            q.n_in = net["n_in"]
+            q.in_sub = arr(net["in_sub"], np.int32)
+            q.n_layers = len(net["widths"])
+            q.width = arr(net["widths"], np.int32)
+            q.params = arr(net["params"])
+            q.x_offset = arr(net["x_offset"])
+            q.x_gain = arr(net["x_gain"])
+            q.x_ymin, q.y_ymin, q.y_gain, q.y_offset = net["x_ymin"], net["y_ymin"], net["y_gain"], net["y_offset"]
+            q.log_output = 1 if net.get("log_output") else 0
+            q.closure = net.get("closure", 0)
+            q.yield_ = arr(net["yield"]) if net.get("yield") is not None else None
+        P.nets = ctypes.addressof(self.net_structs)
+        self.P = P
+        self.S = S
+
+    def voxel(self, conc, bmass, dt):
+        """One voxel's deposited (dC[S], dB[nb]) from its post-collision state."""
+        conc = _f(conc)
+        bmass = _f(bmass)
+        dC = np.zeros(self.S)
+        dB = np.zeros(bmass.size)
+        load().orc_srg_voxel(ctypes.byref(self.P), self.S, bmass.size, _p(conc), _p(bmass), dt, _p(dC), _p(dB))
+        return dC, dB
+
+    def growth(self, k, fin):
+        """Microbe k's network on an uptake row fin[S]."""
+        fin = _f(fin)
+        return load().orc_srg_net_eval(ctypes.byref(self.net_structs[k]), _p(fin), fin.size)
 
 
 # ------------------------------------------------------------------ MLP
