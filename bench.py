@@ -40,6 +40,14 @@ BYTES_KIN4 = 4 * 56 + 12 + 1 + 8  # + biomass read and write
 # few voxels): total/mask update 4 b + 1 mask + 16 total r/w; push decision 1 mask + 8 total
 # + 4 age + 1 flags; gather 1 mask + 1 flags; the sweep's biomass copy 8
 BYTES_CA_SWEEP = 21 + 14 + 2 + 8
+# Surrogate processor kernel, FP32 store, 4 solutes, 2 biomass fields (one surrogate): every
+# reactive voxel reads its mask byte and the surrogate field and writes 4 + 2 increments; a
+# voxel holding the organism also reads its 28 incoming populations.
+BYTES_SRG_REACTIVE = 1 + 4 + 6 * 4
+BYTES_SRG_PRESENT = 28 * 4
+# FP64 flops of the 2-10-10-10-10-1 network per evaluation (mapminmax, 4 tansig layers
+# with exp counted as 20, linear output)
+FLOP_SRG_NET = 2 * 2 + 2 * (2 * 10 + 3 * 100) + 4 * 10 * (20 + 4) + 2 * 10 + 4
 # HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the default
 # workloads, scaled by a copy kernel of known size (profiles/hbm_traffic_fetch_write.md);
 # mean of the odd and even steps. Reported only when the workload is the default one.
@@ -214,6 +222,48 @@ def ca_leg(lat, N, bmax=100.0, grow=2.5):
                 bmax_before=rep["bmax_before"], bmax_after=rep["bmax_after"], mask_changes=rep["mask_changes"])
 
 
+def surrogate_leg(clb, steps, tune_steps):
+    """The kinetics workload (256 x 128 x 128, 4 solutes, FP32 store) with a second biomass
+    field grown by the surrogate processor: the shipped organism's network shape
+    (2-10-10-10-10-1, tansig, random weights) on 1 % seeds, Vmax-limited uptake. Times the
+    whole step and the surrogate kernel alone."""
+    kn = (256, 128, 128)
+    N = kn[0] * kn[1] * kn[2]
+    lat = kinetics_case(clb, *kn, clb.PREC_FP32, 400)
+    mask = lat.download_mask()
+    b0 = lat.biomass()[0]
+    b1 = clb.gen_biomass_seed(mask, pore=2, fraction=0.01, bmin=5.0, bmax=60.0, seed_sites=3, seed_density=2)
+    lat.set_biomass(np.stack([b0, b1]))
+    rng = np.random.default_rng(11)
+    sizes = [2, 10, 10, 10, 10, 1]
+    params = np.concatenate([np.concatenate([rng.normal(scale=0.6, size=(sizes[k + 1], sizes[k])).ravel(),
+                                             rng.normal(scale=0.3, size=sizes[k + 1])]) for k in range(5)])
+    net = dict(n_in=2, in_sub=[0, 1], widths=sizes[1:], params=params, x_offset=[8.9e-4, 3.5e-5],
+               x_gain=[0.2, 4.0], x_ymin=-1.0, y_ymin=-1.0, y_gain=37.9, y_offset=0.0)
+    lat.set_surrogate([1], [[0.05, 0.02, 0.0, 0.0]], [6.94e-8], vmax=[[8.0, 0.4, 0.0, 0.0]])
+    lat.set_surrogate_net(1, net)
+    if tune_steps > 0:
+        lat.autotune(tune_steps)
+    wall, p = timed_steps(lat, steps, 20)
+    m = lat.download_mask()
+    kind, _ = kind_omega(0.8)
+    x = np.arange(N) % kn[0]
+    reactive = (kind[m] == 2) & (m != 0) & (m != 1) & (x >= 1) & (x <= kn[0] - 2)
+    present = reactive & (lat.biomass()[1] > 1e-12)
+    lat.close()
+    sms = p.srg_ms / max(p.srg_launches, 1)
+    nbytes = BYTES_SRG_REACTIVE * reactive.sum() + BYTES_SRG_PRESENT * present.sum()
+    out = dict(workload="kinetics + surrogate 256x128x128, 4 solutes, 2 biomass fields (1 surrogate), FP32 store",
+               ms_per_step=wall * 1e3 / steps, mlups=N * steps / wall / 1e6, srg_kernel_ms=sms,
+               transport_kernel_ms=p.ade_ms / max(p.ade_launches, 1), reactive=int(reactive.sum()),
+               present=int(present.sum()), srg_gbs=nbytes / (sms * 1e-3) / 1e9,
+               srg_frac=nbytes / (sms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+               net_tflops=FLOP_SRG_NET * present.sum() / (sms * 1e-3) / 1e12)
+    log(f"surrogate: {out['mlups']:.1f} MLUPS, surrogate kernel {sms:.3f} ms ({out['present']} voxels with the "
+        f"organism, {out['srg_gbs']:.0f} GB/s algorithmic, net {out['net_tflops']:.2f} TFLOP/s FP64)")
+    return out
+
+
 def timed_steps(lat, steps, warmup):
     lat.step(warmup)
     lat.sync()
@@ -356,6 +406,7 @@ def main():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-kinetics", action="store_true")
     ap.add_argument("--skip-ca", action="store_true")
+    ap.add_argument("--skip-surrogate", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -402,6 +453,8 @@ def main():
                                       frac=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, autotune=ktune)
         log(f"kinetics fp32: {extra['kinetics_fp32']['mlups']:.1f} MLUPS, kernel {kms:.3f} ms, "
             f"{extra['kinetics_fp32']['frac']:.3f} of HBM peak")
+        if not args.skip_surrogate:
+            extra["surrogate_fp32"] = surrogate_leg(clb, args.kin_steps, args.tune_steps)
         if not args.skip_fp64:
             lat = kinetics_case(clb, *kn, clb.PREC_FP64, 2)
             w64, _ = timed_steps(lat, max(args.kin_steps // 4, 10), 5)

@@ -56,7 +56,8 @@ class Perf(ctypes.Structure):
                 ("ade_ms", ctypes.c_double), ("ade_launches", ctypes.c_longlong),
                 ("spec_ms", ctypes.c_double), ("spec_launches", ctypes.c_longlong),
                 ("mlp_ms", ctypes.c_double), ("mlp_launches", ctypes.c_longlong),
-                ("fluid_nodes", ctypes.c_longlong), ("reactive_nodes", ctypes.c_longlong)]
+                ("fluid_nodes", ctypes.c_longlong), ("reactive_nodes", ctypes.c_longlong),
+                ("srg_ms", ctypes.c_double), ("srg_launches", ctypes.c_longlong)]
 
 
 _lib = None

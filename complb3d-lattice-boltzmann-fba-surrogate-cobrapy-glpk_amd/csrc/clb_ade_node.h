@@ -165,19 +165,26 @@ __device__ __forceinline__ void biotic_kinetics(const AdeArgs<R>& a, int n, uint
   if (a.kinetics) defineRxnKinetics(Bv, C, sr, br, (plb::plint)code);
   // the surrogate processor's increments join the kinetics' in the same dC / dB lattices,
   // which update_rxnLattices applies when their sum exceeds 1e-12
-  // (src/complab3d_processors_surrogate.hh:262-283, src/complab.cpp:1294-1313)
+  // (src/complab3d_processors_surrogate.hh:262-283, src/complab.cpp:1294-1313); it left
+  // them where a surrogate organism is present, which Bv tells (same field, same threshold)
+  bool srg = false;
+  if (a.srg_inc) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if ((a.srg_pools >> b) & 1) srg = srg || Bv[b] > 1e-12;
+  }
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     double d = sr[s] * a.dt;
     if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
-    if (a.srg_inc) d += (double)a.srg_inc[(long long)s * g.N + n];
+    if (srg) d += (double)a.srg_inc[(long long)s * g.N + n];
     inject(a, s, p, d);
   }
 #pragma unroll
   for (int b = 0; b < NB; ++b) {
     double dB = br[b] * a.dt;
     if (!(dB > 1e-12 || dB < -1e-12)) dB = 0.0;
-    if (a.srg_inc) dB += (double)a.srg_inc[(long long)(S + b) * g.N + n];
+    if (srg) dB += (double)a.srg_inc[(long long)(S + b) * g.N + n];
     if (dB > 1e-12 || dB < -1e-12) {
       R* bp = &a.bmass[(long long)b * g.N + n];
       *bp = (R)((double)*bp + dB);

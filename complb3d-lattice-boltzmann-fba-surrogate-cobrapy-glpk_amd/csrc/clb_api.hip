@@ -158,7 +158,7 @@ struct clb_ctx {
 
   // measurement
   bool timing = false;
-  EventPool pools[4];  // flow, ade, spec, mlp
+  EventPool pools[5];  // flow, ade, spec, mlp, surrogate
   clb_perf perf{};
 };
 
@@ -222,7 +222,7 @@ struct Timed {
         (void)hipEventElapsedTime(&t, p.ev[i], p.ev[i + 1]);
         ms += t;
       }
-      double* acc[4] = {&c->perf.flow_ms, &c->perf.ade_ms, &c->perf.spec_ms, &c->perf.mlp_ms};
+      double* acc[5] = {&c->perf.flow_ms, &c->perf.ade_ms, &c->perf.spec_ms, &c->perf.mlp_ms, &c->perf.srg_ms};
       *acc[cat] += ms;
       p.used = 0;
     }
@@ -230,8 +230,8 @@ struct Timed {
 };
 
 void count_launch(clb_ctx* c, int cat) {
-  long long* cnt[4] = {&c->perf.flow_launches, &c->perf.ade_launches, &c->perf.spec_launches,
-                       &c->perf.mlp_launches};
+  long long* cnt[5] = {&c->perf.flow_launches, &c->perf.ade_launches, &c->perf.spec_launches,
+                       &c->perf.mlp_launches, &c->perf.srg_launches};
   *cnt[cat] += 1;
 }
 
@@ -262,7 +262,8 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   a.pend_r = nullptr;
   a.pend_w = nullptr;
   a.water = (c->dissol_on && c->d_water) ? (R*)c->d_water : nullptr;
-  a.srg_inc = nullptr;  // set by the step that ran the surrogate kernel
+  a.srg_inc = nullptr;  // set by the step that ran the surrogate kernels
+  a.srg_pools = 0;
   a.geo = c->geo;
   a.S = c->S;
   a.nb = c->nb;
@@ -476,9 +477,13 @@ void enqueue_step(clb_ctx* c) {
   }
   if (c->srg_on) {
     // the surrogate processor reads the state the transport kernel is about to load
-    Timed tm(c, 1);
-    launch_surrogate<R>(a, c->d_srg, (R*)c->d_srg_inc, odd, c->stream);
+    {
+      Timed tm(c, 4);
+      launch_surrogate<R>(a, c->d_srg, c->h_srg.nm, c->h_srg.lds_layers, (R*)c->d_srg_inc, odd, c->stream);
+    }
+    count_launch(c, 4);
     a.srg_inc = (const R*)c->d_srg_inc;
+    for (int k = 0; k < c->h_srg.nm; ++k) a.srg_pools |= 1 << c->h_srg.id[k];
   }
   if (fused) {
     c->pend_dirty = true;
@@ -560,6 +565,10 @@ int srg_prepare(clb_ctx* c) {
   }
   if (c->srg_dirty) {
     SrgTables& t = c->h_srg;
+    t.lds_layers = 1;
+    for (int k = 0; k < t.nm; ++k)
+      if (t.net[k].on) t.lds_layers = std::max(t.lds_layers, t.net[k].n_layers);
+    drop_graph(c);  // the kernel's LDS size follows the networks
     t.totals_active = 0;
     for (int s = 0; s < CLB_MAX_SOLUTES; ++s) {
       t.has_contrib[s] = 0;
@@ -587,8 +596,12 @@ int srg_prepare(clb_ctx* c) {
   return CLB_OK;
 }
 
+// Everything enqueue_step's launches read beyond the state itself: the surrogate tables and
+// increment buffer, the dissolution water, the fused step's buffers or the separate
+// speciation's reactive list. Every caller of enqueue_step runs this first (outside any
+// capture), so no launch ever sees a buffer that is not there yet.
 template <typename R>
-int step_t(clb_ctx* c, int nsteps) {
+int prepare_step(clb_ctx* c) {
   if (c->srg_on) {
     int rc = srg_prepare(c);
     if (rc) return rc;
@@ -602,12 +615,9 @@ int step_t(clb_ctx* c, int nsteps) {
       drop_graph(c);  // captured launches hold the old buffer
     }
   }
-  const bool fused = fused_spec(c);
-  if (fused) {
+  if (fused_spec(c)) {
     int rc = prepare_fused(c);
     if (rc) return rc;
-    // every fused step, eager or replayed from a graph, leaves deferred increments
-    if (nsteps > 0) c->pend_dirty = true;
   } else if (c->have_tableau && c->list_dirty) {
     // the separate speciation kernel takes the reactive list and its length from the
     // host, so a captured graph holds them: rebuild both
@@ -615,6 +625,18 @@ int step_t(clb_ctx* c, int nsteps) {
     if (rc) return rc;
     drop_graph(c);
   }
+  return CLB_OK;
+}
+
+template <typename R>
+int step_t(clb_ctx* c, int nsteps) {
+  {
+    int rc = prepare_step<R>(c);
+    if (rc) return rc;
+  }
+  const bool fused = fused_spec(c);
+  // every fused step, eager or replayed from a graph, leaves deferred increments
+  if (fused && nsteps > 0) c->pend_dirty = true;
   const bool spec_launch = c->have_tableau && (fused || c->n_reactive > 0);
   const int seg = 2 * c->graph_pairs;
   if (c->use_graph && !c->timing && nsteps >= seg) {
@@ -636,12 +658,14 @@ int step_t(clb_ctx* c, int nsteps) {
       c->ade_t = t0;  // capture recorded the steps; they run at replay
       c->perf.ade_launches -= seg;
       if (spec_launch) c->perf.spec_launches -= seg;
+      if (c->srg_on) c->perf.srg_launches -= seg;
     }
     while (nsteps >= seg) {
       CLB_CHECK_HIP(c, hipGraphLaunch(c->graph_exec, c->stream));
       c->ade_t += seg;
       c->perf.ade_launches += seg;
       if (spec_launch) c->perf.spec_launches += seg;
+      if (c->srg_on) c->perf.srg_launches += seg;
       nsteps -= seg;
     }
   }
@@ -652,8 +676,8 @@ int step_t(clb_ctx* c, int nsteps) {
 
 int fold_timing(clb_ctx* c) {
   CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
-  double* acc[4] = {&c->perf.flow_ms, &c->perf.ade_ms, &c->perf.spec_ms, &c->perf.mlp_ms};
-  for (int k = 0; k < 4; ++k) {
+  double* acc[5] = {&c->perf.flow_ms, &c->perf.ade_ms, &c->perf.spec_ms, &c->perf.mlp_ms, &c->perf.srg_ms};
+  for (int k = 0; k < 5; ++k) {
     EventPool& p = c->pools[k];
     for (size_t i = 0; i + 1 < p.used; i += 2) {
       float t = 0;
@@ -1836,8 +1860,8 @@ template <typename R>
 int tune_trial(clb_ctx* c, const TuneSnapshot& s, bool flow, int steps, hipEvent_t e0, hipEvent_t e1, double& ms) {
   int rc = snap_restore(c, s);
   if (rc) return rc;
-  if (!flow && fused_spec(c)) {
-    rc = prepare_fused(c);
+  if (!flow) {
+    rc = prepare_step<R>(c);
     if (rc) return rc;
   }
   auto run = [&](int n) {

@@ -57,8 +57,10 @@ struct AdeArgs {
   // reactive voxel (the water a neighbouring mineral sees), S*N.
   R* water;
   // Surrogate metabolic model (else null): the processor's increments of this step,
-  // S planes of dC then nb planes of dB, written for every reactive voxel before transport.
+  // S planes of dC then nb planes of dB, written before transport for every reactive voxel
+  // holding a surrogate organism (a biomass field of srg_pools above 1e-12), read only there.
   const R* srg_inc;
+  int srg_pools;          // bit b: biomass field b is grown by the surrogate
   Geom geo;
   int S, nb;
   int bc_type[2 * CLB_MAX_SOLUTES];
@@ -203,9 +205,14 @@ struct SrgTables {
   int has_contrib[CLB_MAX_SOLUTES];
   double contrib[CLB_MAX_SOLUTES][CLB_MAX_SOLUTES];  // [substrate][species] coefficient, 0: none
   SrgNet net[CLB_MAX_BIOMASS];
+  int lds_layers;                           // layers per network in the kernel's LDS copy (max n_layers)
 };
+// Evaluates the processor at every reactive voxel holding a surrogate organism, writing
+// its S + nb increments to inc.
+// nm, lds_layers: the tables' values (they size the kernel's LDS copy of the networks).
 template <typename R>
-void launch_surrogate(const AdeArgs<R>& a, const SrgTables* t, R* inc, bool odd, hipStream_t s);
+void launch_surrogate(const AdeArgs<R>& a, const SrgTables* t, int nm, int lds_layers, R* inc, bool odd,
+                      hipStream_t s);
 
 // CA biomass redistribution (clb_ca.hip)
 void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, int pore,

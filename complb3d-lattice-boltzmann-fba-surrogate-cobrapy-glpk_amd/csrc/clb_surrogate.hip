@@ -38,26 +38,42 @@ __device__ __forceinline__ double exhaustion_flux(double C, double B, double mw,
   return v;
 }
 
-__device__ __forceinline__ double tansig(double n) { return 2.0 / (1.0 + exp(-2.0 * n)) - 1.0; }
+// MATLAB's tansig, 2 / (1 + exp(-2n)) - 1, with the division done as a reciprocal
+// estimate refined by two Newton steps (within an ulp or two of the divided value; a
+// saturated exponent gives -1, as the division does).
+__device__ __forceinline__ double tansig(double n) {
+  const double d = 1.0 + exp(-2.0 * n);
+  if (!(d < 1e300)) return (d == d) ? -1.0 : d;
+  double r = __builtin_amdgcn_rcp(d);
+  r = r * (2.0 - d * r);
+  r = r * (2.0 - d * r);
+  return 2.0 * r - 1.0;
+}
+
+// LDS image of one network layer: W (16 x 16, row per neuron, zero padded) then b (16).
+constexpr int kLayerDoubles = CLB_SRG_MAX_WIDTH * CLB_SRG_MAX_WIDTH + CLB_SRG_MAX_WIDTH;
 
 // One microbe's network (the evaluation of surrogateModel.hh:396-427): inputs mapped onto
 // [-1, 1] (mapminmax), tansig hidden layers, one linear output neuron, the reverse map; a
 // growth rate below 1e-8 or not finite is 0. Each neuron sums W x from 0 in input order and
-// then adds its bias, as matMultp / matSum do. The weights are read at uniform addresses
-// (scalar loads); rows and 4-column blocks beyond a layer's width are skipped by uniform
-// branches, so a layer costs its own width, and the zero padding adds exact zeros.
-__device__ __forceinline__ double net_growth(const SrgNet& nt, const double* fin) {
+// then adds its bias, as matMultp / matSum do. The weights come from the block's LDS copy
+// (`w`, kLayerDoubles per layer; every lane reads the same address, a broadcast). Neurons
+// go four at a time and inputs four at a time; groups beyond a layer's width are skipped
+// by uniform branches, and the zero padding inside a group adds exact zeros (a padded
+// neuron is tansig(0) = 0).
+template <int S>
+__device__ __forceinline__ double net_growth(const SrgNet& nt, const double* w, const double* fin) {
   constexpr int W = CLB_SRG_MAX_WIDTH;
   double a[W];
 #pragma unroll
   for (int j = 0; j < W; ++j) a[j] = 0.0;
 #pragma unroll
-  for (int j = 0; j < CLB_MAX_SOLUTES; ++j) {
+  for (int j = 0; j < S; ++j) {  // n_in <= S (an input names a substrate)
     if (j < nt.n_in) {
       const int sub = nt.in_sub[j];
       double x = 0.0;
 #pragma unroll
-      for (int s = 0; s < CLB_MAX_SOLUTES; ++s)
+      for (int s = 0; s < S; ++s)
         if (sub == s) x = fin[s];
       a[j] = (x - nt.x_offset[j]) * nt.x_gain[j] + nt.x_ymin;
     }
@@ -66,22 +82,30 @@ __device__ __forceinline__ double net_growth(const SrgNet& nt, const double* fin
   for (int l = 0; l < nt.n_layers; ++l) {
     const int wl = nt.width[l];
     const bool last = (l == nt.n_layers - 1);
+    const double* Wl = w + l * kLayerDoubles;
     double z[W];
 #pragma unroll
-    for (int r = 0; r < W; ++r) {
-      double acc = 0.0;
-      if (r < wl) {
+    for (int rb = 0; rb < W; rb += 4) {
+      // four neurons at a time: four independent sums (each still in input order)
+      double acc[4] = {0.0, 0.0, 0.0, 0.0};
+      if (rb < wl) {
 #pragma unroll
         for (int cb = 0; cb < W; cb += 4) {
           if (cb < wprev) {
 #pragma unroll
-            for (int c = cb; c < cb + 4; ++c) acc += nt.W[l][r][c] * a[c];
+            for (int c = cb; c < cb + 4; ++c)
+#pragma unroll
+              for (int j = 0; j < 4; ++j) acc[j] += Wl[(rb + j) * W + c] * a[c];
           }
         }
-        acc = nt.b[l][r] + acc;
-        if (!last) acc = tansig(acc);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          acc[j] = Wl[W * W + rb + j] + acc[j];
+          if (!last) acc[j] = tansig(acc[j]);
+        }
       }
-      z[r] = acc;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) z[rb + j] = acc[j];
     }
 #pragma unroll
     for (int r = 0; r < W; ++r) a[r] = z[r];
@@ -134,17 +158,30 @@ __device__ __forceinline__ void draw_down(const SrgTables& t, const double* conc
   }
 }
 
-template <typename R, int S, bool ODD>
-__global__ void __launch_bounds__(256) srg_kernel(AdeArgs<R> a, const SrgTables* __restrict__ tp, R* inc) {
-  const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+// Whether voxel n is reactive (1 <= x <= nx-2, fluid, neither solid nor bounce-back code)
+// and holds a surrogate organism above 1e-12. Branch-free loads (a voxel past the end reads
+// the last one and is rejected), so a tile's passes issue their loads together.
+template <typename R>
+__device__ __forceinline__ bool srg_present(const AdeArgs<R>& a, const SrgTables& t, long long n) {
   const Geom& g = a.geo;
-  if (n64 >= g.N) return;
-  const int n = (int)n64;
+  const long long q = (n < g.N) ? n : g.N - 1;
+  const int x = (int)(q % g.nx);
+  const uint8_t code = a.mask[q];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < CLB_MAX_BIOMASS; ++k)
+    if (k < t.nm) any = any || (double)a.bmass[(long long)t.id[k] * g.N + q] > kThrd;
+  return any && n < g.N && x >= 1 && x <= g.nx - 2 && a.kind[code] == CLB_KIND_FLUID && code != a.solid_code &&
+         code != a.bb_code;
+}
+
+// The processor at one voxel holding a surrogate organism; writes all S + nb of its
+// increments (the transport kernel reads them only at such voxels).
+template <typename R, int S, bool ODD>
+__device__ __forceinline__ void srg_voxel(const AdeArgs<R>& a, const SrgTables& t, const double* wlds, R* inc,
+                                          int n) {
+  const Geom& g = a.geo;
   const int x = n % g.nx;
-  if (x < 1 || x > g.nx - 2) return;
-  const uint8_t code = a.mask[n];
-  if (a.kind[code] != CLB_KIND_FLUID || code == a.solid_code || code == a.bb_code) return;
-  const SrgTables& t = *tp;
   const long long N = g.N;
 
   double bm[CLB_MAX_BIOMASS], dB[CLB_MAX_BIOMASS], dC[S];
@@ -221,9 +258,7 @@ __global__ void __launch_bounds__(256) srg_kernel(AdeArgs<R> a, const SrgTables*
         if (q == k) B = bm[q];
       if (!(B > kThrd)) continue;
       const double mw = t.mw[k];
-      double fin[CLB_MAX_SOLUTES], fout[S];
-#pragma unroll
-      for (int s = 0; s < CLB_MAX_SOLUTES; ++s) fin[s] = 0.0;
+      double fin[S];
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const double kc = t.Kc[k][s];
@@ -235,35 +270,22 @@ __global__ void __launch_bounds__(256) srg_kernel(AdeArgs<R> a, const SrgTables*
         if (t.maxup[k][s] > -1e29 && floor_mag >= 0.0 && v > floor_mag) v = floor_mag;
         if (t.fixlb[s] && cap >= 0.0 && v > cap) v = cap;
         fin[s] = v;
-        fout[s] = v;
       }
       const SrgNet& nt = t.net[k];
       double mu = 0.0;
+      int closure = 0;
       if (nt.on) {
-        const double gr = net_growth(nt, fin);
+        const double gr = net_growth<S>(nt, wlds + k * t.lds_layers * kLayerDoubles, fin);
         mu = gr;
-        if (nt.closure == 1) {
-          // consume what the predicted growth needs, at most the estimate
-#pragma unroll
-          for (int s = 0; s < S; ++s)
-            if (nt.yield[s] > 0.0) {
-              double need = gr / nt.yield[s];
-              if (need > fin[s]) need = fin[s];
-              fout[s] = need;
-            }
-        } else if (nt.closure == 2) {
-          // growth limited by the scarcest substrate, consumption to match
-          double gl = gr;
+        closure = nt.closure;
+        if (closure == 2) {
+          // growth limited by the scarcest substrate
 #pragma unroll
           for (int s = 0; s < S; ++s)
             if (nt.yield[s] > 0.0) {
               const double gmax = fin[s] * nt.yield[s];
-              if (gmax < gl) gl = gmax;
+              if (gmax < mu) mu = gmax;
             }
-          mu = gl;
-#pragma unroll
-          for (int s = 0; s < S; ++s)
-            if (nt.yield[s] > 0.0) fout[s] = gl / nt.yield[s];
         }
       }
       double d = (fabs(mu) < kThrd) ? -t.decay[k] * B * a.dt  // decayToDeltaB, per second
@@ -272,8 +294,17 @@ __global__ void __launch_bounds__(256) srg_kernel(AdeArgs<R> a, const SrgTables*
 #pragma unroll
       for (int q = 0; q < CLB_MAX_BIOMASS; ++q)
         if (q == k) dB[q] = d;
+      // Fout: the estimate, or (closures) what the growth needs -- at most the estimate
+      // (closure 1), exactly growth / yield (closure 2)
 #pragma unroll
-      for (int s = 0; s < S; ++s) draw[s] += -(fout[s] * (B * mw) * a.dt / 3.6e6);  // fluxToDeltaC
+      for (int s = 0; s < S; ++s) {
+        double fout = fin[s];
+        if (closure != 0 && nt.yield[s] > 0.0) {
+          fout = mu / nt.yield[s];
+          if (closure == 1 && fout > fin[s]) fout = fin[s];
+        }
+        draw[s] += -(fout * (B * mw) * a.dt / 3.6e6);  // fluxToDeltaC
+      }
     }
 
     // ---- each substrate's draw, summed over the microbes, clamped once
@@ -304,30 +335,86 @@ __global__ void __launch_bounds__(256) srg_kernel(AdeArgs<R> a, const SrgTables*
   }
 }
 
+// One block per tile of kTile voxels: the block first gathers the tile's voxels holding a
+// surrogate organism into LDS (one LDS atomic per wavefront and pass), then its threads
+// evaluate them, all lanes busy. The organism usually sits in a small part of the domain
+// (biofilm), so most wavefronts of a plain one-voxel-per-thread launch would wait on one
+// or two lanes running the network; here a tile's few voxels share one wavefront, and no
+// global atomics or second launch are needed.
+constexpr int kTileThreads = 256;
+constexpr int kTilePasses = 8;
+constexpr int kTile = kTileThreads * kTilePasses;
+
+template <typename R, int S, bool ODD>
+__global__ void __launch_bounds__(kTileThreads) srg_tile_kernel(AdeArgs<R> a, const SrgTables* __restrict__ tp,
+                                                                R* inc) {
+  __shared__ int slot[kTile];
+  __shared__ unsigned int count;
+  extern __shared__ double wlds[];  // nm x lds_layers layers of the networks
+  const SrgTables& t = *tp;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  if (tid == 0) count = 0u;
+  __syncthreads();
+  const long long base = (long long)blockIdx.x * kTile;
+  bool pres[kTilePasses];
+#pragma unroll
+  for (int v = 0; v < kTilePasses; ++v) pres[v] = srg_present(a, t, base + (long long)v * kTileThreads + tid);
+#pragma unroll
+  for (int v = 0; v < kTilePasses; ++v) {
+    const long long n = base + (long long)v * kTileThreads + tid;
+    const bool present = pres[v];
+    const unsigned long long m = __ballot(present);
+    if (m != 0ull) {
+      const int leader = __ffsll((long long)m) - 1;
+      unsigned int at = 0u;
+      if (lane == leader) at = atomicAdd(&count, (unsigned int)__popcll(m));
+      at = __shfl(at, leader);
+      if (present) slot[at + __popcll(m & ((1ull << lane) - 1ull))] = (int)n;
+    }
+  }
+  __syncthreads();
+  const int total = (int)count;
+  if (total == 0) return;
+  // the networks' weights into LDS (only for tiles with work)
+  const int per = t.lds_layers * kLayerDoubles;
+  for (int i = tid; i < t.nm * per; i += kTileThreads) {
+    const int k = i / per, r = i - k * per, l = r / kLayerDoubles, j = r - l * kLayerDoubles;
+    const SrgNet& nt = t.net[k];
+    wlds[i] = (j < CLB_SRG_MAX_WIDTH * CLB_SRG_MAX_WIDTH) ? (&nt.W[l][0][0])[j]
+                                                          : nt.b[l][j - CLB_SRG_MAX_WIDTH * CLB_SRG_MAX_WIDTH];
+  }
+  __syncthreads();
+  for (int i = tid; i < total; i += kTileThreads) srg_voxel<R, S, ODD>(a, t, wlds, inc, slot[i]);
+}
+
 template <typename R, int S>
-void launch_srg_s(const AdeArgs<R>& a, const SrgTables* t, R* inc, bool odd, hipStream_t s) {
-  const dim3 grid((unsigned)((a.geo.N + 255) / 256));
-  if (odd) hipLaunchKernelGGL((srg_kernel<R, S, true>), grid, dim3(256), 0, s, a, t, inc);
-  else hipLaunchKernelGGL((srg_kernel<R, S, false>), grid, dim3(256), 0, s, a, t, inc);
+void launch_srg_s(const AdeArgs<R>& a, const SrgTables* t, int lds_bytes, R* inc, bool odd, hipStream_t s) {
+  const dim3 grid((unsigned)((a.geo.N + kTile - 1) / kTile));
+  if (odd) hipLaunchKernelGGL((srg_tile_kernel<R, S, true>), grid, dim3(kTileThreads), lds_bytes, s, a, t, inc);
+  else hipLaunchKernelGGL((srg_tile_kernel<R, S, false>), grid, dim3(kTileThreads), lds_bytes, s, a, t, inc);
 }
 
 }  // namespace
 
 template <typename R>
-void launch_surrogate(const AdeArgs<R>& a, const SrgTables* t, R* inc, bool odd, hipStream_t s) {
+void launch_surrogate(const AdeArgs<R>& a, const SrgTables* t, int nm, int lds_layers, R* inc, bool odd,
+                      hipStream_t s) {
+  const int lds_bytes = nm * lds_layers * kLayerDoubles * (int)sizeof(double);
   switch (a.S) {
-    case 1: launch_srg_s<R, 1>(a, t, inc, odd, s); break;
-    case 2: launch_srg_s<R, 2>(a, t, inc, odd, s); break;
-    case 3: launch_srg_s<R, 3>(a, t, inc, odd, s); break;
-    case 4: launch_srg_s<R, 4>(a, t, inc, odd, s); break;
-    case 5: launch_srg_s<R, 5>(a, t, inc, odd, s); break;
-    case 6: launch_srg_s<R, 6>(a, t, inc, odd, s); break;
-    case 7: launch_srg_s<R, 7>(a, t, inc, odd, s); break;
-    default: launch_srg_s<R, 8>(a, t, inc, odd, s); break;
+    case 1: launch_srg_s<R, 1>(a, t, lds_bytes, inc, odd, s); break;
+    case 2: launch_srg_s<R, 2>(a, t, lds_bytes, inc, odd, s); break;
+    case 3: launch_srg_s<R, 3>(a, t, lds_bytes, inc, odd, s); break;
+    case 4: launch_srg_s<R, 4>(a, t, lds_bytes, inc, odd, s); break;
+    case 5: launch_srg_s<R, 5>(a, t, lds_bytes, inc, odd, s); break;
+    case 6: launch_srg_s<R, 6>(a, t, lds_bytes, inc, odd, s); break;
+    case 7: launch_srg_s<R, 7>(a, t, lds_bytes, inc, odd, s); break;
+    default: launch_srg_s<R, 8>(a, t, lds_bytes, inc, odd, s); break;
   }
 }
 
-template void launch_surrogate<float>(const AdeArgs<float>&, const SrgTables*, float*, bool, hipStream_t);
-template void launch_surrogate<double>(const AdeArgs<double>&, const SrgTables*, double*, bool, hipStream_t);
+template void launch_surrogate<float>(const AdeArgs<float>&, const SrgTables*, int, int, float*, bool, hipStream_t);
+template void launch_surrogate<double>(const AdeArgs<double>&, const SrgTables*, int, int, double*, bool,
+                                       hipStream_t);
 
 }  // namespace clb

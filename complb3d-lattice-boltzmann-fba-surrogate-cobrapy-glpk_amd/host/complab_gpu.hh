@@ -153,6 +153,48 @@ class Lattice {
         "clb_node_conversion");
     return changed;
   }
+  // Surrogate metabolic processor (run_surrogate3D, src/complab3d_processors_surrogate.hh:
+  // 74-295): the microbes it grows (by biomass field) with their uptake settings, then each
+  // one's network. Per-microbe tables are n x S, row per microbe; empty vectors take the
+  // defaults (no Vmax, no floor, molar mass 24.6, no held substrates).
+  struct SurrogateNet {
+    int nIn = 0;
+    std::vector<int> inSub;         // input k = Fin[id][inSub[k]]
+    std::vector<int> width;         // neurons per weight layer, the last 1
+    std::vector<double> params;     // per layer W (row per neuron) then b
+    std::vector<double> xOffset, xGain;
+    double xYmin = -1.0, yYmin = -1.0, yGain = 1.0, yOffset = 0.0;
+    bool logOutput = false;
+  };
+  void setSurrogate(const std::vector<int>& ids, const std::vector<double>& Kc, const std::vector<double>& decay,
+                    const std::vector<double>& vmax = {}, const std::vector<double>& maxUptake = {},
+                    const std::vector<double>& molarMass = {}, const std::vector<int>& fixConc = {},
+                    const std::vector<int>& fixLb = {}, bool useTotals = false) {
+    const long long n = (long long)ids.size();
+    need(Kc.size(), n * S_, "Kc");
+    need(decay.size(), n, "decay");
+    if (!vmax.empty()) need(vmax.size(), n * S_, "vmax");
+    if (!maxUptake.empty()) need(maxUptake.size(), n * S_, "maxUptake");
+    if (!molarMass.empty()) need(molarMass.size(), n, "molarMass");
+    if (!fixConc.empty()) need(fixConc.size(), S_, "fixConc");
+    if (!fixLb.empty()) need(fixLb.size(), S_, "fixLb");
+    auto p = [](const auto& v) { return v.empty() ? nullptr : v.data(); };
+    chk(clb_set_surrogate(c_, (int)n, p(ids), p(Kc), p(decay), p(vmax), p(maxUptake), p(molarMass), p(fixConc),
+                          p(fixLb), useTotals ? 1 : 0),
+        "clb_set_surrogate");
+  }
+  void setSurrogateNet(int id, const SurrogateNet& net, int closure = 0, const std::vector<double>& yield = {}) {
+    if (!yield.empty()) need(yield.size(), S_, "yield");
+    chk(clb_set_surrogate_net(c_, id, net.nIn, net.inSub.data(), (int)net.width.size(), net.width.data(),
+                              net.params.data(), net.xOffset.data(), net.xGain.data(), net.xYmin, net.yYmin,
+                              net.yGain, net.yOffset, net.logOutput ? 1 : 0, closure,
+                              yield.empty() ? nullptr : yield.data()),
+        "clb_set_surrogate_net");
+  }
+  void clearSurrogateNet(int id) {
+    chk(clb_set_surrogate_net(c_, id, 0, nullptr, 0, nullptr, nullptr, nullptr, nullptr, 0, 0, 1, 0, 0, 0, nullptr),
+        "clb_set_surrogate_net");
+  }
   void setGraph(bool enable, int stepsPerGraph) { chk(clb_set_graph(c_, enable ? 1 : 0, stepsPerGraph), "clb_set_graph"); }
   void step(int n) { chk(clb_step(c_, n), "clb_step"); }
   void saveState(const std::string& path) { chk(clb_save_state(c_, path.c_str()), "clb_save_state"); }

@@ -73,6 +73,8 @@ typedef struct clb_perf {
   long long mlp_launches;
   long long fluid_nodes; /* fluid nodes of the current mask */
   long long reactive_nodes;
+  double srg_ms;         /* surrogate metabolic processor kernels */
+  long long srg_launches;
 } clb_perf;
 
 /* ---- context (replaces the MultiBlockLattice3D objects main owns, src/complab.cpp:592, 748-775) */

@@ -96,6 +96,46 @@ int main(int argc, char** argv) {
     lat.step(20);
     write_bin(dir + "/conc.bin", lat.concentrations());
     write_bin(dir + "/bmass.bin", lat.biomass());
+
+    // ---- surrogate processor: uniform field at rest, constant growth 0.4 / h, no Vmax.
+    // One step: every reactive voxel's biomass is B (1 + 0.4 dt / 3600); a voxel whose
+    // neighbours all reacted holds C - C^2 / (Kc + C) (the supply-limited draw).
+    {
+      const int sx = 10, sy = 4, sz = 4;
+      const long long sn = (long long)sx * sy * sz;
+      const double B = 0.8, dt = 50.0, Kc = 0.3, g = 0.4;
+      complab_gpu::Lattice sl(sx, sy, sz, CLB_PREC_FP64);
+      sl.setDynamics(kind, om);
+      sl.uploadMask(std::vector<uint8_t>((size_t)sn, 2));
+      sl.setSolutes(2, std::vector<double>(2 * 256, 1.0 / 0.8), {1, 1, 1, 1}, {0, 0, 0, 0});
+      sl.setVelocity(std::vector<double>((size_t)(3 * sn), 0.0));
+      sl.setBiomass(1, std::vector<double>((size_t)sn, B));
+      sl.setKinetics(CLB_KINETICS_NONE, dt, 0, 1);
+      sl.solutesInit({0.5, 0.2});
+      sl.setSurrogate({0}, {Kc, Kc}, {1e-6});
+      complab_gpu::Lattice::SurrogateNet net;
+      net.nIn = 2;
+      net.inSub = {0, 1};
+      net.width = {1};
+      net.params = {0.0, 0.0, 2.0 * g - 1.0};
+      net.xOffset = {0.0, 0.0};
+      net.xGain = {1.0, 1.0};
+      net.yGain = 2.0;
+      sl.setSurrogateNet(0, net);
+      sl.step(1);
+      const std::vector<double> sb = sl.biomass(), sc = sl.concentrations();
+      const double c0[2] = {0.5, 0.2};
+      for (long long q = 0; q < sn; ++q) {
+        const int x = (int)(q % sx);
+        const double wantB = (x >= 1 && x <= sx - 2) ? B + g * B * dt / 3600.0 : B;
+        if (std::fabs(sb[(size_t)q] - wantB) > 1e-14 * wantB) throw std::runtime_error("surrogate biomass");
+        if (x < 2 || x > sx - 3) continue;
+        for (int s = 0; s < 2; ++s) {
+          const double want = c0[s] - c0[s] * c0[s] / (Kc + c0[s]);
+          if (std::fabs(sc[(size_t)(s * sn + q)] - want) > 1e-13 * want) throw std::runtime_error("surrogate draw");
+        }
+      }
+    }
     std::printf("ok %d %lld %.0f %.6e\n", conv_batch ? 1 : 0, (long long)eq.getLastIterations(), st[3],
                 energy.back());
     return 0;

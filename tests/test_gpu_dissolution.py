@@ -153,3 +153,20 @@ def test_dissolution_reopens_voxels(clb):
     assert not ph[reopened].any()
     assert np.array_equal(reopened, rim & (c[MIN] < thr))
     lat.close()
+
+
+@pytest.mark.gpu
+def test_dissolution_autotune_before_the_first_step(clb):
+    """The autotuner runs the step on its own; the water buffer the dissolution kernels read
+    is made for it as for clb_step (regression: it was only made by the first clb_step)."""
+    print("case: dissolution, autotune before any step, then 40 steps vs no autotune", flush=True)
+    out = []
+    for tune in (True, False):
+        lat, ref, mask, phase = setup(clb, closed=False)
+        if tune:
+            assert lat.autotune(2)["trials"] > 0
+            lat.set_block_shape(clb.KERNEL_ADE, 64, 4)
+        lat.step(40)
+        out.append(lat.conc())
+        lat.close()
+    assert np.array_equal(out[0], out[1])

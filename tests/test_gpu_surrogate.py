@@ -240,3 +240,32 @@ def test_setter_checks(clb):
     lat.set_surrogate_net(1, net)
     lat.set_surrogate_net(1, None)
     lat.close()
+
+
+@pytest.mark.gpu
+def test_autotune_before_the_first_step(clb, case):
+    """The autotuner runs the step on its own; the buffers the surrogate kernel reads are set
+    up for it as for clb_step (regression: they were only made by the first clb_step)."""
+    print("case: surrogate, autotune before any step, then 20 steps vs no autotune", flush=True)
+    nets = {1: (geobacter(), 0, None)}
+    pack, kind, u, b = case
+    out = []
+    for tune in (True, False):
+        lat = clb.Lattice(NX, NY, NZ, precision=1)
+        lat.set_dynamics(kind, np.full(256, 1.25))
+        lat.upload_mask(pack)
+        lat.set_solutes(OM, BCT, BCV)
+        lat.set_velocity(u)
+        lat.set_biomass(b.reshape(2, -1))
+        lat.set_kinetics(clb.KINETICS_PLUGIN, DT, 0, 1)
+        lat.solutes_init(C0)
+        lat.set_surrogate(**SHIPPED)
+        lat.set_surrogate_net(1, geobacter())
+        if tune:
+            rep = lat.autotune(2)
+            assert rep["trials"] > 0
+            lat.set_block_shape(clb.KERNEL_ADE, 64, 4)  # the same arithmetic order either way
+        lat.step(20)
+        out.append((lat.conc(), lat.biomass()))
+        lat.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
