@@ -41,10 +41,10 @@ BYTES_KIN4 = 4 * 56 + 12 + 1 + 8  # + biomass read and write
 # + 4 age + 1 flags; gather 1 mask + 1 flags; the sweep's biomass copy 8
 BYTES_CA_SWEEP = 21 + 14 + 2 + 8
 # Surrogate processor kernel, FP32 store, 4 solutes, 2 biomass fields (one surrogate): every
-# reactive voxel reads its mask byte and the surrogate field and writes 4 + 2 increments; a
-# voxel holding the organism also reads its 28 incoming populations.
-BYTES_SRG_REACTIVE = 1 + 4 + 6 * 4
-BYTES_SRG_PRESENT = 28 * 4
+# voxel's mask byte and surrogate field are scanned; a voxel holding the organism also reads its
+# 28 incoming populations and writes its 4 + 2 increments.
+BYTES_SRG_SCAN = 1 + 4
+BYTES_SRG_PRESENT = 28 * 4 + 6 * 4
 # FP64 flops of the 2-10-10-10-10-1 network per evaluation (mapminmax, 4 tansig layers
 # with exp counted as 20, linear output)
 FLOP_SRG_NET = 2 * 2 + 2 * (2 * 10 + 3 * 100) + 4 * 10 * (20 + 4) + 2 * 10 + 4
@@ -52,9 +52,9 @@ FLOP_SRG_NET = 2 * 2 + 2 * (2 * 10 + 3 * 100) + 4 * 10 * (20 + 4) + 2 * 10 + 4
 # workloads, scaled by a copy kernel of known size (profiles/hbm_traffic_fetch_write.md);
 # mean of the odd and even steps. Reported only when the workload is the default one.
 MEASURED_TRAFFIC = {
-    "full_coupled_384x192x192": (6.414e9 + 6.321e9) / 2,
-    "flow_384x192x192": (1.625e9 + 1.542e9) / 2,
-    "kinetics_256x128x128": 1.037e9,
+    "full_coupled_384x192x192": (6.420e9 + 6.356e9) / 2,
+    "flow_384x192x192": (1.588e9 + 1.514e9) / 2,
+    "kinetics_256x128x128": (1.032e9 + 1.009e9) / 2,
 }
 
 
@@ -252,7 +252,7 @@ def surrogate_leg(clb, steps, tune_steps):
     present = reactive & (lat.biomass()[1] > 1e-12)
     lat.close()
     sms = p.srg_ms / max(p.srg_launches, 1)
-    nbytes = BYTES_SRG_REACTIVE * reactive.sum() + BYTES_SRG_PRESENT * present.sum()
+    nbytes = BYTES_SRG_SCAN * N + BYTES_SRG_PRESENT * present.sum()
     out = dict(workload="kinetics + surrogate 256x128x128, 4 solutes, 2 biomass fields (1 surrogate), FP32 store",
                ms_per_step=wall * 1e3 / steps, mlups=N * steps / wall / 1e6, srg_kernel_ms=sms,
                transport_kernel_ms=p.ade_ms / max(p.ade_launches, 1), reactive=int(reactive.sum()),
