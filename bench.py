@@ -216,10 +216,20 @@ def ca_leg(lat, N, bmax=100.0, grow=2.5):
     log(f"CA: {rep['sweeps']} sweeps + {rep['age_passes']} age passes in {wall * 1e3:.2f} ms ({ms:.3f} ms/sweep), "
         f"max {rep['bmax_before']:.1f} -> {rep['bmax_after']:.4f}, status {rep['status']}, "
         f"{rep['mask_changes']} mask flips")
+    # the FD biofilm step on the same state (two mask / total passes around one diffusion step)
+    lat.fd_biomass(0.1)
+    lat.sync()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        lat.fd_biomass(0.1)
+    lat.sync()
+    fd_ms = (time.perf_counter() - t0) * 1e3 / 10
+    log(f"FD biofilm step: {fd_ms:.3f} ms per call")
     return dict(workload="CA redistribution, kinetics case 256x128x128, biomass x2.5", sweeps=rep["sweeps"],
                 age_passes=rep["age_passes"], status=rep["status"], wall_ms=wall * 1e3, ms_per_sweep=ms,
                 voxel_sweeps_per_s=N * sweeps / wall, gbs=BYTES_CA_SWEEP * N / (ms * 1e-3) / 1e9,
-                bmax_before=rep["bmax_before"], bmax_after=rep["bmax_after"], mask_changes=rep["mask_changes"])
+                bmax_before=rep["bmax_before"], bmax_after=rep["bmax_after"], mask_changes=rep["mask_changes"],
+                fd_ms_per_call=fd_ms)
 
 
 def surrogate_leg(clb, steps, tune_steps):

@@ -127,6 +127,7 @@ def load():
                                   c_double, c_int, c_int, vp],
         "clb_ca_redistribute": [vp, c_double, c_int, ctypes.c_ulonglong, vp],
         "clb_ca_update_age": [vp, c_double, c_int, vp],
+        "clb_fd_biomass": [vp, c_double, vp],
         "clb_ca_download_age": [vp, vp],
         "clb_ca_set_age": [vp, vp],
         "clb_flow_stats": [vp, vp],
@@ -622,6 +623,13 @@ class Lattice:
         r = CaReport()
         self._chk(self.lib.clb_ca_redistribute(self.h, float(bmax), int(half), int(seed), ctypes.addressof(r)),
                   "ca_redistribute")
+        return {k: getattr(r, k) for k, _ in CaReport._fields_}
+
+    def fd_biomass(self, nu):
+        """One step of the FD biofilm solver (mask/totals, explicit diffusion, mask/totals);
+        returns the report as a dict."""
+        r = CaReport()
+        self._chk(self.lib.clb_fd_biomass(self.h, float(nu), ctypes.addressof(r)), "fd_biomass")
         return {k: getattr(r, k) for k, _ in CaReport._fields_}
 
     def ca_update_age(self, bmax, max_passes=1000):

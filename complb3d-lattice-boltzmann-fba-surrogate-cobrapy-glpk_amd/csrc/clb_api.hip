@@ -2110,9 +2110,48 @@ int ca_redistribute_t(clb_ctx* c, double bmax, int half, unsigned long long seed
   return CLB_OK;
 }
 
+// The FD biofilm solver's block of the loop (src/complab.cpp:1716-1731): mask and totals,
+// the starting copy, one fdDiffusion3D step, mask and totals again.
+template <typename R>
+int fd_biomass_t(clb_ctx* c, double nu, clb_ca_report* rep) {
+  const size_t bbytes = (size_t)c->nb * (size_t)c->geo.N * (c->fp32 ? sizeof(float) : sizeof(double));
+  if (c->bold_bytes != bbytes) {
+    int rc = dalloc(c, (uint8_t**)&c->d_bold, bbytes);
+    c->bold_bytes = rc ? 0 : bbytes;
+    if (rc) return rc;
+  }
+  *rep = clb_ca_report{};
+  double mx = 0.0;
+  int rc = ca_mask_total<R>(c, false, &mx, &rep->mask_changes);
+  if (rc) return rc;
+  rep->bmax_before = mx;
+  CLB_CHECK_HIP(c, hipMemcpyAsync(c->d_bold, c->d_b, bbytes, hipMemcpyDeviceToDevice, c->stream));
+  launch_fd_biomass<R>((const R*)c->d_bold, (R*)c->d_b, c->nb, c->d_mask, c->geo, nu, c->stream);
+  rep->sweeps = 1;
+  rc = ca_mask_total<R>(c, false, &mx, &rep->mask_changes);
+  if (rc) return rc;
+  rep->bmax_after = mx;
+  if (!std::isfinite(mx)) rep->status = CLB_CA_NONFINITE;
+  return CLB_OK;
+}
+
 }  // namespace
 
 extern "C" {
+
+int clb_fd_biomass(clb_ctx* c, double nu, clb_ca_report* report) {
+  if (!c || !report || !(nu >= 0.0) || !std::isfinite(nu)) return CLB_ERR_ARG;
+  if (!c->ca_ready) return fail(c, CLB_ERR_STATE, "clb_ca_init first");
+  if (c->nb < 1 || !c->d_b) return fail(c, CLB_ERR_STATE, "no biomass");
+  int rc = flush_pending(c);  // the mask may change under deferred increments
+  if (rc) return rc;
+  rc = c->fp32 ? fd_biomass_t<float>(c, nu, report) : fd_biomass_t<double>(c, nu, report);
+  if (report->mask_changes > 0) {
+    c->list_dirty = true;
+    c->fluid_dirty = true;
+  }
+  return rc;
+}
 
 int clb_ca_init(clb_ctx* c) {
   if (!c) return CLB_ERR_ARG;

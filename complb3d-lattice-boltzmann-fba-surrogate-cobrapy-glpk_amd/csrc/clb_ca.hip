@@ -4,6 +4,7 @@
 //   updateAgeDistance3D                            src/complab3d_processors_part2.hh:718-842
 //   createAgeDomain3D                              src/complab3d_processors_part3.hh:347-367
 //   the sweep loop                                 src/complab.cpp:1453-1731
+//   fdDiffusion3D (the FD biofilm solver)          src/complab3d_processors_part2.hh:257-430
 //
 // The reference sweeps voxels in place, in loop order, with std::random_shuffle and
 // std::rand: its result depends on the visiting order and on the clock-seeded generator.
@@ -306,6 +307,63 @@ __global__ void ca_init_kernel(const uint8_t* mask, int* age, uint8_t* dpos, Geo
   dpos[n] = wall ? 0 : 1;
 }
 
+// fdDiffusion3D: one explicit step B += nu * (the 7-point Laplacian of B) for every biomass
+// field, read from the step's starting copy b_old, written to b, at the voxels of
+// 1 <= x <= nx-2 whose mask code is above 1. Per axis, as the reference decides it:
+//  * at the low end of the axis (x = 1, y = 0, z = 0), or when the low neighbour's code is
+//    below 2: the low neighbour is replaced by the voxel itself and the high one is read;
+//  * else at the high end (x = nx-2, y = ny-1, z = nz-1), or when the high neighbour's code
+//    is below 2: the high neighbour is the voxel itself and the low one is read;
+//  * else both are read.
+// (A voxel with walls on both sides of an axis therefore reads its high neighbour, a wall;
+// at y = ny-1 or z = nz-1 that read wraps to the other side, as the lattice does in y and
+// z.) Values below 1e-14 count as 0; a result at or below 1e-14 is not written, so the
+// voxel keeps its value.
+constexpr double kFdThrd = 1e-14;  // COMPLAB_THRD, src/complab_functions.hh:64
+
+__device__ __forceinline__ int fd_axis_mode(int c, int lo, int hi, int m_lo, int m_hi) {
+  // m_lo / m_hi: the neighbours' codes, 0 where the reference does not read them
+  if (c == lo || m_lo < 2) return 1;
+  if (c == hi || m_hi < 2) return 2;
+  return 0;
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) fd_biomass_kernel(const R* b_old, R* b, int nb, const uint8_t* mask, Geom g,
+                                                         double nu) {
+  const long long n = blockIdx.x * 256LL + threadIdx.x;
+  if (n >= g.N) return;
+  const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
+  if (x < 1 || x > g.nx - 2 || mask[n] < 2) return;
+  const long long sy = g.nx, sz = (long long)g.nx * g.ny;
+  const long long xp = n + 1, xm = n - 1;
+  const long long yp = (y == g.ny - 1) ? n - (g.ny - 1) * sy : n + sy;
+  const long long ym = (y == 0) ? n + (g.ny - 1) * sy : n - sy;
+  const long long zp = (z == g.nz - 1) ? n - (g.nz - 1) * sz : n + sz;
+  const long long zm = (z == 0) ? n + (g.nz - 1) * sz : n - sz;
+  const int x_hi = g.nx - 2;
+  const int mx = fd_axis_mode(x, 1, x_hi, (x == 1) ? 0 : mask[xm], (x == x_hi && x != 1) ? 0 : mask[xp]);
+  const int my = fd_axis_mode(y, 0, g.ny - 1, (y == 0) ? 0 : mask[ym], (y == g.ny - 1 && y != 0) ? 0 : mask[yp]);
+  const int mz = fd_axis_mode(z, 0, g.nz - 1, (z == 0) ? 0 : mask[zm], (z == g.nz - 1 && z != 0) ? 0 : mask[zp]);
+  for (int k = 0; k < nb; ++k) {
+    const R* o = b_old + (long long)k * g.N;
+    const double b0r = (double)o[n];
+    double bxp = (mx == 2) ? b0r : (double)o[xp], bxn = (mx == 1) ? b0r : (double)o[xm];
+    double byp = (my == 2) ? b0r : (double)o[yp], byn = (my == 1) ? b0r : (double)o[ym];
+    double bzp = (mz == 2) ? b0r : (double)o[zp], bzn = (mz == 1) ? b0r : (double)o[zm];
+    double b0 = b0r;
+    if (b0 < kFdThrd) b0 = 0.0;
+    if (bxp < kFdThrd) bxp = 0.0;
+    if (bxn < kFdThrd) bxn = 0.0;
+    if (byp < kFdThrd) byp = 0.0;
+    if (byn < kFdThrd) byn = 0.0;
+    if (bzp < kFdThrd) bzp = 0.0;
+    if (bzn < kFdThrd) bzn = 0.0;
+    const double nb_new = b0 + nu * ((bxp - 2 * b0 + bxn) + (byp - 2 * b0 + byn) + (bzp - 2 * b0 + bzn));
+    if (nb_new > kFdThrd) b[(long long)k * g.N + n] = (R)nb_new;
+  }
+}
+
 inline unsigned blocks_of(long long N) { return (unsigned)((N + 255) / 256); }
 // fixed grid of the reducing kernels: a few blocks per CU, each thread over several voxels
 inline unsigned reduce_blocks(long long N) {
@@ -347,6 +405,14 @@ void launch_ca_age(const int* age_in, int* age_out, const uint8_t* dpos, const d
                      changed);
 }
 
+template <typename R>
+void launch_fd_biomass(const R* b_old, R* b, int nb, const uint8_t* mask, const Geom& g, double nu, hipStream_t s) {
+  hipLaunchKernelGGL((fd_biomass_kernel<R>), dim3(blocks_of(g.N)), dim3(256), 0, s, b_old, b, nb, mask, g, nu);
+}
+
+template void launch_fd_biomass<float>(const float*, float*, int, const uint8_t*, const Geom&, double, hipStream_t);
+template void launch_fd_biomass<double>(const double*, double*, int, const uint8_t*, const Geom&, double,
+                                        hipStream_t);
 template void launch_ca_mask_total<float>(const float*, int, uint8_t*, double*, const Geom&, int, int, int, int,
                                           double, bool, unsigned long long*, hipStream_t);
 template void launch_ca_mask_total<double>(const double*, int, uint8_t*, double*, const Geom&, int, int, int, int,

@@ -228,6 +228,9 @@ void launch_ca_gather(const R* b_old, R* b, int nb, const uint8_t* mask, const d
                       const uint8_t* flags, const Geom& g, int solid, int bb, hipStream_t s);
 void launch_ca_age(const int* age_in, int* age_out, const uint8_t* dpos, const double* btot, const Geom& g,
                    double bmax, unsigned long long* changed, hipStream_t s);
+// The FD biofilm solver's explicit diffusion step (fdDiffusion3D) from the copy b_old into b.
+template <typename R>
+void launch_fd_biomass(const R* b_old, R* b, int nb, const uint8_t* mask, const Geom& g, double nu, hipStream_t s);
 
 // MLP kernels (FP32 training, BF16 MFMA inference)
 struct MlpDims {

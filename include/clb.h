@@ -289,6 +289,15 @@ int clb_ca_redistribute(clb_ctx* ctx, double bmax, int half, unsigned long long 
 int clb_ca_update_age(clb_ctx* ctx, double bmax, int max_passes, int* passes);
 int clb_ca_download_age(clb_ctx* ctx, int* age /*N*/);
 int clb_ca_set_age(clb_ctx* ctx, const int* age /*N*/);
+/* The finite-difference biofilm solver's block of the loop (src/complab.cpp:1716-1731):
+ * mask and totals (as in the CA), then one explicit step of fdDiffusion3D
+ * (src/complab3d_processors_part2.hh:257-430), B += nu * (7-point Laplacian of B) for every
+ * biomass field at the voxels of 1 <= x <= nx-2 whose mask code is above 1, with zero flux
+ * against codes 0 and 1 and the domain's ends; then mask and totals again. nu: the biomass
+ * diffusion number (bioNUinPore of the first microbe, src/complab.cpp:729-736). The report's
+ * sweeps is 1; mask_changes counts both updates (nonzero => the caller updates the flow).
+ * Needs clb_ca_init (the totals lattice). */
+int clb_fd_biomass(clb_ctx* ctx, double nu, clb_ca_report* report);
 
 /* ---- batched speciation (the per-voxel solve of EquilibriumChemistry, standalone)
  * totals: n*nc component totals; var_species[nvar] + logk_var[n*nvar]: per-voxel logK of

@@ -300,8 +300,64 @@ ORC_API void orc_ca_redistribute(int nx, int ny, int nz, int nb, double* b, uint
 }
 
 // Age passes alone (src/complab.cpp:1739 runs one); returns passes, negative at the cap.
+// fdDiffusion3D::process (src/complab3d_processors_part2.hh:276-423), bdryGap 1, over the
+// copy `c` into `b` (nb fields). The neighbour reads past the last y or z plane (made only
+// when a voxel there has a wall below it) wrap, as the lattice does in y and z.
+void fd_step(const Box& g, int nb, const double* c, double* b, const uint8_t* mask, double nu) {
+  const int gap = 1;
+  auto M = [&](int x, int y, int z) { return (int)mask[g.at(x, (y + g.ny) % g.ny, (z + g.nz) % g.nz)]; };
+  for (int z = 0; z < g.nz; ++z)
+    for (int y = 0; y < g.ny; ++y)
+      for (int x = gap; x <= g.nx - 1 - gap; ++x) {
+        if (!(M(x, y, z) > 1)) continue;
+        int mxp = 0, mxn = 0, myp = 0, myn = 0, mzp = 0, mzn = 0;
+        if (x == gap) mxp = M(x + 1, y, z);
+        else if (x == g.nx - 1 - gap) mxn = M(x - 1, y, z);
+        else { mxp = M(x + 1, y, z); mxn = M(x - 1, y, z); }
+        if (y == 0) myp = M(x, y + 1, z);
+        else if (y == g.ny - 1) myn = M(x, y - 1, z);
+        else { myp = M(x, y + 1, z); myn = M(x, y - 1, z); }
+        if (z == 0) mzp = M(x, y, z + 1);
+        else if (z == g.nz - 1) mzn = M(x, y, z - 1);
+        else { mzp = M(x, y, z + 1); mzn = M(x, y, z - 1); }
+        for (int k = 0; k < nb; ++k) {
+          const double* ck = c + (long)k * g.N;
+          auto B = [&](int xx, int yy, int zz) { return ck[g.at(xx, (yy + g.ny) % g.ny, (zz + g.nz) % g.nz)]; };
+          double b0 = B(x, y, z), bxp, bxn, byp, byn, bzp, bzn;
+          if (x == gap || mxn < 2) { bxp = B(x + 1, y, z); bxn = b0; }
+          else if (x == g.nx - 1 - gap || mxp < 2) { bxp = b0; bxn = B(x - 1, y, z); }
+          else { bxp = B(x + 1, y, z); bxn = B(x - 1, y, z); }
+          if (y == 0 || myn < 2) { byp = B(x, y + 1, z); byn = b0; }
+          else if (y == g.ny - 1 || myp < 2) { byp = b0; byn = B(x, y - 1, z); }
+          else { byp = B(x, y + 1, z); byn = B(x, y - 1, z); }
+          if (z == 0 || mzn < 2) { bzp = B(x, y, z + 1); bzn = b0; }
+          else if (z == g.nz - 1 || mzp < 2) { bzp = b0; bzn = B(x, y, z - 1); }
+          else { bzp = B(x, y, z + 1); bzn = B(x, y, z - 1); }
+          double* v[7] = {&b0, &bxp, &bxn, &byp, &byn, &bzp, &bzn};
+          for (double* p : v)
+            if (*p < kThrdMask) *p = 0.0;
+          const double nbm = b0 + nu * ((bxp - 2 * b0 + bxn) + (byp - 2 * b0 + byn) + (bzp - 2 * b0 + bzn));
+          if (nbm > kThrdMask) b[(long)k * g.N + g.at(x, y, z)] = nbm;
+        }
+      }
+}
+
 ORC_API int orc_ca_update_age(int nx, int ny, int nz, int* age, const uint8_t* dpos, const double* btot, double bmax,
                               int max_passes) {
   const Box g{nx, ny, nz, (long)nx * ny * nz};
   return age_relax(g, age, dpos, btot, bmax, max_passes);
+}
+
+// The FD biofilm solver's block (src/complab.cpp:1716-1731): mask and totals, the copy, one
+// fdDiffusion3D step, mask and totals. rep[0] mask flips; mx[0] max total before, mx[1] after.
+ORC_API void orc_fd_biomass(int nx, int ny, int nz, int nb, double* b, uint8_t* mask, double* btot, int solid, int bb,
+                            int pore, int biofilm, double bthr, double nu, long long* rep, double* mx) {
+  const Box g{nx, ny, nz, (long)nx * ny * nz};
+  const Codes k{solid, bb, pore, biofilm};
+  long long flips = 0;
+  mx[0] = mask_total(g, k, nb, b, mask, btot, bthr, false, &flips);
+  std::vector<double> copy(b, b + (long)nb * g.N);
+  fd_step(g, nb, copy.data(), b, mask, nu);
+  mx[1] = mask_total(g, k, nb, b, mask, btot, bthr, false, &flips);
+  rep[0] = flips;
 }

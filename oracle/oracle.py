@@ -40,6 +40,7 @@ def load():
         "orc_ca_redistribute": ([ci, ci, ci, ci, vp, vp, vp, vp, vp, ci, ci, ci, ci, cd, cd, ci,
                                  ctypes.c_uint64, ctypes.c_uint64, vp, vp], None),
         "orc_ca_update_age": ([ci, ci, ci, vp, vp, vp, cd, ci], ci),
+        "orc_fd_biomass": ([ci, ci, ci, ci, vp, vp, vp, ci, ci, ci, ci, cd, cd, vp, vp], None),
         "orc_classify_solid": ([ci, ci, ci, vp, vp, ci, ci], None),
         "orc_mlp_forward": ([ci, ci, ci, vp, cl, vp, vp], None),
         "orc_mlp_grad": ([ci, ci, ci, vp, cl, vp, vp, vp], cd),
@@ -272,6 +273,16 @@ class CaState:
                                 _p(rep), _p(mx))
         self.sweep += int(rep[0])
         return int(rep[0]), int(rep[1]), int(rep[2]), int(rep[3]), float(mx[0]), float(mx[1])
+
+    def fd_biomass(self, nu):
+        """The FD biofilm solver's block: mask/totals, one fdDiffusion3D step, mask/totals;
+        returns (mask flips, max total before, max total after)."""
+        rep = np.zeros(1, dtype=np.int64)
+        mx = np.zeros(2)
+        s, bb, p, bf = self.codes
+        load().orc_fd_biomass(self.nx, self.ny, self.nz, self.b.shape[0], _p(self.b), _p(self.mask), _p(self.btot),
+                              s, bb, p, bf, self.bthr, nu, _p(rep), _p(mx))
+        return int(rep[0]), float(mx[0]), float(mx[1])
 
     def update_age(self, bmax, max_passes=1000):
         return load().orc_ca_update_age(self.nx, self.ny, self.nz, _p(self.age), _p(self.dpos), _p(self.btot), bmax,
