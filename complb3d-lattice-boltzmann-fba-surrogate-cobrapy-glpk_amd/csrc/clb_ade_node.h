@@ -28,7 +28,9 @@ namespace clb {
 
 namespace {
 
-template <typename R, int S, bool ODD>
+// XCHK = false for a node with 1 <= x <= nx-2, whose x neighbours always exist: no checks,
+// so the loads and stores need no branches.
+template <typename R, int S, bool ODD, bool XCHK = true>
 __device__ __forceinline__ void load7(const AdeArgs<R>& a, const Nbr& nb, int n, R* p) {
   const long long N = a.geo.N;
 #pragma unroll
@@ -44,7 +46,7 @@ __device__ __forceinline__ void load7(const AdeArgs<R>& a, const Nbr& nb, int n,
       if (!ODD) {
         p[s * 7 + i] = a.g[(long long)(s * 7 + i) * N + n];
       } else {
-        if (nb.xok(-Q7::cx[i]))
+        if (!XCHK || nb.xok(-Q7::cx[i]))
           p[s * 7 + i] = a.g[(long long)(s * 7 + opp_dir(i)) * N + nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i])];
         else
           p[s * 7 + i] = R(0);
@@ -53,7 +55,7 @@ __device__ __forceinline__ void load7(const AdeArgs<R>& a, const Nbr& nb, int n,
   }
 }
 
-template <typename R, int S, bool ODD>
+template <typename R, int S, bool ODD, bool XCHK = true>
 __device__ __forceinline__ void store7(const AdeArgs<R>& a, const Nbr& nb, int n, const R* p) {
   const long long N = a.geo.N;
 #pragma unroll
@@ -67,7 +69,8 @@ __device__ __forceinline__ void store7(const AdeArgs<R>& a, const Nbr& nb, int n
       if (!ODD) {
         a.g[(long long)(s * 7 + opp_dir(i)) * N + n] = p[s * 7 + i];
       } else {
-        if (nb.xok(Q7::cx[i])) a.g[(long long)(s * 7 + i) * N + nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])] = p[s * 7 + i];
+        if (!XCHK || nb.xok(Q7::cx[i]))
+          a.g[(long long)(s * 7 + i) * N + nb.at(Q7::cx[i], Q7::cy[i], Q7::cz[i])] = p[s * 7 + i];
       }
     }
   }
@@ -85,7 +88,7 @@ __device__ __forceinline__ void store7(const AdeArgs<R>& a, const Nbr& nb, int n
 //  * a population entering through an x face is rebuilt by the boundary completion.
 // Face nodes and walls never hold an increment. An immobile solute takes its whole dC in
 // the rest population. Branch-free, like inject.
-template <typename R, int S>
+template <typename R, int S, bool XCHK = true>
 __device__ __forceinline__ void add_pending(const AdeArgs<R>& a, const Nbr& nb, int n, R* p) {
   const long long N = a.geo.N;
   int src[7];
@@ -95,7 +98,7 @@ __device__ __forceinline__ void add_pending(const AdeArgs<R>& a, const Nbr& nb, 
 #pragma unroll
   for (int i = 1; i < 7; ++i) {
     wall[i] = false;
-    if (nb.xok(-Q7::cx[i])) {
+    if (!XCHK || nb.xok(-Q7::cx[i])) {
       const int m = nb.at(-Q7::cx[i], -Q7::cy[i], -Q7::cz[i]);
       wall[i] = a.kind[a.mask[m]] != CLB_KIND_FLUID;
       src[i] = wall[i] ? n : m;

@@ -82,8 +82,8 @@ ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
   if (fluid) {
     const Nbr nb(x, y, z, g);
     R p[S * 7];
-    load7<R, S, ODD>(a, nb, n, p);
-    add_pending<R, S>(a, nb, n, p);
+    load7<R, S, ODD, false>(a, nb, n, p);
+    add_pending<R, S, false>(a, nb, n, p);
 #pragma unroll
     for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&p[s * 7]) + a.c_ref[s];
     ade_node<R, S, true, false>(a, x, y, z, n, code, p, nullptr);
@@ -92,7 +92,7 @@ ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
       for (int s = 0; s < S; ++s) sc[s][slot] = fmax((double)pop_sum(&p[s * 7]) + (double)a.c_ref[s], kMinConc);
       snode[slot] = n;
     }
-    store7<R, S, ODD>(a, nb, n, p);
+    store7<R, S, ODD, false>(a, nb, n, p);
   }
   if (valid && (x == 1 || x == g.nx - 2)) {
     const int xb = (x == 1) ? 0 : g.nx - 1;

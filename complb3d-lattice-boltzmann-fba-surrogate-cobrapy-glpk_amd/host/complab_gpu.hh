@@ -215,6 +215,48 @@ class Lattice {
     chk(clb_download_biomass(c_, b.data()), "clb_download_biomass");
     return b;
   }
+  // Execution choices with no reference counterpart (clb.h documents each).
+  void setSpeciationMode(int mode) { chk(clb_set_speciation_mode(c_, mode), "clb_set_speciation_mode"); }
+  void setFusedBudget(int evals) { chk(clb_set_fused_budget(c_, evals), "clb_set_fused_budget"); }
+  void setBlockShape(int kernel, int bx, int by) { chk(clb_set_block_shape(c_, kernel, bx, by), "clb_set_block_shape"); }
+
+  // Mineral dissolution (surfaceDissolutionKinetics3D) and one solute's field at rest.
+  void setDissolution(bool enable, const std::vector<int>& phaseIds, const std::vector<int>& substrates,
+                      bool surfaceOnly = true) {
+    need(substrates.size(), (long long)phaseIds.size(), "substrates");
+    chk(clb_set_dissolution(c_, enable ? 1 : 0, (int)phaseIds.size(), phaseIds.data(), substrates.data(),
+                            surfaceOnly ? 1 : 0),
+        "clb_set_dissolution");
+  }
+  void setSoluteField(int solute, const std::vector<double>& c) {
+    need(c.size(), nodes(), "field");
+    chk(clb_set_solute_field(c_, solute, c.data()), "clb_set_solute_field");
+  }
+
+  // Biofilm: CA redistribution (the CA solver) and the FD step (the FD solver).
+  void caInit() { chk(clb_ca_init(c_), "clb_ca_init"); }
+  clb_ca_report caRedistribute(double bmax, bool half = false, unsigned long long seed = 1) {
+    clb_ca_report r{};
+    chk(clb_ca_redistribute(c_, bmax, half ? 1 : 0, seed, &r), "clb_ca_redistribute");
+    return r;
+  }
+  int caUpdateAge(double bmax, int maxPasses = 1) {
+    int passes = 0;
+    chk(clb_ca_update_age(c_, bmax, maxPasses, &passes), "clb_ca_update_age");
+    return passes;
+  }
+  std::vector<int> caAge() const {
+    std::vector<int> a((size_t)nodes());
+    chk(clb_ca_download_age(c_, a.data()), "clb_ca_download_age");
+    return a;
+  }
+  void caSetAge(const std::vector<int>& age) { need(age.size(), nodes(), "age"); chk(clb_ca_set_age(c_, age.data()), "clb_ca_set_age"); }
+  clb_ca_report fdBiomass(double nu) {
+    clb_ca_report r{};
+    chk(clb_fd_biomass(c_, nu, &r), "clb_fd_biomass");
+    return r;
+  }
+
   std::array<long long, 3> speciationStats() const {
     std::array<long long, 3> s{};
     chk(clb_speciation_stats(c_, s.data()), "clb_speciation_stats");

@@ -136,6 +136,29 @@ int main(int argc, char** argv) {
         }
       }
     }
+
+    // ---- FD biofilm step through the mirror: a spike in an open box keeps its mass and
+    // sends nu b to each face neighbour
+    {
+      const int fx = 9, fy = 7, fz = 7;
+      const long long fn = (long long)fx * fy * fz;
+      complab_gpu::Lattice fl(fx, fy, fz, CLB_PREC_FP64);
+      fl.setDynamics(kind, om);
+      fl.uploadMask(std::vector<uint8_t>((size_t)fn, 2));
+      std::vector<double> fb((size_t)fn, 0.0);
+      const long long c = 4 + fx * (3 + fy * 3);
+      fb[(size_t)c] = 10.0;
+      fl.setBiomass(1, fb);
+      fl.setKinetics(CLB_KINETICS_NONE, 1.0, 0, 1);
+      fl.setMaskUpdate(true, 1e9, 2, 3);
+      fl.caInit();
+      const clb_ca_report r = fl.fdBiomass(0.1);
+      const std::vector<double> out = fl.biomass();
+      double total = 0.0;
+      for (double v : out) total += v;
+      if (r.sweeps != 1 || out[(size_t)c] != 4.0 || out[(size_t)(c + 1)] != 1.0 || std::fabs(total - 10.0) > 1e-12)
+        throw std::runtime_error("fd biomass");
+    }
     std::printf("ok %d %lld %.0f %.6e\n", conv_batch ? 1 : 0, (long long)eq.getLastIterations(), st[3],
                 energy.back());
     return 0;
