@@ -53,7 +53,9 @@ __device__ __forceinline__ void regularize(R* p, R rhobar) {
   }
 }
 
-template <typename R, bool ODD>
+// XCHK = false for a node with 1 <= x <= nx-2, whose x neighbours always exist: no checks,
+// so the loads and stores need no branches.
+template <typename R, bool ODD, bool XCHK = true>
 __device__ __forceinline__ void load19(const R* __restrict__ f, long long N, const Nbr& nb, int n, R* p) {
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
@@ -61,22 +63,48 @@ __device__ __forceinline__ void load19(const R* __restrict__ f, long long N, con
       p[i] = f[(long long)i * N + n];
     } else {
       const int o = opp_dir(i);
-      if (nb.xok(-Q19::cx[i])) p[i] = f[(long long)o * N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])];
+      if (!XCHK || nb.xok(-Q19::cx[i])) p[i] = f[(long long)o * N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])];
       else p[i] = R(0);
     }
   }
 }
 
-template <typename R, bool ODD>
+template <typename R, bool ODD, bool XCHK = true>
 __device__ __forceinline__ void store19(R* __restrict__ f, long long N, const Nbr& nb, int n, const R* p) {
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
     if (!ODD) {
       f[(long long)opp_dir(i) * N + n] = p[i];
     } else {
-      if (nb.xok(Q19::cx[i])) f[(long long)i * N + nb.at(Q19::cx[i], Q19::cy[i], Q19::cz[i])] = p[i];
+      if (!XCHK || nb.xok(Q19::cx[i])) f[(long long)i * N + nb.at(Q19::cx[i], Q19::cy[i], Q19::cz[i])] = p[i];
     }
   }
+}
+
+// One fluid node: load, boundary regularization on an x face (FACE -1: x = 0, +1: x = nx-1,
+// 0: interior), BGK collision, store. Returns |u|^2 / 2 for the energy tracer.
+template <typename R, bool ODD, int FACE>
+__device__ __forceinline__ double flow_node(const FlowArgs<R>& a, int x, int y, int z, int n, uint8_t code) {
+  const Geom& g = a.g;
+  const Nbr nb(x, y, z, g);
+  R p[19];
+  load19<R, ODD, FACE != 0>(a.f, g.N, nb, n, p);
+  if (FACE == -1) regularize<R, -1>(p, a.rhobar_in);
+  if (FACE == 1) regularize<R, 1>(p, a.rhobar_out);
+  R rhobar = 0, jx = 0, jy = 0, jz = 0;
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    rhobar += p[i];
+    jx += R(Q19::cx[i]) * p[i];
+    jy += R(Q19::cy[i]) * p[i];
+    jz += R(Q19::cz[i]) * p[i];
+  }
+  const R om = a.omega[code];
+  const R jsq = jx * jx + jy * jy + jz * jz;
+#pragma unroll
+  for (int i = 0; i < 19; ++i) p[i] = (R(1) - om) * p[i] + om * feq19<R>(i, rhobar, jx, jy, jz, jsq);
+  store19<R, ODD, FACE != 0>(a.f, g.N, nb, n, p);
+  return 0.5 * (double)jsq;
 }
 
 template <typename R, bool ODD, bool ENERGY>
@@ -90,25 +118,11 @@ __global__ void __launch_bounds__(256) flow_step_kernel(FlowArgs<R> a) {
     const int n = x + g.nx * (y + g.ny * z);
     const uint8_t code = a.mask[n];
     if (a.kind[code] == CLB_KIND_FLUID) {
-      const Nbr nb(x, y, z, g);
-      R p[19];
-      load19<R, ODD>(a.f, g.N, nb, n, p);
-      if (x == 0) regularize<R, -1>(p, a.rhobar_in);
-      else if (x == g.nx - 1) regularize<R, 1>(p, a.rhobar_out);
-      R rhobar = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-      for (int i = 0; i < 19; ++i) {
-        rhobar += p[i];
-        jx += R(Q19::cx[i]) * p[i];
-        jy += R(Q19::cy[i]) * p[i];
-        jz += R(Q19::cz[i]) * p[i];
-      }
-      const R om = a.omega[code];
-      const R jsq = jx * jx + jy * jy + jz * jz;
-#pragma unroll
-      for (int i = 0; i < 19; ++i) p[i] = (R(1) - om) * p[i] + om * feq19<R>(i, rhobar, jx, jy, jz, jsq);
-      store19<R, ODD>(a.f, g.N, nb, n, p);
-      if (ENERGY) { e = 0.5 * (double)jsq; cnt = 1.0; }
+      double ek;
+      if (x == 0) ek = flow_node<R, ODD, -1>(a, x, y, z, n, code);
+      else if (x == g.nx - 1) ek = flow_node<R, ODD, 1>(a, x, y, z, n, code);
+      else ek = flow_node<R, ODD, 0>(a, x, y, z, n, code);
+      if (ENERGY) { e = ek; cnt = 1.0; }
     }
   }
   if (ENERGY) {
