@@ -14,7 +14,7 @@ Beside it, on the same GPU: the kinetics + biomass case at 256 x 128 x 128 (4 so
 and FP64 stores), D3Q19 flow MLUPS, plain FP64 Newton solves/s on the 128^3 Gaussian-
 smoothed totals field of the 8 / 24 tableau (seed 3), and MLP-warm vs plain Newton solves/s
 at the same 1e-10 residual on fresh samples of the 12 -> 128 -> 128 -> 8 warm-start setting.
-The CPU checker is timed on a bounded slab of the headline workload on the host cores.
+The CPU oracle is timed on a bounded slab of the headline workload on the host cores.
 """
 import argparse
 import json
@@ -354,7 +354,7 @@ def speciation_legs(clb, n_side, n_train, epochs):
 
 
 def cpu_baseline(oracle_mod, clb, nx, ny, nz, steps):
-    """The FP64 checker (the reference's algorithms, PCF + Anderson speciation) on a slab of
+    """The FP64 oracle (the reference's algorithms, PCF + Anderson speciation) on a slab of
     the headline workload: same generator and porosity, same 6 solutes, kinetics, biomass and
     tableau; a uniform pore velocity in place of the flow solve."""
     pack, mask, b, S, lk, c0, cin = full_case_inputs(clb, nx, ny, nz)
@@ -369,7 +369,7 @@ def cpu_baseline(oracle_mod, clb, nx, ny, nz, steps):
     run.run(steps)
     dt = time.perf_counter() - t0
     return dict(value=nx * ny * nz * steps / dt / 1e6, unit="MLUPS", cores=oracle_mod.num_threads(), kind="port",
-                sample=f"FP64 checker, the coupled step (6 solutes, kinetics, PCF+Anderson speciation) on a "
+                sample=f"FP64 oracle, the coupled step (6 solutes, kinetics, PCF+Anderson speciation) on a "
                        f"{nx}x{ny}x{nz} slab of the same workload, {steps} steps ({dt:.1f} s)")
 
 
@@ -420,7 +420,7 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (the multi-GPU slab "
+        print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1 (the multi-GPU slab "
                                    "decomposition is a design, not built)"}))
         sys.exit(2)
 

@@ -3,7 +3,7 @@
  * Written against the reference's abiotic kinetics interface
  * (defineAbioticKinetics.hh:130-134 of CompLB3D): the same free function, by-value
  * std::vector of concentrations, by-reference rates, plb::plint mask. The GPU build
- * compiles this file unchanged into device code; the CPU checker compiles it as host C++.
+ * compiles this file unchanged into device code; the CPU oracle compiles it as host C++.
  *
  *     R = k * C[a] * C[b]         C[a] and C[b] consumed, C[p] produced (mol/L/s)
  *

@@ -3,7 +3,7 @@
  * Written against the reference's compiled-in kinetics interface
  * (defineKinetics.hh:223-228 of CompLB3D): the same free function, the same by-value
  * std::vector inputs and by-reference outputs, the same plb::plint mask argument. The
- * GPU build compiles this file unchanged into device code; the CPU checker compiles it
+ * GPU build compiles this file unchanged into device code; the CPU oracle compiles it
  * as ordinary host C++.
  *
  *     r     = k_max * B * D/(Ks_D + D) * A/(Ks_A + A)      (per second)

@@ -28,7 +28,7 @@ extern "C" __attribute__((visibility("default"))) void orc_abiotic_plugin(const 
   for (int i = 0; i < ns; ++i) subsR[i] = sr[i];
 }
 
-// defineDissolutionRate (defineAbioticKinetics.hh:292 of the reference) for the checker.
+// defineDissolutionRate (defineAbioticKinetics.hh:292 of the reference) for the oracle.
 extern "C" __attribute__((visibility("default"))) void orc_dissolution_plugin(long long phase_id, const double* C,
                                                                                int ns, double mineral, double* subsR,
                                                                                double* mineralR, long long mask) {

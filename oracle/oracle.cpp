@@ -1,6 +1,6 @@
 // oracle.cpp -- FP64 CPU restatement of CompLB3D's per-timestep hot path.
 //
-// TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP path: only
+// TEST INFRASTRUCTURE ONLY. This library is the oracle for the HIP path: only
 // tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it. The
 // product never links, loads or calls it.
 //

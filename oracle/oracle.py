@@ -1,4 +1,4 @@
-"""ctypes loader for the FP64 CPU checker (liboracle.so).
+"""ctypes loader for the FP64 CPU oracle (liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
 cpu_baseline leg, never by the product. See oracle.cpp for what each function restates.
@@ -249,7 +249,7 @@ def ca_init(mask, solid=0, bb=1, pore=2):
 
 
 class CaState:
-    """The CA's lattices for the checker: biomass (nb, N) FP64, mask, age, distance sign and
+    """The CA's lattices for the oracle: biomass (nb, N) FP64, mask, age, distance sign and
     total biomass, advanced in place by redistribute()."""
 
     def __init__(self, nx, ny, nz, b, mask, solid=0, bb=1, pore=2, biofilm=3, bthr=10.0):
@@ -392,7 +392,7 @@ class AdeRun:
 
 # ------------------------------------------------------------------ surrogate processor
 class Surrogate:
-    """The surrogate processor's settings for the checker (surrogate.cpp). nets: one entry per
+    """The surrogate processor's settings for the oracle (surrogate.cpp). nets: one entry per
     microbe, a dict (n_in, in_sub, widths, params, x_offset, x_gain, x_ymin, y_ymin, y_gain,
     y_offset, log_output; optional closure, yield) or None for no network."""
 

@@ -1,6 +1,6 @@
 // surrogate.cpp -- FP64 CPU restatement of the surrogate metabolic processor.
 //
-// TEST INFRASTRUCTURE ONLY: part of the checker library (liboracle.so), loaded by tests/,
+// TEST INFRASTRUCTURE ONLY: part of the oracle library (liboracle.so), loaded by tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the product.
 //
 // Restates, per voxel:
@@ -216,7 +216,7 @@ double exhaustion_flux(const orc_srg& P, int k, double C, double B, double dt) {
 
 extern "C" {
 
-// The network alone (for the checker's own tests).
+// The network alone (for the oracle's own tests).
 __attribute__((visibility("default"))) double orc_srg_net_eval(const orc_srg_net* nt, const double* fin, int S) {
   std::vector<double> f(fin, fin + S);
   return net_eval(*nt, f);

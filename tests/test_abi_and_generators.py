@@ -25,7 +25,7 @@ def test_sphere_pack_spec(clb, oracle):
     # inlet / outlet planes duplicate the first / last interior slice (pore vs solid)
     assert np.array_equal(v[:, :, 0] == 2, v[:, :, 1] == 2)
     assert np.array_equal(v[:, :, nx - 1] == 2, v[:, :, nx - 2] == 2)
-    # solid classification agrees with the checker's rule (pore D3Q19 neighbour -> 1)
+    # solid classification agrees with the oracle's rule (pore D3Q19 neighbour -> 1)
     solid = np.where(m == 2, 2, 0).astype(np.uint8)
     is_pore = np.zeros(256, dtype=np.uint8)
     is_pore[2] = 1

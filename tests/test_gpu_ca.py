@@ -1,4 +1,4 @@
-"""CA biomass redistribution on the GPU against the checker (oracle/ca.cpp), through the C ABI.
+"""CA biomass redistribution on the GPU against the oracle (oracle/ca.cpp), through the C ABI.
 
 Both sides run the same order-independent sweep (decisions from the sweep's starting state,
 neighbour orders from the same counter-based draws) with no FMA contraction, so with FP64
@@ -30,7 +30,7 @@ def gpu_lattice(clb, nx, ny, nz, mask, b, prec=0, bthr=10.0):
 @pytest.mark.gpu
 @pytest.mark.parametrize("half", [0, 1])
 def test_ca_bit_exact_vs_oracle(clb, half):
-    print(f"case: CA FP64 vs checker, {NX}x{NY}x{NZ}, half={half}", flush=True)
+    print(f"case: CA FP64 vs oracle, {NX}x{NY}x{NZ}, half={half}", flush=True)
     mask, b = random_case(5, NX, NY, NZ)  # with boxed-in biofilm: both stall the same way
     lat = gpu_lattice(clb, NX, NY, NZ, mask, b)
     ref = oracle.CaState(NX, NY, NZ, b, mask)

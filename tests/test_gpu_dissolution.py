@@ -1,10 +1,10 @@
 """Mineral dissolution (surfaceDissolutionKinetics3D, src/dissolutionVOP.hh:229-459) on the
-GPU against the checker's restatement, through the C ABI.
+GPU against the oracle's restatement, through the C ABI.
 
 A block of solid voxels carrying phase 1 holds an inventory in an immobile solute; the
 compiled hook of kinetics/abiotic_second_order dissolves it first order in the attacking
 solute (1) and in the mineral, releasing solute 0 one for one. Pinned by:
-  * FP64 GPU vs the checker (same step order: water after the biotic kinetics, products
+  * FP64 GPU vs the oracle (same step order: water after the biotic kinetics, products
     and inventory loss joined to the step's increments before streaming), 1e-10;
   * the element balances the stoichiometry fixes, in a box closed by walls: mineral +
     released solute, and attacker + released solute, both conserved to rounding;
@@ -78,8 +78,8 @@ def setup(clb, closed, prec=0, mode=None, tableau=False):
 
 
 @pytest.mark.gpu
-def test_dissolution_fp64_vs_checker(clb):
-    print("case: dissolution FP64 vs checker, open channel, 120 steps", flush=True)
+def test_dissolution_fp64_vs_oracle(clb):
+    print("case: dissolution FP64 vs oracle, open channel, 120 steps", flush=True)
     lat, ref, mask, phase = setup(clb, closed=False)
     lat.step(120)
     ref.run(120)
@@ -121,7 +121,7 @@ def test_dissolution_element_balance(clb, prec, tol):
 
 @pytest.mark.gpu
 def test_dissolution_fused_matches_separate(clb):
-    print("case: dissolution with speciation, fused (graph) vs separate vs checker", flush=True)
+    print("case: dissolution with speciation, fused (graph) vs separate vs oracle", flush=True)
     out = []
     for mode, graph in ((0, True), (1, False)):
         lat, ref, mask, phase = setup(clb, closed=False, mode=mode, tableau=True)

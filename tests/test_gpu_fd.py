@@ -1,5 +1,5 @@
 """The FD biofilm step (fdDiffusion3D between two mask / total updates, src/complab.cpp:1716-1731)
-on the GPU against the checker (oracle/ca.cpp), through the C ABI.
+on the GPU against the oracle (oracle/ca.cpp), through the C ABI.
 
 Both sides evaluate the same expression with no FMA contraction, so with FP64 biomass the
 fields, masks and reports agree bit for bit over several calls; with FP32 biomass the fields
@@ -16,7 +16,7 @@ NX, NY, NZ = 40, 20, 16
 
 @pytest.mark.gpu
 def test_fd_bit_exact_vs_oracle(clb):
-    print(f"case: FD biomass FP64 vs checker, {NX}x{NY}x{NZ}, 6 calls", flush=True)
+    print(f"case: FD biomass FP64 vs oracle, {NX}x{NY}x{NZ}, 6 calls", flush=True)
     mask, b = random_case(8, NX, NY, NZ)
     b = b * 0.1  # under the CA's cap: only diffusion and the mask rule act
     lat = gpu_lattice(clb, NX, NY, NZ, mask, b)
@@ -36,7 +36,7 @@ def test_fd_bit_exact_vs_oracle(clb):
 
 @pytest.mark.gpu
 def test_fd_fp32_and_line_fixtures(clb):
-    print("case: FD biomass FP32 vs checker; closed-form lines on the GPU", flush=True)
+    print("case: FD biomass FP32 vs oracle; closed-form lines on the GPU", flush=True)
     mask, b = random_case(9, NX, NY, NZ)
     b = b * 0.1
     lat = gpu_lattice(clb, NX, NY, NZ, mask, b.astype(np.float32).astype(np.float64), prec=1)

@@ -148,7 +148,7 @@ def test_solutes_reset(clb, oracle, case):
     fl = (pack == 2) & interior.ravel()
     for s in range(3):
         assert np.max(np.abs(c[s][fl] - c_new[s])) < 1e-15
-    # the same reset applied to the checker's state, then both continue
+    # the same reset applied to the oracle's state, then both continue
     run = oracle_run(oracle, case, 31, 0, imm)
     g = run.g.reshape(4, 7, -1)
     w = np.array([0.25] + [0.125] * 6)

@@ -1,18 +1,18 @@
 """The surrogate metabolic processor (run_surrogate3D, src/complab3d_processors_surrogate.hh:
-74-295) on the GPU against the checker's restatement (oracle/surrogate.cpp), through the C ABI.
+74-295) on the GPU against the oracle's restatement (oracle/surrogate.cpp), through the C ABI.
 
 Case: the 64 x 32 x 32 sphere pack with four solutes (donor D, acceptor A, products P1, P2)
 and two biomass fields. Field 0 follows the compiled dual-Monod kinetics; field 1 is the
 reference's shipped surrogate organism (Geobacter, the trained 2-10-10-10-10-1 network of
 surrogateModel.hh, weights from tests/golden/surrogate_geobacter.json), whose increments
 join the kinetics' in the same step. Pinned by:
-  * FP64 GPU vs the checker, 1e-10, and FP32 storage, 1e-5 (both with the shipped network,
+  * FP64 GPU vs the oracle, 1e-10, and FP32 storage, 1e-5 (both with the shipped network,
     Vmax keeping its inputs inside its training box);
   * two surrogate organisms alone (no kinetics plugin): a small random network with the
     growth-limiting closure and a substrate floor, and one without a network (decay), with a
     held substrate (fix_concentration);
   * component totals with the fused speciation path (deferred increments) and the separate
-    one, against the checker;
+    one, against the oracle;
   * a captured graph replays the eager steps bit for bit; switching the processor off
     restores the plain kinetics run bit for bit;
   * the argument checks of the two setters."""
@@ -200,7 +200,7 @@ def totals_setup(clb, mode):
 
 @pytest.mark.gpu
 def test_component_totals_with_speciation(clb):
-    print("case: surrogate on component totals, fused (graph) and separate speciation vs checker", flush=True)
+    print("case: surrogate on component totals, fused (graph) and separate speciation vs oracle", flush=True)
     out = []
     for mode, graph in ((0, True), (1, False)):
         lat, ref = totals_setup(clb, mode)

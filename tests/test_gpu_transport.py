@@ -1,8 +1,8 @@
 """GPU D3Q7 transport + kinetics + biomass threshold mask vs the FP64 oracle.
 
-configs[0]-style transport: 2 solutes on the 64 x 32 x 32 pack, tau 0.8, west Dirichlet
-1.0, east Neumann, c0 = 0, 1000 steps after the flow. configs[1]-style reactions on a
-reduced domain: 4 solutes, dual-Monod (Ks 1e-2, k_max dt 1e-3, Y 0.4), biomass seeded in
+Flow + transport case: 2 solutes on the 64 x 32 x 32 pack, tau 0.8, west Dirichlet 1.0,
+east Neumann, c0 = 0, 1000 steps after the flow. Kinetics case reactions on a reduced
+domain: 4 solutes, dual-Monod (Ks 1e-2, k_max dt 1e-3, Y 0.4), biomass seeded in
 1 % of pore voxels with B ~ U[10, 100], pore <-> biofilm reclassification at a total
 biomass of 50 (chosen so growth moves voxels across it within the run). FP64 to
 rounding, FP32 store within rel-L2 1e-5, the mask bit-exact."""

@@ -1,4 +1,4 @@
-"""CPU pins of the checker's CA biomass redistribution (oracle/ca.cpp).
+"""CPU pins of the oracle's CA biomass redistribution (oracle/ca.cpp).
 
 The reference's push/pull CA (src/complab3d_processors_part1.hh:432-894, sweep loop
 src/complab.cpp:1680-1731) has no golden vectors: its sweeps depend on loop order and on

@@ -1,4 +1,4 @@
-"""CPU pins of the checker's finite-difference biofilm step (oracle/ca.cpp, fd_step and
+"""CPU pins of the oracle's finite-difference biofilm step (oracle/ca.cpp, fd_step and
 orc_fd_biomass): fdDiffusion3D (src/complab3d_processors_part2.hh:257-430) between two mask /
 total updates, as the loop runs it (src/complab.cpp:1716-1731).
 

@@ -1,9 +1,9 @@
-"""CPU pins of the checker's surrogate processor (oracle/surrogate.cpp).
+"""CPU pins of the oracle's surrogate processor (oracle/surrogate.cpp).
 
 The reference holds no golden vectors for run_surrogate3D
 (src/complab3d_processors_surrogate.hh:74-295) or for defineSurrogateModel
 (surrogateModel.hh:252-500). Every rule is pinned here by a closed form the rule fixes:
-  * the network: a one-neuron tanh net written with np.tanh (the checker uses MATLAB's
+  * the network: a one-neuron tanh net written with np.tanh (the oracle uses MATLAB's
     tansig form), an asymmetric two-input layer (a transposed weight matrix fails), the
     mapminmax direction (the training box maps onto [-1, 1]), the 1e-8 / non-finite guards;
   * the processor: with no Vmax the uptake is the supply-limited flux, and the units of
