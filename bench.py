@@ -52,9 +52,9 @@ FLOP_SRG_NET = 2 * 2 + 2 * (2 * 10 + 3 * 100) + 4 * 10 * (20 + 4) + 2 * 10 + 4
 # workloads, scaled by a copy kernel of known size (profiles/hbm_traffic_fetch_write.md);
 # mean of the odd and even steps. Reported only when the workload is the default one.
 MEASURED_TRAFFIC = {
-    "full_coupled_384x192x192": (6.420e9 + 6.356e9) / 2,
-    "flow_384x192x192": (1.588e9 + 1.514e9) / 2,
-    "kinetics_256x128x128": (1.032e9 + 1.009e9) / 2,
+    "full_coupled_384x192x192": (6.584e9 + 6.486e9) / 2,
+    "flow_384x192x192": (1.619e9 + 1.540e9) / 2,
+    "kinetics_256x128x128": (1.019e9 + 1.005e9) / 2,
 }
 
 
