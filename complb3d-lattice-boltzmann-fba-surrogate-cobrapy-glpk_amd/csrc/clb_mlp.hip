@@ -160,91 +160,108 @@ struct MlpSmem {
   unsigned short w2[HID * HPAD];
   unsigned short w3[NOUT * HPAD];
   float b1[HID], b2[HID], b3[NOUT];
-  unsigned short act[4][32 * HPAD];  // per wavefront: 32 samples x HID activations
 };
 
+// Four consecutive BF16 values (8-byte aligned in LDS) as one 64-bit LDS read.
 __device__ __forceinline__ bf16x4 lds_bf16x4(const unsigned short* p) {
-  bf16x4 r;
-  r[0] = (short)p[0]; r[1] = (short)p[1]; r[2] = (short)p[2]; r[3] = (short)p[3];
-  return r;
+  return *reinterpret_cast<const bf16x4*>(p);
 }
 
-// D[32 x 32] += A[32 x K] * W^T where A rows are samples (LDS, row stride lda) and W is
-// [n][k] (LDS, row stride ldw); lane l holds sample (l % 32), k-group (l / 32).
+__device__ __forceinline__ float fast_tanh(float x) {
+  // 1 - 2 / (e^{2x} + 1): saturates to +-1 (e^{2x} overflows to inf or underflows to 0);
+  // its error is far below the BF16 rounding the activation gets next
+  return 1.f - 2.f * __frcp_rn(__expf(2.f * x) + 1.f);
+}
+
+__device__ __forceinline__ short bf16s(float f) { return (short)f2bf(f); }
+
+// The layers are computed transposed, Z^T = W A^T: the weights (LDS) are the A operand,
+// the activations the B operand, so a 32 x 32 tile covers 32 units x 32 samples. In the
+// accumulator, lane l holds sample l % 32 and units 8 (r >> 2) + 4 (l >> 5) + (r & 3) of
+// its tile (r = 0..15); the B operand of k-step ks wants units 8 ks + 4 (l >> 5) + j, which
+// are entries r = 4 (ks % 4) + j of tile ks / 4 on the same lane. So each layer's output,
+// through tanh and BF16, feeds the next layer straight from registers.
 template <int KSTEPS>
-__device__ __forceinline__ f32x16 mfma_tile(const unsigned short* A, int lda, const unsigned short* W, int ldw,
-                                            int n0, f32x16 acc) {
+__device__ __forceinline__ f32x16 mfma_wt(const unsigned short* W, int ldw, int n0, const bf16x4* b, f32x16 acc) {
   const int lane = threadIdx.x & 63;
   const int i = lane & 31, kg = lane >> 5;
 #pragma unroll
   for (int ks = 0; ks < KSTEPS; ++ks) {
-    const int k = ks * 8 + kg * 4;
-    const bf16x4 a = lds_bf16x4(A + i * lda + k);
-    const bf16x4 b = lds_bf16x4(W + (n0 + i) * ldw + k);
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    const bf16x4 a = lds_bf16x4(W + (n0 + i) * ldw + ks * 8 + kg * 4);
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[ks], acc, 0, 0, 0);
+    // keep the weight reads of later k-steps from being hoisted: they would hold registers
+    // that the other wavefronts of the SIMD need
+    __builtin_amdgcn_sched_barrier(0);
   }
   return acc;
 }
 
-// Output element r of the 32x32 accumulator on this lane: row (sample) and column.
-__device__ __forceinline__ int acc_row(int r, int lane) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
-
-// Fused inference for 32 samples per wavefront. X holds din scaled inputs per sample
-// (fp32, row-major); Y receives dout scaled outputs.
-__device__ void mlp_tile(MlpSmem& sm, int din, int dout, const float* X, float* Y, long long s0, long long n) {
+// Fused inference for 32 samples per wavefront (no LDS besides the weights, no barriers).
+// X holds din scaled inputs per sample (fp32, row-major); Y receives dout scaled outputs.
+__device__ void mlp_tile(const MlpSmem& sm, int din, int dout, const float* X, float* Y, long long s0, long long n) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  unsigned short* act = sm.act[wave];
-  // stage inputs (bf16) into the activation buffer as a 32 x KIN matrix
-  for (int e = lane; e < 32 * KIN; e += 64) {
-    const int i = e / KIN, k = e % KIN;
-    const long long s = s0 + i;
-    float v = 0.f;
-    if (s < n && k < din) v = X[s * din + k];
-    act[i * HPAD + k] = f2bf(v);
-  }
-  __syncthreads();
-  // layer 1: 4 column tiles of 32 hidden units, K = 16
-  f32x16 h[4];
+  const int smp = lane & 31, kg = lane >> 5;
+  const long long s = s0 + smp;
+  const bool live = s < n;
+  // layer 1 B operand: inputs 8 ks + 4 kg + j of this lane's sample
+  bf16x4 bx[KIN / 8];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) {
+  for (int ks = 0; ks < KIN / 8; ++ks)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = ks * 8 + kg * 4 + j;
+      bx[ks][j] = bf16s((live && k < din) ? X[s * din + k] : 0.f);
+    }
+  // hidden layers: h1 / h2 hold the B operand of the next layer, 16 k-steps of 4 units;
+  // each 32-unit tile goes through tanh as soon as it is done
+  bf16x4 h1[HID / 8], h2[HID / 8];
+#pragma unroll
+  for (int t = 0; t < HID / 32; ++t) {
     f32x16 z = {0};
-    h[t] = mfma_tile<KIN / 8>(act, HPAD, sm.w1, KPAD, t * 32, z);
-  }
-  __syncthreads();
+    z = mfma_wt<KIN / 8>(sm.w1, KPAD, t * 32, bx, z);
 #pragma unroll
-  for (int t = 0; t < 4; ++t)
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int unit = t * 32 + 8 * q + 4 * kg + j;
+        h1[t * 4 + q][j] = bf16s(fast_tanh(z[q * 4 + j] + sm.b1[unit]));
+      }
+  }
+  {
+    // layer 2, k outer: the four 32-unit output tiles accumulate side by side
+    const int i = lane & 31;
+    f32x16 z[HID / 32];
+#pragma unroll
+    for (int t = 0; t < HID / 32; ++t) z[t] = f32x16{0};
+#pragma unroll
+    for (int ks = 0; ks < HID / 8; ++ks) {
+#pragma unroll
+      for (int t = 0; t < HID / 32; ++t) {
+        const bf16x4 a = lds_bf16x4(sm.w2 + (t * 32 + i) * HPAD + ks * 8 + kg * 4);
+        z[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, h1[ks], z[t], 0, 0, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int t = 0; t < HID / 32; ++t)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int unit = t * 32 + 8 * q + 4 * kg + j;
+          h2[t * 4 + q][j] = bf16s(fast_tanh(z[t][q * 4 + j] + sm.b2[unit]));
+        }
+  }
+  // output layer: one tile of 32 (dout used)
+  f32x16 acc = {0};
+  const f32x16 o = mfma_wt<HID / 8>(sm.w3, HPAD, 0, h2, acc);
+  if (live) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int row = acc_row(r, lane), col = t * 32 + (lane & 31);
-      act[row * HPAD + col] = f2bf(tanhf(h[t][r] + sm.b1[col]));
+      const int out = 8 * (r >> 2) + 4 * kg + (r & 3);
+      if (out < dout) Y[s * dout + out] = o[r] + sm.b3[out];
     }
-  __syncthreads();
-  // layer 2: K = 128
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    f32x16 z = {0};
-    h[t] = mfma_tile<HID / 8>(act, HPAD, sm.w2, HPAD, t * 32, z);
   }
-  __syncthreads();
-#pragma unroll
-  for (int t = 0; t < 4; ++t)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int row = acc_row(r, lane), col = t * 32 + (lane & 31);
-      act[row * HPAD + col] = f2bf(tanhf(h[t][r] + sm.b2[col]));
-    }
-  __syncthreads();
-  // layer 3: one 32-wide output tile
-  f32x16 z = {0};
-  f32x16 o = mfma_tile<HID / 8>(act, HPAD, sm.w3, HPAD, 0, z);
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = acc_row(r, lane), col = lane & 31;
-    const long long s = s0 + row;
-    if (s < n && col < dout) Y[s * dout + col] = o[r] + sm.b3[col];
-  }
-  __syncthreads();
 }
 
 __device__ void stage_weights(MlpSmem& sm, const MlpDims& d, const float* theta) {
@@ -278,11 +295,9 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(MlpDims d, long long n, 
   stage_weights(sm, d, theta);
   const int wave = threadIdx.x >> 6;
   const long long tiles = (n + 31) / 32;
-  // all four wavefronts of a block step together so the block-wide barriers match
-  for (long long t0 = (long long)blockIdx.x * 4; t0 < tiles; t0 += (long long)gridDim.x * 4) {
-    const long long t = t0 + wave;
-    mlp_tile(sm, d.din, d.dout, X, Y, t * 32, (t < tiles) ? n : 0);
-  }
+  // each wavefront takes its own tiles (after the staging barrier, none is needed)
+  for (long long t = (long long)blockIdx.x * 4 + wave; t < tiles; t += (long long)gridDim.x * 4)
+    mlp_tile(sm, d.din, d.dout, X, Y, t * 32, n);
 }
 
 // Warm start for the batched Newton solve: inputs are log10 totals and the per-voxel logK
