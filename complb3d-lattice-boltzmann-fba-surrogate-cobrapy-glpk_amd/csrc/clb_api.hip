@@ -704,8 +704,10 @@ int ensure_work(clb_ctx* c, size_t bytes) {
 }
 
 // Forward + backward over one batch with the FP32 kernels. Buffers in `w`:
-// H1[B*H], H2[B*H], Y[B*dout], dY[B*dout], dH2[B*H], dH1[B*H].
-int mlp_fwd_bwd(clb_ctx* c, long long B, const float* X, const float* T, float* grad, double* d_loss, float* w) {
+// H1[B*H], H2[B*H], Y[B*dout], dY[B*dout], dH2[B*H], dH1[B*H]. The gradient is left as
+// `splits` partial sums over consecutive row chunks, partial z at grad + z P.
+int mlp_fwd_bwd(clb_ctx* c, long long B, const float* X, const float* T, float* grad, double* d_loss, float* w,
+                int splits = 1) {
   const MlpDims d = c->md;
   const int H = d.H;
   float* H1 = w;
@@ -733,16 +735,17 @@ int mlp_fwd_bwd(clb_ctx* c, long long B, const float* X, const float* T, float* 
   launch_gemm(0, 1, Bi, d.dout, H, H2, H, W3, H, Y, d.dout, b3, 0, 0.f, s);
   if (!T) return CLB_OK;
   launch_mse_grad(B * d.dout, Y, T, 2.0f / (float)(B * d.dout), dY, d_loss, s);
-  launch_gemm(1, 0, d.dout, H, Bi, dY, d.dout, H2, H, gW3, H, nullptr, 0, 0.f, s);
-  launch_colsum(Bi, d.dout, dY, gb3, s);
+  const long long P = c->mlp_P;
+  launch_gemm_split(1, 0, d.dout, H, Bi, dY, d.dout, H2, H, gW3, H, nullptr, 0, 0.f, splits, P, s);
+  launch_colsum(Bi, d.dout, dY, gb3, splits, P, s);
   launch_gemm(0, 0, Bi, H, d.dout, dY, d.dout, W3, H, dZ2, H, nullptr, 0, 0.f, s);
   launch_tanh_backward(B * H, dZ2, H2, s);
-  launch_gemm(1, 0, H, H, Bi, dZ2, H, H1, H, gW2, H, nullptr, 0, 0.f, s);
-  launch_colsum(Bi, H, dZ2, gb2, s);
+  launch_gemm_split(1, 0, H, H, Bi, dZ2, H, H1, H, gW2, H, nullptr, 0, 0.f, splits, P, s);
+  launch_colsum(Bi, H, dZ2, gb2, splits, P, s);
   launch_gemm(0, 0, Bi, H, H, dZ2, H, W2, H, dZ1, H, nullptr, 0, 0.f, s);
   launch_tanh_backward(B * H, dZ1, H1, s);
-  launch_gemm(1, 0, H, d.din, Bi, dZ1, H, X, d.din, gW1, d.din, nullptr, 0, 0.f, s);
-  launch_colsum(Bi, H, dZ1, gb1, s);
+  launch_gemm_split(1, 0, H, d.din, Bi, dZ1, H, X, d.din, gW1, d.din, nullptr, 0, 0.f, splits, P, s);
+  launch_colsum(Bi, H, dZ1, gb1, splits, P, s);
   return CLB_OK;
 }
 
@@ -755,7 +758,10 @@ int mlp_train_device(clb_ctx* c, long long n, const float* dX, const float* dT, 
   if (batch < 1) batch = 1;
   if (batch > n) batch = (int)n;
   const long long P = c->mlp_P;
-  const size_t nfl = (mlp_fwd_bwd_floats(d, batch) + P + 1) & ~(size_t)1;  // keep the double 8-byte aligned
+  // the weight gradients reduce over the batch in chunks of 32 rows (at most 16 chunks),
+  // which spreads each gradient GEMM over many workgroups
+  const int splits = (int)std::min<long long>(16, std::max<long long>(1, batch / 32));
+  const size_t nfl = (mlp_fwd_bwd_floats(d, batch) + splits * P + 1) & ~(size_t)1;  // keep the double 8-byte aligned
   int rc = ensure_work(c, nfl * sizeof(float) + sizeof(double) * 2);
   if (rc) return rc;
   float* w = c->d_work;
@@ -779,10 +785,10 @@ int mlp_train_device(clb_ctx* c, long long n, const float* dX, const float* dT, 
       const long long r0 = order[bi] * batch;
       const float* Xp = dX + r0 * d.din;
       const float* Tp = dT + r0 * d.dout;
-      mlp_fwd_bwd(c, batch, Xp, Tp, grad, d_loss, w);
+      mlp_fwd_bwd(c, batch, Xp, Tp, grad, d_loss, w, splits);
       ++tstep;
       const float b1 = 0.9f, b2 = 0.999f;
-      launch_adam(P, c->d_theta, grad, c->d_adam_m, c->d_adam_v, (float)lr_ep, b1, b2, 1e-8f,
+      launch_adam(P, c->d_theta, grad, splits, c->d_adam_m, c->d_adam_v, (float)lr_ep, b1, b2, 1e-8f,
                   (float)std::pow(0.9, (double)tstep), (float)std::pow(0.999, (double)tstep), c->stream);
     }
     double h = 0;
@@ -1438,11 +1444,9 @@ int clb_batch_solve(clb_ctx* c, int warm, const double* logc_in) {
     if (!c->d_theta || !c->mlp_scaled) return fail(c, CLB_ERR_STATE, "MLP not configured / scaled");
     if (c->md.din != nc + c->bnvar || c->md.dout != nc || c->md.H != 128)
       return fail(c, CLB_ERR_UNSUP, "MLP shape does not match the batch (din = nc + nvar, dout = nc, H = 128)");
-    int rc = ensure_work(c, (size_t)n * (c->md.din + c->md.dout) * sizeof(float) + (size_t)n + 256);
-    if (rc) return rc;
     Timed tm(c, 3);
     launch_mlp_warm(c->md, n, c->d_theta, c->d_box, c->d_box + 32, c->d_box + 64, c->d_box + 96, c->tb, sa,
-                    c->d_work, c->stream);
+                    c->stream);
     count_launch(c, 3);
   } else if (warm != CLB_WARM_REF) {
     return CLB_ERR_ARG;

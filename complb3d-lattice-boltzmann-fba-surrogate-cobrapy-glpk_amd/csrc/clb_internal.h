@@ -238,17 +238,22 @@ struct MlpDims {
 };
 void launch_gemm(int transA, int transB, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
                  float* C, int ldc, const float* bias, int act, float beta, hipStream_t s);
+// K split in `splits` chunks, chunk z written to C + z sc
+void launch_gemm_split(int transA, int transB, int M, int N, int K, const float* A, int lda, const float* B,
+                       int ldb, float* C, int ldc, const float* bias, int act, float beta, int splits, long long sc,
+                       hipStream_t s);
 void launch_tanh_backward(long long n, float* dz, const float* h, hipStream_t s);
-void launch_colsum(int rows, int cols, const float* A, float* out, hipStream_t s);
+// column sums over rows split in `splits` chunks, chunk z written to out + z so
+void launch_colsum(int rows, int cols, const float* A, float* out, int splits, long long so, hipStream_t s);
 void launch_mse_grad(long long n, const float* Y, const float* T, float scale, float* dY, double* loss,
                      hipStream_t s);
-void launch_adam(long long n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
+// gradient = sum of `splits` partials at g + z n
+void launch_adam(long long n, float* p, const float* g, int splits, float* m, float* v, float lr, float b1, float b2,
                  float eps, float b1t, float b2t, hipStream_t s);
 void launch_mlp_infer_bf16(const MlpDims& d, long long n, const float* theta, const float* X, float* Y,
                            hipStream_t s);
 void launch_mlp_warm(const MlpDims& d, long long n, const float* theta, const double* xmin, const double* xmax,
-                     const double* ymin, const double* ymax, const Tableau& tb, const SpecArgs& sa, float* work,
-                     hipStream_t s);
+                     const double* ymin, const double* ymax, const Tableau& tb, const SpecArgs& sa, hipStream_t s);
 void launch_mlp_make_dataset(const MlpDims& d, long long n, const Tableau& tb, const SpecArgs& sa, float* X,
                              float* T, const double* xmin, const double* xmax, const double* ymin,
                              const double* ymax, hipStream_t s);

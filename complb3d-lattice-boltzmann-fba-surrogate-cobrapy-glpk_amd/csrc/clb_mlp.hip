@@ -16,23 +16,31 @@ namespace {
 // A is M x K (or K x M when transA), B is K x N (or N x K when transB); row-major with
 // leading dimensions lda / ldb / ldc. act: 0 none, 1 tanh. Tile 64 x 64 x 16, 256
 // threads, 4 x 4 outputs per thread.
+//
+// Split K: block z sums k in [z kc, (z + 1) kc) into its own C + z sc (the weight
+// gradients reduce over the batch rows; with one split per 32 rows a 128 x 128 gradient
+// runs on 64 workgroups instead of 4, and the partials are summed in a fixed order by
+// the optimizer step).
 constexpr int TM = 64, TN = 64, TK = 16;
 
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
                                                    const float* __restrict__ B, int ldb, float* __restrict__ C,
-                                                   int ldc, const float* __restrict__ bias, int act, float beta) {
+                                                   int ldc, const float* __restrict__ bias, int act, float beta,
+                                                   int kc, long long sc) {
   __shared__ float As[TK][TM + 4];
   __shared__ float Bs[TK][TN + 4];
   const int tid = threadIdx.x;
   const int tx = tid % 16, ty = tid / 16;
   const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int kbeg = blockIdx.z * kc, kend = min(K, kbeg + kc);
+  C += blockIdx.z * sc;
   float acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = 0.f;
-  for (int k0 = 0; k0 < K; k0 += TK) {
+  for (int k0 = kbeg; k0 < kend; k0 += TK) {
     // load A tile: TM x TK elements, 1024 -> 4 per thread
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -40,7 +48,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const fl
       const int mm = e / TK, kk = e % TK;
       const int gm = m0 + mm, gk = k0 + kk;
       float v = 0.f;
-      if (gm < M && gk < K) v = TA ? A[(long long)gk * lda + gm] : A[(long long)gm * lda + gk];
+      if (gm < M && gk < kend) v = TA ? A[(long long)gk * lda + gm] : A[(long long)gm * lda + gk];
       As[kk][mm] = v;
     }
 #pragma unroll
@@ -49,7 +57,7 @@ __global__ void __launch_bounds__(256) gemm_kernel(int M, int N, int K, const fl
       const int nn = e / TK, kk = e % TK;
       const int gn = n0 + nn, gk = k0 + kk;
       float v = 0.f;
-      if (gn < N && gk < K) v = TB ? B[(long long)gn * ldb + gk] : B[(long long)gk * ldb + gn];
+      if (gn < N && gk < kend) v = TB ? B[(long long)gn * ldb + gk] : B[(long long)gk * ldb + gn];
       Bs[kk][nn] = v;
     }
     __syncthreads();
@@ -89,11 +97,14 @@ __global__ void tanh_backward_kernel(long long n, float* dz, const float* h) {
   if (i < n) dz[i] *= (1.f - h[i] * h[i]);
 }
 
-// out[c] = sum_r A[r, c]; one block per column chunk, shared-memory reduction.
-__global__ void colsum_kernel(int rows, int cols, const float* A, float* out) {
+// out[c] = sum_r A[r, c]; one block per column, shared-memory reduction. Split rows:
+// block y sums rows [y kc, (y + 1) kc) into out + y so.
+__global__ void colsum_kernel(int rows, int cols, const float* A, float* out, int kc, long long so) {
   const int c = blockIdx.x;
+  const int r1 = min(rows, (int)blockIdx.y * kc + kc);
+  out += blockIdx.y * so;
   float s = 0.f;
-  for (int r = threadIdx.x; r < rows; r += blockDim.x) s += A[(long long)r * cols + c];
+  for (int r = blockIdx.y * kc + threadIdx.x; r < r1; r += blockDim.x) s += A[(long long)r * cols + c];
   __shared__ float sh[256];
   sh[threadIdx.x] = s;
   __syncthreads();
@@ -124,11 +135,13 @@ __global__ void mse_grad_kernel(long long n, const float* Y, const float* T, flo
   }
 }
 
-__global__ void adam_kernel(long long n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
-                            float eps, float b1t, float b2t) {
+// Adam step; the gradient is the sum of `splits` partial gradients g[z n + i], z ascending.
+__global__ void adam_kernel(long long n, float* p, const float* g, int splits, float* m, float* v, float lr,
+                            float b1, float b2, float eps, float b1t, float b2t) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const float gi = g[i];
+  float gi = g[i];
+  for (int z = 1; z < splits; ++z) gi += g[z * n + i];
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
@@ -196,22 +209,13 @@ __device__ __forceinline__ f32x16 mfma_wt(const unsigned short* W, int ldw, int 
   return acc;
 }
 
-// Fused inference for 32 samples per wavefront (no LDS besides the weights, no barriers).
-// X holds din scaled inputs per sample (fp32, row-major); Y receives dout scaled outputs.
-__device__ void mlp_tile(const MlpSmem& sm, int din, int dout, const float* X, float* Y, long long s0, long long n) {
+// The three layers for 32 samples per wavefront (no LDS besides the weights, no
+// barriers). bx is layer 1's B operand: lane l holds inputs 8 ks + 4 (l >> 5) + j of sample
+// l % 32 in bx[ks][j]. Returns the output layer's accumulator without its bias: lane l
+// holds outputs 8 (r >> 2) + 4 (l >> 5) + (r & 3) of sample l % 32 in entry r.
+__device__ __forceinline__ f32x16 mlp_layers(const MlpSmem& sm, const bf16x4* bx) {
   const int lane = threadIdx.x & 63;
-  const int smp = lane & 31, kg = lane >> 5;
-  const long long s = s0 + smp;
-  const bool live = s < n;
-  // layer 1 B operand: inputs 8 ks + 4 kg + j of this lane's sample
-  bf16x4 bx[KIN / 8];
-#pragma unroll
-  for (int ks = 0; ks < KIN / 8; ++ks)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = ks * 8 + kg * 4 + j;
-      bx[ks][j] = bf16s((live && k < din) ? X[s * din + k] : 0.f);
-    }
+  const int kg = lane >> 5;
   // hidden layers: h1 / h2 hold the B operand of the next layer, 16 k-steps of 4 units;
   // each 32-unit tile goes through tanh as soon as it is done
   bf16x4 h1[HID / 8], h2[HID / 8];
@@ -254,7 +258,25 @@ __device__ void mlp_tile(const MlpSmem& sm, int din, int dout, const float* X, f
   }
   // output layer: one tile of 32 (dout used)
   f32x16 acc = {0};
-  const f32x16 o = mfma_wt<HID / 8>(sm.w3, HPAD, 0, h2, acc);
+  return mfma_wt<HID / 8>(sm.w3, HPAD, 0, h2, acc);
+}
+
+// Inference for one tile: X holds din scaled inputs per sample (fp32, row-major); Y
+// receives dout scaled outputs.
+__device__ void mlp_tile(const MlpSmem& sm, int din, int dout, const float* X, float* Y, long long s0, long long n) {
+  const int lane = threadIdx.x & 63;
+  const int smp = lane & 31, kg = lane >> 5;
+  const long long s = s0 + smp;
+  const bool live = s < n;
+  bf16x4 bx[KIN / 8];
+#pragma unroll
+  for (int ks = 0; ks < KIN / 8; ++ks)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = ks * 8 + kg * 4 + j;
+      bx[ks][j] = bf16s((live && k < din) ? X[s * din + k] : 0.f);
+    }
+  const f32x16 o = mlp_layers(sm, bx);
   if (live) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
@@ -304,29 +326,64 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(MlpDims d, long long n, 
 // values, scaled with the training box; outputs unscaled into log activities. A voxel with
 // any input outside the box gets NaN, which the Newton kernel replaces by the reference
 // rule.
-__global__ void mlp_features_kernel(long long n, int nc, int nvar, const double* totals, const double* logk_var,
-                                    const double* xmin, const double* xmax, float* X, unsigned char* inbox) {
-  const long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (v >= n) return;
+//
+// One tile of the fused warm start: the inputs are scaled in registers (each lane computes
+// the inputs its B operand holds) and the outputs are unscaled as they leave the
+// accumulator, so neither the features nor the scaled outputs go through memory. The box
+// test runs in FP32 on the FP32 inputs: a voxel within FP32 rounding of the box edge may
+// go either way, and both starts converge.
+__device__ void mlp_warm_tile(const MlpSmem& sm, int nc, int nvar, int dout, const double* totals,
+                              const double* logk_var, const double* xmin, const double* xmax, const double* ymin,
+                              const double* ymax, double* logc, long long s0, long long n) {
+  const int lane = threadIdx.x & 63;
+  const int smp = lane & 31, kg = lane >> 5;
+  const long long s = s0 + smp;
+  const bool live = s < n;
   const int din = nc + nvar;
   bool ok = true;
-  for (int k = 0; k < din; ++k) {
-    const double raw = (k < nc) ? log10(fmax(totals[v * nc + k], 1e-30)) : logk_var[v * nvar + (k - nc)];
-    if (raw < xmin[k] || raw > xmax[k]) ok = false;
-    const double span = xmax[k] - xmin[k];
-    X[v * din + k] = (float)((span > 0) ? 2.0 * (raw - xmin[k]) / span - 1.0 : 0.0);
+  bf16x4 bx[KIN / 8];
+#pragma unroll
+  for (int ks = 0; ks < KIN / 8; ++ks)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int k = ks * 8 + kg * 4 + j;
+      float f = 0.f;
+      if (live && k < din) {
+        // FP32 logarithm: the input is rounded to BF16 (8-bit mantissa) next
+        const float raw = (k < nc) ? __log10f((float)fmax(totals[s * nc + k], 1e-30))
+                                   : (float)logk_var[s * nvar + (k - nc)];
+        const float lo = (float)xmin[k], hi = (float)xmax[k];
+        if (raw < lo || raw > hi) ok = false;
+        f = (hi > lo) ? (raw - lo) * (2.f / (hi - lo)) - 1.f : 0.f;
+      }
+      bx[ks][j] = bf16s(f);
+    }
+  // lanes l and l ^ 32 hold the two halves of a sample's inputs
+  ok = ok && (__shfl_xor((int)ok, 32) != 0);
+  const f32x16 o = mlp_layers(sm, bx);
+  if (live) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int out = 8 * (r >> 2) + 4 * kg + (r & 3);
+      if (out < dout) {
+        const double y = (double)(o[r] + sm.b3[out]);
+        logc[s * dout + out] = ok ? ymin[out] + 0.5 * (y + 1.0) * (ymax[out] - ymin[out]) : __builtin_nan("");
+      }
+    }
   }
-  if (inbox) inbox[v] = ok ? 1 : 0;
 }
 
-__global__ void mlp_unscale_kernel(long long n, int dout, const float* Y, const unsigned char* inbox,
-                                   const double* ymin, const double* ymax, double* logc) {
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e >= n * dout) return;
-  const long long v = e / dout;
-  const int k = (int)(e % dout);
-  const double y = (double)Y[e];
-  logc[e] = inbox[v] ? ymin[k] + 0.5 * (y + 1.0) * (ymax[k] - ymin[k]) : __builtin_nan("");
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) mlp_warm_kernel(MlpDims d, long long n, const float* theta, int nc, int nvar,
+                                                       const double* totals, const double* logk_var,
+                                                       const double* xmin, const double* xmax, const double* ymin,
+                                                       const double* ymax, double* logc) {
+  extern __shared__ unsigned char smraw[];
+  MlpSmem& sm = *reinterpret_cast<MlpSmem*>(smraw);
+  stage_weights(sm, d, theta);
+  const int wave = threadIdx.x >> 6;
+  const long long tiles = (n + 31) / 32;
+  for (long long t = (long long)blockIdx.x * 4 + wave; t < tiles; t += (long long)gridDim.x * 4)
+    mlp_warm_tile(sm, nc, nvar, d.dout, totals, logk_var, xmin, xmax, ymin, ymax, logc, t * 32, n);
 }
 
 // Training pairs from a solved batch: X = scaled (log10 T, logK), T = scaled log activities.
@@ -349,29 +406,40 @@ __global__ void mlp_dataset_kernel(long long n, int nc, int nvar, const double* 
 
 }  // namespace
 
+void launch_gemm_split(int transA, int transB, int M, int N, int K, const float* A, int lda, const float* B,
+                       int ldb, float* C, int ldc, const float* bias, int act, float beta, int splits, long long sc,
+                       hipStream_t s) {
+  const int kc = (K + splits - 1) / splits;
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
+#define CLB_GEMM(TA_, TB_) \
+  hipLaunchKernelGGL((gemm_kernel<TA_, TB_>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta, kc, sc)
+  if (!transA && !transB) CLB_GEMM(false, false);
+  else if (!transA && transB) CLB_GEMM(false, true);
+  else if (transA && !transB) CLB_GEMM(true, false);
+  else CLB_GEMM(true, true);
+#undef CLB_GEMM
+}
+
 void launch_gemm(int transA, int transB, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
                  float* C, int ldc, const float* bias, int act, float beta, hipStream_t s) {
-  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
-  if (!transA && !transB) hipLaunchKernelGGL((gemm_kernel<false, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta);
-  else if (!transA && transB) hipLaunchKernelGGL((gemm_kernel<false, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta);
-  else if (transA && !transB) hipLaunchKernelGGL((gemm_kernel<true, false>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta);
-  else hipLaunchKernelGGL((gemm_kernel<true, true>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta);
+  launch_gemm_split(transA, transB, M, N, K, A, lda, B, ldb, C, ldc, bias, act, beta, 1, 0, s);
 }
 
 void launch_tanh_backward(long long n, float* dz, const float* h, hipStream_t s) {
   hipLaunchKernelGGL(tanh_backward_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, dz, h);
 }
-void launch_colsum(int rows, int cols, const float* A, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(colsum_kernel, dim3(cols), dim3(256), 0, s, rows, cols, A, out);
+void launch_colsum(int rows, int cols, const float* A, float* out, int splits, long long so, hipStream_t s) {
+  const int kc = (rows + splits - 1) / splits;
+  hipLaunchKernelGGL(colsum_kernel, dim3(cols, splits), dim3(256), 0, s, rows, cols, A, out, kc, so);
 }
 void launch_mse_grad(long long n, const float* Y, const float* T, float scale, float* dY, double* loss,
                      hipStream_t s) {
   hipLaunchKernelGGL(mse_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, Y, T, scale, dY, loss);
 }
-void launch_adam(long long n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2, float eps,
-                 float b1t, float b2t, hipStream_t s) {
-  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, p, g, m, v, lr, b1, b2, eps,
-                     b1t, b2t);
+void launch_adam(long long n, float* p, const float* g, int splits, float* m, float* v, float lr, float b1, float b2,
+                 float eps, float b1t, float b2t, hipStream_t s) {
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, p, g, splits, m, v, lr, b1,
+                     b2, eps, b1t, b2t);
 }
 
 // The fused kernel stages ~82 KB of weights and activations in LDS (CDNA4: 160 KB per
@@ -400,18 +468,11 @@ void launch_mlp_infer_bf16(const MlpDims& d, long long n, const float* theta, co
 }
 
 void launch_mlp_warm(const MlpDims& d, long long n, const float* theta, const double* xmin, const double* xmax,
-                     const double* ymin, const double* ymax, const Tableau& tb, const SpecArgs& sa, float* work,
-                     hipStream_t s) {
-  // work: n*din features, n*dout outputs, n in-box flags
-  float* X = work;
-  float* Y = X + n * d.din;
-  unsigned char* inbox = reinterpret_cast<unsigned char*>(Y + n * d.dout);
-  hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, tb.nc, sa.nvar,
-                     sa.totals, sa.logk_var, xmin, xmax, X, inbox);
+                     const double* ymin, const double* ymax, const Tableau& tb, const SpecArgs& sa, hipStream_t s) {
+  // one fused kernel: features, the three layers and the unscaling per 32-sample tile
   mlp_smem_attr();
-  hipLaunchKernelGGL(mlp_infer_kernel, dim3(mlp_grid(n)), dim3(256), sizeof(MlpSmem), s, d, n, theta, X, Y);
-  hipLaunchKernelGGL(mlp_unscale_kernel, dim3((unsigned)((n * d.dout + 255) / 256)), dim3(256), 0, s, n, d.dout, Y,
-                     inbox, ymin, ymax, sa.logc);
+  hipLaunchKernelGGL(mlp_warm_kernel, dim3(mlp_grid(n)), dim3(256), sizeof(MlpSmem), s, d, n, theta, tb.nc, sa.nvar,
+                     sa.totals, sa.logk_var, xmin, xmax, ymin, ymax, sa.logc);
 }
 
 void launch_mlp_make_dataset(const MlpDims& d, long long n, const Tableau& tb, const SpecArgs& sa, float* X, float* T,
