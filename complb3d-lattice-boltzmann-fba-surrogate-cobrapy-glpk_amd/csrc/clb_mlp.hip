@@ -151,7 +151,7 @@ __global__ void adam_kernel(long long n, float* p, const float* g, int splits, f
 }
 
 // ------------------------------------------------------------------ BF16 MFMA inference
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -162,46 +162,52 @@ __device__ __forceinline__ unsigned short f2bf(float f) {
 }
 
 constexpr int HID = 128;     // hidden width of the MFMA kernel
-constexpr int KIN = 16;      // padded input width
+constexpr int KIN = 16;      // padded input width (one K = 16 step)
 constexpr int NOUT = 32;     // padded output width
-constexpr int HPAD = HID + 4;
-constexpr int KPAD = KIN + 4;
+constexpr int HPAD = HID + 8;  // row strides keep 16-byte rows and spread the rows over the banks
+constexpr int KPAD = KIN + 8;
 
-// Weights staged once per block into LDS as BF16, [out][in] with padded rows.
+// The layers use v_mfma_f32_32x32x16_bf16 on the transposed product Z^T = W A^T: the
+// weights (LDS) are the A operand, the activations the B operand, so a 32 x 32 tile covers
+// 32 units x 32 samples. Lane l works on sample l % 32 with lane group g = l >> 5. The
+// accumulator of a tile holds, in entry r, unit 8 (r >> 2) + 4 g + (r & 3). In k-step ks
+// the MFMA pairs A[i][8 g + e] with B[8 g + e][n] (e = 0..7), and any assignment of units
+// to these slots gives the same sum when A and B use the same one. Slot (ks, g, e) is given
+// unit 16 ks + 8 (e >> 2) + 4 g + (e & 3): then a lane's B operand for k-step ks is entries
+// 8 (ks % 2) .. 8 (ks % 2) + 7 of tile ks / 2 of the previous layer's accumulator, on the
+// same lane, and each layer's output feeds the next one from registers. The weights are
+// staged with their columns in slot order, so an A operand is one 16-byte LDS read.
+__host__ __device__ constexpr int slot_unit(int p) {  // column p = 16 ks + 8 g + e
+  return (p & ~15) + 8 * ((p & 7) >> 2) + 4 * ((p >> 3) & 1) + (p & 3);
+}
+
+// Weights staged once per block into LDS as BF16, [out][slot] with padded rows.
 struct MlpSmem {
-  unsigned short w1[HID * KPAD];
-  unsigned short w2[HID * HPAD];
-  unsigned short w3[NOUT * HPAD];
+  alignas(16) unsigned short w1[HID * KPAD];
+  alignas(16) unsigned short w2[HID * HPAD];
+  alignas(16) unsigned short w3[NOUT * HPAD];
   float b1[HID], b2[HID], b3[NOUT];
 };
 
-// Four consecutive BF16 values (8-byte aligned in LDS) as one 64-bit LDS read.
-__device__ __forceinline__ bf16x4 lds_bf16x4(const unsigned short* p) {
-  return *reinterpret_cast<const bf16x4*>(p);
+__device__ __forceinline__ bf16x8 lds_bf16x8(const unsigned short* p) {
+  return *reinterpret_cast<const bf16x8*>(p);
 }
 
 __device__ __forceinline__ float fast_tanh(float x) {
-  // 1 - 2 / (e^{2x} + 1): saturates to +-1 (e^{2x} overflows to inf or underflows to 0);
-  // its error is far below the BF16 rounding the activation gets next
-  return 1.f - 2.f * __frcp_rn(__expf(2.f * x) + 1.f);
+  // 1 - 2 / (e^{2x} + 1) with the hardware reciprocal (1 ulp; a correctly rounded divide
+  // costs ten instructions more): saturates to +-1 (e^{2x} overflows to inf or underflows
+  // to 0); its error is far below the BF16 rounding the activation gets next
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(__expf(2.f * x) + 1.f);
 }
 
-__device__ __forceinline__ short bf16s(float f) { return (short)f2bf(f); }
-
-// The layers are computed transposed, Z^T = W A^T: the weights (LDS) are the A operand,
-// the activations the B operand, so a 32 x 32 tile covers 32 units x 32 samples. In the
-// accumulator, lane l holds sample l % 32 and units 8 (r >> 2) + 4 (l >> 5) + (r & 3) of
-// its tile (r = 0..15); the B operand of k-step ks wants units 8 ks + 4 (l >> 5) + j, which
-// are entries r = 4 (ks % 4) + j of tile ks / 4 on the same lane. So each layer's output,
-// through tanh and BF16, feeds the next layer straight from registers.
 template <int KSTEPS>
-__device__ __forceinline__ f32x16 mfma_wt(const unsigned short* W, int ldw, int n0, const bf16x4* b, f32x16 acc) {
+__device__ __forceinline__ f32x16 mfma_wt(const unsigned short* W, int ldw, int n0, const bf16x8* b, f32x16 acc) {
   const int lane = threadIdx.x & 63;
-  const int i = lane & 31, kg = lane >> 5;
+  const int i = lane & 31, g = lane >> 5;
 #pragma unroll
   for (int ks = 0; ks < KSTEPS; ++ks) {
-    const bf16x4 a = lds_bf16x4(W + (n0 + i) * ldw + ks * 8 + kg * 4);
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[ks], acc, 0, 0, 0);
+    const bf16x8 a = lds_bf16x8(W + (n0 + i) * ldw + ks * 16 + g * 8);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b[ks], acc, 0, 0, 0);
     // keep the weight reads of later k-steps from being hoisted: they would hold registers
     // that the other wavefronts of the SIMD need
     __builtin_amdgcn_sched_barrier(0);
@@ -209,83 +215,89 @@ __device__ __forceinline__ f32x16 mfma_wt(const unsigned short* W, int ldw, int 
   return acc;
 }
 
+// Hidden activation of one 32-unit tile into the next layer's B operand (k-steps 2t, 2t+1).
+__device__ __forceinline__ void tile_to_operand(const f32x16& z, const float* bias, int t, bf16x8* h) {
+  const int g = (threadIdx.x & 63) >> 5;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int unit = t * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+    h[2 * t + (r >> 3)][r & 7] = (__bf16)fast_tanh(z[r] + bias[unit]);
+  }
+}
+
 // The three layers for 32 samples per wavefront (no LDS besides the weights, no
-// barriers). bx is layer 1's B operand: lane l holds inputs 8 ks + 4 (l >> 5) + j of sample
-// l % 32 in bx[ks][j]. Returns the output layer's accumulator without its bias: lane l
-// holds outputs 8 (r >> 2) + 4 (l >> 5) + (r & 3) of sample l % 32 in entry r.
-__device__ __forceinline__ f32x16 mlp_layers(const MlpSmem& sm, const bf16x4* bx) {
+// barriers). bx is layer 1's B operand: lane l holds input slot_unit(8 g + e) of sample
+// l % 32 in bx[e]. Returns the output layer's accumulator without its bias: lane l holds
+// outputs 8 (r >> 2) + 4 g + (r & 3) of sample l % 32 in entry r.
+__device__ __forceinline__ f32x16 mlp_layers(const MlpSmem& sm, const bf16x8& bx) {
   const int lane = threadIdx.x & 63;
-  const int kg = lane >> 5;
-  // hidden layers: h1 / h2 hold the B operand of the next layer, 16 k-steps of 4 units;
-  // each 32-unit tile goes through tanh as soon as it is done
-  bf16x4 h1[HID / 8], h2[HID / 8];
+  const int i = lane & 31, g = lane >> 5;
+  bf16x8 h1[HID / 16], h2[HID / 16];
 #pragma unroll
   for (int t = 0; t < HID / 32; ++t) {
     f32x16 z = {0};
-    z = mfma_wt<KIN / 8>(sm.w1, KPAD, t * 32, bx, z);
-#pragma unroll
-    for (int q = 0; q < 4; ++q)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int unit = t * 32 + 8 * q + 4 * kg + j;
-        h1[t * 4 + q][j] = bf16s(fast_tanh(z[q * 4 + j] + sm.b1[unit]));
-      }
+    z = mfma_wt<1>(sm.w1, KPAD, t * 32, &bx, z);
+    tile_to_operand(z, sm.b1, t, h1);
   }
   {
     // layer 2, k outer: the four 32-unit output tiles accumulate side by side
-    const int i = lane & 31;
     f32x16 z[HID / 32];
 #pragma unroll
     for (int t = 0; t < HID / 32; ++t) z[t] = f32x16{0};
 #pragma unroll
-    for (int ks = 0; ks < HID / 8; ++ks) {
+    for (int ks = 0; ks < HID / 16; ++ks) {
 #pragma unroll
       for (int t = 0; t < HID / 32; ++t) {
-        const bf16x4 a = lds_bf16x4(sm.w2 + (t * 32 + i) * HPAD + ks * 8 + kg * 4);
-        z[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, h1[ks], z[t], 0, 0, 0);
+        const bf16x8 a = lds_bf16x8(sm.w2 + (t * 32 + i) * HPAD + ks * 16 + g * 8);
+        z[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, h1[ks], z[t], 0, 0, 0);
       }
       __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
-    for (int t = 0; t < HID / 32; ++t)
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int unit = t * 32 + 8 * q + 4 * kg + j;
-          h2[t * 4 + q][j] = bf16s(fast_tanh(z[t][q * 4 + j] + sm.b2[unit]));
-        }
+    for (int t = 0; t < HID / 32; ++t) tile_to_operand(z[t], sm.b2, t, h2);
   }
   // output layer: one tile of 32 (dout used)
   f32x16 acc = {0};
-  return mfma_wt<HID / 8>(sm.w3, HPAD, 0, h2, acc);
+  return mfma_wt<HID / 16>(sm.w3, HPAD, 0, h2, acc);
 }
 
 // Inference for one tile: X holds din scaled inputs per sample (fp32, row-major); Y
 // receives dout scaled outputs.
 __device__ void mlp_tile(const MlpSmem& sm, int din, int dout, const float* X, float* Y, long long s0, long long n) {
   const int lane = threadIdx.x & 63;
-  const int smp = lane & 31, kg = lane >> 5;
+  const int smp = lane & 31, g = lane >> 5;
   const long long s = s0 + smp;
   const bool live = s < n;
-  bf16x4 bx[KIN / 8];
+  bf16x8 bx;
 #pragma unroll
-  for (int ks = 0; ks < KIN / 8; ++ks)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = ks * 8 + kg * 4 + j;
-      bx[ks][j] = bf16s((live && k < din) ? X[s * din + k] : 0.f);
-    }
+  for (int e = 0; e < 8; ++e) {
+    const int k = slot_unit(8 * g + e);
+    bx[e] = (__bf16)((live && k < din) ? X[s * din + k] : 0.f);
+  }
   const f32x16 o = mlp_layers(sm, bx);
   if (live) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int out = 8 * (r >> 2) + 4 * kg + (r & 3);
+      const int out = 8 * (r >> 2) + 4 * g + (r & 3);
       if (out < dout) Y[s * dout + out] = o[r] + sm.b3[out];
     }
   }
 }
 
+// Column of unit k: slot_unit swaps bits 2 and 3 of the index, so it is its own inverse
+// and keeps groups of four consecutive units together.
+__device__ __forceinline__ int unit_slot(int k) { return slot_unit(k); }
+
+// Four FP32 weights of consecutive units as BF16 into LDS (one 8-byte store).
+__device__ __forceinline__ void store_bf16x4(unsigned short* p, const float4& v) {
+  const unsigned int lo = (unsigned int)f2bf(v.x) | ((unsigned int)f2bf(v.y) << 16);
+  const unsigned int hi = (unsigned int)f2bf(v.z) | ((unsigned int)f2bf(v.w) << 16);
+  *reinterpret_cast<uint2*>(p) = make_uint2(lo, hi);
+}
+
+// Stage the weights (256 threads). W2 and W3 rows are read as float4 (every row starts on
+// a 16-byte boundary: theta is 16-byte aligned and the offsets are multiples of 128
+// floats), all of a thread's loads issued before its stores.
 __device__ void stage_weights(MlpSmem& sm, const MlpDims& d, const float* theta) {
   const float* W1 = theta;
   const float* b1 = W1 + HID * d.din;
@@ -293,26 +305,57 @@ __device__ void stage_weights(MlpSmem& sm, const MlpDims& d, const float* theta)
   const float* b2 = W2 + HID * HID;
   const float* W3 = b2 + HID;
   const float* b3 = W3 + d.dout * HID;
-  for (int e = threadIdx.x; e < HID * KPAD; e += blockDim.x) {
-    const int o = e / KPAD, k = e % KPAD;
-    sm.w1[e] = f2bf((k < d.din) ? W1[o * d.din + k] : 0.f);
+  const int tid = threadIdx.x;
+  {
+    constexpr int PER = HID * HID / 4 / 256;
+    float4 v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) v[q] = reinterpret_cast<const float4*>(W2)[tid + q * 256];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = tid + q * 256, o = idx >> 5, k = (idx & 31) * 4;
+      store_bf16x4(&sm.w2[o * HPAD + unit_slot(k)], v[q]);
+    }
   }
-  for (int e = threadIdx.x; e < HID * HPAD; e += blockDim.x) {
-    const int o = e / HPAD, k = e % HPAD;
-    sm.w2[e] = f2bf((k < HID) ? W2[o * HID + k] : 0.f);
+  {
+    constexpr int PER = NOUT * HID / 4 / 256;
+    float4 v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = tid + q * 256, o = idx >> 5;
+      v[q] = (o < d.dout) ? reinterpret_cast<const float4*>(W3)[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int idx = tid + q * 256, o = idx >> 5, k = (idx & 31) * 4;
+      store_bf16x4(&sm.w3[o * HPAD + unit_slot(k)], v[q]);
+    }
   }
-  for (int e = threadIdx.x; e < NOUT * HPAD; e += blockDim.x) {
-    const int o = e / HPAD, k = e % HPAD;
-    sm.w3[e] = f2bf((o < d.dout && k < HID) ? W3[o * HID + k] : 0.f);
+  {
+    constexpr int PER = HID * KIN / 256;
+    float v[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * 256, o = e / KIN, k = slot_unit(e % KIN);
+      v[q] = (k < d.din) ? W1[o * d.din + k] : 0.f;
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+      const int e = tid + q * 256;
+      sm.w1[(e / KIN) * KPAD + e % KIN] = f2bf(v[q]);
+    }
   }
-  for (int e = threadIdx.x; e < HID; e += blockDim.x) { sm.b1[e] = b1[e]; sm.b2[e] = b2[e]; }
-  for (int e = threadIdx.x; e < NOUT; e += blockDim.x) sm.b3[e] = (e < d.dout) ? b3[e] : 0.f;
+  if (tid < HID) {
+    sm.b1[tid] = b1[tid];
+    sm.b2[tid] = b2[tid];
+  }
+  if (tid < NOUT) sm.b3[tid] = (tid < d.dout) ? b3[tid] : 0.f;
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) mlp_infer_kernel(MlpDims d, long long n, const float* theta, const float* X,
-                                                        float* Y) {
-  extern __shared__ unsigned char smraw[];
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+mlp_infer_kernel(MlpDims d, long long n, const float* theta, const float* X, float* Y) {
+  extern __shared__ __align__(16) unsigned char smraw[];
   MlpSmem& sm = *reinterpret_cast<MlpSmem*>(smraw);
   stage_weights(sm, d, theta);
   const int wave = threadIdx.x >> 6;
@@ -336,35 +379,33 @@ __device__ void mlp_warm_tile(const MlpSmem& sm, int nc, int nvar, int dout, con
                               const double* logk_var, const double* xmin, const double* xmax, const double* ymin,
                               const double* ymax, double* logc, long long s0, long long n) {
   const int lane = threadIdx.x & 63;
-  const int smp = lane & 31, kg = lane >> 5;
+  const int smp = lane & 31, g = lane >> 5;
   const long long s = s0 + smp;
   const bool live = s < n;
   const int din = nc + nvar;
   bool ok = true;
-  bf16x4 bx[KIN / 8];
+  bf16x8 bx;
 #pragma unroll
-  for (int ks = 0; ks < KIN / 8; ++ks)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = ks * 8 + kg * 4 + j;
-      float f = 0.f;
-      if (live && k < din) {
-        // FP32 logarithm: the input is rounded to BF16 (8-bit mantissa) next
-        const float raw = (k < nc) ? __log10f((float)fmax(totals[s * nc + k], 1e-30))
-                                   : (float)logk_var[s * nvar + (k - nc)];
-        const float lo = (float)xmin[k], hi = (float)xmax[k];
-        if (raw < lo || raw > hi) ok = false;
-        f = (hi > lo) ? (raw - lo) * (2.f / (hi - lo)) - 1.f : 0.f;
-      }
-      bx[ks][j] = bf16s(f);
+  for (int e = 0; e < 8; ++e) {
+    const int k = slot_unit(8 * g + e);
+    float f = 0.f;
+    if (live && k < din) {
+      // FP32 logarithm: the input is rounded to BF16 (8-bit mantissa) next
+      const float raw = (k < nc) ? __log10f((float)fmax(totals[s * nc + k], 1e-30))
+                                 : (float)logk_var[s * nvar + (k - nc)];
+      const float lo = (float)xmin[k], hi = (float)xmax[k];
+      if (raw < lo || raw > hi) ok = false;
+      f = (hi > lo) ? (raw - lo) * (2.f / (hi - lo)) - 1.f : 0.f;
     }
+    bx[e] = (__bf16)f;
+  }
   // lanes l and l ^ 32 hold the two halves of a sample's inputs
   ok = ok && (__shfl_xor((int)ok, 32) != 0);
   const f32x16 o = mlp_layers(sm, bx);
   if (live) {
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int out = 8 * (r >> 2) + 4 * kg + (r & 3);
+      const int out = 8 * (r >> 2) + 4 * g + (r & 3);
       if (out < dout) {
         const double y = (double)(o[r] + sm.b3[out]);
         logc[s * dout + out] = ok ? ymin[out] + 0.5 * (y + 1.0) * (ymax[out] - ymin[out]) : __builtin_nan("");
@@ -373,11 +414,11 @@ __device__ void mlp_warm_tile(const MlpSmem& sm, int nc, int nvar, int dout, con
   }
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) mlp_warm_kernel(MlpDims d, long long n, const float* theta, int nc, int nvar,
-                                                       const double* totals, const double* logk_var,
-                                                       const double* xmin, const double* xmax, const double* ymin,
-                                                       const double* ymax, double* logc) {
-  extern __shared__ unsigned char smraw[];
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+mlp_warm_kernel(MlpDims d, long long n, const float* theta, int nc, int nvar, const double* totals,
+                const double* logk_var, const double* xmin, const double* xmax, const double* ymin,
+                const double* ymax, double* logc) {
+  extern __shared__ __align__(16) unsigned char smraw[];
   MlpSmem& sm = *reinterpret_cast<MlpSmem*>(smraw);
   stage_weights(sm, d, theta);
   const int wave = threadIdx.x >> 6;
@@ -442,21 +483,33 @@ void launch_adam(long long n, float* p, const float* g, int splits, float* m, fl
                      b2, eps, b1t, b2t);
 }
 
-// The fused kernel stages ~82 KB of weights and activations in LDS (CDNA4: 160 KB per
-// CU); raise the per-launch dynamic LDS limit once.
+// The inference kernels stage ~50 KB of BF16 weights in LDS per block (three blocks per
+// CU); allow the dynamic LDS size explicitly once.
 static void mlp_smem_attr() {
   static bool done = false;
   if (!done) {
     (void)hipFuncSetAttribute((const void*)mlp_infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)sizeof(MlpSmem));
+    (void)hipFuncSetAttribute((const void*)mlp_warm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)sizeof(MlpSmem));
     done = true;
   }
 }
 
+// One resident wave of blocks (three per CU): each block stages the weights once and its
+// wavefronts then take tiles in turn.
 static int mlp_grid(long long n) {
+  static int resident = 0;
+  if (!resident) {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess || cus < 1)
+      cus = 256;
+    resident = 3 * cus;
+  }
   const long long tiles = (n + 31) / 32;
   long long blocks = (tiles + 3) / 4;
-  if (blocks > 2048) blocks = 2048;
+  if (blocks > resident) blocks = resident;
   if (blocks < 1) blocks = 1;
   return (int)blocks;
 }
