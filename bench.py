@@ -373,6 +373,29 @@ def cpu_baseline(oracle_mod, clb, nx, ny, nz, steps):
                        f"{nx}x{ny}x{nz} slab of the same workload, {steps} steps ({dt:.1f} s)")
 
 
+def speciation_cpu_baseline(oracle_mod, clb, n_side, budget_s=8.0):
+    """The FP64 oracle's speciation (the reference's PCF + Anderson solve) on a bounded sample
+    of the same smoothed totals field as the GPU's field leg (8 / 24 tableau, 1e-10 judge): a
+    2 000-voxel probe sizes the sample to about `budget_s` seconds."""
+    S, lk = clb.gen_tableau(8, 24, seed=3)
+    tot = clb.gen_totals_field(n_side, n_side, n_side, 8, sigma=4.0, seed=4)
+
+    def run(m):
+        conc = np.zeros((m, 24))
+        conc[:, :8] = tot[:m]
+        t0 = time.perf_counter()
+        _, it, _, cv, _ = oracle_mod.speciate_pcf(S, lk, list(range(8)), conc, tol=1e-10, max_iter=200)
+        return time.perf_counter() - t0, it, cv
+
+    dt, _, _ = run(2000)
+    m = int(min(tot.shape[0], max(2000, 2000 * budget_s / max(dt, 1e-6))))
+    dt, it, cv = run(m)
+    return dict(value=m / dt, unit="solves/s", cores=oracle_mod.num_threads(), kind="port",
+                sample=f"FP64 oracle (PCF + Anderson) on the first {m} voxels of the {n_side}^3 totals field, "
+                       f"8 / 24 tableau, 1e-10 judge ({dt:.1f} s, mean {it.mean():.1f} iterations, "
+                       f"converged {cv.mean():.4f})")
+
+
 def copy_calibration():
     """Device-to-device copies of known size (the HIP runtime's copy kernel), for scaling the
     HBM traffic counters: each copy reads and writes 1 GiB."""
@@ -489,6 +512,10 @@ def main():
         import oracle as orc
         cpu = cpu_baseline(orc, clb, args.nx, args.ny, args.cpu_nz, args.cpu_steps)
         log(f"cpu baseline: {cpu['value']:.2f} MLUPS on {cpu['cores']} threads")
+        if not args.skip_speciation:
+            sc = speciation_cpu_baseline(orc, clb, args.spec_side)
+            extra["speciation"]["cpu_baseline"] = sc
+            log(f"cpu speciation baseline: {sc['value']:.0f} solves/s on {sc['cores']} threads ({sc['sample']})")
     line = {
         "metric": METRIC,
         "value": round(fc["mlups"], 2),
