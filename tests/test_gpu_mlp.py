@@ -148,3 +148,32 @@ def test_warm_start_gain_at_full_size(clb):
           f"kernel time {out['plain'][0]:.2f} -> {out['mlp'][0]:.2f} ms ({time_ratio:.2f}x)", flush=True)
     assert eval_ratio >= 3.0
     assert time_ratio >= 2.9
+
+
+@pytest.mark.gpu
+def test_warm_start_small_tableau_without_logk_inputs(clb):
+    """The fused warm start with padded shapes: 4 components and no varying logK, so the
+    network is 4 -> 128 -> 128 -> 4 and most input slots and output rows of the MFMA tiles
+    are padding. Same solutions, fewer evaluations, and a sample count that leaves a
+    partial 32-sample tile."""
+    print("case: mlp warm start 4/8, no logK inputs", flush=True)
+    S, lk = clb.gen_tableau(4, 8, seed=11)
+    lat = clb.Lattice(4, 1, 1)
+    lat.set_tableau(S, lk, list(range(4)), tol=1e-10, max_iter=200)
+    rng = np.random.default_rng(12)
+    lat.batch_load(10 ** rng.uniform(-6, -2, (100000, 4)))
+    lat.batch_solve(clb.WARM_REF)
+    lat.mlp_config(4, H, 4)
+    loss = lat.mlp_train_from_batch(epochs=20, batch=512, lr=2e-3, seed=7)
+    assert loss < 0.01
+    T = 10 ** np.random.default_rng(13).uniform(-6, -2, (20011, 4))
+    lat.batch_load(T)
+    lat.batch_solve(clb.WARM_REF)
+    x0, it0, r0, cv0 = lat.batch_fetch()
+    lat.batch_solve(clb.WARM_MLP)
+    x1, it1, r1, cv1 = lat.batch_fetch()
+    lat.close()
+    assert cv0.all() and cv1.all() and np.all(r1 < 1e-10)
+    assert np.max(np.abs(x1 - x0)) < 1e-6
+    print(f"mean iterations: reference start {it0.mean():.2f}, MLP start {it1.mean():.2f}", flush=True)
+    assert it1.mean() < it0.mean()
