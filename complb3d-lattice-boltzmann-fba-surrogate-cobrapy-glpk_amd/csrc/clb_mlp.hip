@@ -486,27 +486,29 @@ void launch_adam(long long n, float* p, const float* g, int splits, float* m, fl
 // The inference kernels stage ~50 KB of BF16 weights in LDS per block (three blocks per
 // CU); allow the dynamic LDS size explicitly once.
 static void mlp_smem_attr() {
-  static bool done = false;
-  if (!done) {
+  // a function-local static is initialised once, thread-safely
+  static const bool done = [] {
     (void)hipFuncSetAttribute((const void*)mlp_infer_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)sizeof(MlpSmem));
     (void)hipFuncSetAttribute((const void*)mlp_warm_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                               (int)sizeof(MlpSmem));
-    done = true;
-  }
+    return true;
+  }();
+  (void)done;
 }
 
 // One resident wave of blocks (three per CU): each block stages the weights once and its
 // wavefronts then take tiles in turn.
 static int mlp_grid(long long n) {
-  static int resident = 0;
-  if (!resident) {
+  // sized once (thread-safe static initialisation); any grid size is correct, since the
+  // wavefronts loop over the tiles
+  static const int resident = [] {
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
                                                  hipSuccess || cus < 1)
       cus = 256;
-    resident = 3 * cus;
-  }
+    return 3 * cus;
+  }();
   const long long tiles = (n + 31) / 32;
   long long blocks = (tiles + 3) / 4;
   if (blocks > resident) blocks = resident;
