@@ -73,10 +73,14 @@ template <typename R, bool ODD, bool XCHK = true>
 __device__ __forceinline__ void store19(R* __restrict__ f, long long N, const Nbr& nb, int n, const R* p) {
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
+    // non-temporal stores: the 19 populations of the whole lattice (1 GB at 384 x 192 x 192)
+    // exceed the last-level cache, so a stored line is evicted before the next step reads it
+    // and would only displace lines that this step's neighbour loads still use
     if (!ODD) {
-      f[(long long)opp_dir(i) * N + n] = p[i];
+      __builtin_nontemporal_store(p[i], &f[(long long)opp_dir(i) * N + n]);
     } else {
-      if (!XCHK || nb.xok(Q19::cx[i])) f[(long long)i * N + nb.at(Q19::cx[i], Q19::cy[i], Q19::cz[i])] = p[i];
+      if (!XCHK || nb.xok(Q19::cx[i]))
+        __builtin_nontemporal_store(p[i], &f[(long long)i * N + nb.at(Q19::cx[i], Q19::cy[i], Q19::cz[i])]);
     }
   }
 }
