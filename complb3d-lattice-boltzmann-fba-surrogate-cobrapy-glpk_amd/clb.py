@@ -57,7 +57,8 @@ class Perf(ctypes.Structure):
                 ("spec_ms", ctypes.c_double), ("spec_launches", ctypes.c_longlong),
                 ("mlp_ms", ctypes.c_double), ("mlp_launches", ctypes.c_longlong),
                 ("fluid_nodes", ctypes.c_longlong), ("reactive_nodes", ctypes.c_longlong),
-                ("srg_ms", ctypes.c_double), ("srg_launches", ctypes.c_longlong)]
+                ("srg_ms", ctypes.c_double), ("srg_launches", ctypes.c_longlong),
+                ("stragglers", ctypes.c_longlong)]
 
 
 _lib = None
@@ -82,6 +83,7 @@ def load():
         "clb_upload_mask": [vp, vp],
         "clb_download_mask": [vp, vp],
         "clb_set_dynamics": [vp, vp, vp],
+        "clb_set_solute_dynamics": [vp, vp],
         "clb_flow_init": [vp, c_double, c_double],
         "clb_flow_steps": [vp, c_int, vp],
         "clb_flow_download_velocity": [vp, vp, vp],
@@ -439,6 +441,13 @@ class Lattice:
         o = _f64(flow_omega)
         assert k.size == 256 and o.size == 256
         self._chk(self.lib.clb_set_dynamics(self.h, _ptr(k), _ptr(o)), "set_dynamics")
+
+    def set_solute_dynamics(self, kind):
+        """The solute lattices' own per-code dynamics (soluteDomainSetup): a biofilm code
+        stays advection-diffusion here even when it is bounce-back for the flow."""
+        k = np.ascontiguousarray(kind, dtype=np.int32)
+        assert k.size == 256
+        self._chk(self.lib.clb_set_solute_dynamics(self.h, _ptr(k)), "set_solute_dynamics")
 
     # flow
     def flow_init(self, rho_in, rho_out):

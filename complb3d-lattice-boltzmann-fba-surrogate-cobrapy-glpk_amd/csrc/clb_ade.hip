@@ -176,11 +176,12 @@ __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int
       phase[n] = 0;
       flip = (int)m != pore_code;
     }
-    if (flip && a.mask[n] == (uint8_t)pore_code && a.kind[m] != CLB_KIND_FLUID) {
+    if (flip && a.mask[n] == (uint8_t)pore_code) {
       // a reopened wall voxel starts at rest: flow at the reference density, mobile
-      // solutes at zero (its wall slots held only bounced populations)
-      write_rest_state(a, ODD, x, y, z, n, nullptr);
-      if (flowf) {
+      // solutes at zero (its wall slots held only bounced populations); each lattice
+      // judges "wall" by its own dynamics table
+      if (a.kind[m] != CLB_KIND_FLUID) write_rest_state(a, ODD, x, y, z, n, nullptr);
+      if (flowf && a.flow_kind[m] != CLB_KIND_FLUID) {
         const Nbr nb(x, y, z, g);
 #pragma unroll
         for (int i = 0; i < 19; ++i) {

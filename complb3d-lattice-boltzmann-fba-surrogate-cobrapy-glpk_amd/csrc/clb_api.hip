@@ -33,8 +33,11 @@ struct clb_ctx {
   std::string err;
 
   uint8_t* d_mask = nullptr;
-  uint8_t* d_kind = nullptr;
+  uint8_t* d_kind = nullptr;          // flow lattice dynamics per mask code (NSdomainSetup)
   int h_kind[256] = {0};
+  uint8_t* d_ade_kind = nullptr;      // solute lattices' dynamics per mask code (soluteDomainSetup)
+  int h_ade_kind[256] = {0};
+  bool ade_kind_set = false;          // set by clb_set_solute_dynamics (else follows h_kind)
   double h_flow_omega[256] = {0};
   void* d_flow_omega = nullptr;
 
@@ -254,7 +257,8 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   AdeArgs<R> a;
   a.g = (R*)c->d_g;
   a.mask = c->d_mask;
-  a.kind = c->d_kind;
+  a.kind = c->d_ade_kind;
+  a.flow_kind = c->d_kind;
   a.omega = (const R*)c->d_ade_omega;
   a.u = (const R*)c->d_u;
   a.bmass = (R*)c->d_b;
@@ -291,6 +295,8 @@ int upload_tables(clb_ctx* c) {
   uint8_t k8[256];
   for (int i = 0; i < 256; ++i) k8[i] = (uint8_t)c->h_kind[i];
   CLB_CHECK_HIP(c, hipMemcpy(c->d_kind, k8, 256, hipMemcpyHostToDevice));
+  for (int i = 0; i < 256; ++i) k8[i] = (uint8_t)c->h_ade_kind[i];
+  CLB_CHECK_HIP(c, hipMemcpy(c->d_ade_kind, k8, 256, hipMemcpyHostToDevice));
   if (c->fp32) {
     float o[256];
     for (int i = 0; i < 256; ++i) o[i] = (float)c->h_flow_omega[i];
@@ -379,7 +385,7 @@ int build_reactive_list(clb_ctx* c) {
   if (rc) return rc;
   launch_fill_u32(c->d_counter, 1, 0u, c->stream);
   hipLaunchKernelGGL(reactive_list_kernel, dim3((unsigned)((c->geo.N + 255) / 256)), dim3(256), 0, c->stream,
-                     c->d_mask, c->d_kind, c->geo, c->solid_code, c->bb_code, c->d_list, c->d_counter);
+                     c->d_mask, c->d_ade_kind, c->geo, c->solid_code, c->bb_code, c->d_list, c->d_counter);
   unsigned int cnt = 0;
   CLB_CHECK_HIP(c, hipMemcpyAsync(&cnt, c->d_counter, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
   CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
@@ -405,6 +411,21 @@ int prepare_fused(clb_ctx* c) {
   if (rc) return rc;
   const long long nw = fused_waves(c->geo, c->spec_block);
   if (c->blk_stats_n != nw) {
+    // a new block shape: fold the fused solves counted so far into the totals first
+    if (c->d_blk_stats && c->blk_stats_n > 0) {
+      std::vector<unsigned long long> b((size_t)(2 * c->blk_stats_n));
+      unsigned long long h[3];
+      CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+      CLB_CHECK_HIP(c, hipMemcpy(b.data(), c->d_blk_stats, sizeof(unsigned long long) * b.size(),
+                                 hipMemcpyDeviceToHost));
+      CLB_CHECK_HIP(c, hipMemcpy(h, c->d_stats, sizeof(h), hipMemcpyDeviceToHost));
+      for (long long k = 0; k < c->blk_stats_n; ++k) {
+        h[0] += b[2 * k];
+        h[1] += b[2 * k];
+        h[2] += b[2 * k + 1];
+      }
+      CLB_CHECK_HIP(c, hipMemcpy(c->d_stats, h, sizeof(h), hipMemcpyHostToDevice));
+    }
     rc = dalloc(c, &c->d_blk_stats, (size_t)(2 * nw));
     if (rc) return rc;
     launch_fill_u64(c->d_blk_stats, 2 * nw, 0ull, c->stream);
@@ -829,6 +850,7 @@ int clb_create(const clb_config* cfg, clb_ctx** out) {
   int rc = CLB_OK;
   if (!rc) rc = dalloc(c, &c->d_mask, (size_t)N);
   if (!rc) rc = dalloc(c, &c->d_kind, 256);
+  if (!rc) rc = dalloc(c, &c->d_ade_kind, 256);
   if (!rc) rc = dalloc(c, (uint8_t**)&c->d_flow_omega, 256 * rs);
   if (!rc) rc = dalloc(c, &c->d_energy, 2);
   if (!rc) c->energy_n = 2;
@@ -841,6 +863,7 @@ int clb_create(const clb_config* cfg, clb_ctx** out) {
   // default material numbers: 0 no dynamics, 1 bounce-back, others fluid with tau = 0.8
   for (int i = 0; i < 256; ++i) {
     c->h_kind[i] = (i == 0) ? CLB_KIND_NONE : (i == 1 ? CLB_KIND_BB : CLB_KIND_FLUID);
+    c->h_ade_kind[i] = c->h_kind[i];
     c->h_flow_omega[i] = 1.0 / 0.8;
   }
   rc = upload_tables(c);
@@ -858,7 +881,7 @@ int clb_destroy(clb_ctx* c) {
   if (!c) return CLB_ERR_ARG;
   drop_graph(c);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  void* ptrs[] = {c->d_mask, c->d_kind, c->d_flow_omega, c->d_f, c->d_energy, c->d_ade_omega, c->d_cref, c->d_g,
+  void* ptrs[] = {c->d_mask, c->d_kind, c->d_ade_kind, c->d_flow_omega, c->d_f, c->d_energy, c->d_ade_omega, c->d_cref, c->d_g,
                   c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
                   c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
@@ -911,13 +934,39 @@ int clb_set_dynamics(clb_ctx* c, const int* kind, const double* flow_omega) {
   if (!c || !kind) return CLB_ERR_ARG;
   for (int i = 0; i < 256; ++i) {
     if (kind[i] < CLB_KIND_NONE || kind[i] > CLB_KIND_FLUID) return fail(c, CLB_ERR_ARG, "bad dynamics kind");
-    c->h_kind[i] = kind[i];
-    if (flow_omega) {
-      if (kind[i] == CLB_KIND_FLUID && !(flow_omega[i] > 0.0 && flow_omega[i] < 2.0))
-        return fail(c, CLB_ERR_ARG, "flow omega outside (0, 2)");
-      c->h_flow_omega[i] = flow_omega[i];
-    }
+    if (flow_omega && kind[i] == CLB_KIND_FLUID && !(flow_omega[i] > 0.0 && flow_omega[i] < 2.0))
+      return fail(c, CLB_ERR_ARG, "flow omega outside (0, 2)");
   }
+  if (!c->ade_kind_set) {
+    // the solute lattices follow this table until clb_set_solute_dynamics gives them
+    // their own: the deferred increments were routed by the old one
+    const int rc = flush_pending(c);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < 256; ++i) {
+    c->h_kind[i] = kind[i];
+    if (!c->ade_kind_set) c->h_ade_kind[i] = kind[i];
+    if (flow_omega) c->h_flow_omega[i] = flow_omega[i];
+  }
+  c->list_dirty = true;
+  return upload_tables(c);
+}
+
+int clb_set_solute_dynamics(clb_ctx* c, const int* kind) {
+  if (!c || !kind) return CLB_ERR_ARG;
+  for (int i = 0; i < 256; ++i)
+    if (kind[i] < CLB_KIND_NONE || kind[i] > CLB_KIND_FLUID) return fail(c, CLB_ERR_ARG, "bad dynamics kind");
+  if (c->S > 0)
+    for (int s = 0; s < c->S; ++s)
+      for (int i = 0; i < 256; ++i)
+        if (kind[i] == CLB_KIND_FLUID && !(c->ade_omega[(size_t)s * 256 + i] > 0.0 && c->ade_omega[(size_t)s * 256 + i] < 2.0))
+          return fail(c, CLB_ERR_ARG, "solute omega outside (0, 2) for a code this table makes fluid");
+  {
+    const int rc = flush_pending(c);  // routed by the old table
+    if (rc) return rc;
+  }
+  for (int i = 0; i < 256; ++i) c->h_ade_kind[i] = kind[i];
+  c->ade_kind_set = true;
   c->list_dirty = true;
   return upload_tables(c);
 }
@@ -973,7 +1022,7 @@ int clb_set_solutes(clb_ctx* c, int S, const double* omega, const int* bc_type, 
   }
   for (int s = 0; s < S; ++s)
     for (int i = 0; i < 256; ++i)
-      if (c->h_kind[i] == CLB_KIND_FLUID && !(omega[s * 256 + i] > 0.0 && omega[s * 256 + i] < 2.0))
+      if (c->h_ade_kind[i] == CLB_KIND_FLUID && !(omega[s * 256 + i] > 0.0 && omega[s * 256 + i] < 2.0))
         return fail(c, CLB_ERR_ARG, "solute omega outside (0, 2)");
   for (int i = 0; i < 2 * S; ++i)
     if (bc_type[i] != CLB_BC_DIRICHLET && bc_type[i] != CLB_BC_NEUMANN) return fail(c, CLB_ERR_ARG, "bad bc type");
@@ -1785,6 +1834,9 @@ int clb_get_perf(clb_ctx* c, clb_perf* p) {
     if (rc) return rc;
   }
   *p = c->perf;
+  unsigned long long st[4];
+  CLB_CHECK_HIP(c, hipMemcpy(st, c->d_stats, sizeof(st), hipMemcpyDeviceToHost));
+  p->stragglers = (long long)st[3];
   return CLB_OK;
 }
 

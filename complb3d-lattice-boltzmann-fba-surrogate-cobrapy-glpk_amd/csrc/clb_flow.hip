@@ -130,20 +130,24 @@ __global__ void __launch_bounds__(256) flow_step_kernel(FlowArgs<R> a) {
     }
   }
   if (ENERGY) {
-    __shared__ double se[256], sc[256];
-    const int t = threadIdx.x + blockDim.x * threadIdx.y;
-    const int nt = blockDim.x * blockDim.y;
-    se[t] = e; sc[t] = cnt;
-    __syncthreads();
-    for (int s = nt / 2; s > 0; s >>= 1) {
-      if (t < s) { se[t] += se[t + s]; sc[t] += sc[t + s]; }
-      __syncthreads();
+    // within each wavefront (xor butterfly), then the wavefronts' partials in a fixed
+    // order: right for any block of whole wavefronts (64 x 3 as well as 64 x 4)
+    for (int off = 32; off > 0; off >>= 1) {
+      e += __shfl_xor(e, off);
+      cnt += __shfl_xor(cnt, off);
     }
+    __shared__ double se[4], sc[4];
+    const int t = threadIdx.x + blockDim.x * threadIdx.y;
+    const int nw = (blockDim.x * blockDim.y) >> 6;
+    if ((t & 63) == 0) { se[t >> 6] = e; sc[t >> 6] = cnt; }
+    __syncthreads();
     if (t == 0) {
+      double es = 0.0, cs = 0.0;
+      for (int w = 0; w < nw; ++w) { es += se[w]; cs += sc[w]; }
       // one partial per block (a deterministic sum, and no atomics on one address)
       const long long b = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-      a.energy[2 + 2 * b] = se[0];
-      a.energy[3 + 2 * b] = sc[0];
+      a.energy[2 + 2 * b] = es;
+      a.energy[3 + 2 * b] = cs;
     }
   }
 }

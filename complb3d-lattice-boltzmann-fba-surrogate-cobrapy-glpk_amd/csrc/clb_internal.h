@@ -44,7 +44,8 @@ template <typename R>
 struct AdeArgs {
   R* g;                   // S*7*N population-major; stored as f_i - t_i*c_ref[s]
   uint8_t* mask;          // updated in place by the biomass threshold rule
-  const uint8_t* kind;    // 256
+  const uint8_t* kind;    // 256: the solute lattices' dynamics per mask code (soluteDomainSetup)
+  const uint8_t* flow_kind;  // 256: the flow lattice's (NSdomainSetup); node conversion only
   const R* omega;         // S*256
   const R* u;             // 3*N (x, y, z planes)
   R* bmass;               // nb*N

@@ -802,6 +802,8 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
     atomicAdd(&sa.stats[0], my_solves);
     atomicAdd(&sa.stats[1], my_conv);
     atomicAdd(&sa.stats[2], my_iters);
+    // voxels continued from the fused step's trial points: its stragglers
+    if (MODE == 1 && sa.logc) atomicAdd(&sa.stats[3], my_solves);
   }
 }
 

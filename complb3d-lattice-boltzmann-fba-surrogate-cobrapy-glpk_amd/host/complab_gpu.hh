@@ -97,6 +97,11 @@ class Lattice {
   void setDynamics(const std::array<int, 256>& kind, const std::array<double, 256>& flowOmega) {
     chk(clb_set_dynamics(c_, kind.data(), flowOmega.data()), "clb_set_dynamics");
   }
+  // soluteDomainSetup's own map: every biofilm code is advection-diffusion for the solutes,
+  // whether or not the flow sees it as bounce-back
+  void setSoluteDynamics(const std::array<int, 256>& kind) {
+    chk(clb_set_solute_dynamics(c_, kind.data()), "clb_set_solute_dynamics");
+  }
 
   // D3Q19 flow
   void flowInit(double rhoIn, double rhoOut) { chk(clb_flow_init(c_, rhoIn, rhoOut), "clb_flow_init"); }

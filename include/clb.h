@@ -75,6 +75,7 @@ typedef struct clb_perf {
   long long reactive_nodes;
   double srg_ms;         /* surrogate metabolic processor kernels */
   long long srg_launches;
+  long long stragglers;  /* voxels the fused speciation handed to the persistent kernel */
 } clb_perf;
 
 /* ---- context (replaces the MultiBlockLattice3D objects main owns, src/complab.cpp:592, 748-775) */
@@ -88,6 +89,15 @@ int clb_device_name(char* buf, int len);
 int clb_upload_mask(clb_ctx* ctx, const uint8_t* mask);          /* N codes */
 int clb_download_mask(clb_ctx* ctx, uint8_t* mask);
 int clb_set_dynamics(clb_ctx* ctx, const int* kind /*256*/, const double* flow_omega /*256*/);
+/* The solute lattices' own dynamics per mask code (soluteDomainSetup,
+ * src/complab_functions.hh:296-340): there the bounce-back code is BounceBack, the
+ * no-dynamics code NoDynamics, and pore and every biofilm code AdvectionDiffusionBGK --
+ * also a biofilm that is impermeable to the flow (permRatio = 0 makes it BounceBack only on
+ * the flow lattice, :271-283). Until this is called the solute lattices follow
+ * clb_set_dynamics' table; afterwards each lattice keeps its own. Transport, kinetics,
+ * speciation, dissolution and the surrogate processor act on the voxels this table makes
+ * CLB_KIND_FLUID. */
+int clb_set_solute_dynamics(clb_ctx* ctx, const int* kind /*256*/);
 
 /* ---- D3Q19 flow (NSdomainSetup + collideAndStream loops, src/complab_functions.hh:245-294,
  *      src/complab.cpp:597-667). Density rho_in on x = 0, rho_out on x = nx-1. */

@@ -621,7 +621,8 @@ extern "C" void orc_srg_voxel(const orc_srg* P, int S, int nb, const double* con
 
 struct orc_ade_params {
   int nx, ny, nz, S;       // grid and number of solutes
-  const int* kind;         // [256] dynamics kind per mask code
+  const int* kind;         // [256] the solute lattices' dynamics kind per mask code (soluteDomainSetup:
+                           //       every biofilm code is advection-diffusion, whatever the flow sees)
   const double* omega;     // [S*256] relaxation per solute and mask code
   const int* bc_type;      // [2*S] west/east per solute: 0 Dirichlet, 1 Neumann
   const double* bc_value;  // [2*S] Dirichlet concentration (Neumann: unused)

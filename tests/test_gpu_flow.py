@@ -94,3 +94,21 @@ def test_bounce_back_nodes_untouched(clb, pack):
     assert np.all(u[pack != 2] == 0)
     assert np.abs(u[pack == 2, 0]).max() > 0
     lat.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("block", [(64, 3), (192, 1), (64, 1)])
+def test_energy_any_block_shape(clb, pack, oracle_flow, block):
+    """The device-reduced energy (the ValueTracer's input) is the same for block shapes whose
+    thread count is not a power of two: the per-block sum covers every wavefront."""
+    print(f"case: flow fp64 energy with block {block}", flush=True)
+    lat = clb.Lattice(NX, NY, NZ, precision=clb.PREC_FP64)
+    kind, om = d3q19_kind_omega(0.8)
+    lat.set_block_shape(clb.KERNEL_FLOW, *block)
+    lat.set_dynamics(kind, om)
+    lat.upload_mask(pack)
+    lat.flow_init(RHO_IN, RHO_OUT)
+    e = lat.flow_steps(301, energy=True)
+    lat.close()
+    _, eo = oracle_flow[301]
+    assert np.max(np.abs(e - eo) / eo) < 1e-9

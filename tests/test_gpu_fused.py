@@ -84,3 +84,30 @@ def test_launch_settings_rejected(clb):
     lat.set_block_shape(clb.KERNEL_FLOW, 128, 2)
     lat.set_fused_budget(5)
     lat.close()
+
+
+@pytest.mark.gpu
+def test_statistics_survive_a_block_shape_change(clb, oracle):
+    """Fused solves counted before a block-shape change stay in clb_speciation_stats."""
+    print("case: speciation statistics across a fused block-shape change", flush=True)
+    nx, ny, nz = 32, 16, 16
+    pack = clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.5, seed=0)
+    kind, om = d3q19_kind_omega(0.8)
+    c_in, c_0 = carbonate_exact(2e-3, 4e-4), carbonate_exact(1e-3, 1e-4)
+    lat = clb.Lattice(nx, ny, nz, precision=0)
+    lat.set_dynamics(kind, om)
+    lat.upload_mask(pack)
+    lat.set_solutes(np.full((4, 256), 1.0 / 0.8), [0, 1] * 4, [c_in[0], 0, c_in[1], 0, c_in[2], 0, c_in[3], 0])
+    lat.set_velocity(np.zeros(3 * pack.size))
+    lat.set_tableau(CARB_S, CARB_LOGK, CARB_COMP, tol=1e-10, max_iter=200)
+    lat.solutes_init(c_0)
+    lat.step(3)
+    s1 = lat.speciation_stats()
+    lat.set_block_shape(clb.KERNEL_FUSED, 64, 4)   # the default is 64 x 8: new wavefront count
+    lat.step(2)
+    s2 = lat.speciation_stats()
+    lat.close()
+    x = np.arange(pack.size) % nx
+    react = int(((pack == 2) & (x >= 1) & (x <= nx - 2)).sum())
+    assert s1[0] == s1[1] == 3 * react
+    assert s2[0] == s2[1] == 5 * react
