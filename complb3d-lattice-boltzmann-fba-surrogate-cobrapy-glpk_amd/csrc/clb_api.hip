@@ -20,6 +20,11 @@
 
 using namespace clb;
 
+// Fixed device scratch of clb_batch_solve_noalloc: [0] its work-queue head. Every GPU holds
+// its own copy until the process ends; host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (clb_create keeps it per context).
+static __device__ __align__(256) unsigned int scratch[64];
+
 struct EventPool {
   std::vector<hipEvent_t> ev;  // start/stop pairs
   size_t used = 0;
@@ -32,6 +37,7 @@ struct clb_ctx {
   hipStream_t stream = nullptr;
   std::string err;
 
+  unsigned int* d_scratch = nullptr;  // this device's `scratch` (clb_batch_solve_noalloc)
   uint8_t* d_mask = nullptr;
   uint8_t* d_kind = nullptr;          // flow lattice dynamics per mask code (NSdomainSetup)
   int h_kind[256] = {0};
@@ -848,6 +854,11 @@ int clb_create(const clb_config* cfg, clb_ctx** out) {
   }
   const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
   int rc = CLB_OK;
+  if (hipGetSymbolAddress((void**)&c->d_scratch, HIP_SYMBOL(scratch)) != hipSuccess || !c->d_scratch) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CLB_ERR_HIP;
+  }
   if (!rc) rc = dalloc(c, &c->d_mask, (size_t)N);
   if (!rc) rc = dalloc(c, &c->d_kind, 256);
   if (!rc) rc = dalloc(c, &c->d_ade_kind, 256);
@@ -1506,6 +1517,46 @@ int clb_batch_solve(clb_ctx* c, int warm, const double* logc_in) {
     launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->stream);
   }
   count_launch(c, 2);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int clb_batch_solve_noalloc(clb_ctx* c, long n, const double* d_totals, int nvar, const int* var_species,
+                            const double* d_logk_var, int warm, double* d_logc, int* d_iters, double* d_resid,
+                            uint8_t* d_conv) {
+  if (!c) return CLB_ERR_ARG;
+  if (n < 1 || n > CLB_NOALLOC_MAX_SOLVES)
+    return fail(c, CLB_ERR_ARG, "clb_batch_solve_noalloc: n must be in [1, " +
+                                    std::to_string(CLB_NOALLOC_MAX_SOLVES) + "] (the work-queue head is 32 bits)");
+  if (!d_totals || !d_logc || nvar < 0 || nvar > 4 || (nvar > 0 && (!var_species || !d_logk_var)))
+    return fail(c, CLB_ERR_ARG, "clb_batch_solve_noalloc: null buffer or bad nvar");
+  if (!c->have_tableau) return fail(c, CLB_ERR_STATE, "clb_set_tableau first");
+  if (!c->d_scratch) return fail(c, CLB_ERR_STATE, "no device scratch");
+  for (int v = 0; v < nvar; ++v)
+    if (var_species[v] < 0 || var_species[v] >= c->tb.ns) return fail(c, CLB_ERR_ARG, "bad var_species");
+  SpecArgs sa{};
+  sa.n = n;
+  sa.totals = d_totals;
+  sa.logk_var = d_logk_var;
+  sa.nvar = nvar;
+  for (int v = 0; v < 4; ++v) sa.var_species[v] = (v < nvar) ? var_species[v] : 0;
+  sa.logc = d_logc;
+  sa.iters = d_iters;
+  sa.resid = d_resid;
+  sa.conv = d_conv;
+  sa.counter = c->d_scratch;  // the queue head: scratch[0] of this device
+  sa.stats = nullptr;
+  if (warm == CLB_WARM_MLP) {
+    if (!c->d_theta || !c->mlp_scaled) return fail(c, CLB_ERR_STATE, "MLP not configured / scaled");
+    if (c->md.din != c->tb.nc + nvar || c->md.dout != c->tb.nc || c->md.H != 128)
+      return fail(c, CLB_ERR_UNSUP, "MLP shape does not match the batch (din = nc + nvar, dout = nc, H = 128)");
+    launch_mlp_warm(c->md, n, c->d_theta, c->d_box, c->d_box + 32, c->d_box + 64, c->d_box + 96, c->tb, sa,
+                    c->stream);
+  } else if (warm != CLB_WARM_REF && warm != CLB_WARM_USER) {
+    return fail(c, CLB_ERR_ARG, "bad warm-start mode");
+  }
+  launch_fill_u32(c->d_scratch, 1, 0u, c->stream);  // a fill kernel, no memset node
+  launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->stream);
   CLB_CHECK_HIP(c, hipGetLastError());
   return CLB_OK;
 }

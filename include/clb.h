@@ -317,6 +317,21 @@ int clb_batch_load(clb_ctx* ctx, long n, const double* totals, int nvar, const i
                    const double* logk_var);
 int clb_batch_solve(clb_ctx* ctx, int warm, const double* logc_in);
 int clb_batch_fetch(clb_ctx* ctx, double* logc, int* iters, double* resid, uint8_t* conv);
+/* The batched solve on caller-owned device buffers, allocating nothing. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Same kernels and
+ * results as clb_batch_load + clb_batch_solve: d_totals[n*nc] and (nvar > 0)
+ * d_logk_var[n*nvar] in device memory; d_logc[n*nc] out (the start for CLB_WARM_USER);
+ * d_iters[n], d_resid[n], d_conv[n] optional device outputs (NULL: not written). Needs
+ * clb_set_tableau (and for CLB_WARM_MLP a trained, scaled network), set up beforehand.
+ * Enqueued on the context's stream, no host synchronisation. Its work-queue head lives in a
+ * fixed 256-byte library scratch per GPU that every call on that GPU shares: calls that
+ * overlap on one GPU race. n <= CLB_NOALLOC_MAX_SOLVES (the queue head is a 32-bit
+ * counter that runs past n by up to the grid's lane count). */
+#define CLB_NOALLOC_MAX_SOLVES 1073741824L
+int clb_batch_solve_noalloc(clb_ctx* ctx, long n, const double* d_totals, int nvar, const int* var_species,
+                            const double* d_logk_var, int warm, double* d_logc, int* d_iters, double* d_resid,
+                            uint8_t* d_conv);
 
 /* ---- MLP warm start: din -> H -> H -> dout, tanh, tanh, linear; inputs and outputs
  * min-max scaled to [-1, 1] (surrogate_training/trainSurrogate.py:182-201 convention).
