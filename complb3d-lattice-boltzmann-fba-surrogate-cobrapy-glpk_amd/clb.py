@@ -27,6 +27,7 @@ WARM_REF, WARM_USER, WARM_MLP = 0, 1, 2
 SPEC_FUSED, SPEC_SEPARATE = 0, 1
 CA_DONE, CA_STALLED, CA_SWEEP_CAP, CA_AGE_CAP, CA_NONFINITE = 0, 1, 2, 3, 4
 KERNEL_FLOW, KERNEL_ADE, KERNEL_FUSED = 0, 1, 2
+LAYOUT_COMPACT, LAYOUT_DENSE = 0, 1
 
 
 class ClbError(RuntimeError):
@@ -117,6 +118,7 @@ def load():
         "clb_node_conversion": [vp, c_int, c_int, c_int, c_int, c_double, c_double, vp],
         "clb_set_speciation_mode": [vp, c_int],
         "clb_set_fused_budget": [vp, c_int],
+        "clb_set_layout": [vp, c_int],
         "clb_set_block_shape": [vp, c_int, c_int, c_int],
         "clb_autotune": [vp, c_int, vp],
         "clb_ca_init": [vp],
@@ -559,6 +561,9 @@ class Lattice:
 
     def set_speciation_mode(self, mode):
         self._chk(self.lib.clb_set_speciation_mode(self.h, int(mode)), "set_speciation_mode")
+
+    def set_layout(self, layout):
+        self._chk(self.lib.clb_set_layout(self.h, layout), "set_layout")
 
     def set_fused_budget(self, evals):
         self._chk(self.lib.clb_set_fused_budget(self.h, int(evals)), "set_fused_budget")

@@ -33,7 +33,7 @@ __global__ void __launch_bounds__(256) ade_step_kernel(AdeArgs<R> a) {
     load7<R, S, ODD, false>(a, nb, n, p);
 #pragma unroll
     for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&p[s * 7]) + a.c_ref[s];
-    ade_node<R, S, KIN, CBUF>(a, x, y, z, n, code, p, nullptr);
+    ade_node<R, S, KIN, CBUF>(a, x, y, z, n, code, p, nullptr, n, g.N);
     store7<R, S, ODD, false>(a, nb, n, p);
   }
   if (x == 1 || x == g.nx - 2) {
@@ -49,8 +49,8 @@ __global__ void __launch_bounds__(256) ade_step_kernel(AdeArgs<R> a) {
 #pragma unroll
         for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&q[s * 7]) + a.c_ref[s];
       }
-      if (xb == 0) ade_node<R, S, false, false, 1>(a, xb, y, z, nbn, cb, q, rho_in);
-      else ade_node<R, S, false, false, 2>(a, xb, y, z, nbn, cb, q, rho_in);
+      if (xb == 0) ade_node<R, S, false, false, 1>(a, xb, y, z, nbn, cb, q, rho_in, nbn, g.N);
+      else ade_node<R, S, false, false, 2>(a, xb, y, z, nbn, cb, q, rho_in, nbn, g.N);
       store7<R, S, ODD>(a, nb, nbn, q);
     }
   }

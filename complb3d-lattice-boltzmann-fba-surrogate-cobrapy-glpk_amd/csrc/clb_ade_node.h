@@ -216,14 +216,16 @@ __device__ __forceinline__ void complete_face(const AdeArgs<R>& a, R* p, const R
 // post-kinetics concentration output and biomass threshold rule for one fluid node.
 // FACE: 0 interior node (1 <= x <= nx-2), 1 the inlet node x = 0, 2 the outlet node
 // x = nx-1 (known at compile time, so every population index is a constant). rho_bc[s] is
-// the Neumann density (interior neighbour's pre-collision density) for face nodes.
+// the Neumann density (interior neighbour's pre-collision density) for face nodes. The
+// advecting velocity is a.u[ui + k us], k = 0, 1, 2 (dense: ui = n, us = N; compact layout:
+// the node's slot and the capacity).
 template <typename R, int S, bool KIN, bool CBUF, int FACE = 0>
 __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int y, int z, int n, uint8_t code, R* p,
-                                         const R* rho_bc) {
+                                         const R* rho_bc, long long ui, long long us) {
   const Geom& g = a.geo;
   if (FACE == 1) complete_face<R, S, 1>(a, p, rho_bc);
   if (FACE == 2) complete_face<R, S, 2>(a, p, rho_bc);
-  const R ux = a.u[n], uy = a.u[g.N + n], uz = a.u[2 * g.N + n];
+  const R ux = a.u[ui], uy = a.u[us + ui], uz = a.u[2 * us + ui];
 #pragma unroll
   for (int s = 0; s < S; ++s) {
     if (a.immobile[s]) continue;

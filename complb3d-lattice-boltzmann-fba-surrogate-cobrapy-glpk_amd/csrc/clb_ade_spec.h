@@ -14,6 +14,7 @@
 #pragma once
 
 #include "clb_ade_node.h"
+#include "clb_compact.h"
 #include "clb_internal.h"
 #include "clb_spec_kernel.h"
 
@@ -39,6 +40,69 @@ namespace {
 // choice.
 template <typename R, int S, int NC>
 constexpr int fused_waves_per_eu() { return (sizeof(R) == 4 && S <= 6 && NC <= 3) ? 4 : 1; }
+
+// Phase 2 of the fused step, shared by the dense and compact kernels: the first `count`
+// threads of the block each solve one slot (post-kinetics concentrations sc, node index
+// snode), write the limited increments to pend_w (stride: N dense, ccap compact), or, for a
+// voxel that has not met the judge within io.evals evaluations, zeros there, its
+// concentrations to cbuf and its next trial point to io.sx, and append it to the straggler
+// list; then the per-wavefront statistics of block bid.
+template <typename R, int S, int NC, int BT>
+__device__ __forceinline__ void fused_solve_phase(const AdeArgs<R>& a, const Tableau& tb, const FusedSpecIO& io,
+                                                  int lane, int count, double (&sc)[S][BT], double (&scr)[S][BT],
+                                                  const int (&snode)[BT], long long stride, long long bid) {
+  constexpr int NW = BT / 64;
+  const int wlane = lane & 63, wave = lane >> 6;
+  bool solved = false, straggler = false;
+  int evals = 0, node = 0;
+  double xs[NC];
+#pragma unroll
+  for (int j = 0; j < NC; ++j) xs[j] = 0.0;
+  if (lane < count) {
+    double c[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) c[s] = sc[s][lane];
+    node = snode[lane];
+    solved = spec_fused<NC, S, BT>(tb, c, xs, evals, io.evals, &scr[0][lane]);
+    straggler = !solved;
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      double d = 0.0;
+      if (solved) {
+        // the judged evaluation that converged was at xs: its species concentrations are
+        // still in scr (clamping them equals calc_species' clamp of log c to [-30, 1])
+        d = spec_increment_eq(tb.is_eq[s], scr[s][lane], sc[s][lane]);
+        if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
+      } else {
+        a.cbuf[(long long)s * stride + node] = sc[s][lane];
+      }
+      a.pend_w[(long long)s * stride + node] = (R)d;
+    }
+  }
+  const unsigned long long sb = __ballot(straggler);
+  if (sb) {
+    unsigned int base = 0;
+    if (wlane == 0) base = atomicAdd(io.scount, (unsigned int)__popcll(sb));
+    base = __shfl(base, 0);
+    if (straggler) {
+      const unsigned int k = base + __popcll(sb & ((1ull << wlane) - 1ull));
+      io.slist[k] = node;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) io.sx[(long long)k * 4 + j] = (j < NC) ? xs[j] : 0.0;
+    }
+  }
+  // ---- statistics: per-wavefront accumulators (solves, evaluations), no atomics
+  if (lane < ((count + 63) & ~63)) {
+    unsigned long long it = solved ? (unsigned long long)evals : 0ull;
+    for (int off = 32; off > 0; off >>= 1) it += __shfl_xor(it, off);
+    const unsigned long long nsol = __popcll(__ballot(solved));
+    if (wlane == 0 && nsol) {
+      unsigned long long* st = io.blk_stats + 2 * (bid * NW + wave);
+      st[0] += nsol;
+      st[1] += it;
+    }
+  }
+}
 
 template <typename R, int S, bool ODD, int NC, int BT>
 __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(fused_waves_per_eu<R, S, NC>(), 8)))
@@ -86,7 +150,7 @@ ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
     add_pending<R, S, false>(a, nb, n, p);
 #pragma unroll
     for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&p[s * 7]) + a.c_ref[s];
-    ade_node<R, S, true, false>(a, x, y, z, n, code, p, nullptr);
+    ade_node<R, S, true, false>(a, x, y, z, n, code, p, nullptr, n, g.N);
     if (react) {
 #pragma unroll
       for (int s = 0; s < S; ++s) sc[s][slot] = fmax((double)pop_sum(&p[s * 7]) + (double)a.c_ref[s], kMinConc);
@@ -108,63 +172,119 @@ ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
 #pragma unroll
         for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&q[s * 7]) + a.c_ref[s];
       }
-      if (xb == 0) ade_node<R, S, false, false, 1>(a, xb, y, z, nbn, cb, q, rho_in);
-      else ade_node<R, S, false, false, 2>(a, xb, y, z, nbn, cb, q, rho_in);
+      if (xb == 0) ade_node<R, S, false, false, 1>(a, xb, y, z, nbn, cb, q, rho_in, nbn, g.N);
+      else ade_node<R, S, false, false, 2>(a, xb, y, z, nbn, cb, q, rho_in, nbn, g.N);
       store7<R, S, ODD>(a, nb, nbn, q);
     }
   }
   __syncthreads();
+  fused_solve_phase<R, S, NC, BT>(a, tb, io, lane, count, sc, scr, snode, g.N,
+                                  blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z));
+}
 
-  // ---- 2. solve the block's reactive voxels on its first `count` threads
-  bool solved = false, straggler = false;
-  int evals = 0, node = 0;
-  double xs[NC];
+// ade_spec_step_kernel over the compact layout (clb_compact.h): one thread per fluid node
+// of a 1-D block; the same two phases. A face node whose interior neighbour is fluid is
+// updated by that neighbour's thread; one next to a wall updates itself with its own density.
+template <typename R, int S, bool ODD, int NC, int BT>
+__global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(fused_waves_per_eu<R, S, NC>(), 8)))
+ade_spec_step_compact_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
+  constexpr int NW = BT / 64;
+  __shared__ double sc[S][BT];
+  __shared__ double scr[S][BT];
+  __shared__ int snode[BT];
+  __shared__ unsigned int woff[NW + 1];
+  const Geom& g = a.geo;
+  const int lane = threadIdx.x;
+  const int wlane = lane & 63, wave = lane >> 6;
+  const int nwaves = (blockDim.x + 63) >> 6;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = j < (int)*a.cnf;
+  CPos q{0, 0, 0, 0};
+  uint8_t code = (uint8_t)a.solid_code;
+  if (valid) {
+    q = cpos(a, j);
+    code = a.mask[q.n];
+  }
+  const bool face = valid && (q.x == 0 || q.x == g.nx - 1);
+  const bool react = valid && !face && code != a.solid_code && code != a.bb_code;
+  const unsigned long long rb = __ballot(react);
+  if (wlane == 0) woff[wave] = (unsigned int)__popcll(rb);
+  __syncthreads();
+  if (lane == 0) {
+    unsigned int acc = 0;
+    for (int w = 0; w < nwaves; ++w) {
+      const unsigned int c = woff[w];
+      woff[w] = acc;
+      acc += c;
+    }
+    woff[NW] = acc;
+  }
+  __syncthreads();
+  const int slot = (int)woff[wave] + __popcll(rb & ((1ull << wlane) - 1ull));
+  const int count = (int)woff[NW];
+
+  // ---- 1. transport
+  if (valid) {
+    const CSlots L = cslots(a, j);
+    R rho_in[S];
+    if (face) {
+      const int inner = L.s[q.x == 0 ? 1 : 2];
+      if (!(inner >= 0 && inner < a.ccap)) {
+        R f[S * 7];
+        cload7<R, S, ODD>(a, L, f);
+        cadd_pending<R, S>(a, L, f);
 #pragma unroll
-  for (int j = 0; j < NC; ++j) xs[j] = 0.0;
-  if (lane < count) {
-    double c[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) c[s] = sc[s][lane];
-    node = snode[lane];
-    solved = spec_fused<NC, S, BT>(tb, c, xs, evals, io.evals, &scr[0][lane]);
-    straggler = !solved;
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      double d = 0.0;
-      if (solved) {
-        // the judged evaluation that converged was at xs: its species concentrations are
-        // still in scr (clamping them equals calc_species' clamp of log c to [-30, 1])
-        d = spec_increment_eq(tb.is_eq[s], scr[s][lane], sc[s][lane]);
-        if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
-      } else {
-        a.cbuf[(long long)s * g.N + node] = sc[s][lane];
+        for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&f[s * 7]) + a.c_ref[s];
+        if (q.x == 0) ade_node<R, S, false, false, 1>(a, q.x, q.y, q.z, q.n, code, f, rho_in, j, a.ccap);
+        else ade_node<R, S, false, false, 2>(a, q.x, q.y, q.z, q.n, code, f, rho_in, j, a.ccap);
+        cstore7<R, S, ODD>(a, L, f);
       }
-      a.pend_w[(long long)s * g.N + node] = (R)d;
-    }
-  }
-  const unsigned long long sb = __ballot(straggler);
-  if (sb) {
-    unsigned int base = 0;
-    if (wlane == 0) base = atomicAdd(io.scount, (unsigned int)__popcll(sb));
-    base = __shfl(base, 0);
-    if (straggler) {
-      const unsigned int k = base + __popcll(sb & ((1ull << wlane) - 1ull));
-      io.slist[k] = node;
+    } else {
+      {
+        R p[S * 7];
+        cload7<R, S, ODD, false>(a, L, p);
+        cadd_pending<R, S, false>(a, L, p);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) io.sx[(long long)k * 4 + j] = (j < NC) ? xs[j] : 0.0;
+        for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&p[s * 7]) + a.c_ref[s];
+        ade_node<R, S, true, false>(a, q.x, q.y, q.z, q.n, code, p, nullptr, j, a.ccap);
+        if (react) {
+#pragma unroll
+          for (int s = 0; s < S; ++s) sc[s][slot] = fmax((double)pop_sum(&p[s * 7]) + (double)a.c_ref[s], kMinConc);
+          snode[slot] = j;
+        }
+        cstore7<R, S, ODD, false>(a, L, p);
+      }
+      if (q.x == 1 || q.x == g.nx - 2) {
+        const int fj = L.s[q.x == 1 ? 2 : 1];
+        if (fj >= 0 && fj < a.ccap) {
+          const CSlots F = cslots(a, fj);
+          const int xb = (q.x == 1) ? 0 : g.nx - 1;
+          const int nb = q.n + xb - q.x;
+          const uint8_t cb = a.mask[nb];
+          R f[S * 7];
+          cload7<R, S, ODD>(a, F, f);
+          cadd_pending<R, S>(a, F, f);
+          if (xb == 0) ade_node<R, S, false, false, 1>(a, xb, q.y, q.z, nb, cb, f, rho_in, fj, a.ccap);
+          else ade_node<R, S, false, false, 2>(a, xb, q.y, q.z, nb, cb, f, rho_in, fj, a.ccap);
+          cstore7<R, S, ODD>(a, F, f);
+        }
+      }
     }
   }
-  // ---- statistics: per-wavefront accumulators (solves, evaluations), no atomics
-  if (lane < ((count + 63) & ~63)) {
-    unsigned long long it = solved ? (unsigned long long)evals : 0ull;
-    for (int off = 32; off > 0; off >>= 1) it += __shfl_xor(it, off);
-    const unsigned long long nsol = __popcll(__ballot(solved));
-    if (wlane == 0 && nsol) {
-      const long long bid = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
-      unsigned long long* st = io.blk_stats + 2 * (bid * NW + wave);
-      st[0] += nsol;
-      st[1] += it;
-    }
+  __syncthreads();
+  fused_solve_phase<R, S, NC, BT>(a, tb, io, lane, count, sc, scr, snode, a.ccap, blockIdx.x);
+}
+
+template <typename R, int S, int NC>
+void launch_fused_shape_compact(const AdeArgs<R>& a, bool odd, int bt, const Tableau& tb, const FusedSpecIO& io,
+                                hipStream_t st) {
+  const unsigned blocks = (unsigned)((a.ccap + bt - 1) / bt);
+  if (bt <= 256) {
+    if (odd) hipLaunchKernelGGL((ade_spec_step_compact_kernel<R, S, true, NC, 256>), dim3(blocks), dim3(bt), 0, st, a, tb, io);
+    else hipLaunchKernelGGL((ade_spec_step_compact_kernel<R, S, false, NC, 256>), dim3(blocks), dim3(bt), 0, st, a, tb, io);
+  } else {
+    if (odd) hipLaunchKernelGGL((ade_spec_step_compact_kernel<R, S, true, NC, 512>), dim3(blocks), dim3(bt), 0, st, a, tb, io);
+    else hipLaunchKernelGGL((ade_spec_step_compact_kernel<R, S, false, NC, 512>), dim3(blocks), dim3(bt), 0, st, a, tb, io);
   }
 }
 
@@ -199,6 +319,30 @@ bool dispatch_fused(const AdeArgs<R>& a, bool odd, dim3 block, const Tableau& tb
   if (NC > S) return false;
 #define CLB_FUSED(S_, NC_) \
   case S_ * 10 + NC_: launch_fused_shape<R, S_, NC_>(a, odd, block, tb, io, st); return true;
+  switch (S * 10 + NC) {
+    CLB_FUSED(2, 2)
+    CLB_FUSED(3, 2) CLB_FUSED(3, 3)
+    CLB_FUSED(4, 2) CLB_FUSED(4, 3) CLB_FUSED(4, 4)
+    CLB_FUSED(5, 2) CLB_FUSED(5, 3) CLB_FUSED(5, 4)
+    CLB_FUSED(6, 2) CLB_FUSED(6, 3) CLB_FUSED(6, 4)
+    CLB_FUSED(7, 2) CLB_FUSED(7, 3) CLB_FUSED(7, 4)
+    CLB_FUSED(8, 2) CLB_FUSED(8, 3) CLB_FUSED(8, 4)
+    default: return false;
+  }
+#undef CLB_FUSED
+}
+
+// The compiled shapes of dispatch_fused, on the compact layout (bt threads per 1-D block).
+template <typename R>
+bool dispatch_fused_compact(const AdeArgs<R>& a, bool odd, int bt, const Tableau& tb, const FusedSpecIO& io,
+                            hipStream_t st) {
+  const int S = a.S, nc = tb.nc;
+  if (tb.ns != S || nc < 1 || nc > 4) return false;
+  if (bt > 512 || bt < 64 || (bt & 63) || io.evals < 2) return false;
+  const int NC = (nc <= 2) ? 2 : nc;
+  if (NC > S) return false;
+#define CLB_FUSED(S_, NC_) \
+  case S_ * 10 + NC_: launch_fused_shape_compact<R, S_, NC_>(a, odd, bt, tb, io, st); return true;
   switch (S * 10 + NC) {
     CLB_FUSED(2, 2)
     CLB_FUSED(3, 2) CLB_FUSED(3, 3)

@@ -129,6 +129,24 @@ struct clb_ctx {
   bool pend_dirty = false;
   bool fluid_dirty = false;
 
+  // compact (fluid-only) solute layout (clb_compact.hip): the step kernels run on it when the
+  // configuration allows (compact_ok); the dense arrays stay the layout every other kernel uses
+  int layout = CLB_LAYOUT_COMPACT;    // requested (clb_set_layout)
+  bool cmp_active = false;            // the solute state lives in the compact arrays
+  bool cmp_dirty = true;              // mask or solute dynamics changed since the table was built
+  bool cmp_u_dirty = true;            // the dense velocity changed since it was gathered
+  bool graph_cmp = false;             // the captured graph's layout
+  long long ccap = 0, cgcap = 0, cstride = 0;  // node capacity, ghost capacity, population stride
+  int cmp_S = 0;                      // solutes the compact arrays were sized for
+  int* d_cnode = nullptr;
+  int* d_cnbr = nullptr;
+  unsigned int* d_cnf = nullptr;
+  void* d_cg = nullptr;               // compact populations (S * 7 * cstride)
+  void* d_cu = nullptr;               // compact velocity (3 * ccap)
+  void* d_cpend = nullptr;            // compact deferred increments (2 * S * ccap)
+  CompactBuild cb{};                  // build buffers (flags, scans)
+  long long cb_N = 0;
+
   // CA biomass redistribution
   bool ca_ready = false;
   int* d_age = nullptr;               // age (wavefront) lattice, two buffers for Jacobi passes
@@ -275,6 +293,11 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   a.srg_inc = nullptr;  // set by the step that ran the surrogate kernels
   a.srg_pools = 0;
   a.geo = c->geo;
+  a.cnode = nullptr;
+  a.cnbr = nullptr;
+  a.cnf = nullptr;
+  a.ccap = 0;
+  a.cstride = 0;
   a.S = c->S;
   a.nb = c->nb;
   for (int i = 0; i < 2 * CLB_MAX_SOLUTES; ++i) {
@@ -294,6 +317,25 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   a.bthreshold = c->bthr;
   a.pore_code = c->pore_code;
   a.biofilm_code = c->biofilm_code;
+  return a;
+}
+
+// Whether the step can run on the compact layout: no surrogate processor or dissolution
+// (their kernels read the dense populations), the speciation (if any) fused into the
+// transport kernel, and a pore <-> biofilm rule that never turns a fluid voxel into a wall
+// for the solutes.
+bool compact_ok(const clb_ctx* c);
+
+template <typename R>
+AdeArgs<R> compact_args(clb_ctx* c) {
+  AdeArgs<R> a = ade_args<R>(c);
+  a.g = (R*)c->d_cg;
+  a.u = (const R*)c->d_cu;
+  a.cnode = c->d_cnode;
+  a.cnbr = c->d_cnbr;
+  a.cnf = c->d_cnf;
+  a.ccap = c->ccap;
+  a.cstride = c->cstride;
   return a;
 }
 
@@ -409,14 +451,133 @@ bool fused_spec(const clb_ctx* c) {
   return c->tb.ns == S && nc >= 1 && nc <= 4 && (nc < 2 ? 2 : nc) <= S;
 }
 
+void drop_graph(clb_ctx* c);
+
+bool compact_ok(const clb_ctx* c) {
+  if (c->layout != CLB_LAYOUT_COMPACT || !c->ade_ready || c->S < 1) return false;
+  if (c->srg_on || c->dissol_on) return false;
+  if (c->have_tableau && !fused_spec(c)) return false;
+  if (c->mask_update && c->h_ade_kind[c->pore_code & 255] != c->h_ade_kind[c->biofilm_code & 255]) return false;
+  return true;
+}
+
+// The solute state back in the dense arrays (with the table it was gathered with).
+template <typename R>
+int to_dense_t(clb_ctx* c) {
+  AdeArgs<R> ca = compact_args<R>(c);
+  launch_compact_move<R>(false, ca, (R*)c->d_g, nullptr, c->d_pend ? (R*)c->d_pend : nullptr,
+                         c->d_pend ? (R*)c->d_cpend : nullptr, c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->cmp_active = false;
+  return CLB_OK;
+}
+
+int to_dense(clb_ctx* c) {
+  if (!c->cmp_active) return CLB_OK;
+  return c->fp32 ? to_dense_t<float>(c) : to_dense_t<double>(c);
+}
+
+// Node table of the current mask and solute dynamics; buffers (re)allocated when the fluid
+// nodes or bounce-back links outgrow them (captured graphs hold the old ones then).
+template <typename R>
+int build_compact(clb_ctx* c) {
+  const long long N = c->geo.N;
+  CompactBuild& b = c->cb;
+  int rc = CLB_OK;
+  if (c->cb_N != N) {
+    rc = dalloc(c, &b.flag, (size_t)N);
+    if (!rc) rc = dalloc(c, &b.cj, (size_t)N);
+    if (!rc) rc = dalloc(c, &b.lk, (size_t)(6 * N + 1));
+    if (!rc) rc = dalloc(c, &b.cg, (size_t)(6 * N + 1));
+    const size_t tb = compact_scan_bytes(6 * N + 1);
+    if (!rc) rc = dalloc(c, (uint8_t**)&b.scan_tmp, tb);
+    if (rc) return rc;
+    b.scan_tmp_bytes = tb;
+    launch_fill_u32(b.lk, 6 * N + 1, 0u, c->stream);
+    c->cb_N = N;
+  }
+  launch_compact_flags(c->d_mask, c->d_ade_kind, c->geo, b, c->stream);
+  launch_compact_scan(b, N, c->stream);
+  unsigned int hf[2], hg[7];
+  CLB_CHECK_HIP(c, hipMemcpyAsync(&hf[0], b.flag + (N - 1), sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipMemcpyAsync(&hf[1], b.cj + (N - 1), sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  for (int d = 0; d <= 6; ++d)
+    CLB_CHECK_HIP(c, hipMemcpyAsync(&hg[d], b.cg + (long long)d * N, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                                    c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  const long long nf = (long long)hf[0] + hf[1];
+  long long maxlink = 0;
+  for (int d = 0; d < 6; ++d) maxlink = std::max<long long>(maxlink, (long long)hg[d + 1] - hg[d]);
+  if (nf > c->ccap || maxlink > c->cgcap || c->cmp_S != c->S || !c->d_cg) {
+    // headroom, so that a few node conversions do not force a reallocation; at most N, since
+    // the stragglers' concentrations share the dense cbuf (S * N) at the compact stride
+    const long long cap = std::min<long long>(N, std::max<long long>(256, ((nf + nf / 16 + 1024) + 255) / 256 * 256));
+    const long long gcap = ((maxlink + maxlink / 4 + 1024) + 63) / 64 * 64;
+    const size_t rs = sizeof(R);
+    c->ccap = cap;
+    c->cgcap = gcap;
+    c->cstride = cap + gcap;
+    rc = dalloc(c, &c->d_cnode, (size_t)cap);
+    if (!rc) rc = dalloc(c, &c->d_cnbr, (size_t)(6 * cap));
+    if (!rc && !c->d_cnf) rc = dalloc(c, &c->d_cnf, 1);
+    if (!rc) rc = dalloc(c, (uint8_t**)&c->d_cg, (size_t)c->S * 7 * (size_t)c->cstride * rs);
+    if (!rc) rc = dalloc(c, (uint8_t**)&c->d_cu, (size_t)3 * cap * rs);
+    if (!rc) rc = dalloc(c, (uint8_t**)&c->d_cpend, (size_t)2 * c->S * cap * rs);
+    if (rc) {
+      c->ccap = c->cgcap = c->cstride = 0;
+      c->cmp_S = 0;
+      return rc;
+    }
+    c->cmp_S = c->S;
+    drop_graph(c);  // captured launches hold the old buffers
+  }
+  launch_compact_table(b, c->geo, c->d_cnode, c->d_cnbr, c->ccap, c->d_cnf, c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->cmp_dirty = false;
+  return CLB_OK;
+}
+
+// The solute state in the compact arrays: rebuild the table if the geometry changed (from
+// the dense state), gather, or only refresh the velocity.
+template <typename R>
+int to_compact(clb_ctx* c) {
+  if (c->cmp_active && !c->cmp_dirty) {
+    if (c->cmp_u_dirty) {
+      AdeArgs<R> ca = compact_args<R>(c);
+      launch_compact_move<R>(true, ca, nullptr, (const R*)c->d_u, nullptr, nullptr, c->stream);
+      c->cmp_u_dirty = false;
+    }
+    return CLB_OK;
+  }
+  int rc = to_dense(c);  // before a rebuild: scattered with the table it was gathered with
+  if (rc) return rc;
+  if (c->cmp_dirty || c->cmp_S != c->S || !c->d_cg) {
+    rc = build_compact<R>(c);
+    if (rc) return rc;
+  }
+  AdeArgs<R> ca = compact_args<R>(c);
+  R* pc = (R*)c->d_cpend;
+  if (!c->d_pend)  // no deferred increments yet: the compact ones start at zero
+    launch_fill_u32((unsigned int*)pc, (long long)(2 * (size_t)c->S * (size_t)c->ccap * sizeof(R) / 4), 0u,
+                    c->stream);
+  launch_compact_move<R>(true, ca, (R*)c->d_g, (const R*)c->d_u, c->d_pend ? (R*)c->d_pend : nullptr,
+                         c->d_pend ? pc : nullptr, c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->cmp_active = true;
+  c->cmp_u_dirty = false;
+  return CLB_OK;
+}
+
 // Buffers of the fused step (allocated outside any capture).
 int prepare_fused(clb_ctx* c) {
   int rc = CLB_OK;
   if (!c->d_slist) rc = dalloc(c, &c->d_slist, (size_t)c->geo.N);
   if (!rc && !c->d_sx) rc = dalloc(c, &c->d_sx, (size_t)4 * (size_t)c->geo.N);
   if (rc) return rc;
-  const long long nw = fused_waves(c->geo, c->spec_block);
-  if (c->blk_stats_n != nw) {
+  const int bt = (int)(c->spec_block.x * c->spec_block.y);
+  const long long nw = std::max(fused_waves(c->geo, c->spec_block),
+                                c->ccap > 0 ? fused_waves_compact(c->ccap, bt) : 0ll);
+  if (c->blk_stats_n < nw) {
     // a new block shape: fold the fused solves counted so far into the totals first
     if (c->d_blk_stats && c->blk_stats_n > 0) {
       std::vector<unsigned long long> b((size_t)(2 * c->blk_stats_n));
@@ -473,12 +634,15 @@ int flush_pending_t(clb_ctx* c) {
 }
 
 int flush_pending(clb_ctx* c) {
+  const int rc = to_dense(c);  // whatever reads or rewrites the solute state works on the dense layout
+  if (rc) return rc;
   if (!c->pend_dirty || !c->d_pend || !c->ade_ready) return CLB_OK;
   return c->fp32 ? flush_pending_t<float>(c) : flush_pending_t<double>(c);
 }
 
 // Drop the deferred increments (the populations are being overwritten).
 int clear_pending(clb_ctx* c) {
+  c->cmp_active = false;  // the caller overwrites the whole solute state
   if (c->d_pend) launch_fill_u32((unsigned int*)c->d_pend, (long long)(c->pend_bytes / 4), 0u, c->stream);
   c->pend_dirty = false;
   return CLB_OK;
@@ -492,9 +656,63 @@ DissolArgs dissol_args(const clb_ctx* c) {
   return d;
 }
 
+// The pending buffer of the compact layout the step of parity `odd` reads or writes.
+template <typename R>
+R* cpend_buf(clb_ctx* c, bool odd, bool w) {
+  const bool which = w ? !odd : odd;
+  return (R*)c->d_cpend + (which ? (size_t)c->S * (size_t)c->ccap : 0);
+}
+
+// One coupled step on the compact layout (the state was gathered by prepare_step).
+template <typename R>
+void enqueue_step_compact(clb_ctx* c) {
+  const bool odd = (c->ade_t & 1) != 0;
+  AdeArgs<R> a = compact_args<R>(c);
+  if (fused_spec(c)) {
+    a.pend_r = cpend_buf<R>(c, odd, false);
+    a.pend_w = cpend_buf<R>(c, odd, true);
+    c->pend_dirty = true;
+    launch_fill_u32(c->d_counter + 2, 1, 0u, c->stream);
+    const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats, c->d_sx, c->fused_evals};
+    {
+      Timed tm(c, 1);
+      launch_ade_spec_step_compact<R>(a, odd, (int)(c->spec_block.x * c->spec_block.y), c->tb, io, c->stream);
+    }
+    count_launch(c, 1);
+    // the stragglers, in the same step; their concentrations and increments use the compact
+    // stride
+    launch_fill_u32(c->d_counter, 1, 0u, c->stream);
+    SpecArgs sa{};
+    sa.n = 0;
+    sa.n_dev = c->d_counter + 2;
+    sa.iters = c->d_slist;
+    sa.logc = c->d_sx;
+    sa.counter = c->d_counter;
+    sa.stats = c->d_stats;
+    AdeArgs<R> as = a;
+    as.geo.N = c->ccap;
+    {
+      Timed tm(c, 2);
+      launch_spec_coupled(as, odd, c->tb, sa, c->spec_grid, c->stream);
+    }
+    count_launch(c, 2);
+  } else {
+    {
+      Timed tm(c, 1);
+      launch_ade_step_compact<R>(a, odd, (int)(c->ade_block.x * c->ade_block.y), c->stream);
+    }
+    count_launch(c, 1);
+  }
+  c->ade_t++;
+}
+
 // One coupled step on the stream, no host synchronisation (so it can be captured).
 template <typename R>
 void enqueue_step(clb_ctx* c) {
+  if (c->cmp_active) {
+    enqueue_step_compact<R>(c);
+    return;
+  }
   const bool odd = (c->ade_t & 1) != 0;
   AdeArgs<R> a = ade_args<R>(c);
   const bool fused = fused_spec(c);
@@ -642,6 +860,10 @@ int prepare_step(clb_ctx* c) {
       drop_graph(c);  // captured launches hold the old buffer
     }
   }
+  {
+    const int rc = compact_ok(c) ? to_compact<R>(c) : to_dense(c);
+    if (rc) return rc;
+  }
   if (fused_spec(c)) {
     int rc = prepare_fused(c);
     if (rc) return rc;
@@ -666,6 +888,7 @@ int step_t(clb_ctx* c, int nsteps) {
   if (fused && nsteps > 0) c->pend_dirty = true;
   const bool spec_launch = c->have_tableau && (fused || c->n_reactive > 0);
   const int seg = 2 * c->graph_pairs;
+  if (c->graph_valid && c->graph_cmp != c->cmp_active) drop_graph(c);  // captured on the other layout
   if (c->use_graph && !c->timing && nsteps >= seg) {
     // Replay a captured graph of `seg` steps (it starts on an even step, as the AA
     // kernels alternate): one launch per segment instead of two or three per step.
@@ -682,6 +905,7 @@ int step_t(clb_ctx* c, int nsteps) {
       c->graph = g;
       CLB_CHECK_HIP(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
       c->graph_valid = true;
+      c->graph_cmp = c->cmp_active;
       c->ade_t = t0;  // capture recorded the steps; they run at replay
       c->perf.ade_launches -= seg;
       if (spec_launch) c->perf.spec_launches -= seg;
@@ -897,7 +1121,8 @@ int clb_destroy(clb_ctx* c) {
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
                   c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
                   c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared,
-                  c->d_caflag, c->d_water, c->d_srg, c->d_srg_inc};
+                  c->d_caflag, c->d_water, c->d_srg, c->d_srg_inc, c->d_cnode, c->d_cnbr, c->d_cnf, c->d_cg,
+                  c->d_cu, c->d_cpend, c->cb.flag, c->cb.cj, c->cb.lk, c->cb.cg, c->cb.scan_tmp};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -931,6 +1156,7 @@ int clb_upload_mask(clb_ctx* c, const uint8_t* mask) {
   for (long long n = 0; n < c->geo.N; ++n) fluid += (c->h_kind[mask[n]] == CLB_KIND_FLUID);
   c->perf.fluid_nodes = fluid;
   c->list_dirty = true;
+  c->cmp_dirty = true;
   return CLB_OK;
 }
 
@@ -960,6 +1186,7 @@ int clb_set_dynamics(clb_ctx* c, const int* kind, const double* flow_omega) {
     if (flow_omega) c->h_flow_omega[i] = flow_omega[i];
   }
   c->list_dirty = true;
+  c->cmp_dirty = true;
   return upload_tables(c);
 }
 
@@ -979,6 +1206,7 @@ int clb_set_solute_dynamics(clb_ctx* c, const int* kind) {
   for (int i = 0; i < 256; ++i) c->h_ade_kind[i] = kind[i];
   c->ade_kind_set = true;
   c->list_dirty = true;
+  c->cmp_dirty = true;
   return upload_tables(c);
 }
 
@@ -1110,6 +1338,7 @@ int clb_couple_velocity(clb_ctx* c) {
     launch_flow_velocity<double>(flow_args<double>(c, nullptr), odd_next, nullptr, nullptr, (double*)c->d_u,
                                  c->stream);
   CLB_CHECK_HIP(c, hipGetLastError());
+  c->cmp_u_dirty = true;
   return CLB_OK;
 }
 
@@ -1128,6 +1357,7 @@ int clb_set_velocity(clb_ctx* c, const double* u) {
       for (int k = 0; k < 3; ++k) p[k * N + n] = u[3 * n + k];
     CLB_CHECK_HIP(c, hipMemcpy(c->d_u, p.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice));
   }
+  c->cmp_u_dirty = true;
   return CLB_OK;
 }
 
@@ -1191,6 +1421,7 @@ int clb_set_kinetics(clb_ctx* c, int kind, double dt, int solid_code, int bb_cod
   c->solid_code = solid_code;
   c->bb_code = bb_code;
   c->list_dirty = true;
+  c->cmp_dirty = true;
   return CLB_OK;
 }
 
@@ -1244,6 +1475,7 @@ int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* l
     if (rc) return rc;
   }
   c->list_dirty = true;
+  c->cmp_dirty = true;
   return CLB_OK;
 }
 
@@ -1292,6 +1524,7 @@ int clb_set_speciation_mode(clb_ctx* c, int mode) {
   if (mode != c->spec_mode) drop_graph(c);
   c->spec_mode = mode;
   c->list_dirty = true;
+  c->cmp_dirty = true;
   return CLB_OK;
 }
 
@@ -1442,6 +1675,15 @@ int clb_set_block_shape(clb_ctx* c, int kernel, int bx, int by) {
   dim3& b = (kernel == CLB_KERNEL_FLOW) ? c->flow_block : (kernel == CLB_KERNEL_ADE) ? c->ade_block : c->spec_block;
   if ((int)b.x != bx || (int)b.y != by) drop_graph(c);  // captured launches hold the old grid
   b = dim3((unsigned)bx, (unsigned)by, 1);
+  return CLB_OK;
+}
+
+int clb_set_layout(clb_ctx* c, int layout) {
+  if (!c || (layout != CLB_LAYOUT_COMPACT && layout != CLB_LAYOUT_DENSE)) return CLB_ERR_ARG;
+  const int rc = to_dense(c);
+  if (rc) return rc;
+  if (layout != c->layout) drop_graph(c);
+  c->layout = layout;
   return CLB_OK;
 }
 
@@ -1816,6 +2058,7 @@ int clb_node_conversion(clb_ctx* c, int mineral_solute, int pore_code, int preci
     // the captured step reads the mask on the device; only the separate speciation
     // path's host-side reactive list has to be rebuilt (step_t does it)
     c->list_dirty = true;
+    c->cmp_dirty = true;
     c->fluid_dirty = true;
   }
   return CLB_OK;
@@ -1959,6 +2202,7 @@ int snap_restore(clb_ctx* c, const TuneSnapshot& s) {
   c->flow_t = s.flow_t;
   c->ade_t = s.ade_t;
   c->pend_dirty = s.pend_dirty;
+  c->cmp_active = false;  // the restored dense copy is the state
   return CLB_OK;
 }
 
@@ -2083,6 +2327,7 @@ int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
   c->perf = s.perf;
   c->timing = timing;
   c->list_dirty = true;   // the mask rule may have run during the trials
+  c->cmp_dirty = true;
   c->fluid_dirty = true;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -2255,6 +2500,7 @@ int clb_fd_biomass(clb_ctx* c, double nu, clb_ca_report* report) {
   rc = c->fp32 ? fd_biomass_t<float>(c, nu, report) : fd_biomass_t<double>(c, nu, report);
   if (report->mask_changes > 0) {
     c->list_dirty = true;
+    c->cmp_dirty = true;
     c->fluid_dirty = true;
   }
   return rc;
@@ -2287,6 +2533,7 @@ int clb_ca_redistribute(clb_ctx* c, double bmax, int half, unsigned long long se
                : ca_redistribute_t<double>(c, bmax, half, seed, report);
   if (report->mask_changes > 0) {
     c->list_dirty = true;
+    c->cmp_dirty = true;
     c->fluid_dirty = true;
   }
   return rc;
@@ -2447,6 +2694,7 @@ int clb_load_state(clb_ctx* c, const char* path) {
   c->ade_t = h.ade_t;
   c->ca_sweep = h.ca_sweep;
   c->list_dirty = true;
+  c->cmp_dirty = true;
   c->fluid_dirty = true;
   drop_graph(c);  // a captured segment starts on the parity it was captured for
   return CLB_OK;

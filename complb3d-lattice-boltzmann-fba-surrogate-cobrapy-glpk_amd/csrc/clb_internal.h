@@ -63,6 +63,17 @@ struct AdeArgs {
   const R* srg_inc;
   int srg_pools;          // bit b: biomass field b is grown by the surrogate
   Geom geo;
+  // Compact (fluid-only) layout, when cnode is set (clb_compact.hip). Then g, u, pend_r,
+  // pend_w and cbuf hold the ADE-fluid voxels only, in natural order (j = 0 .. nf-1), with
+  // stride ccap (populations: cstride), while mask and biomass stay dense (index cnode[j]).
+  // A population array (s, i) has ccap node slots, then the ghost slots of direction i: for
+  // a fluid node whose neighbour at +c_i is a wall, the wall's slot i of the dense layout
+  // (the bounce-back link's storage).
+  const int* cnode;          // [ccap] dense index of fluid node j
+  const int* cnbr;           // [6 * ccap] direction d = 1..6 at (d-1)*ccap + j: the slot at +c_d,
+                             //   a fluid node's j (< ccap), a ghost slot (>= ccap), or -1 beyond an x face
+  const unsigned int* cnf;   // number of fluid nodes (device memory, so captured steps survive a rebuild)
+  long long ccap, cstride;
   int S, nb;
   int bc_type[2 * CLB_MAX_SOLUTES];
   R bc_value[2 * CLB_MAX_SOLUTES];
@@ -148,6 +159,37 @@ void launch_spec_coupled(const AdeArgs<float>& a, bool odd_done, const Tableau& 
                          hipStream_t s);
 void launch_spec_coupled(const AdeArgs<double>& a, bool odd_done, const Tableau& tb, SpecArgs sa, int grid,
                          hipStream_t s);
+
+// Compact layout (clb_compact.hip). Build: fluid flags and bounce-back links of every voxel,
+// scans, node table. The caller sizes ccap / ghost capacity from the counts.
+struct CompactBuild {
+  unsigned int* flag;        // [N] 1: fluid
+  unsigned int* cj;          // [N] exclusive scan of flag
+  unsigned int* lk;          // [6N] 1: fluid voxel with a wall at +c_d (direction-major)
+  unsigned int* cg;          // [6N + 1] exclusive scan of lk
+  void* scan_tmp;            // scan workspace
+  size_t scan_tmp_bytes;
+};
+size_t compact_scan_bytes(long long n);  // workspace the scans need for n items
+void launch_compact_flags(const uint8_t* mask, const uint8_t* kind, const Geom& g, const CompactBuild& b,
+                          hipStream_t s);
+void launch_compact_scan(const CompactBuild& b, long long N, hipStream_t s);
+// node table and the fluid count (written to cnf)
+void launch_compact_table(const CompactBuild& b, const Geom& g, int* cnode, int* cnbr, long long ccap,
+                          unsigned int* cnf, hipStream_t s);
+// Dense <-> compact copies of the solute state, for the fluid nodes of ca's table:
+// populations with the ghost slots (to_compact: from g_dense into ca.g; else back), both
+// deferred-increment buffers (2 * S planes each, dense stride N, compact stride ccap; null:
+// none) and, towards the compact layout, the velocity (u_dense into ca.u).
+template <typename R>
+void launch_compact_move(bool to_compact, const AdeArgs<R>& ca, R* g_dense, const R* u_dense, R* pend_dense,
+                         R* pend_compact, hipStream_t s);
+template <typename R>
+void launch_ade_step_compact(const AdeArgs<R>& a, bool odd, int bt, hipStream_t s);
+template <typename R>
+bool launch_ade_spec_step_compact(const AdeArgs<R>& a, bool odd, int bt, const Tableau& tb, const FusedSpecIO& io,
+                                  hipStream_t s);
+long long fused_waves_compact(long long ccap, int bt);
 
 // Returns false when no fused shape is compiled for (S, nc): the caller then uses the
 // transport kernel + spec_kernel over the reactive list.

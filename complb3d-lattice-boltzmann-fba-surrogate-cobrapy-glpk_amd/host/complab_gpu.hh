@@ -223,6 +223,7 @@ class Lattice {
   // Execution choices with no reference counterpart (clb.h documents each).
   void setSpeciationMode(int mode) { chk(clb_set_speciation_mode(c_, mode), "clb_set_speciation_mode"); }
   void setFusedBudget(int evals) { chk(clb_set_fused_budget(c_, evals), "clb_set_fused_budget"); }
+  void setLayout(int layout) { chk(clb_set_layout(c_, layout), "clb_set_layout"); }
   void setBlockShape(int kernel, int bx, int by) { chk(clb_set_block_shape(c_, kernel, bx, by), "clb_set_block_shape"); }
 
   // Mineral dissolution (surfaceDissolutionKinetics3D) and one solute's field at rest.

@@ -140,6 +140,17 @@ int clb_speciation_stats(clb_ctx* ctx, long long* stats /*3: solves, converged, 
  * separate path runs. CLB_SPEC_SEPARATE: transport kernel, then the persistent kernel over
  * every reactive voxel. Both meet the same judge. */
 int clb_set_speciation_mode(clb_ctx* ctx, int mode);
+/* Layout of the solute lattices in the step kernels (no reference counterpart: a memory
+ * layout choice). CLB_LAYOUT_COMPACT (default): the step runs on the ADE-fluid voxels only,
+ * stored contiguously in natural order with a neighbour table (the dense layout moves the
+ * cache lines of walls too: in a pack of porosity 0.35 nearly every line holds a fluid
+ * voxel); used when neither the surrogate processor nor dissolution is on, the speciation
+ * (if any) is fused, and the pore <-> biofilm rule keeps every voxel fluid for the solutes,
+ * else the dense layout runs. The state moves between the layouts on its own; results are
+ * the same. CLB_LAYOUT_DENSE: always the dense layout. */
+#define CLB_LAYOUT_COMPACT 0
+#define CLB_LAYOUT_DENSE 1
+int clb_set_layout(clb_ctx* ctx, int layout);
 /* Evaluation budget of the fused solve, 2..64 (no reference counterpart: a tuning choice,
  * default 3). A voxel that has not met the judge within it is finished by the persistent
  * kernel in the same step, from the point its fused solve reached; a small budget keeps the
