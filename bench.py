@@ -6,9 +6,11 @@ Headline ("value"): coupled MLUPS of the full coupled case -- 384 x 192 x 192 sp
 6 D3Q7 solutes (tau 0.8) carrying dual-Monod kinetics on biomass seeded in 1 % of the pore
 voxels and per-voxel FP64 equilibrium speciation on a 3-component / 6-species tableau
 (seed 11) every step, precipitation / dissolution node conversion every 100 steps followed
-by a flow re-solve, steps replayed from captured HIP graphs. One step = collide + kinetics
-+ biomass threshold rule + speciation + stream for every solute; the node conversions and
-flow re-solves of the run are inside the timed region.
+by a flow re-solve, steps replayed from captured HIP graphs. One coupled step = collide +
+kinetics + biomass threshold rule + speciation + stream for every solute. One bench step is
+one segment: 100 coupled steps, the node conversion, and the flow re-solve of the
+reference's loop (until the driver's ValueTracer converges, the tracer kept across
+re-solves); every segment of the timed region is inside the timing.
 
 Beside it, on the same GPU: the kinetics + biomass case at 256 x 128 x 128 (4 solutes, FP32
 and FP64 stores), D3Q19 flow MLUPS, plain FP64 Newton solves/s on the 128^3 Gaussian-
@@ -49,12 +51,14 @@ BYTES_SRG_PRESENT = 28 * 4 + 6 * 4
 # with exp counted as 20, linear output)
 FLOP_SRG_NET = 2 * 2 + 2 * (2 * 10 + 3 * 100) + 4 * 10 * (20 + 4) + 2 * 10 + 4
 # HBM bytes per dispatch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE on the default
-# workloads, scaled by a copy kernel of known size (profiles/hbm_traffic_fetch_write.md);
-# mean of the odd and even steps. Reported only when the workload is the default one.
+# workloads, scaled by a copy kernel of known size; mean of the odd and even steps. Not
+# measured in this run (the counters need the profiler): the file they come from is named.
+# Reported only when the workload is the default one.
+MEASURED_TRAFFIC_SOURCE = "profiles/hbm_traffic_fetch_write.md"
 MEASURED_TRAFFIC = {
-    "full_coupled_384x192x192": (6.584e9 + 6.486e9) / 2,
+    "full_coupled_384x192x192": None,
     "flow_384x192x192": (1.619e9 + 1.540e9) / 2,
-    "kinetics_256x128x128": (1.019e9 + 1.005e9) / 2,
+    "kinetics_256x128x128": None,
 }
 
 
@@ -93,6 +97,11 @@ def full_case_inputs(clb, nx, ny, nz):
 
 
 def full_coupled(clb, args):
+    """The headline workload. One bench step is one segment of `interval` coupled steps,
+    then the precipitation / dissolution node conversion and, when the mask changed, the
+    flow re-solve of the reference's loop (src/complab.cpp:1782-1845): collideAndStream
+    until the driver's tracer (ValueTracer(1, 1000, ns_converge_iT2), kept across re-solves)
+    has converged, at most ns_max_iT2 steps, then the percolation test when it did not."""
     nx, ny, nz = args.nx, args.ny, args.nz
     N = nx * ny * nz
     pack, mask, b, S, lk, c0, cin = full_case_inputs(clb, nx, ny, nz)
@@ -101,7 +110,9 @@ def full_coupled(clb, args):
     lat.set_dynamics(kind, om)
     lat.upload_mask(mask)
     lat.flow_init(1.0, 0.999)
-    lat.flow_steps(args.flow_steps)
+    # set-up: the initial flow solve of the driver (src/complab.cpp:593, 614-620), to its own
+    # tracer ns_convg1 = ValueTracer(1, 1000, ns_converge_iT1), at most ns_max_iT1 steps
+    flow0_steps, flow0_conv = lat.flow_resolve(clb.ValueTracer(1.0, 1000.0, args.ns_converge1), args.ns_max1)
     lat.set_solutes(np.full((6, 256), 1.0 / 0.8), [0, 1] * 6, sum([[cin[s], 0.0] for s in range(6)], []))
     lat.couple_velocity()
     lat.set_biomass(b)
@@ -111,33 +122,40 @@ def full_coupled(clb, args):
     lat.solutes_init(c0)
     lat.set_phase(np.zeros(N, dtype=np.uint8))
     # mineral threshold: the 0.01 % upper tail of species 5 after the start-up transient
-    lat.step(args.warmup)
+    lat.step(args.startup)
     c5 = lat.conc()[5][mask >= 2]
     thr = float(np.quantile(c5, 0.9999))
     # launch shapes and fused budget picked on this state (set-up: the state is restored)
     tune = lat.autotune(args.tune_steps) if args.tune_steps > 0 else None
     lat.set_graph(True, args.interval)
-    lat.step(args.interval)  # capture the graph
-    lat.sync()
+    tracer = clb.ValueTracer(1.0, 1000.0, args.ns_converge)  # the driver's ns_convg2
+    st = dict(conversions=0, resolves=0, resolve_steps=0, unconverged=0, percolation=False)
 
-    def block_run(nblocks):
-        conv = res = 0
-        for _ in range(nblocks):
+    def segments(n):
+        for _ in range(n):
             lat.step(args.interval)
             ch = lat.node_conversion(5, 2, 1, 9, thr, 0.5 * thr)
-            conv += ch
-            if ch:
-                lat.flow_steps(args.resolve_steps)
+            st["conversions"] += ch
+            if ch and not st["percolation"]:
+                k, conv = lat.flow_resolve(tracer, args.ns_max)
+                st["resolves"] += 1
+                st["resolve_steps"] += k
+                if not conv:
+                    st["unconverged"] += 1
+                    st["percolation"] = not lat.percolates()
                 lat.couple_velocity()
-                res += args.resolve_steps
-        return conv, res
 
-    nblocks = max(args.steps // args.interval, 1)
-    steps = nblocks * args.interval
+    # warm-up segments (the first re-solve fills the tracer's window of 500 steps here)
+    segments(args.warmup)
+    warm = dict(st)
+    for key in st:
+        st[key] = 0 if key != "percolation" else st[key]
+    lat.sync()
     t0 = time.perf_counter()
-    conversions, resolves = block_run(nblocks)
+    segments(args.steps)
     lat.sync()
     wall = time.perf_counter() - t0
+    steps = args.steps * args.interval
     # per-kernel durations on the library's stream (HIP events), uncaptured steps
     lat.reset_perf()
     s0 = lat.speciation_stats()
@@ -153,9 +171,20 @@ def full_coupled(clb, args):
     spec_ms = p.spec_ms / max(p.spec_launches, 1)
     flow_ms = p.flow_ms / max(p.flow_launches, 1)
     solves_step = s1[0] / 20.0
-    return dict(N=N, steps=steps, wall=wall, ms_per_step=wall * 1e3 / steps, mlups=N * steps / wall / 1e6,
-                ade_ms=ade_ms, straggler_ms=spec_ms, flow_ms=flow_ms, fluid=int(p.fluid_nodes),
-                conversions=int(conversions), resolve_steps=int(resolves), solves_per_step=solves_step,
+    fluid = int(p.fluid_nodes)
+    # the whole configured run (5000 coupled steps) with the warm-up's first re-solve in it
+    run_steps = 5000
+    run_ms = (wall * 1e3 / steps) * run_steps + warm["resolve_steps"] * flow_ms
+    return dict(N=N, flow0_steps=int(flow0_steps), flow0_converged=bool(flow0_conv), threshold=thr,
+                segments=args.steps, steps=steps, wall=wall, ms_per_segment=wall * 1e3 / args.steps,
+                ms_per_step=wall * 1e3 / steps, mlups=N * steps / wall / 1e6,
+                ade_ms=ade_ms, straggler_ms=spec_ms, flow_ms=flow_ms, fluid=fluid, reactive=int(p.reactive_nodes),
+                conversions=int(st["conversions"]), resolves=int(st["resolves"]),
+                resolve_steps=int(st["resolve_steps"]), unconverged_resolves=int(st["unconverged"]),
+                percolation=bool(st["percolation"]), warmup_resolve_steps=int(warm["resolve_steps"]),
+                warmup_resolves=int(warm["resolves"]),
+                run_5000_ms_per_step=run_ms / run_steps, run_5000_mlups=N * run_steps / (run_ms * 1e-3) / 1e6,
+                solves_per_step=solves_step, stragglers_per_step=p.stragglers / 20.0,
                 converged=int(s1[1]), solves=int(s1[0]), mean_evals=float(s1[2] / max(s1[0], 1)),
                 coupled_solves_per_s=solves_step / (wall / steps), autotune=tune)
 
@@ -419,14 +448,18 @@ def copy_calibration():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=5, help="timed steps: segments of --interval coupled steps, each "
+                                                         "followed by node conversion and the flow re-solve")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed segments before them")
     ap.add_argument("--nx", type=int, default=384)
     ap.add_argument("--ny", type=int, default=192)
     ap.add_argument("--nz", type=int, default=192)
     ap.add_argument("--interval", type=int, default=100, help="steps between node conversions")
-    ap.add_argument("--resolve-steps", type=int, default=100, help="flow steps after a geometry change")
-    ap.add_argument("--flow-steps", type=int, default=1000)
+    ap.add_argument("--startup", type=int, default=100, help="coupled steps before the mineral threshold is drawn")
+    ap.add_argument("--ns-converge", type=float, default=1e-4, help="ns_converge_iT2 of the re-solve tracer")
+    ap.add_argument("--ns-max", type=int, default=2000, help="ns_max_iT2: at most this many re-solve steps")
+    ap.add_argument("--ns-converge1", type=float, default=1e-6, help="ns_converge_iT1 of the initial flow solve")
+    ap.add_argument("--ns-max1", type=int, default=20000, help="ns_max_iT1: at most this many initial flow steps")
     ap.add_argument("--tune-steps", type=int, default=10, help="steps per autotuner trial (0: default shapes)")
     ap.add_argument("--kin-steps", type=int, default=200)
     ap.add_argument("--spec-side", type=int, default=128)
@@ -455,7 +488,11 @@ def main():
     # ---- headline: the full coupled case
     fc = full_coupled(clb, args)
     N = fc["N"]
-    achieved = BYTES_COUPLED6 * N / (fc["ade_ms"] * 1e-3) / 1e9
+    # the fused kernel runs on the compact layout: its algorithmic bytes are those of the
+    # fluid voxels it updates (349 B each: 6 solutes x 7 populations read + written, the
+    # velocity, the mask byte)
+    nf = fc["fluid"]
+    achieved = BYTES_COUPLED6 * nf / (fc["ade_ms"] * 1e-3) / 1e9
     tune = fc["autotune"]
     fused_bt = 512 if tune is None or tune["fused_bx"] * tune["fused_by"] > 256 else 256
     if tune:
@@ -463,12 +500,19 @@ def main():
             f"{tune['flow_ms']:.3f} ms, fused {tune['fused_bx']}x{tune['fused_by']} budget {tune['fused_budget']} "
             f"{tune['fused_ms']:.3f} ms")
     log(f"full coupled: {fc['ms_per_step']:.3f} ms/step, {fc['mlups']:.1f} MLUPS; transport+speciation kernel "
-        f"{fc['ade_ms']:.3f} ms ({achieved:.0f} GB/s algorithmic), stragglers {fc['straggler_ms']:.3f} ms, "
-        f"{fc['solves_per_step']:.0f} solves/step, {fc['mean_evals']:.2f} evaluations/solve")
+        f"{fc['ade_ms']:.3f} ms ({achieved:.0f} GB/s algorithmic over {nf} fluid voxels), stragglers "
+        f"{fc['straggler_ms']:.3f} ms, {fc['solves_per_step']:.0f} solves/step, {fc['mean_evals']:.2f} "
+        f"evaluations/solve; {fc['resolves']} re-solves of {fc['resolve_steps']} flow steps in the timed segments "
+        f"(warm-up: {fc['warmup_resolve_steps']}); 5000-step run {fc['run_5000_ms_per_step']:.3f} ms/step; "
+        f"initial flow {fc['flow0_steps']} steps (converged {fc['flow0_converged']}); {fc['conversions']} voxels "
+        f"converted in the timed segments")
     extra = dict(full_coupled=fc)
-    extra["flow_fp32"] = dict(ms_per_step=fc["flow_ms"], mlups=N / (fc["flow_ms"] * 1e-3) / 1e6,
-                              gbs=BYTES_D3Q19 * N / (fc["flow_ms"] * 1e-3) / 1e9,
-                              frac=BYTES_D3Q19 * N / (fc["flow_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+    fl_ms = fc["flow_ms"]
+    extra["flow_fp32"] = dict(ms_per_step=fl_ms, mlups=N / (fl_ms * 1e-3) / 1e6,
+                              gbs_dense=BYTES_D3Q19 * N / (fl_ms * 1e-3) / 1e9,
+                              frac_dense=BYTES_D3Q19 * N / (fl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              gbs_fluid=BYTES_D3Q19 * nf / (fl_ms * 1e-3) / 1e9,
+                              frac_fluid=BYTES_D3Q19 * nf / (fl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
     # ---- kinetics + biomass, 256 x 128 x 128, 4 solutes
     if not args.skip_kinetics:
         kn = (256, 128, 128)
@@ -480,12 +524,15 @@ def main():
             extra["ca"] = ca_leg(lat, Nk)
         lat.close()
         kms = p.ade_ms / max(p.ade_launches, 1)
+        kf = int(p.fluid_nodes)
+        # compact layout: the algorithmic bytes are the fluid voxels'
         extra["kinetics_fp32"] = dict(workload="kinetics + biomass 256x128x128, 4 solutes, FP32 store",
                                       ms_per_step=wall * 1e3 / args.kin_steps, mlups=Nk * args.kin_steps / wall / 1e6,
-                                      kernel_ms=kms, gbs=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9,
-                                      frac=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, autotune=ktune)
+                                      kernel_ms=kms, fluid=kf, gbs=BYTES_KIN4 * kf / (kms * 1e-3) / 1e9,
+                                      frac=BYTES_KIN4 * kf / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                      gbs_dense=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9, autotune=ktune)
         log(f"kinetics fp32: {extra['kinetics_fp32']['mlups']:.1f} MLUPS, kernel {kms:.3f} ms, "
-            f"{extra['kinetics_fp32']['frac']:.3f} of HBM peak")
+            f"{extra['kinetics_fp32']['frac']:.3f} of HBM peak (fluid bytes)")
         if not args.skip_surrogate:
             extra["surrogate_fp32"] = surrogate_leg(clb, args.kin_steps, args.tune_steps)
         if not args.skip_fp64:
@@ -521,9 +568,9 @@ def main():
         "value": round(fc["mlups"], 2),
         "unit": "MLUPS",
         "n_gpus": 1,
-        "steps": fc["steps"],
+        "steps": fc["segments"],
         "warmup": args.warmup,
-        "ms_per_step": round(fc["ms_per_step"], 4),
+        "ms_per_step": round(fc["ms_per_segment"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -531,16 +578,21 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"full coupled case {args.nx}x{args.ny}x{args.nz}, sphere pack phi 0.35, D3Q19 + "
                                f"6 D3Q7 solutes (FP32 deviation store), dual-Monod kinetics + biomass, FP64 "
-                               f"speciation 3/6 tableau every step, node conversion every {args.interval} steps "
-                               f"+ {args.resolve_steps}-step flow re-solve, captured graphs",
+                               f"speciation 3/6 tableau every step; one step = {args.interval} coupled steps "
+                               f"(captured graphs) + node conversion + flow re-solve to the driver's tracer "
+                               f"(ValueTracer(1, 1000, {args.ns_converge}), at most {args.ns_max} steps)",
+                   "segment_steps": args.interval, "coupled_steps": fc["steps"],
+                   "coupled_ms_per_step": round(fc["ms_per_step"], 4),
                    "nodes": N, "fluid_nodes": fc["fluid"], "solutes": 6, "speciation": "fp64",
                    "speciation_solves_per_s": round(fc["coupled_solves_per_s"], 1), "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (MEASURED_TRAFFIC["full_coupled_384x192x192"]
                                  if (args.nx, args.ny, args.nz) == (384, 192, 192) else None),
-                     "kernel": f"ade_spec_step_kernel<float,6,*,3,{fused_bt}>", "kernel_ms": round(fc["ade_ms"], 4),
-                     "bytes_per_launch": int(BYTES_COUPLED6 * N)},
+                     "traffic_source": MEASURED_TRAFFIC_SOURCE,
+                     "kernel": f"ade_spec_step_compact_kernel<float,6,*,3,{fused_bt}>",
+                     "kernel_ms": round(fc["ade_ms"], 4), "units": "fluid voxels", "units_per_launch": nf,
+                     "bytes_per_unit": BYTES_COUPLED6, "bytes_per_launch": int(BYTES_COUPLED6 * nf)},
         "cpu_baseline": cpu,
         "extra": extra,
     }

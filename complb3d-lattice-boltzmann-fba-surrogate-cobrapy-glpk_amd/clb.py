@@ -11,6 +11,7 @@ Names follow the reference's operator surface where one exists:
   * ``Lattice.step`` runs the reference's per-iteration order (src/complab.cpp:1252-1849).
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -393,6 +394,51 @@ def device_name():
     return buf.value.decode()
 
 
+# ------------------------------------------------------------------ flow convergence (host)
+class ValueTracer:
+    """Palabos' util::ValueTracer (Palabos v2.3.0, src/core/util.h; not in the reference tree),
+    restated from its published behaviour, as the reference's flow loops use it
+    (src/complab.cpp:593, 609-610, 1149, 1797-1798): ValueTracer(u, L, epsilon) keeps the last
+    deltaT = int(L / u / 2) values; it has converged once that window is full and
+    |stddev / mean| < epsilon, stddev with n - 1; a NaN ratio counts as converged (Palabos
+    reports "simulation diverged" and stops). The driver builds its tracers with u = 1,
+    L = 1000: a window of 500 values."""
+
+    def __init__(self, u=1.0, L=1000.0, epsilon=1e-4):
+        self.delta_t = int(L / u / 2.0)
+        self.epsilon = epsilon
+        self.values = []
+        self.t = 0
+
+    def take_value(self, v):
+        self.values.append(v)
+        if len(self.values) > abs(self.delta_t):
+            self.values.pop(0)
+        self.t += 1
+
+    def reset(self):
+        self.values = []
+        self.t = 0
+
+    def average(self):
+        return sum(self.values) / len(self.values)
+
+    def std_dev(self, avg):
+        n = len(self.values)
+        return math.sqrt(sum((v - avg) * (v - avg) for v in self.values) / (n - 1))
+
+    def has_converged(self):
+        if len(self.values) < abs(self.delta_t):
+            return False
+        avg = self.average()
+        sd = self.std_dev(avg)
+        # IEEE division, as in C++: 0/0 is NaN, x/0 infinite
+        ratio = sd / avg if avg != 0.0 else (float("nan") if sd == 0.0 else math.copysign(float("inf"), sd))
+        if math.isnan(ratio):
+            return True
+        return abs(ratio) < self.epsilon
+
+
 # ------------------------------------------------------------------ context
 class Lattice:
     """One GPU's lattice: mask, D3Q19 flow, D3Q7 solutes, reactions, batched speciation."""
@@ -551,6 +597,28 @@ class Lattice:
         s = np.zeros(4)
         self._chk(self.lib.clb_flow_stats(self.h, _ptr(s)), "flow_stats")
         return s
+
+    def flow_resolve(self, tracer, max_steps):
+        """A flow solve of the reference's loops (src/complab.cpp:607-620, 1794-1800,
+        1836-1840): collideAndStream, the stored average energy into the tracer, stop when it
+        has converged, at most max_steps. The tracer keeps its window across calls, as the
+        driver's ns_convg2 does (declared once, src/complab.cpp:1149, never reset). Steps that
+        cannot end the solve (the window not yet full) run as one batch. Returns (steps run,
+        converged)."""
+        done = 0
+        while done < max_steps:
+            k = min(max(1, tracer.delta_t - len(tracer.values)), max_steps - done)
+            for v in self.flow_steps(k, energy=True):
+                tracer.take_value(float(v))
+            done += k
+            if tracer.has_converged():
+                return done, True
+        return done, False
+
+    def percolates(self, thrd=1e-12):
+        """The percolation test after a re-solve that used all its steps (src/complab.cpp:
+        1801-1805): the mean x velocity over the plane x = nx-2 above thrd."""
+        return float(self.flow_stats()[2]) > thrd
 
     def solutes_reset(self, c0):
         c = _f64(c0)

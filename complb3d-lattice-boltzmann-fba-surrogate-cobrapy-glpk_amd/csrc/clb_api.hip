@@ -2241,6 +2241,10 @@ template <typename R>
 int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
   static const int shapes256[][2] = {{64, 4}, {128, 2}, {256, 1}, {32, 8}, {64, 2}, {128, 1}};
   static const int shapes512[][2] = {{64, 8}, {64, 4}, {128, 4}, {32, 8}, {128, 2}, {256, 2}};
+  // on the compact layout a block is a run of fluid nodes: only its thread count matters
+  static const int cshapes256[][2] = {{64, 1}, {128, 1}, {256, 1}};
+  static const int cshapes512[][2] = {{256, 1}, {512, 1}};
+  const bool cmp = compact_ok(c);
   static const int budgets[] = {2, 3, 4};
   TuneSnapshot s;
   int rc = snap_take(c, s);
@@ -2280,8 +2284,11 @@ int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
       if (fused) {
         dim3 pick = c->spec_block;
         int pick_b = c->fused_evals;
+        const int (*sh512)[2] = cmp ? cshapes512 : shapes512;
+        const int n512 = cmp ? 2 : 6;
         for (int b : budgets)
-          for (auto& sh : shapes512) {
+          for (int k = 0; k < n512; ++k) {
+            const int* sh = sh512[k];
             c->spec_block = dim3(sh[0], sh[1], 1);
             c->fused_evals = b;
             double ms = 0;
@@ -2298,7 +2305,10 @@ int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
         rep->fused_ms = best;
       } else {
         dim3 pick = c->ade_block;
-        for (auto& sh : shapes256) {
+        const int (*sh256)[2] = cmp ? cshapes256 : shapes256;
+        const int n256 = cmp ? 3 : 6;
+        for (int k = 0; k < n256; ++k) {
+          const int* sh = sh256[k];
           c->ade_block = dim3(sh[0], sh[1], 1);
           double ms = 0;
           int r = tune_trial<R>(c, s, false, steps, e0, e1, ms);

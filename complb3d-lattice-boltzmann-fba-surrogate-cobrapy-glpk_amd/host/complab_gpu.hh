@@ -69,6 +69,51 @@ inline clb_ctx* create(int device, int precision, int nx, int ny, int nz) {
 }  // namespace detail
 
 // ------------------------------------------------------------------------------------------
+// ValueTracer: Palabos' util::ValueTracer (Palabos v2.3.0, src/core/util.h; not in the
+// reference tree), restated from its published behaviour, as the driver's flow loops use it
+// (src/complab.cpp:593, 609-610, 1149, 1797-1798): ValueTracer(u, L, epsilon) keeps the last
+// deltaT = (int)(L / u / 2) values; converged once the window is full and |stddev / mean| <
+// epsilon (stddev with n - 1), or when that ratio is NaN (a diverged run). Host only.
+class ValueTracer {
+ public:
+  ValueTracer(double u, double L, double epsilon) : deltaT_((int)(L / u / 2.)), epsilon_(epsilon) {}
+  int deltaT() const { return deltaT_; }
+  int size() const { return (int)values_.size(); }
+  void takeValue(double v) {
+    values_.push_back(v);
+    if ((int)values_.size() > std::abs(deltaT_)) values_.erase(values_.begin());
+    ++t_;
+  }
+  void resetValues() {
+    values_.clear();
+    t_ = 0;
+  }
+  bool hasConverged() const {
+    if ((int)values_.size() < std::abs(deltaT_)) return false;
+    const double avg = average();
+    const double ratio = stdDev(avg) / avg;
+    if (std::isnan(ratio)) return true;
+    return std::fabs(ratio) < epsilon_;
+  }
+  double average() const {
+    double s = 0.0;
+    for (double v : values_) s += v;
+    return s / (double)values_.size();
+  }
+  double stdDev(double avg) const {
+    double s = 0.0;
+    for (double v : values_) s += (v - avg) * (v - avg);
+    return std::sqrt(s / (double)(values_.size() - 1));
+  }
+
+ private:
+  int deltaT_;
+  double epsilon_;
+  std::vector<double> values_;
+  long long t_ = 0;
+};
+
+// ------------------------------------------------------------------------------------------
 // Lattice: the device-resident state of one coupled simulation.
 class Lattice {
  public:
@@ -119,6 +164,30 @@ class Lattice {
     chk(clb_flow_stats(c_, s.data()), "clb_flow_stats");
     return s;
   }
+  // A flow solve of the driver's loops (src/complab.cpp:607-620, 1794-1800, 1836-1840):
+  // collideAndStream, the stored average energy into the tracer, stop once it has converged,
+  // at most maxSteps; the tracer keeps its window across calls, as ns_convg2 does. Steps
+  // that cannot end the solve (window not yet full) run as one batch. Returns the steps run;
+  // *converged tells whether the tracer stopped it.
+  long long flowResolve(ValueTracer& tracer, long long maxSteps, bool* converged = nullptr) {
+    long long done = 0;
+    std::vector<double> e;
+    while (done < maxSteps) {
+      const long long k = std::min<long long>(std::max(1, tracer.deltaT() - tracer.size()), maxSteps - done);
+      flowSteps((int)k, &e);
+      for (double v : e) tracer.takeValue(v);
+      done += k;
+      if (tracer.hasConverged()) {
+        if (converged) *converged = true;
+        return done;
+      }
+    }
+    if (converged) *converged = false;
+    return done;
+  }
+  // The percolation test after a re-solve that used all its steps (src/complab.cpp:1801-1805):
+  // the mean x velocity over the plane x = nx-2 above thrd.
+  bool percolates(double thrd = 1e-12) const { return flowStats()[2] > thrd; }
 
   // D3Q7 solutes and reactions
   void setSolutes(int S, const std::vector<double>& omega, const std::vector<int>& bcType,

@@ -15,13 +15,14 @@ from conftest import d3q19_kind_omega, rel_l2
 from test_oracle_speciation import CARB_COMP, CARB_LOGK, CARB_S, carbonate_exact
 
 
-def build(clb, mode, prec):
+def build(clb, mode, prec, layout=0):
     nx, ny, nz = 48, 24, 16
     pack = clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.5, seed=0)
     kind, om = d3q19_kind_omega(0.8)
     b = clb.gen_biomass_seed(pack, pore=2, fraction=0.05, bmin=10, bmax=100, seed_sites=1, seed_density=2)
     c_in, c_0 = carbonate_exact(2e-3, 4e-4), carbonate_exact(1e-3, 1e-4)
     lat = clb.Lattice(nx, ny, nz, precision=prec)
+    lat.set_layout(layout)
     lat.set_speciation_mode(mode)
     lat.set_dynamics(kind, om)
     lat.upload_mask(pack)
@@ -44,11 +45,11 @@ def snapshot(lat):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode,prec", [(0, 1), (1, 1), (0, 0)])
-def test_autotune_restores_state_and_matches(clb, mode, prec):
-    print(f"case: autotune, speciation mode {mode}, precision {prec}", flush=True)
-    lat = build(clb, mode, prec)
-    ref = build(clb, mode, prec)
+@pytest.mark.parametrize("mode,prec,layout", [(0, 1, 0), (1, 1, 0), (0, 0, 0), (0, 1, 1)])
+def test_autotune_restores_state_and_matches(clb, mode, prec, layout):
+    print(f"case: autotune, speciation mode {mode}, precision {prec}, layout {layout}", flush=True)
+    lat = build(clb, mode, prec, layout)
+    ref = build(clb, mode, prec, layout)
     before = snapshot(lat)
     rep = lat.autotune(4)
     after = snapshot(lat)
@@ -60,7 +61,9 @@ def test_autotune_restores_state_and_matches(clb, mode, prec):
         assert rep["ade_bx"] == 0 and rep["fused_ms"] > 0
     else:
         assert rep["ade_bx"] * rep["ade_by"] in (64, 128, 256) and rep["fused_bx"] == 0
-    assert rep["trials"] == 6 + (18 if mode == 0 else 6)
+    # flow: 6 shapes; fused on the compact layout: 2 block sizes x 3 budgets, on the dense one
+    # 6 shapes x 3 budgets; the separate path's transport (dense): 6 shapes
+    assert rep["trials"] == 6 + ((6 if layout == 0 else 18) if mode == 0 else 6)
     # continue both runs: tuned shapes (and budget) against the defaults
     for l in (lat, ref):
         l.step(40)
