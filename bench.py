@@ -107,6 +107,8 @@ def full_coupled(clb, args):
     pack, mask, b, S, lk, c0, cin = full_case_inputs(clb, nx, ny, nz)
     kind, om = kind_omega(0.8)
     lat = clb.Lattice(nx, ny, nz, precision=clb.PREC_FP32)
+    if args.brick:
+        lat.set_compact_order(*[int(v) for v in args.brick.split(",")])
     lat.set_dynamics(kind, om)
     lat.upload_mask(mask)
     lat.flow_init(1.0, 0.999)
@@ -125,8 +127,13 @@ def full_coupled(clb, args):
     lat.step(args.startup)
     c5 = lat.conc()[5][mask >= 2]
     thr = float(np.quantile(c5, 0.9999))
-    # launch shapes and fused budget picked on this state (set-up: the state is restored)
+    # launch shapes and fused budget picked on this state (set-up: the state is restored), or
+    # the given ones
     tune = lat.autotune(args.tune_steps) if args.tune_steps > 0 else None
+    if args.fused_block:
+        lat.set_block_shape(clb.KERNEL_FUSED, args.fused_block, 1)
+    if args.fused_budget:
+        lat.set_fused_budget(args.fused_budget)
     lat.set_graph(True, args.interval)
     tracer = clb.ValueTracer(1.0, 1000.0, args.ns_converge)  # the driver's ns_convg2
     st = dict(conversions=0, resolves=0, resolve_steps=0, unconverged=0, percolation=False)
@@ -461,6 +468,9 @@ def main():
     ap.add_argument("--ns-converge1", type=float, default=1e-6, help="ns_converge_iT1 of the initial flow solve")
     ap.add_argument("--ns-max1", type=int, default=20000, help="ns_max_iT1: at most this many initial flow steps")
     ap.add_argument("--tune-steps", type=int, default=10, help="steps per autotuner trial (0: default shapes)")
+    ap.add_argument("--fused-block", type=int, default=0, help="fused kernel threads per block (0: tuned/default)")
+    ap.add_argument("--fused-budget", type=int, default=0, help="fused evaluation budget (0: tuned/default)")
+    ap.add_argument("--brick", default="", help="bx,by,bz: order of the compact layout's voxels (default: the library's)")
     ap.add_argument("--kin-steps", type=int, default=200)
     ap.add_argument("--spec-side", type=int, default=128)
     ap.add_argument("--train-solves", type=int, default=2000000)
@@ -495,6 +505,8 @@ def main():
     achieved = BYTES_COUPLED6 * nf / (fc["ade_ms"] * 1e-3) / 1e9
     tune = fc["autotune"]
     fused_bt = 512 if tune is None or tune["fused_bx"] * tune["fused_by"] > 256 else 256
+    if args.fused_block:
+        fused_bt = 512 if args.fused_block > 256 else 256
     if tune:
         log(f"autotune ({tune['trials']} trials): flow {tune['flow_bx']}x{tune['flow_by']} "
             f"{tune['flow_ms']:.3f} ms, fused {tune['fused_bx']}x{tune['fused_by']} budget {tune['fused_budget']} "

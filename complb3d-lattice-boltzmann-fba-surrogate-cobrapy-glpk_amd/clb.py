@@ -120,6 +120,7 @@ def load():
         "clb_set_speciation_mode": [vp, c_int],
         "clb_set_fused_budget": [vp, c_int],
         "clb_set_layout": [vp, c_int],
+        "clb_set_compact_order": [vp, c_int, c_int, c_int],
         "clb_set_block_shape": [vp, c_int, c_int, c_int],
         "clb_autotune": [vp, c_int, vp],
         "clb_ca_init": [vp],
@@ -632,6 +633,9 @@ class Lattice:
 
     def set_layout(self, layout):
         self._chk(self.lib.clb_set_layout(self.h, layout), "set_layout")
+
+    def set_compact_order(self, bx, by, bz):
+        self._chk(self.lib.clb_set_compact_order(self.h, bx, by, bz), "set_compact_order")
 
     def set_fused_budget(self, evals):
         self._chk(self.lib.clb_set_fused_budget(self.h, int(evals)), "set_fused_budget")

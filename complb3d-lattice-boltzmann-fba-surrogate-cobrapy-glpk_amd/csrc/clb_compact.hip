@@ -2,9 +2,15 @@
 // device, moving the solute state between the dense and compact layouts, and the transport
 // step without fused speciation on it (clb_compact.h).
 //
-// Build, all on the device (it runs after every geometry change, inside a timed loop):
-//  1. per voxel: 1 if ADE-fluid; per voxel and direction d: 1 if fluid with a wall at +c_d
-//     (a bounce-back link, direction-major);
+// The fluid nodes are numbered in brick order: the lattice cut into bricks of bx x by x bz
+// voxels, the bricks in x-fastest order and the voxels natural inside each. A workgroup's run
+// of nodes then covers a compact block of the lattice, so most y and z neighbours of its nodes
+// are its own nodes (bx = nx, by = bz = 1 is the natural order).
+//
+// Build, all on the device (it runs after every geometry change, inside a timed loop), over
+// the brick-order key of every voxel (padded to whole bricks):
+//  1. 1 if ADE-fluid; per direction d: 1 if fluid with a wall at +c_d (a bounce-back link,
+//     direction-major);
 //  2. exclusive scans of both (the fluid voxel's compact index, each link's ghost slot);
 //  3. the node table: dense index per node, and per direction the slot at +c_d.
 #include <hipcub/hipcub.hpp>
@@ -29,37 +35,56 @@ __device__ __forceinline__ long long nb_index(int x, int y, int z, int d, const 
   return (long long)(x + Q7::cx[d]) + (long long)g.nx * (yy + (long long)g.ny * zz);
 }
 
-__global__ void compact_flags_kernel(const uint8_t* mask, const uint8_t* kind, Geom g, unsigned int* flag,
+// Brick-order keys: key(x, y, z) and its inverse; voxels of the padding are outside the lattice.
+__device__ __forceinline__ long long brick_key(int x, int y, int z, const BrickOrder& o) {
+  const long long b = (long long)(x / o.bx) + (long long)o.nbx * ((y / o.by) + (long long)o.nby * (z / o.bz));
+  return b * o.vol + (x % o.bx) + (long long)o.bx * ((y % o.by) + (long long)o.by * (z % o.bz));
+}
+
+__device__ __forceinline__ bool brick_voxel(long long k, const BrickOrder& o, const Geom& g, int& x, int& y, int& z) {
+  const long long b = k / o.vol, r = k % o.vol;
+  x = (int)((b % o.nbx) * o.bx + r % o.bx);
+  y = (int)(((b / o.nbx) % o.nby) * o.by + (r / o.bx) % o.by);
+  z = (int)((b / ((long long)o.nbx * o.nby)) * o.bz + r / ((long long)o.bx * o.by));
+  return x < g.nx && y < g.ny && z < g.nz;
+}
+
+__global__ void compact_flags_kernel(const uint8_t* mask, const uint8_t* kind, Geom g, BrickOrder o, unsigned int* flag,
                                      unsigned int* lk) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (n >= g.N) return;
-  const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
-  const bool f = kind[mask[n]] == CLB_KIND_FLUID;
-  flag[n] = f ? 1u : 0u;
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= o.nkey) return;
+  int x, y, z;
+  const bool in = brick_voxel(k, o, g, x, y, z);
+  const long long n = in ? (long long)x + (long long)g.nx * (y + (long long)g.ny * z) : 0;
+  const bool f = in && kind[mask[n]] == CLB_KIND_FLUID;
+  flag[k] = f ? 1u : 0u;
 #pragma unroll
   for (int d = 1; d < 7; ++d) {
     bool link = false;
     if (f && nb_in_x(x, d, g.nx)) link = kind[mask[nb_index(x, y, z, d, g)]] != CLB_KIND_FLUID;
-    lk[(long long)(d - 1) * g.N + n] = link ? 1u : 0u;
+    lk[(long long)(d - 1) * o.nkey + k] = link ? 1u : 0u;
   }
 }
 
 __global__ void compact_table_kernel(const unsigned int* flag, const unsigned int* cj, const unsigned int* cg, Geom g,
-                                     int* cnode, int* cnbr, long long ccap, unsigned int* cnf) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (n >= g.N) return;
-  if (n == g.N - 1) *cnf = cj[n] + flag[n];
-  if (!flag[n]) return;
-  const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
-  const long long j = cj[n];
-  cnode[j] = (int)n;
+                                     BrickOrder o, int* cnode, int* cnbr, long long ccap, unsigned int* cnf) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= o.nkey) return;
+  if (k == o.nkey - 1) *cnf = cj[k] + flag[k];
+  if (!flag[k]) return;
+  int x, y, z;
+  (void)brick_voxel(k, o, g, x, y, z);
+  const long long j = cj[k];
+  cnode[j] = (int)((long long)x + (long long)g.nx * (y + (long long)g.ny * z));
 #pragma unroll
   for (int d = 1; d < 7; ++d) {
     int v = -1;
     if (nb_in_x(x, d, g.nx)) {
       const long long m = nb_index(x, y, z, d, g);
-      const long long base = (long long)(d - 1) * g.N;
-      v = flag[m] ? (int)cj[m] : (int)(ccap + (cg[base + n] - cg[base]));
+      const int mx = (int)(m % g.nx), my = (int)((m / g.nx) % g.ny), mz = (int)(m / ((long long)g.nx * g.ny));
+      const long long km = brick_key(mx, my, mz, o);
+      const long long base = (long long)(d - 1) * o.nkey;
+      v = flag[km] ? (int)cj[km] : (int)(ccap + (cg[base + k] - cg[base]));
     }
     cnbr[(long long)(d - 1) * ccap + j] = v;
   }
@@ -111,6 +136,39 @@ __global__ void compact_move_kernel(AdeArgs<R> ca, R* gd, const R* ud, R* pd, R*
   }
 }
 
+// pending_flush_kernel (clb_ade.hip) on the compact layout: for a next step of parity ODD,
+// each fluid node adds to the slots it reads then the increments add_pending would add, and
+// to each population it sent towards a wall (its ghost slot) t_i of its own previous-step
+// increment. Each slot belongs to one node: no races.
+template <typename R, bool ODD>
+__global__ void compact_flush_kernel(AdeArgs<R> a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= (int)*a.cnf) return;
+  const long long st = a.cstride, cap = a.ccap;
+  const CSlots L = cslots(a, j);
+  for (int s = 0; s < a.S; ++s) {
+    const R* P1 = a.pend_r + (long long)s * cap;
+    const R* P2 = a.pend_w + (long long)s * cap;
+    if (a.immobile[s]) {
+      a.g[(long long)(s * 7) * st + j] += P1[j];
+      continue;
+    }
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+      const int src = L.s[opp_dir(i)];  // the slot at -c_i (i = 0: the node's own)
+      if (src >= 0) {
+        const R v = (i == 0) ? P1[j] : (src < cap ? P1[src] : P2[j]);
+        R* q = ODD ? &a.g[(long long)(s * 7 + opp_dir(i)) * st + src] : &a.g[(long long)(s * 7 + i) * st + j];
+        *q += R(Q7::w(i)) * v;
+      }
+      if (i > 0 && L.s[i] >= cap) {
+        R* q = ODD ? &a.g[(long long)(s * 7 + opp_dir(i)) * st + j] : &a.g[(long long)(s * 7 + i) * st + L.s[i]];
+        *q += R(Q7::w(i)) * P1[j];
+      }
+    }
+  }
+}
+
 template <typename R, int S>
 void launch_ade_step_compact_s(const AdeArgs<R>& a, bool odd, int bt, hipStream_t st) {
   const unsigned blocks = (unsigned)((a.ccap + bt - 1) / bt);
@@ -133,24 +191,24 @@ size_t compact_scan_bytes(long long n) {
   return bytes;
 }
 
-void launch_compact_flags(const uint8_t* mask, const uint8_t* kind, const Geom& g, const CompactBuild& b,
-                          hipStream_t s) {
-  hipLaunchKernelGGL(compact_flags_kernel, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, mask, kind, g,
+void launch_compact_flags(const uint8_t* mask, const uint8_t* kind, const Geom& g, const BrickOrder& o,
+                          const CompactBuild& b, hipStream_t s) {
+  hipLaunchKernelGGL(compact_flags_kernel, dim3((unsigned)((o.nkey + 255) / 256)), dim3(256), 0, s, mask, kind, g, o,
                      b.flag, b.lk);
 }
 
-void launch_compact_scan(const CompactBuild& b, long long N, hipStream_t s) {
+void launch_compact_scan(const CompactBuild& b, long long nkey, hipStream_t s) {
   size_t bytes = b.scan_tmp_bytes;
-  (void)hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, bytes, b.flag, b.cj, (int)N, s);
+  (void)hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, bytes, b.flag, b.cj, (int)nkey, s);
   bytes = b.scan_tmp_bytes;
-  // lk holds 6N flags and one zero, so cg[6N] is the total number of links
-  (void)hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, bytes, b.lk, b.cg, (int)(6 * N + 1), s);
+  // lk holds 6 nkey flags and one zero, so cg[6 nkey] is the total number of links
+  (void)hipcub::DeviceScan::ExclusiveSum(b.scan_tmp, bytes, b.lk, b.cg, (int)(6 * nkey + 1), s);
 }
 
-void launch_compact_table(const CompactBuild& b, const Geom& g, int* cnode, int* cnbr, long long ccap,
-                          unsigned int* cnf, hipStream_t s) {
-  hipLaunchKernelGGL(compact_table_kernel, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, b.flag, b.cj, b.cg,
-                     g, cnode, cnbr, ccap, cnf);
+void launch_compact_table(const CompactBuild& b, const Geom& g, const BrickOrder& o, int* cnode, int* cnbr,
+                          long long ccap, unsigned int* cnf, hipStream_t s) {
+  hipLaunchKernelGGL(compact_table_kernel, dim3((unsigned)((o.nkey + 255) / 256)), dim3(256), 0, s, b.flag, b.cj,
+                     b.cg, g, o, cnode, cnbr, ccap, cnf);
 }
 
 template <typename R>
@@ -163,6 +221,13 @@ void launch_compact_move(bool to_compact, const AdeArgs<R>& ca, R* g_dense, cons
   else
     hipLaunchKernelGGL((compact_move_kernel<R, false>), dim3(blocks), dim3(256), 0, s, ca, g_dense, nullptr,
                        pend_dense, pend_compact);
+}
+
+template <typename R>
+void launch_compact_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s) {
+  const unsigned blocks = (unsigned)((a.ccap + 255) / 256);
+  if (odd_next) hipLaunchKernelGGL((compact_flush_kernel<R, true>), dim3(blocks), dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((compact_flush_kernel<R, false>), dim3(blocks), dim3(256), 0, s, a);
 }
 
 template <typename R>
@@ -184,6 +249,8 @@ template void launch_compact_move<float>(bool, const AdeArgs<float>&, float*, co
                                          hipStream_t);
 template void launch_compact_move<double>(bool, const AdeArgs<double>&, double*, const double*, double*, double*,
                                           hipStream_t);
+template void launch_compact_flush<float>(const AdeArgs<float>&, bool, hipStream_t);
+template void launch_compact_flush<double>(const AdeArgs<double>&, bool, hipStream_t);
 template void launch_ade_step_compact<float>(const AdeArgs<float>&, bool, int, hipStream_t);
 template void launch_ade_step_compact<double>(const AdeArgs<double>&, bool, int, hipStream_t);
 

@@ -153,25 +153,36 @@ __global__ void __launch_bounds__(256) flow_step_kernel(FlowArgs<R> a) {
 }
 
 // energy[0..1] = the sums of the per-block partials energy[2 + 2b], energy[3 + 2b], in a
-// fixed order (one block).
-__global__ void __launch_bounds__(256) energy_reduce_kernel(double* energy, long long nblocks) {
-  __shared__ double se[256], sc[256];
-  double e = 0.0, c = 0.0;
-  for (long long b = threadIdx.x; b < nblocks; b += 256) {
-    e += energy[2 + 2 * b];
-    c += energy[3 + 2 * b];
+// fixed order (one block of 1024 threads: thread t sums the partials t, t + 1024, ... into
+// four accumulators in turn, so four loads are in flight; then a tree over the threads).
+constexpr int kEnergyThreads = 1024;
+__global__ void __launch_bounds__(kEnergyThreads) energy_reduce_kernel(double* energy, long long nblocks) {
+  __shared__ double se[kEnergyThreads], sc[kEnergyThreads];
+  double e[4] = {0.0, 0.0, 0.0, 0.0}, c[4] = {0.0, 0.0, 0.0, 0.0};
+  const long long t = threadIdx.x;
+  long long b = t;
+  for (; b + 3 * kEnergyThreads < nblocks; b += 4 * kEnergyThreads) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      e[k] += energy[2 + 2 * (b + k * kEnergyThreads)];
+      c[k] += energy[3 + 2 * (b + k * kEnergyThreads)];
+    }
   }
-  se[threadIdx.x] = e;
-  sc[threadIdx.x] = c;
+  for (int k = 0; b < nblocks; b += kEnergyThreads, ++k) {
+    e[k & 3] += energy[2 + 2 * b];
+    c[k & 3] += energy[3 + 2 * b];
+  }
+  se[t] = (e[0] + e[1]) + (e[2] + e[3]);
+  sc[t] = (c[0] + c[1]) + (c[2] + c[3]);
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      se[threadIdx.x] += se[threadIdx.x + s];
-      sc[threadIdx.x] += sc[threadIdx.x + s];
+  for (int s = kEnergyThreads / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      se[t] += se[t + s];
+      sc[t] += sc[t + s];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  if (t == 0) {
     energy[0] = se[0];
     energy[1] = sc[0];
   }
@@ -297,7 +308,7 @@ void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s)
   if (a.energy) {
     if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((flow_step_kernel<R, false, true>), grid, block, 0, s, a);
-    hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(256), 0, s, a.energy,
+    hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(kEnergyThreads), 0, s, a.energy,
                        (long long)grid.x * grid.y * grid.z);
   } else {
     if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, false>), grid, block, 0, s, a);

@@ -162,21 +162,27 @@ void launch_spec_coupled(const AdeArgs<double>& a, bool odd_done, const Tableau&
 
 // Compact layout (clb_compact.hip). Build: fluid flags and bounce-back links of every voxel,
 // scans, node table. The caller sizes ccap / ghost capacity from the counts.
+// The order of the compact nodes: bricks of bx x by x bz voxels in x-fastest order, natural
+// inside each (bx = nx, by = bz = 1: the natural order). nkey = nbx nby nbz vol >= N.
+struct BrickOrder {
+  int bx, by, bz, nbx, nby, nbz;
+  long long vol, nkey;
+};
 struct CompactBuild {
-  unsigned int* flag;        // [N] 1: fluid
-  unsigned int* cj;          // [N] exclusive scan of flag
-  unsigned int* lk;          // [6N] 1: fluid voxel with a wall at +c_d (direction-major)
-  unsigned int* cg;          // [6N + 1] exclusive scan of lk
+  unsigned int* flag;        // [nkey] 1: fluid (brick order)
+  unsigned int* cj;          // [nkey] exclusive scan of flag
+  unsigned int* lk;          // [6 nkey + 1] 1: fluid voxel with a wall at +c_d (direction-major)
+  unsigned int* cg;          // [6 nkey + 1] exclusive scan of lk
   void* scan_tmp;            // scan workspace
   size_t scan_tmp_bytes;
 };
 size_t compact_scan_bytes(long long n);  // workspace the scans need for n items
-void launch_compact_flags(const uint8_t* mask, const uint8_t* kind, const Geom& g, const CompactBuild& b,
-                          hipStream_t s);
-void launch_compact_scan(const CompactBuild& b, long long N, hipStream_t s);
+void launch_compact_flags(const uint8_t* mask, const uint8_t* kind, const Geom& g, const BrickOrder& o,
+                          const CompactBuild& b, hipStream_t s);
+void launch_compact_scan(const CompactBuild& b, long long nkey, hipStream_t s);
 // node table and the fluid count (written to cnf)
-void launch_compact_table(const CompactBuild& b, const Geom& g, int* cnode, int* cnbr, long long ccap,
-                          unsigned int* cnf, hipStream_t s);
+void launch_compact_table(const CompactBuild& b, const Geom& g, const BrickOrder& o, int* cnode, int* cnbr,
+                          long long ccap, unsigned int* cnf, hipStream_t s);
 // Dense <-> compact copies of the solute state, for the fluid nodes of ca's table:
 // populations with the ghost slots (to_compact: from g_dense into ca.g; else back), both
 // deferred-increment buffers (2 * S planes each, dense stride N, compact stride ccap; null:
@@ -184,6 +190,10 @@ void launch_compact_table(const CompactBuild& b, const Geom& g, int* cnode, int*
 template <typename R>
 void launch_compact_move(bool to_compact, const AdeArgs<R>& ca, R* g_dense, const R* u_dense, R* pend_dense,
                          R* pend_compact, hipStream_t s);
+// pending_flush on the compact layout (a.pend_r / pend_w: the compact buffers, as for the
+// next step of parity odd_next)
+template <typename R>
+void launch_compact_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s);
 template <typename R>
 void launch_ade_step_compact(const AdeArgs<R>& a, bool odd, int bt, hipStream_t s);
 template <typename R>

@@ -151,6 +151,11 @@ int clb_set_speciation_mode(clb_ctx* ctx, int mode);
 #define CLB_LAYOUT_COMPACT 0
 #define CLB_LAYOUT_DENSE 1
 int clb_set_layout(clb_ctx* ctx, int layout);
+/* Order of the fluid voxels in the compact layout (no reference counterpart: a memory layout
+ * choice): bricks of bx x by x bz voxels, in x-fastest order, natural inside each, so that a
+ * workgroup's run of voxels covers a compact block of the lattice (bx = nx, by = bz = 1: the
+ * natural order; 0: the default). Results do not depend on it. */
+int clb_set_compact_order(clb_ctx* ctx, int bx, int by, int bz);
 /* Evaluation budget of the fused solve, 2..64 (no reference counterpart: a tuning choice,
  * default 3). A voxel that has not met the judge within it is finished by the persistent
  * kernel in the same step, from the point its fused solve reached; a small budget keeps the
