@@ -54,11 +54,11 @@ FLOP_SRG_NET = 2 * 2 + 2 * (2 * 10 + 3 * 100) + 4 * 10 * (20 + 4) + 2 * 10 + 4
 # workloads, scaled by a copy kernel of known size; mean of the odd and even steps. Not
 # measured in this run (the counters need the profiler): the file they come from is named.
 # Reported only when the workload is the default one.
-MEASURED_TRAFFIC_SOURCE = "profiles/hbm_traffic_fetch_write.md"
+MEASURED_TRAFFIC_SOURCE = "profiles/hbm_traffic_full_coupled_384x192x192_kinetics_256x128x128.md"
 MEASURED_TRAFFIC = {
-    "full_coupled_384x192x192": None,
-    "flow_384x192x192": (1.619e9 + 1.540e9) / 2,
-    "kinetics_256x128x128": None,
+    "full_coupled_384x192x192": (3.867e9 + 2.953e9) / 2,
+    "flow_384x192x192": (1.956e9 + 1.867e9) / 2,
+    "kinetics_256x128x128": (0.841e9 + 0.570e9) / 2,
 }
 
 
@@ -524,7 +524,9 @@ def main():
                               gbs_dense=BYTES_D3Q19 * N / (fl_ms * 1e-3) / 1e9,
                               frac_dense=BYTES_D3Q19 * N / (fl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                               gbs_fluid=BYTES_D3Q19 * nf / (fl_ms * 1e-3) / 1e9,
-                              frac_fluid=BYTES_D3Q19 * nf / (fl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS)
+                              frac_fluid=BYTES_D3Q19 * nf / (fl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                              traffic=(MEASURED_TRAFFIC["flow_384x192x192"]
+                                       if (args.nx, args.ny, args.nz) == (384, 192, 192) else None))
     # ---- kinetics + biomass, 256 x 128 x 128, 4 solutes
     if not args.skip_kinetics:
         kn = (256, 128, 128)
@@ -542,7 +544,8 @@ def main():
                                       ms_per_step=wall * 1e3 / args.kin_steps, mlups=Nk * args.kin_steps / wall / 1e6,
                                       kernel_ms=kms, fluid=kf, gbs=BYTES_KIN4 * kf / (kms * 1e-3) / 1e9,
                                       frac=BYTES_KIN4 * kf / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                      gbs_dense=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9, autotune=ktune)
+                                      gbs_dense=BYTES_KIN4 * Nk / (kms * 1e-3) / 1e9,
+                                      traffic=MEASURED_TRAFFIC["kinetics_256x128x128"], autotune=ktune)
         log(f"kinetics fp32: {extra['kinetics_fp32']['mlups']:.1f} MLUPS, kernel {kms:.3f} ms, "
             f"{extra['kinetics_fp32']['frac']:.3f} of HBM peak (fluid bytes)")
         if not args.skip_surrogate:
