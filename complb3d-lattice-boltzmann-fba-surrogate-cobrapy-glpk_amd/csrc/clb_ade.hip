@@ -151,11 +151,13 @@ __device__ __forceinline__ void write_rest_state(const AdeArgs<R>& a, bool next_
 // stamped with phase_id; any other voxel stamped with phase_id whose mineral falls below
 // reopen becomes pore_code with phase 0, whatever its mask was. changed[0] counts the
 // voxels whose mask code changed (a flow re-solve trigger); a wall voxel that reopens
-// restarts at rest.
+// restarts at rest. The flow's slots are stored relative to the reference profile
+// 1 + fref0 + x * frdx of their own voxel (clb_flow.hip), so a population at rest at
+// density 1 in a slot of voxel x is -t_i (fref0 + x * frdx).
 template <typename R, bool ODD>
 __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int pore_code, int precip_code,
                                        int phase_id, double precip_max, double reopen, unsigned int* changed,
-                                       R* flowf, bool flow_next_odd) {
+                                       R* flowf, bool flow_next_odd, double fref0, double frdx) {
   const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const Geom& g = a.geo;
   bool flip = false;
@@ -185,9 +187,13 @@ __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int
         const Nbr nb(x, y, z, g);
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-          if (!flow_next_odd) flowf[(long long)i * g.N + n] = R(0);
-          else if (nb.xok(-Q19::cx[i]))
-            flowf[(long long)opp_dir(i) * g.N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])] = R(0);
+          if (!flow_next_odd) {
+            flowf[(long long)i * g.N + n] = (R)(-Q19::w(i) * (fref0 + frdx * (double)x));
+          } else if (nb.xok(-Q19::cx[i])) {
+            const int xs = x - Q19::cx[i];
+            flowf[(long long)opp_dir(i) * g.N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])] =
+                (R)(-Q19::w(i) * (fref0 + frdx * (double)xs));
+          }
         }
       }
     }
@@ -335,15 +341,15 @@ void launch_ade_conc(const AdeArgs<R>& a, bool odd_next, double* c, hipStream_t 
 template <typename R>
 void launch_node_conversion(const AdeArgs<R>& a, bool odd_next, uint8_t* phase, int ms, int pore_code,
                             int precip_code, int phase_id, double precip_max, double reopen, unsigned int* changed,
-                            R* flowf, bool flow_next_odd, hipStream_t s) {
+                            R* flowf, bool flow_next_odd, double fref0, double frdx, hipStream_t s) {
   const int bs = 256;
   dim3 grid((unsigned)((a.geo.N + bs - 1) / bs));
   if (odd_next)
     hipLaunchKernelGGL((node_conversion_kernel<R, true>), grid, dim3(bs), 0, s, a, phase, ms, pore_code, precip_code,
-                       phase_id, precip_max, reopen, changed, flowf, flow_next_odd);
+                       phase_id, precip_max, reopen, changed, flowf, flow_next_odd, fref0, frdx);
   else
     hipLaunchKernelGGL((node_conversion_kernel<R, false>), grid, dim3(bs), 0, s, a, phase, ms, pore_code,
-                       precip_code, phase_id, precip_max, reopen, changed, flowf, flow_next_odd);
+                       precip_code, phase_id, precip_max, reopen, changed, flowf, flow_next_odd, fref0, frdx);
 }
 
 template <typename R>
@@ -380,8 +386,8 @@ template void launch_ade_step<double>(const AdeArgs<double>&, bool, dim3, hipStr
 template void launch_ade_conc<float>(const AdeArgs<float>&, bool, double*, hipStream_t);
 template void launch_ade_conc<double>(const AdeArgs<double>&, bool, double*, hipStream_t);
 template void launch_node_conversion<float>(const AdeArgs<float>&, bool, uint8_t*, int, int, int, int, double,
-                                            double, unsigned int*, float*, bool, hipStream_t);
+                                            double, unsigned int*, float*, bool, double, double, hipStream_t);
 template void launch_node_conversion<double>(const AdeArgs<double>&, bool, uint8_t*, int, int, int, int, double,
-                                             double, unsigned int*, double*, bool, hipStream_t);
+                                             double, unsigned int*, double*, bool, double, double, hipStream_t);
 
 }  // namespace clb

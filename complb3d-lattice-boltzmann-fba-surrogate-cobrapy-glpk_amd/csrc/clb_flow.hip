@@ -6,13 +6,24 @@
 //
 // HBM traffic per fluid node and step: 19 populations read + 19 written
 // (2*19*sizeof(R)) + 1 mask byte; bounce-back / no-dynamics nodes cost the mask byte.
+//
+// Storage: each slot holds f_i - t_i (1 + rref(x)) for the x of the slot's own voxel, where
+// rref is the linear density profile of the initial state (clb_flow_init). The pressure drop
+// along x makes the density, not the velocity, the largest part of f_i - t_i; taking the
+// profile out leaves the local deviation from it and the momentum, so an FP32 store keeps
+// about as many bits of the velocity as the profile is larger than that deviation. In a
+// node's registers the populations are relative to the node's own rref(x); a population
+// that moves one voxel along x changes reference by t_i c_ix rdx. The collision is unchanged
+// in form: with q_i = f_i - t_i (1 + rref(x)), sum q_i = rho - 1 - rref(x) and
+// sum c_i q_i = j (sum c_i t_i = 0), and q_eq has the f_eq - t_i form with that density.
 #include "clb_lattice.h"
 
 namespace clb {
 
 namespace {
 
-// fbar_eq_i = t_i [rhobar + 3 c.j + 4.5 (c.j)^2 - 1.5 j^2]   (rho0 = 1)
+// f_eq,i - t_i (1 + r) = t_i [(rho - 1 - r) + 3 c.j + 4.5 (c.j)^2 - 1.5 j^2]   (rho0 = 1), for
+// any reference r: called with rhobar = rho - 1 - rref(x)
 template <typename R>
 __device__ __forceinline__ R feq19(int i, R rhobar, R jx, R jy, R jz, R jsq) {
   const R cj = R(Q19::cx[i]) * jx + R(Q19::cy[i]) * jy + R(Q19::cz[i]) * jz;
@@ -54,23 +65,29 @@ __device__ __forceinline__ void regularize(R* p, R rhobar) {
 }
 
 // XCHK = false for a node with 1 <= x <= nx-2, whose x neighbours always exist: no checks,
-// so the loads and stores need no branches.
+// so the loads and stores need no branches. Odd steps move populations between voxels: one
+// arriving from x - c_ix changes reference by -t_i c_ix rdx (and one leaving for x + c_ix,
+// in store19, by the same).
 template <typename R, bool ODD, bool XCHK = true>
-__device__ __forceinline__ void load19(const R* __restrict__ f, long long N, const Nbr& nb, int n, R* p) {
+__device__ __forceinline__ void load19(const R* __restrict__ f, long long N, const Nbr& nb, int n, R* p, R rdx) {
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
     if (!ODD) {
       p[i] = f[(long long)i * N + n];
     } else {
       const int o = opp_dir(i);
-      if (!XCHK || nb.xok(-Q19::cx[i])) p[i] = f[(long long)o * N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])];
-      else p[i] = R(0);
+      if (!XCHK || nb.xok(-Q19::cx[i])) {
+        p[i] = f[(long long)o * N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])];
+        if (Q19::cx[i] != 0) p[i] -= R(Q19::w(i) * Q19::cx[i]) * rdx;
+      } else {
+        p[i] = R(0);
+      }
     }
   }
 }
 
 template <typename R, bool ODD, bool XCHK = true>
-__device__ __forceinline__ void store19(R* __restrict__ f, long long N, const Nbr& nb, int n, const R* p) {
+__device__ __forceinline__ void store19(R* __restrict__ f, long long N, const Nbr& nb, int n, const R* p, R rdx) {
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
     // non-temporal stores: the 19 populations of the whole lattice (1 GB at 384 x 192 x 192)
@@ -79,8 +96,10 @@ __device__ __forceinline__ void store19(R* __restrict__ f, long long N, const Nb
     if (!ODD) {
       __builtin_nontemporal_store(p[i], &f[(long long)opp_dir(i) * N + n]);
     } else {
-      if (!XCHK || nb.xok(Q19::cx[i]))
-        __builtin_nontemporal_store(p[i], &f[(long long)i * N + nb.at(Q19::cx[i], Q19::cy[i], Q19::cz[i])]);
+      if (!XCHK || nb.xok(Q19::cx[i])) {
+        const R v = (Q19::cx[i] != 0) ? p[i] - R(Q19::w(i) * Q19::cx[i]) * rdx : p[i];
+        __builtin_nontemporal_store(v, &f[(long long)i * N + nb.at(Q19::cx[i], Q19::cy[i], Q19::cz[i])]);
+      }
     }
   }
 }
@@ -92,9 +111,9 @@ __device__ __forceinline__ double flow_node(const FlowArgs<R>& a, int x, int y, 
   const Geom& g = a.g;
   const Nbr nb(x, y, z, g);
   R p[19];
-  load19<R, ODD, FACE != 0>(a.f, g.N, nb, n, p);
-  if (FACE == -1) regularize<R, -1>(p, a.rhobar_in);
-  if (FACE == 1) regularize<R, 1>(p, a.rhobar_out);
+  load19<R, ODD, FACE != 0>(a.f, g.N, nb, n, p, a.rdx);
+  if (FACE == -1) regularize<R, -1>(p, a.dface_in);
+  if (FACE == 1) regularize<R, 1>(p, a.dface_out);
   R rhobar = 0, jx = 0, jy = 0, jz = 0;
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
@@ -107,7 +126,7 @@ __device__ __forceinline__ double flow_node(const FlowArgs<R>& a, int x, int y, 
   const R jsq = jx * jx + jy * jy + jz * jz;
 #pragma unroll
   for (int i = 0; i < 19; ++i) p[i] = (R(1) - om) * p[i] + om * feq19<R>(i, rhobar, jx, jy, jz, jsq);
-  store19<R, ODD, FACE != 0>(a.f, g.N, nb, n, p);
+  store19<R, ODD, FACE != 0>(a.f, g.N, nb, n, p, a.rdx);
   return 0.5 * (double)jsq;
 }
 
@@ -188,14 +207,14 @@ __global__ void __launch_bounds__(kEnergyThreads) energy_reduce_kernel(double* e
   }
 }
 
+// The initial state, at rest with rho linear in x from rho_in to rho_out, is the reference
+// profile itself: every slot holds zero.
 template <typename R>
-__global__ void flow_init_kernel(R* f, Geom g, double rho_in, double rho_out) {
+__global__ void flow_init_kernel(R* f, Geom g) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= g.N) return;
-  const int x = (int)(n % g.nx);
-  const double rho = rho_in + (rho_out - rho_in) * (double)x / (double)(g.nx - 1);
 #pragma unroll
-  for (int i = 0; i < 19; ++i) f[(long long)i * g.N + n] = (R)(Q19::w(i) * (rho - 1.0));
+  for (int i = 0; i < 19; ++i) f[(long long)i * g.N + n] = R(0);
 }
 
 // Velocity (u = j, rho0 = 1) and density from the pre-collision populations of the next
@@ -212,18 +231,20 @@ __global__ void flow_velocity_kernel(FlowArgs<R> a, double* u, double* rho, R* a
   if (a.kind[a.mask[n]] == CLB_KIND_FLUID) {
     const Nbr nb(x, y, z, g);
     R p[19];
-    load19<R, ODD>(a.f, g.N, nb, n, p);
+    load19<R, ODD>(a.f, g.N, nb, n, p, a.rdx);
     if (x == 0 || x == g.nx - 1) {
       const int side = (x == 0) ? -1 : 1;
       rb = (double)((x == 0) ? a.rhobar_in : a.rhobar_out);
+      const double d = (double)((x == 0) ? a.dface_in : a.dface_out);
       double onwall = 0, normal = 0;
 #pragma unroll
       for (int i = 0; i < 19; ++i) {
         if (Q19::cx[i] == 0) onwall += (double)p[i];
         else if (Q19::cx[i] == side) normal += (double)p[i];
       }
-      jx = (side < 0) ? (rb - onwall - 2.0 * normal) : -(rb - onwall - 2.0 * normal);
+      jx = (side < 0) ? (d - onwall - 2.0 * normal) : -(d - onwall - 2.0 * normal);
     } else {
+      rb = a.rref0 + a.rrdx * (double)x;
 #pragma unroll
       for (int i = 0; i < 19; ++i) {
         rb += (double)p[i];
@@ -252,7 +273,7 @@ __global__ void flow_stats_kernel(FlowArgs<R> a, double* acc) {
     if (a.kind[a.mask[n]] == CLB_KIND_FLUID && x > 0 && x < g.nx - 1) {
       const Nbr nb(x, y, z, g);
       R p[19];
-      load19<R, ODD>(a.f, g.N, nb, n, p);
+      load19<R, ODD>(a.f, g.N, nb, n, p, a.rdx);
       double jx = 0, jy = 0, jz = 0;
 #pragma unroll
       for (int i = 0; i < 19; ++i) {
@@ -296,10 +317,9 @@ template void launch_flow_stats<float>(const FlowArgs<float>&, bool, double*, hi
 template void launch_flow_stats<double>(const FlowArgs<double>&, bool, double*, hipStream_t);
 
 template <typename R>
-void launch_flow_init(const Geom& g, R* f, double rho_in, double rho_out, hipStream_t s) {
+void launch_flow_init(const Geom& g, R* f, hipStream_t s) {
   const int bs = 256;
-  hipLaunchKernelGGL(flow_init_kernel<R>, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, f, g, rho_in,
-                     rho_out);
+  hipLaunchKernelGGL(flow_init_kernel<R>, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, f, g);
 }
 
 template <typename R>
@@ -324,8 +344,8 @@ void launch_flow_velocity(const FlowArgs<R>& a, bool odd_next, double* u, double
   else hipLaunchKernelGGL((flow_velocity_kernel<R, false>), grid, dim3(bs), 0, s, a, u, rho, ade_u);
 }
 
-template void launch_flow_init<float>(const Geom&, float*, double, double, hipStream_t);
-template void launch_flow_init<double>(const Geom&, double*, double, double, hipStream_t);
+template void launch_flow_init<float>(const Geom&, float*, hipStream_t);
+template void launch_flow_init<double>(const Geom&, double*, hipStream_t);
 long long flow_blocks(const Geom& g, dim3 block) {
   return (long long)((g.nx + block.x - 1) / block.x) * ((g.ny + block.y - 1) / block.y) * g.nz;
 }

@@ -31,12 +31,18 @@ struct Geom {
 
 template <typename R>
 struct FlowArgs {
-  R* f;                   // 19*N, population-major, deviation from the weights
+  // 19*N, population-major. Slot (i, m) holds f_i - t_i (1 + rref(x_m)): the deviation from
+  // the weights at the reference density 1 + rref(x) of the slot's own voxel, where
+  // rref(x) = rref0 + x * rdx is the linear profile of the flow's initial state.
+  R* f;
   const uint8_t* mask;    // N
   const uint8_t* kind;    // 256
   const R* omega;         // 256
   Geom g;
-  R rhobar_in, rhobar_out;
+  R rhobar_in, rhobar_out;  // the faces' densities - 1
+  R dface_in, dface_out;    // the same, less rref at x = 0 and x = nx-1
+  R rdx;                    // rref(x + 1) - rref(x)
+  double rref0, rrdx;       // rref(0) and rdx in FP64, for the reported densities
   double* energy;         // [0..1] sum |u|^2/2 and count; [2 + 2b..] per-block partials; or null
 };
 
@@ -127,7 +133,7 @@ struct FusedSpecIO {
 
 // ------------------------------------------------------------------ launchers
 template <typename R>
-void launch_flow_init(const Geom& g, R* f, double rho_in, double rho_out, hipStream_t s);
+void launch_flow_init(const Geom& g, R* f, hipStream_t s);
 // Per-block energy partials a launch of this block shape writes (the energy buffer holds
 // 2 + 2 * flow_blocks doubles).
 long long flow_blocks(const Geom& g, dim3 block);
@@ -142,7 +148,7 @@ void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c
 template <typename R>
 void launch_node_conversion(const AdeArgs<R>& a, bool odd_next, uint8_t* phase, int ms, int pore_code,
                             int precip_code, int phase_id, double precip_max, double reopen, unsigned int* changed,
-                            R* flowf, bool flow_next_odd, hipStream_t s);
+                            R* flowf, bool flow_next_odd, double fref0, double frdx, hipStream_t s);
 template <typename R>
 void launch_ade_reset(const AdeArgs<R>& a, bool next_odd, const double* c0, hipStream_t s);
 template <typename R>

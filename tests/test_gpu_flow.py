@@ -56,7 +56,8 @@ def test_flow_fp64_matches_oracle(clb, pack, oracle_flow):
     (uo, ro), eo = oracle_flow[301]
     assert rel_l2(u301, uo) < 1e-10
     assert np.max(np.abs(r301 - ro)) < 1e-12
-    assert np.max(np.abs(e1 - eo) / eo) < 1e-9
+    # the first step's energy is zero (the state at rest); the oracle's holds rounding noise
+    assert np.all(np.abs(e1 - eo) <= 1e-9 * eo + 1e-30)
     lat.flow_steps(1699)
     u, r = lat.velocity()
     (uo, ro), _ = oracle_flow[2000]
@@ -111,4 +112,4 @@ def test_energy_any_block_shape(clb, pack, oracle_flow, block):
     e = lat.flow_steps(301, energy=True)
     lat.close()
     _, eo = oracle_flow[301]
-    assert np.max(np.abs(e - eo) / eo) < 1e-9
+    assert np.all(np.abs(e - eo) <= 1e-9 * eo + 1e-30)

@@ -52,6 +52,27 @@ def test_batch_newton_residual_every_voxel(clb, oracle, tableau, totals):
 
 
 @pytest.mark.gpu
+def test_batch_newton_residual_every_voxel_128cubed(clb, oracle, tableau):
+    """The batched speciation workload at full size: the 128^3 Gaussian-smoothed totals field
+    (seed 4) on the 8 / 24 tableau, every one of its 2 097 152 answers judged by the oracle's
+    restatement of the reference's PCF residual (<= 1e-10 in the 2-norm)."""
+    print("case: batch newton 8/24, 128^3 voxels, oracle judge on every voxel", flush=True)
+    S, lk = tableau
+    T = clb.gen_totals_field(128, 128, 128, 8, sigma=4.0, seed=4)
+    lat = clb.Lattice(4, 1, 1)
+    lat.set_tableau(S, lk, list(range(8)), tol=TOL, max_iter=200)
+    lat.batch_load(T)
+    lat.batch_solve(clb.WARM_REF)
+    x, it, r, cv = lat.batch_fetch()
+    lat.close()
+    assert x.shape == (128 ** 3, 8)
+    assert cv.all(), f"{(~cv.astype(bool)).sum()} voxels did not converge"
+    f = np.linalg.norm(oracle.pcf_residual(S, lk, x, T), axis=1)
+    assert np.all(np.isfinite(f))
+    assert f.max() < TOL, f"worst voxel {int(f.argmax())}: {f.max():.3e}"
+
+
+@pytest.mark.gpu
 def test_batch_carbonate_closed_form(clb, oracle):
     print("case: batch newton carbonate", flush=True)
     rng = np.random.default_rng(12)
