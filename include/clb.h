@@ -30,7 +30,8 @@ extern "C" {
 #define CLB_ERR_UNSUP (-5)   /* configuration outside what the kernels are built for */
 
 #define CLB_PREC_FP64 0      /* FP64 storage and arithmetic */
-#define CLB_PREC_FP32 1      /* FP32 storage (D3Q19 as deviations from the weights), FP32 collision */
+#define CLB_PREC_FP32 1      /* FP32 storage (deviations from the weights; D3Q19 at the initial density
+                                profile of clb_flow_init), FP32 collision */
 
 #define CLB_KIND_NONE 0      /* Palabos NoDynamics */
 #define CLB_KIND_BB 1        /* Palabos BounceBack (node-based, full way) */
