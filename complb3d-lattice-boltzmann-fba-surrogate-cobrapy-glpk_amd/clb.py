@@ -63,17 +63,19 @@ class Perf(ctypes.Structure):
                 ("stragglers", ctypes.c_longlong)]
 
 
-_lib = None
+_libs = {}
 
 
-def load():
-    """Load libclb.so (raises ClbError if it has not been built)."""
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise ClbError(f"{LIB_PATH} is missing: build it first (__graft_entry__.build() or make)")
-    lib = ctypes.CDLL(LIB_PATH)
+def load(path=None):
+    """Load libclb.so, or the library at path: a build of the same sources with other kinetics
+    plugins compiled in (make KINETICS=... ABIOTIC=... OUT=path). Each is loaded once; raises
+    ClbError if it has not been built."""
+    path = LIB_PATH if path is None else os.path.abspath(path)
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise ClbError(f"{path} is missing: build it first (__graft_entry__.build() or make)")
+    lib = ctypes.CDLL(path)
     P = ctypes.POINTER
     c_int, c_double, c_long, c_u64 = ctypes.c_int, ctypes.c_double, ctypes.c_long, ctypes.c_uint64
     vp = ctypes.c_void_p
@@ -153,7 +155,7 @@ def load():
         fn.restype = c_int
     lib.clb_last_error.argtypes = [vp]
     lib.clb_last_error.restype = ctypes.c_char_p
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -444,8 +446,10 @@ class ValueTracer:
 class Lattice:
     """One GPU's lattice: mask, D3Q19 flow, D3Q7 solutes, reactions, batched speciation."""
 
-    def __init__(self, nx, ny, nz, precision=PREC_FP64, device=0):
-        self.lib = load()
+    def __init__(self, nx, ny, nz, precision=PREC_FP64, device=0, library=None):
+        """library: the path of another build of the library (other kinetics plugins), or None
+        for libclb.so."""
+        self.lib = load(library)
         self.nx, self.ny, self.nz = nx, ny, nz
         self.N = nx * ny * nz
         cfg = _Config(device, precision, nx, ny, nz)

@@ -1316,7 +1316,10 @@ int clb_flow_download_velocity(clb_ctx* c, double* u, double* rho) {
 
 int clb_set_solutes(clb_ctx* c, int S, const double* omega, const int* bc_type, const double* bc_value) {
   if (c) drop_graph(c);
-  if (!c || S < 1 || S > CLB_MAX_SOLUTES || !omega || !bc_type || !bc_value) return CLB_ERR_ARG;
+  if (!c || S < 1 || !omega || !bc_type || !bc_value) return CLB_ERR_ARG;
+  if (S > CLB_MAX_SOLUTES)
+    return fail(c, CLB_ERR_ARG, "at most " + std::to_string(CLB_MAX_SOLUTES) + " solutes (asked for " +
+                                    std::to_string(S) + ")");
   {
     const int rc0 = flush_pending(c);
     if (rc0) return rc0;
@@ -1425,7 +1428,10 @@ int clb_set_velocity(clb_ctx* c, const double* u) {
 
 int clb_set_biomass(clb_ctx* c, int nb, const double* b) {
   if (c) drop_graph(c);
-  if (!c || nb < 0 || nb > CLB_MAX_BIOMASS || (nb > 0 && !b)) return CLB_ERR_ARG;
+  if (!c || nb < 0 || (nb > 0 && !b)) return CLB_ERR_ARG;
+  if (nb > CLB_MAX_BIOMASS)
+    return fail(c, CLB_ERR_ARG, "at most " + std::to_string(CLB_MAX_BIOMASS) + " biomass fields (asked for " +
+                                    std::to_string(nb) + ")");
   c->nb = nb;
   if (nb == 0) return CLB_OK;
   const long long n = (long long)nb * c->geo.N;

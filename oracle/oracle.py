@@ -13,16 +13,22 @@ LIB_PATH = os.path.join(HERE, "liboracle.so")
 
 KIND_NONE, KIND_BB, KIND_FLUID = 0, 1, 2
 
-_lib = None
+# the hook-variant oracle: built with the reference's own example kinetics headers
+# (`make -C oracle ref-hooks`, only where the reference tree is present)
+REF_HOOKS_PATH = os.path.join(HERE, "_ref", "liboracle_ref_hooks.so")
+
+_libs = {}
 
 
-def load():
-    global _lib
-    if _lib is not None:
-        return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
-    lib = ctypes.CDLL(LIB_PATH)
+def load(path=None):
+    """The oracle library at path (default: liboracle.so, built with the package's own
+    kinetics plugins); each is loaded once."""
+    path = LIB_PATH if path is None else path
+    if path in _libs:
+        return _libs[path]
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run `make -C oracle`")
+    lib = ctypes.CDLL(path)
     vp, ci, cd, cl = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
     sigs = {
         "orc_flow_init": ([ci, ci, ci, cd, cd, vp], None),
@@ -54,7 +60,7 @@ def load():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
-    _lib = lib
+    _libs[path] = lib
     return lib
 
 
@@ -192,31 +198,31 @@ _KIN_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
                            ctypes.c_void_p, ctypes.c_longlong)
 
 
-def kinetics_plugin_ptr():
-    lib = load()
+def kinetics_plugin_ptr(library=None):
+    lib = load(library)
     return ctypes.cast(lib.orc_kinetics_plugin, ctypes.c_void_p).value
 
 
-def abiotic_plugin_ptr():
-    lib = load()
+def abiotic_plugin_ptr(library=None):
+    lib = load(library)
     return ctypes.cast(lib.orc_abiotic_plugin, ctypes.c_void_p).value
 
 
-def dissolution_plugin_ptr():
-    lib = load()
+def dissolution_plugin_ptr(library=None):
+    lib = load(library)
     return ctypes.cast(lib.orc_dissolution_plugin, ctypes.c_void_p).value
 
 
-def abiotic_plugin(C, mask_code):
-    lib = load()
+def abiotic_plugin(C, mask_code, library=None):
+    lib = load(library)
     C = _f(C)
     sr = np.zeros(C.size)
     lib.orc_abiotic_plugin(_p(C), C.size, _p(sr), mask_code)
     return sr
 
 
-def kinetics_plugin(B, C, mask_code):
-    lib = load()
+def kinetics_plugin(B, C, mask_code, library=None):
+    lib = load(library)
     B = _f(B)
     C = _f(C)
     sr = np.zeros(C.size)
@@ -294,8 +300,8 @@ class AdeRun:
 
     def __init__(self, nx, ny, nz, mask, kind, omega, bc_type, bc_value, u, c0, solid_code=0, bb_code=1,
                  kinetics=False, dt=1.0, biomass=None, tableau=None, eq_tol=1e-10, eq_max_iter=200,
-                 mask_update=None, immobile=None, abiotic=0, dissolution=None, surrogate=None):
-        self.lib = load()
+                 mask_update=None, immobile=None, abiotic=0, dissolution=None, surrogate=None, library=None):
+        self.lib = load(library)
         self.nx, self.ny, self.nz = nx, ny, nz
         self.N = N = nx * ny * nz
         self.mask = np.ascontiguousarray(mask, dtype=np.uint8).copy()
@@ -316,7 +322,7 @@ class AdeRun:
         P.kind, P.omega = _p(self.kind).value, _p(self.omega).value
         P.bc_type, P.bc_value, P.u = _p(self.bc_type).value, _p(self.bc_value).value, _p(self.u).value
         P.solid_code, P.bb_code = solid_code, bb_code
-        P.kinetics = kinetics_plugin_ptr() if kinetics else None
+        P.kinetics = kinetics_plugin_ptr(library) if kinetics else None
         P.dt, P.nb = dt, nb
         if tableau is not None:
             Sm, lk, cs = tableau
@@ -335,7 +341,7 @@ class AdeRun:
             np.ascontiguousarray(immobile, dtype=np.int32)
         P.immobile = _p(self.immobile).value
         if abiotic:
-            P.abiotic = abiotic_plugin_ptr()
+            P.abiotic = abiotic_plugin_ptr(library)
             P.abiotic_surface = 1 if abiotic == 2 else 0
         if dissolution is not None:
             # (phase ids per voxel, {phase id: immobile solute}, surface_only)
@@ -344,7 +350,7 @@ class AdeRun:
             self.phase_sub = np.full(16, -1, dtype=np.int32)
             for pid, sub in subs.items():
                 self.phase_sub[pid] = sub
-            P.dissol = dissolution_plugin_ptr()
+            P.dissol = dissolution_plugin_ptr(library)
             P.phase, P.phase_sub = _p(self.phase).value, _p(self.phase_sub).value
             P.dissol_surface = 1 if surf else 0
         if surrogate is not None:
