@@ -101,6 +101,7 @@ def load(path=None):
         "clb_set_kinetics": [vp, c_int, c_double, c_int, c_int],
         "clb_set_tableau": [vp, c_int, c_int, vp, vp, vp, c_double, c_int],
         "clb_set_mask_update": [vp, c_int, c_double, c_int, c_int],
+        "clb_set_mask_codes": [vp, c_int, vp, c_int, vp],
         "clb_step": [vp, c_int],
         "clb_speciation_stats": [vp, vp],
         "clb_batch_load": [vp, c_long, vp, c_int, vp, vp],
@@ -570,6 +571,13 @@ class Lattice:
     def set_mask_update(self, enable, threshold, pore_code=2, biofilm_code=3):
         self._chk(self.lib.clb_set_mask_update(self.h, int(enable), threshold, pore_code, biofilm_code),
                   "set_mask_update")
+
+    def set_mask_codes(self, pore_codes, bio_codes):
+        """The pore <-> biofilm rule with several material types (pore_dynamics, and
+        bio_dynamics[i][0] per biomass field i): see clb_set_mask_codes."""
+        p = np.ascontiguousarray(pore_codes, dtype=np.int32)
+        b = np.ascontiguousarray(bio_codes, dtype=np.int32)
+        self._chk(self.lib.clb_set_mask_codes(self.h, p.size, _ptr(p), b.size, _ptr(b)), "set_mask_codes")
 
     def step(self, n=1):
         self._chk(self.lib.clb_step(self.h, n), "step")

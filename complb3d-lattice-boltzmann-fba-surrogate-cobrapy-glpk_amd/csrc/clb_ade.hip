@@ -7,6 +7,7 @@
 // (1 per immobile one), 3 velocity components, 1 mask byte (+ biomass read/write and the
 // post-kinetics concentrations when the separate speciation kernel follows).
 #include "clb_ade_node.h"
+#include "clb_mask_rule.h"
 
 namespace clb {
 
@@ -202,6 +203,34 @@ __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int
   if ((threadIdx.x & 63) == 0 && bal) atomicAdd(changed, (unsigned int)__popcll(bal));
 }
 
+// updateLocalMaskNtotalLattices3D with several pore types, after the step: every solute-fluid
+// voxel of the planes 1 <= x <= nx-2 that is neither solid nor bounce-back decides from prev,
+// a copy of the mask before the pass, as the transport kernel's own rule (one pore type)
+// decides for its voxel.
+template <typename R>
+__global__ void __launch_bounds__(256) mask_rule_kernel(const R* b, int nb, const uint8_t* prev, uint8_t* mask,
+                                                        const uint8_t* kind, Geom g, int solid, int bb, MaskRule r,
+                                                        double thr) {
+  const long long n = blockIdx.x * 256LL + threadIdx.x;
+  if (n >= g.N) return;
+  const int x = (int)(n % g.nx);
+  if (x < 1 || x > g.nx - 2) return;
+  const int m = prev[n];
+  if (m == solid || m == bb || kind[m] != CLB_KIND_FLUID) return;
+  double btot = 0.0;
+  int label = 0;
+  for (int i = 0; i < nb; ++i) {
+    const double v = (double)b[(long long)i * g.N + n];
+    if (v > 1e-14) {
+      btot += v;
+      label += r.bio[i];
+    }
+  }
+  const int y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
+  const int nm = mask_rule_code(r, prev, g, x, y, z, m, btot, label, thr, solid, bb);
+  if (nm != m) mask[n] = (uint8_t)nm;
+}
+
 // Fold the deferred speciation increments into the stored populations (before anything
 // other than the next step reads them). For a next step of parity ODD, every fluid node
 // adds to the slots load7 will read the contributions add_pending would add (pend_r: the
@@ -353,6 +382,13 @@ void launch_node_conversion(const AdeArgs<R>& a, bool odd_next, uint8_t* phase, 
 }
 
 template <typename R>
+void launch_mask_rule(const R* b, int nb, const uint8_t* prev, uint8_t* mask, const uint8_t* kind, const Geom& g,
+                      int solid, int bb, const MaskRule& r, double thr, hipStream_t s) {
+  hipLaunchKernelGGL((mask_rule_kernel<R>), dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, b, nb, prev, mask,
+                     kind, g, solid, bb, r, thr);
+}
+
+template <typename R>
 void launch_ade_reset(const AdeArgs<R>& a, bool next_odd, const double* c0, hipStream_t s) {
   const int bs = 256;
   hipLaunchKernelGGL(ade_reset_kernel<R>, dim3((unsigned)((a.geo.N + bs - 1) / bs)), dim3(bs), 0, s, a, next_odd, c0);
@@ -385,6 +421,10 @@ template void launch_ade_step<float>(const AdeArgs<float>&, bool, dim3, hipStrea
 template void launch_ade_step<double>(const AdeArgs<double>&, bool, dim3, hipStream_t);
 template void launch_ade_conc<float>(const AdeArgs<float>&, bool, double*, hipStream_t);
 template void launch_ade_conc<double>(const AdeArgs<double>&, bool, double*, hipStream_t);
+template void launch_mask_rule<float>(const float*, int, const uint8_t*, uint8_t*, const uint8_t*, const Geom&, int,
+                                      int, const MaskRule&, double, hipStream_t);
+template void launch_mask_rule<double>(const double*, int, const uint8_t*, uint8_t*, const uint8_t*, const Geom&, int,
+                                       int, const MaskRule&, double, hipStream_t);
 template void launch_node_conversion<float>(const AdeArgs<float>&, bool, uint8_t*, int, int, int, int, double,
                                             double, unsigned int*, float*, bool, double, double, hipStream_t);
 template void launch_node_conversion<double>(const AdeArgs<double>&, bool, uint8_t*, int, int, int, int, double,

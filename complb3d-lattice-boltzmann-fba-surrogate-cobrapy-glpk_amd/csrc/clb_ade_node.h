@@ -15,7 +15,7 @@
 //     update_abiotic_rxnLattices (src/complab3d_processors_part1.hh:158-336,
 //     src/precipitationVOP.hh:53-145) with the compiled defineAbioticRxnKinetics hook;
 //   * the biomass threshold reclassification of updateLocalMaskNtotalLattices3D
-//     (src/complab3d_processors_part2.hh:99-210, single pore type).
+//     (src/complab3d_processors_part2.hh:99-210, one pore type).
 // Populations are stored as g_i = f_i - t_i * c_ref[s]. An immobile solute keeps its whole
 // concentration in the rest population, whose AA slot is the node's own in both step
 // parities, so it costs one read and one write per node and never moves.
@@ -281,15 +281,26 @@ __device__ __forceinline__ void ade_node(const AdeArgs<R>& a, int x, int y, int 
     }
   }
   if (react && a.mask_update && a.nb > 0) {
+    // updateLocalMaskNtotalLattices3D with one pore type (several: mask_rule_kernel after
+    // the step): the fields above 1e-14 make the total and the biofilm label
     double btot = 0;
+    int label = 0;
 #pragma unroll
     for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
       if (b < a.nb) {
         const double bv = (double)a.bmass[(long long)b * g.N + n];
-        if (bv > 1e-14) btot += bv;
+        if (bv > 1e-14) {
+          btot += bv;
+          label += a.mrule.bio[b];
+        }
       }
-    if (code == a.pore_code && btot >= a.bthreshold) a.mask[n] = (uint8_t)a.biofilm_code;
-    else if (code == a.biofilm_code && btot < a.bthreshold) a.mask[n] = (uint8_t)a.pore_code;
+    const bool pore = mr_is_pore(a.mrule, code);
+    if (!pore && btot < a.bthreshold) {
+      a.mask[n] = (uint8_t)a.mrule.pore[0];
+    } else if (pore && btot >= a.bthreshold) {
+      const int lab = a.mrule.sum_labels ? label : a.mrule.bio[0];
+      if (lab > 0) a.mask[n] = (uint8_t)lab;
+    }
   }
   if (CBUF && react) {
 #pragma unroll

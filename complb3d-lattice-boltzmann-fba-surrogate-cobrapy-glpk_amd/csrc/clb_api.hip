@@ -30,6 +30,17 @@ struct EventPool {
   size_t used = 0;
 };
 
+// The rule with one pore type and one biofilm code (clb_set_mask_update).
+MaskRule single_rule(int pore_code, int biofilm_code) {
+  MaskRule r{};
+  r.npore = 1;
+  r.pore[0] = pore_code & 255;
+  r.pore_bits[(pore_code >> 6) & 3] |= 1ull << (pore_code & 63);
+  for (int b = 0; b < CLB_MAX_BIOMASS; ++b) r.bio[b] = biofilm_code & 255;
+  r.sum_labels = 0;
+  return r;
+}
+
 struct clb_ctx {
   clb_config cfg{};
   Geom geo{};
@@ -107,6 +118,8 @@ struct clb_ctx {
   int mask_update = 0;
   double bthr = 0;
   int pore_code = 2, biofilm_code = 3;
+  MaskRule mrule = single_rule(2, 3);
+  uint8_t* d_mask_prev = nullptr;  // the mask before a pass of the rule with several pore types
 
   // coupled speciation
   bool have_tableau = false;
@@ -330,10 +343,12 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   a.dt = c->dt;
   a.solid_code = c->solid_code;
   a.bb_code = c->bb_code;
-  a.mask_update = c->mask_update;
+  // several pore types: the rule runs after the step, over a copy of the mask
+  a.mask_update = (c->mask_update && c->mrule.npore <= 1) ? 1 : 0;
   a.bthreshold = c->bthr;
   a.pore_code = c->pore_code;
   a.biofilm_code = c->biofilm_code;
+  a.mrule = c->mrule;
   return a;
 }
 
@@ -474,7 +489,19 @@ bool compact_ok(const clb_ctx* c) {
   if (c->layout != CLB_LAYOUT_COMPACT || !c->ade_ready || c->S < 1) return false;
   if (c->srg_on || c->dissol_on) return false;
   if (c->have_tableau && !fused_spec(c)) return false;
-  if (c->mask_update && c->h_ade_kind[c->pore_code & 255] != c->h_ade_kind[c->biofilm_code & 255]) return false;
+  if (c->mask_update) {
+    // the rule must keep every voxel it relabels a solute-lattice fluid voxel: each pore type
+    // and each biofilm label it can write
+    const MaskRule& r = c->mrule;
+    for (int k = 0; k < r.npore; ++k)
+      if (c->h_ade_kind[r.pore[k] & 255] != CLB_KIND_FLUID) return false;
+    for (int set = 1; set < (1 << CLB_MAX_BIOMASS); ++set) {
+      int lab = 0;
+      for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
+        if ((set >> b) & 1) lab += r.sum_labels ? r.bio[b] : (b == 0 ? r.bio[0] : 0);
+      if (lab > 0 && lab < 256 && c->h_ade_kind[lab] != CLB_KIND_FLUID) return false;
+    }
+  }
   return true;
 }
 
@@ -723,6 +750,17 @@ DissolArgs dissol_args(const clb_ctx* c) {
   return d;
 }
 
+// updateLocalMaskNtotalLattices3D with several pore types, after the step's transport and
+// reactions: every voxel decides from a copy of the mask (the label of a pore neighbour
+// included), so the result does not depend on the order of the voxels.
+template <typename R>
+void enqueue_mask_rule(clb_ctx* c) {
+  if (!c->mask_update || c->nb < 1 || c->mrule.npore <= 1) return;
+  (void)hipMemcpyAsync(c->d_mask_prev, c->d_mask, (size_t)c->geo.N, hipMemcpyDeviceToDevice, c->stream);
+  launch_mask_rule<R>((const R*)c->d_b, c->nb, c->d_mask_prev, c->d_mask, c->d_ade_kind, c->geo, c->solid_code,
+                      c->bb_code, c->mrule, c->bthr, c->stream);
+}
+
 // One coupled step on the compact layout (the state was gathered by prepare_step).
 template <typename R>
 void enqueue_step_compact(clb_ctx* c) {
@@ -763,6 +801,7 @@ void enqueue_step_compact(clb_ctx* c) {
     }
     count_launch(c, 1);
   }
+  enqueue_mask_rule<R>(c);
   c->ade_t++;
 }
 
@@ -814,6 +853,7 @@ void enqueue_step(clb_ctx* c) {
     }
     count_launch(c, 2);
     if (c->dissol_on) launch_dissolution<R>(a, dissol_args(c), odd, c->stream);
+    enqueue_mask_rule<R>(c);
     c->ade_t++;
     return;
   }
@@ -836,6 +876,7 @@ void enqueue_step(clb_ctx* c) {
     count_launch(c, 2);
   }
   if (c->dissol_on) launch_dissolution<R>(a, dissol_args(c), odd, c->stream);
+  enqueue_mask_rule<R>(c);
   c->ade_t++;
 }
 
@@ -1182,7 +1223,7 @@ int clb_destroy(clb_ctx* c) {
                   c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
                   c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared,
                   c->d_caflag, c->d_water, c->d_srg, c->d_srg_inc, c->d_cnode, c->d_cnbr, c->d_cnf, c->d_cg,
-                  c->d_cu, c->d_cpend, c->cb.flag, c->cb.cj, c->cb.lk, c->cb.cg, c->cb.scan_tmp};
+                  c->d_cu, c->d_cpend, c->cb.flag, c->cb.cj, c->cb.lk, c->cb.cg, c->cb.scan_tmp, c->d_mask_prev};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1556,10 +1597,48 @@ int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* l
 int clb_set_mask_update(clb_ctx* c, int enable, double threshold, int pore_code, int biofilm_code) {
   if (c) drop_graph(c);
   if (!c) return CLB_ERR_ARG;
+  if (pore_code < 0 || pore_code > 255 || biofilm_code < 0 || biofilm_code > 255)
+    return fail(c, CLB_ERR_ARG, "material codes are 0..255");
   c->mask_update = enable ? 1 : 0;
   c->bthr = threshold;
   c->pore_code = pore_code;
   c->biofilm_code = biofilm_code;
+  c->mrule = single_rule(pore_code, biofilm_code);
+  c->cmp_dirty = true;
+  return CLB_OK;
+}
+
+int clb_set_mask_codes(clb_ctx* c, int npore, const int* pore_codes, int nbio, const int* bio_codes) {
+  if (c) drop_graph(c);
+  if (!c || npore < 1 || !pore_codes || nbio < 0 || (nbio > 0 && !bio_codes)) return CLB_ERR_ARG;
+  if (npore > CLB_MAX_PORE_CODES || nbio > CLB_MAX_BIOMASS)
+    return fail(c, CLB_ERR_ARG, "at most " + std::to_string(CLB_MAX_PORE_CODES) + " pore types and " +
+                                    std::to_string(CLB_MAX_BIOMASS) + " biofilm codes");
+  MaskRule r{};
+  r.npore = npore;
+  for (int k = 0; k < npore; ++k) {
+    const int m = pore_codes[k];
+    if (m < 0 || m > 255 || m == c->solid_code || m == c->bb_code)
+      return fail(c, CLB_ERR_ARG, "a pore type must be a code 0..255 other than the solid and bounce-back codes");
+    r.pore[k] = m;
+    r.pore_bits[m >> 6] |= 1ull << (m & 63);
+  }
+  int sum = 0;
+  for (int b = 0; b < nbio; ++b) {
+    if (bio_codes[b] < 0 || bio_codes[b] > 255) return fail(c, CLB_ERR_ARG, "biofilm codes are 0..255");
+    r.bio[b] = bio_codes[b];
+    sum += bio_codes[b];
+  }
+  if (sum > 255) return fail(c, CLB_ERR_ARG, "the biofilm codes of all fields together exceed 255");
+  r.sum_labels = 1;
+  if (npore > 1 && !c->d_mask_prev) {
+    int rc = dalloc(c, &c->d_mask_prev, (size_t)c->geo.N);
+    if (rc) return rc;
+  }
+  c->mrule = r;
+  c->pore_code = r.pore[0];
+  c->biofilm_code = r.bio[0];
+  c->cmp_dirty = true;
   return CLB_OK;
 }
 
@@ -2459,8 +2538,14 @@ namespace {
 template <typename R>
 int ca_mask_total(clb_ctx* c, bool fresh, double* mx, long long* changes) {
   launch_fill_u64(c->d_cared, 2, 0ull, c->stream);
-  launch_ca_mask_total<R>((const R*)c->d_b, c->nb, c->d_mask, c->d_btot, c->geo, c->solid_code, c->bb_code,
-                          c->pore_code, c->biofilm_code, c->bthr, fresh, c->d_cared, c->stream);
+  // several pore types: every voxel decides from the mask before the pass
+  const uint8_t* prev = c->d_mask;
+  if (c->mrule.npore > 1) {
+    CLB_CHECK_HIP(c, hipMemcpyAsync(c->d_mask_prev, c->d_mask, (size_t)c->geo.N, hipMemcpyDeviceToDevice, c->stream));
+    prev = c->d_mask_prev;
+  }
+  launch_ca_mask_total<R>((const R*)c->d_b, c->nb, prev, c->d_mask, c->d_btot, c->geo, c->solid_code, c->bb_code,
+                          c->mrule, c->bthr, fresh, c->d_cared, c->stream);
   unsigned long long h[2];
   CLB_CHECK_HIP(c, hipMemcpyAsync(h, c->d_cared, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
@@ -2523,7 +2608,7 @@ int ca_redistribute_t(clb_ctx* c, double bmax, int half, unsigned long long seed
   for (int whilecount = 0; mx - bmax > thrd; ++whilecount) {
     CLB_CHECK_HIP(c, hipMemcpyAsync(c->d_bold, c->d_b, bbytes, hipMemcpyDeviceToDevice, c->stream));
     launch_ca_push(c->d_mask, c->d_btot, c->d_age, c->d_caout, c->d_caflag, c->geo, c->solid_code, c->bb_code,
-                   c->pore_code, bmax, half, seed, c->ca_sweep, c->stream);
+                   c->mrule, bmax, half, seed, c->ca_sweep, c->stream);
     launch_ca_gather<R>((const R*)c->d_bold, (R*)c->d_b, c->nb, c->d_mask, c->d_btot, c->d_caout, c->d_caflag,
                         c->geo, c->solid_code, c->bb_code, c->stream);
     ++c->ca_sweep;
@@ -2612,7 +2697,7 @@ int clb_ca_init(clb_ctx* c) {
   if (!rc) rc = dalloc(c, &c->d_btot, N);
   if (!rc && !c->d_cared) rc = dalloc(c, &c->d_cared, (size_t)4);
   if (rc) return rc;
-  launch_ca_init(c->d_mask, c->d_age, c->d_dpos, c->geo, c->solid_code, c->bb_code, c->pore_code, c->stream);
+  launch_ca_init(c->d_mask, c->d_age, c->d_dpos, c->geo, c->solid_code, c->bb_code, c->mrule, c->stream);
   launch_fill_u64((unsigned long long*)c->d_btot, (long long)N, 0ull, c->stream);
   CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   CLB_CHECK_HIP(c, hipGetLastError());

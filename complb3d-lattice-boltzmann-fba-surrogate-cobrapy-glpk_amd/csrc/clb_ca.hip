@@ -23,6 +23,7 @@
 // FP contraction is off in this file: the CPU restatement reproduces every operation, so
 // with FP64 storage the two agree bit for bit.
 #include "clb_internal.h"
+#include "clb_mask_rule.h"
 
 namespace clb {
 namespace {
@@ -56,40 +57,45 @@ __device__ __forceinline__ long long ca_nbr(const Geom& g, int x, int y, int z, 
 
 // Total biomass and mask (updateLocalMaskNtotalLattices3D): bmass = sum of species above
 // 1e-14; the stored total follows it when they differ by more than 1e-14 (always at
-// `fresh`). Pore -> biofilm at bmass >= bthr, biofilm -> pore below it, on the interior
-// planes 1 <= x <= nx-2 (as the transport kernel's mask rule). red[0]: max total over those
-// planes (bits; the inlet / outlet planes take no part in the CA), red[1]: mask changes.
+// `fresh`). The pore <-> biofilm rule (mask_rule_code) on the interior planes
+// 1 <= x <= nx-2 (as the transport kernel's mask rule), each voxel deciding from prev (the
+// mask itself with one pore type, a copy of it taken before the pass with several).
+// red[0]: max total over those planes (bits; the inlet / outlet planes take no part in the
+// CA), red[1]: mask changes.
 // Grid-stride over a fixed grid with one atomic per block: per-wavefront atomics on one
 // address serialise (65 k of them took 0.8 ms at 4 M voxels).
 template <typename R>
-__global__ void __launch_bounds__(256) ca_mask_total_kernel(const R* b, int nb, uint8_t* mask, double* btot, Geom g,
-                                                            int solid, int bb, int pore, int biofilm, double bthr,
-                                                            bool fresh, unsigned long long* red) {
+__global__ void __launch_bounds__(256) ca_mask_total_kernel(const R* b, int nb, const uint8_t* prev, uint8_t* mask,
+                                                            double* btot, Geom g, int solid, int bb, MaskRule r,
+                                                            double bthr, bool fresh, unsigned long long* red) {
   __shared__ unsigned long long smax[4];
   __shared__ unsigned int schg[4];
   unsigned long long bits = 0;  // max over this thread's voxels (non-negative doubles order as bits)
   unsigned int changed = 0;
   for (long long n = blockIdx.x * 256LL + threadIdx.x; n < g.N; n += (long long)gridDim.x * 256) {
-    const int m = mask[n];
+    const int m = prev[n];
     const int xn = (int)(n % g.nx);
     const bool inner = xn >= 1 && xn <= g.nx - 2;
     double t = btot[n];
     if (m != solid && m != bb) {
       double bm = 0.0;
+      int label = 0;
       for (int i = 0; i < nb; ++i) {
         const double v = (double)b[(long long)i * g.N + n];
-        if (v > kCaThrdMask) bm += v;
+        if (v > kCaThrdMask) {
+          bm += v;
+          label += r.bio[i];
+        }
       }
       if (fresh || fabs(bm - t) > kCaThrdMask) {
         t = bm;
         btot[n] = bm;
       }
       if (inner) {
-        if (m == pore && bm >= bthr) {
-          mask[n] = (uint8_t)biofilm;
-          ++changed;
-        } else if (m == biofilm && bm < bthr) {
-          mask[n] = (uint8_t)pore;
+        const int y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
+        const int nm = mask_rule_code(r, prev, g, xn, y, z, m, bm, label, bthr, solid, bb);
+        if (nm != m) {
+          mask[n] = (uint8_t)nm;
           ++changed;
         }
       }
@@ -128,7 +134,7 @@ __global__ void __launch_bounds__(256) ca_mask_total_kernel(const R* b, int nb, 
 // half: halfPushExcessBiomass3D (biofilm voxels send half their biomass, the age rule
 // accepts neighbours of equal age) instead of pushExcessBiomass3D (the excess over Bmax).
 __global__ void ca_push_kernel(const uint8_t* mask, const double* btot, const int* age, double* out, uint8_t* flags,
-                               Geom g, int solid, int bb, int pore, double bmax, int half, unsigned long long seed,
+                               Geom g, int solid, int bb, MaskRule r, double bmax, int half, unsigned long long seed,
                                unsigned long long sweep) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= g.N) return;
@@ -137,7 +143,7 @@ __global__ void ca_push_kernel(const uint8_t* mask, const double* btot, const in
   const int m = mask[n];
   const double T = btot[n];
   bool push = x >= 1 && x <= g.nx - 2 && m != solid && m != bb && T > bmax;
-  if (half) push = push && m != pore;
+  if (half) push = push && !mr_is_pore(r, m);
   double bMd = half ? T * 0.5 : T - bmax;
   if (!half) push = push && bMd > kCaThrd;
   if (push) {
@@ -164,7 +170,7 @@ __global__ void ca_push_kernel(const uint8_t* mask, const double* btot, const in
       for (int t = 0; t < nk && !done; ++t) {
         const int k = idx[t];
         const long long q = nn[k];
-        const double nt = (mask[q] == pore) ? 0.0 : btot[q];
+        const double nt = mr_is_pore(r, mask[q]) ? 0.0 : btot[q];
         if (nt < bmax) {
           const double hold = bmax - nt;
           double part;
@@ -298,12 +304,12 @@ __global__ void __launch_bounds__(256) ca_age_kernel(const int* age_in, int* age
 // createAgeDomain3D (-1 wall, 0 pore, 1 other) and the distance field's sign (the
 // distance is -1 on solid, 0 on bounce-back, positive elsewhere: calculateDistanceFromSolid,
 // src/complab_functions.hh:169-230).
-__global__ void ca_init_kernel(const uint8_t* mask, int* age, uint8_t* dpos, Geom g, int solid, int bb, int pore) {
+__global__ void ca_init_kernel(const uint8_t* mask, int* age, uint8_t* dpos, Geom g, int solid, int bb, MaskRule r) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= g.N) return;
   const int m = mask[n];
   const bool wall = m == solid || m == bb;
-  age[n] = wall ? -1 : (m == pore ? 0 : 1);
+  age[n] = wall ? -1 : (mr_is_pore(r, m) ? 0 : 1);
   dpos[n] = wall ? 0 : 1;
 }
 
@@ -373,23 +379,24 @@ inline unsigned reduce_blocks(long long N) {
 
 }  // namespace
 
-void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, int pore,
+void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, const MaskRule& r,
                     hipStream_t s) {
-  hipLaunchKernelGGL(ca_init_kernel, dim3(blocks_of(g.N)), dim3(256), 0, s, mask, age, dpos, g, solid, bb, pore);
+  hipLaunchKernelGGL(ca_init_kernel, dim3(blocks_of(g.N)), dim3(256), 0, s, mask, age, dpos, g, solid, bb, r);
 }
 
 template <typename R>
-void launch_ca_mask_total(const R* b, int nb, uint8_t* mask, double* btot, const Geom& g, int solid, int bb, int pore,
-                          int biofilm, double bthr, bool fresh, unsigned long long* red, hipStream_t s) {
-  hipLaunchKernelGGL((ca_mask_total_kernel<R>), dim3(reduce_blocks(g.N)), dim3(256), 0, s, b, nb, mask, btot, g,
-                     solid, bb, pore, biofilm, bthr, fresh, red);
+void launch_ca_mask_total(const R* b, int nb, const uint8_t* prev, uint8_t* mask, double* btot, const Geom& g,
+                          int solid, int bb, const MaskRule& r, double bthr, bool fresh, unsigned long long* red,
+                          hipStream_t s) {
+  hipLaunchKernelGGL((ca_mask_total_kernel<R>), dim3(reduce_blocks(g.N)), dim3(256), 0, s, b, nb, prev, mask, btot,
+                     g, solid, bb, r, bthr, fresh, red);
 }
 
 void launch_ca_push(const uint8_t* mask, const double* btot, const int* age, double* out, uint8_t* flags,
-                    const Geom& g, int solid, int bb, int pore, double bmax, int half, unsigned long long seed,
-                    unsigned long long sweep, hipStream_t s) {
+                    const Geom& g, int solid, int bb, const MaskRule& r, double bmax, int half,
+                    unsigned long long seed, unsigned long long sweep, hipStream_t s) {
   hipLaunchKernelGGL(ca_push_kernel, dim3(blocks_of(g.N)), dim3(256), 0, s, mask, btot, age, out, flags, g, solid, bb,
-                     pore, bmax, half, seed, sweep);
+                     r, bmax, half, seed, sweep);
 }
 
 template <typename R>
@@ -413,10 +420,10 @@ void launch_fd_biomass(const R* b_old, R* b, int nb, const uint8_t* mask, const 
 template void launch_fd_biomass<float>(const float*, float*, int, const uint8_t*, const Geom&, double, hipStream_t);
 template void launch_fd_biomass<double>(const double*, double*, int, const uint8_t*, const Geom&, double,
                                         hipStream_t);
-template void launch_ca_mask_total<float>(const float*, int, uint8_t*, double*, const Geom&, int, int, int, int,
-                                          double, bool, unsigned long long*, hipStream_t);
-template void launch_ca_mask_total<double>(const double*, int, uint8_t*, double*, const Geom&, int, int, int, int,
-                                           double, bool, unsigned long long*, hipStream_t);
+template void launch_ca_mask_total<float>(const float*, int, const uint8_t*, uint8_t*, double*, const Geom&, int, int,
+                                          const MaskRule&, double, bool, unsigned long long*, hipStream_t);
+template void launch_ca_mask_total<double>(const double*, int, const uint8_t*, uint8_t*, double*, const Geom&, int,
+                                           int, const MaskRule&, double, bool, unsigned long long*, hipStream_t);
 template void launch_ca_gather<float>(const float*, float*, int, const uint8_t*, const double*, const double*,
                                       const uint8_t*, const Geom&, int, int, hipStream_t);
 template void launch_ca_gather<double>(const double*, double*, int, const uint8_t*, const double*, const double*,

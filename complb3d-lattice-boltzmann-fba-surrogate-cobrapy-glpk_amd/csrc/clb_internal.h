@@ -46,6 +46,23 @@ struct FlowArgs {
   double* energy;         // [0..1] sum |u|^2/2 and count; [2 + 2b..] per-block partials; or null
 };
 
+// The pore <-> biofilm rule of updateLocalMaskNtotalLattices3D
+// (src/complab3d_processors_part2.hh:99-210): the pore types (pore[0] the type a biofilm voxel
+// returns to by default), each biomass field's biofilm code, and whether the biofilm label is
+// the sum of the codes of the fields present (the reference's rule) or bio[0] (one biofilm
+// code).
+struct MaskRule {
+  unsigned long long pore_bits[4];  // bit m: code m is a pore type
+  int npore;
+  int pore[CLB_MAX_PORE_CODES];
+  int bio[CLB_MAX_BIOMASS];
+  int sum_labels;
+};
+
+__host__ __device__ inline bool mr_is_pore(const MaskRule& r, int m) {
+  return ((r.pore_bits[(m >> 6) & 3] >> (m & 63)) & 1ull) != 0;
+}
+
 template <typename R>
 struct AdeArgs {
   R* g;                   // S*7*N population-major; stored as f_i - t_i*c_ref[s]
@@ -89,9 +106,10 @@ struct AdeArgs {
   int abiotic;            // 0 none, 1 plugin everywhere, 2 plugin at pore voxels next to a wall
   double dt;
   int solid_code, bb_code;
-  int mask_update;
+  int mask_update;        // the rule inside the transport kernel (one pore type only)
   double bthreshold;
   int pore_code, biofilm_code;
+  MaskRule mrule;
 };
 
 struct Tableau {
@@ -145,6 +163,9 @@ void launch_flow_velocity(const FlowArgs<R>& a, bool odd_next, double* u, double
 template <typename R>
 void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c_ref, const int* immobile,
                      hipStream_t s);
+template <typename R>
+void launch_mask_rule(const R* b, int nb, const uint8_t* prev, uint8_t* mask, const uint8_t* kind, const Geom& g,
+                      int solid, int bb, const MaskRule& r, double thr, hipStream_t s);
 template <typename R>
 void launch_node_conversion(const AdeArgs<R>& a, bool odd_next, uint8_t* phase, int ms, int pore_code,
                             int precip_code, int phase_id, double precip_max, double reopen, unsigned int* changed,
@@ -274,14 +295,15 @@ void launch_surrogate(const AdeArgs<R>& a, const SrgTables* t, int nm, int lds_l
                       hipStream_t s);
 
 // CA biomass redistribution (clb_ca.hip)
-void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, int pore,
+void launch_ca_init(const uint8_t* mask, int* age, uint8_t* dpos, const Geom& g, int solid, int bb, const MaskRule& r,
                     hipStream_t s);
 template <typename R>
-void launch_ca_mask_total(const R* b, int nb, uint8_t* mask, double* btot, const Geom& g, int solid, int bb, int pore,
-                          int biofilm, double bthr, bool fresh, unsigned long long* red, hipStream_t s);
+void launch_ca_mask_total(const R* b, int nb, const uint8_t* prev, uint8_t* mask, double* btot, const Geom& g,
+                          int solid, int bb, const MaskRule& r, double bthr, bool fresh, unsigned long long* red,
+                          hipStream_t s);
 void launch_ca_push(const uint8_t* mask, const double* btot, const int* age, double* out, uint8_t* flags,
-                    const Geom& g, int solid, int bb, int pore, double bmax, int half, unsigned long long seed,
-                    unsigned long long sweep, hipStream_t s);
+                    const Geom& g, int solid, int bb, const MaskRule& r, double bmax, int half,
+                    unsigned long long seed, unsigned long long sweep, hipStream_t s);
 template <typename R>
 void launch_ca_gather(const R* b_old, R* b, int nb, const uint8_t* mask, const double* btot, const double* out,
                       const uint8_t* flags, const Geom& g, int solid, int bb, hipStream_t s);

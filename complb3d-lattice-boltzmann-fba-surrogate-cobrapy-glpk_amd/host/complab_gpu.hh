@@ -219,6 +219,13 @@ class Lattice {
   void setMaskUpdate(bool enable, double threshold, int poreCode, int biofilmCode) {
     chk(clb_set_mask_update(c_, enable ? 1 : 0, threshold, poreCode, biofilmCode), "clb_set_mask_update");
   }
+  // updateLocalMaskNtotalLattices3D with several material types: pore_dynamics and
+  // bio_dynamics[i][0] of each microbe (after setMaskUpdate)
+  void setMaskCodes(const std::vector<int>& poreCodes, const std::vector<int>& bioCodes) {
+    chk(clb_set_mask_codes(c_, (int)poreCodes.size(), poreCodes.data(), (int)bioCodes.size(),
+                           bioCodes.empty() ? nullptr : bioCodes.data()),
+        "clb_set_mask_codes");
+  }
   void setPhase(const std::vector<uint8_t>& ph) { need(ph.size(), nodes(), "phase"); chk(clb_set_phase(c_, ph.data()), "clb_set_phase"); }
   long long nodeConversion(int mineralSolute, int poreCode, int precipCode, int phaseId, double precipMax,
                            double reopen) {

@@ -54,6 +54,7 @@ extern "C" {
 #define CLB_MAX_BIOMASS 4
 #define CLB_MAX_COMPONENTS 8
 #define CLB_MAX_SPECIES 32
+#define CLB_MAX_PORE_CODES 8
 
 typedef struct clb_ctx clb_ctx;
 
@@ -127,11 +128,26 @@ int clb_download_biomass(clb_ctx* ctx, double* b /*nb*N*/);
  *   index named like component j, or -1. Solved by the FP64 Newton kernel to the
  *   reference's convergence judge (PCF residual 2-norm < tol).
  * clb_set_mask_update: pore <-> biofilm reclassification on total biomass
- *   (updateLocalMaskNtotalLattices3D, src/complab3d_processors_part2.hh:99-210). */
+ *   (updateLocalMaskNtotalLattices3D, src/complab3d_processors_part2.hh:99-210) with one pore
+ *   code and one biofilm code: a pore voxel whose biomass (the fields above 1e-14) reaches
+ *   threshold becomes biofilm_code; any other voxel that is neither solid nor bounce-back
+ *   becomes pore_code below it. On the planes 1 <= x <= nx-2.
+ * clb_set_mask_codes: the same rule with the reference's several material types, after
+ *   clb_set_mask_update (which it keeps the switch and threshold of): pore_codes[0..npore)
+ *   are the pore types (pore_dynamics; pore_codes[0] the default), bio_codes[i] the biofilm
+ *   code of biomass field i (bio_dynamics[i][0]; fields beyond nbio count 0). A pore-type
+ *   voxel whose biomass reaches the threshold takes the sum of the codes of the fields
+ *   present there (src/complab3d_processors_part2.hh:117-125, 190-198); a voxel of no pore
+ *   type drops below it to pore_codes[0], or with several pore types to the type of its first
+ *   face neighbour of a pore type in the order +x -x +y -y +z -z, no wrap (:141-186). Every
+ *   voxel decides from the mask before the pass (the reference's in-place loop lets a voxel
+ *   see the new type of an already visited neighbour; its MPI blocks change which). Every
+ *   sum of bio_codes must stay within 1..255. The CA (clb_ca_*) uses the same pore types. */
 int clb_set_kinetics(clb_ctx* ctx, int kind, double dt, int solid_code, int bb_code);
 int clb_set_tableau(clb_ctx* ctx, int nc, int ns, const double* S, const double* logK, const int* comp_species,
                     double tol, int max_iter);
 int clb_set_mask_update(clb_ctx* ctx, int enable, double threshold, int pore_code, int biofilm_code);
+int clb_set_mask_codes(clb_ctx* ctx, int npore, const int* pore_codes, int nbio, const int* bio_codes);
 int clb_step(clb_ctx* ctx, int nsteps);                       /* collide -> kinetics -> speciation -> stream */
 int clb_speciation_stats(clb_ctx* ctx, long long* stats /*3: solves, converged, iterations*/);
 /* Where the coupled step solves the speciation (no reference counterpart: an execution

@@ -16,6 +16,8 @@
 #include <cstring>
 #include <vector>
 
+#include "mask_rule.h"
+
 #define ORC_API extern "C" __attribute__((visibility("default")))
 
 namespace {
@@ -50,32 +52,41 @@ struct Box {
 };
 
 struct Codes {
-  int solid, bb, pore, biofilm;
+  int solid, bb;
+  OrcMaskRule rule;
   bool wall(int m) const { return m == solid || m == bb; }
+  bool pore(int m) const { return rule.is_pore(m); }
 };
 
 // updateLocalMaskNtotalLattices3D; returns the max total over the planes 1 <= x <= nx-2,
-// adds mask flips to *flips.
+// adds mask flips to *flips. With several pore types every voxel decides from the mask
+// before the pass.
 double mask_total(const Box& g, const Codes& k, int nb, const double* b, uint8_t* mask, double* btot, double bthr,
                   bool fresh, long long* flips) {
   double mx = 0.0;
   bool nan = false;
+  std::vector<uint8_t> snapshot;
+  if (k.rule.npore > 1) snapshot.assign(mask, mask + g.N);
+  const uint8_t* prev = k.rule.npore > 1 ? snapshot.data() : mask;
   for (long n = 0; n < g.N; ++n) {
-    const int m = mask[n];
+    const int m = prev[n];
     if (!k.wall(m)) {
       double bm = 0.0;
+      int label = 0;
       for (int i = 0; i < nb; ++i) {
         const double v = b[(long)i * g.N + n];
-        if (v > kThrdMask) bm += v;
+        if (v > kThrdMask) {
+          bm += v;
+          label += (i < 4) ? k.rule.bio[i] : 0;
+        }
       }
       if (fresh || std::fabs(bm - btot[n]) > kThrdMask) btot[n] = bm;
       const int x = (int)(n % g.nx);
       if (x >= 1 && x <= g.nx - 2) {
-        if (m == k.pore && bm >= bthr) {
-          mask[n] = (uint8_t)k.biofilm;
-          ++*flips;
-        } else if (m == k.biofilm && bm < bthr) {
-          mask[n] = (uint8_t)k.pore;
+        const int y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long)g.nx * g.ny));
+        const int nm = orc_mask_rule(k.rule, prev, g.nx, g.ny, g.nz, x, y, z, m, bm, label, bthr, k.solid, k.bb);
+        if (nm != m) {
+          mask[n] = (uint8_t)nm;
           ++*flips;
         }
       }
@@ -97,7 +108,7 @@ void push_decide(const Box& g, const Codes& k, const uint8_t* mask, const double
     const int x = (int)(n % g.nx);
     const int m = mask[n];
     if (x < 1 || x > g.nx - 2 || k.wall(m)) continue;
-    if (half && m == k.pore) continue;
+    if (half && k.pore(m)) continue;
     const double T = btot[n];
     if (!(T > bmax)) continue;
     double rest = half ? T * 0.5 : T - bmax;
@@ -123,7 +134,7 @@ void push_decide(const Box& g, const Codes& k, const uint8_t* mask, const double
     bool placed = false;
     for (int t = 0; t < nk && !placed; ++t) {
       const int s = order[t];
-      const double room_used = (mask[nbrs[s]] == k.pore) ? 0.0 : btot[nbrs[s]];
+      const double room_used = k.pore(mask[nbrs[s]]) ? 0.0 : btot[nbrs[s]];
       if (room_used < bmax) {
         const double room = bmax - room_used;
         double part;
@@ -233,11 +244,14 @@ int age_relax(const Box& g, int* age, const uint8_t* dpos, const double* btot, d
 }  // namespace
 
 // createAgeDomain3D and the distance field's sign, from the mask.
-ORC_API void orc_ca_init(long N, const uint8_t* mask, int solid, int bb, int pore, int* age, uint8_t* dpos) {
+// rule: {npore, pore[8], bio[4], sum_labels} of several material types, or null (pore_code).
+ORC_API void orc_ca_init(long N, const uint8_t* mask, int solid, int bb, int pore, int* age, uint8_t* dpos,
+                         const int* rule) {
+  const OrcMaskRule r = OrcMaskRule::from(rule, pore, pore);
   for (long n = 0; n < N; ++n) {
     const int m = mask[n];
     const bool wall = m == solid || m == bb;
-    age[n] = wall ? -1 : (m == pore ? 0 : 1);
+    age[n] = wall ? -1 : (r.is_pore(m) ? 0 : 1);
     dpos[n] = wall ? 0 : 1;
   }
 }
@@ -249,9 +263,9 @@ ORC_API void orc_ca_init(long N, const uint8_t* mask, int solid, int bb, int por
 ORC_API void orc_ca_redistribute(int nx, int ny, int nz, int nb, double* b, uint8_t* mask, int* age,
                                  const uint8_t* dpos, double* btot, int solid, int bb, int pore, int biofilm,
                                  double bthr, double bmax, int half, uint64_t seed, uint64_t sweep0, long long* rep,
-                                 double* mx) {
+                                 double* mx, const int* rule) {
   const Box g{nx, ny, nz, (long)nx * ny * nz};
-  const Codes k{solid, bb, pore, biofilm};
+  const Codes k{solid, bb, OrcMaskRule::from(rule, pore, biofilm)};
   for (int i = 0; i < 4; ++i) rep[i] = 0;
   long long flips = 0;
   double m = mask_total(g, k, nb, b, mask, btot, bthr, true, &flips);
@@ -351,9 +365,10 @@ ORC_API int orc_ca_update_age(int nx, int ny, int nz, int* age, const uint8_t* d
 // The FD biofilm solver's block (src/complab.cpp:1716-1731): mask and totals, the copy, one
 // fdDiffusion3D step, mask and totals. rep[0] mask flips; mx[0] max total before, mx[1] after.
 ORC_API void orc_fd_biomass(int nx, int ny, int nz, int nb, double* b, uint8_t* mask, double* btot, int solid, int bb,
-                            int pore, int biofilm, double bthr, double nu, long long* rep, double* mx) {
+                            int pore, int biofilm, double bthr, double nu, long long* rep, double* mx,
+                            const int* rule) {
   const Box g{nx, ny, nz, (long)nx * ny * nz};
-  const Codes k{solid, bb, pore, biofilm};
+  const Codes k{solid, bb, OrcMaskRule::from(rule, pore, biofilm)};
   long long flips = 0;
   mx[0] = mask_total(g, k, nb, b, mask, btot, bthr, false, &flips);
   std::vector<double> copy(b, b + (long)nb * g.N);

@@ -47,6 +47,8 @@
 #include <limits>
 #include <vector>
 
+#include "mask_rule.h"
+
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -655,6 +657,9 @@ struct orc_ade_params {
   int dissol_surface;        // 1: enclosed phase voxels do not react
   // surrogate metabolic processor (run_surrogate3D), null: none
   const orc_srg* srg;
+  // the mask rule with several material types: {npore, pore[8], bio[4], sum_labels}; npore 0
+  // (or mask_rule null): one pore type pore_code, one biofilm code biofilm_code
+  const int* mask_rule;
 };
 
 // surfaceDissolutionKinetics3D (src/dissolutionVOP.hh:229-459) over the post-collision
@@ -747,6 +752,7 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
   const long plane = (long)ny * nz;
   std::vector<double> gtmp((size_t)S * 7 * N);
   Tableau tb{P->nc, S, P->eqS, P->eqLogK, P->eqCompSpecies};
+  const OrcMaskRule rule = OrcMaskRule::from(P->mask_rule, P->pore_code, P->biofilm_code);
   // with dissolution the step has three passes: update every voxel, dissolve, stream
   const bool dissol = P->dissol != nullptr;
   std::vector<double> post(dissol ? (size_t)S * 7 * N : 0), water(dissol ? (size_t)S * N : 0);
@@ -907,15 +913,21 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
                   for (int i = 0; i < 7; ++i) p[s * 7 + i] += dC * W7[i];
               }
             }
-            // ---- biomass threshold reclassification (single pore type, no CA)
-            if (react && P->mask_update && P->nb > 0) {
+            // ---- biomass threshold reclassification in the voxel's own update (one pore type;
+            // several: a pass over a copy of the mask after the step, below)
+            if (react && P->mask_update && P->nb > 0 && rule.npore <= 1) {
               double tot = 0;
+              int label = 0;
               for (int b = 0; b < P->nb; ++b) {
                 double bv = bmass[(long)b * N + n];
-                if (bv > 1e-14) tot += bv;
+                if (bv > 1e-14) {
+                  tot += bv;
+                  label += (b < 4) ? rule.bio[b] : 0;
+                }
               }
-              if (code == P->pore_code && tot >= P->bmass_threshold) mask[n] = (uint8_t)P->biofilm_code;
-              else if (code == P->biofilm_code && tot < P->bmass_threshold) mask[n] = (uint8_t)P->pore_code;
+              const int nm = orc_mask_rule(rule, mask, nx, ny, nz, x, y, z, code, tot, label, P->bmass_threshold,
+                                           P->solid_code, P->bb_code);
+              if (nm != code) mask[n] = (uint8_t)nm;
             }
           } else if (k == KIND_BB) {
             for (int s = 0; s < S; ++s) {
@@ -943,6 +955,30 @@ ORC_API void orc_ade_run(const orc_ade_params* P, uint8_t* mask, double* g, doub
           }
     }
     std::memcpy(g, gtmp.data(), sizeof(double) * S * 7 * N);
+    // ---- the mask rule with several pore types: every fluid voxel of 1 <= x <= nx-2 that is
+    // neither solid nor bounce-back decides from the mask before this pass
+    if (P->mask_update && P->nb > 0 && rule.npore > 1) {
+      const std::vector<uint8_t> prev(mask, mask + N);
+      for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+          for (int x = 1; x <= nx - 2; ++x) {
+            const long n = gr.idx(x, y, z);
+            const int m = prev[n];
+            if (m == P->solid_code || m == P->bb_code || P->kind[m] != KIND_FLUID) continue;
+            double tot = 0;
+            int label = 0;
+            for (int b = 0; b < P->nb; ++b) {
+              double bv = bmass[(long)b * N + n];
+              if (bv > 1e-14) {
+                tot += bv;
+                label += (b < 4) ? rule.bio[b] : 0;
+              }
+            }
+            const int nm = orc_mask_rule(rule, prev.data(), nx, ny, nz, x, y, z, m, tot, label, P->bmass_threshold,
+                                         P->solid_code, P->bb_code);
+            if (nm != m) mask[n] = (uint8_t)nm;
+          }
+    }
     // ---- Neumann faces: density of the adjacent interior plane after stream
     for (int s = 0; s < S; ++s)
       for (int side = 0; side < 2; ++side) {

@@ -42,11 +42,11 @@ def load(path=None):
         "orc_ade_density": ([cl, ci, vp, vp], None),
         "orc_ade_run": ([vp, vp, vp, vp, vp, ci, vp], None),
         "orc_node_conversion": ([cl, vp, vp, vp, ci, ci, ci, ci, ci, cd, cd], cl),
-        "orc_ca_init": ([cl, vp, ci, ci, ci, vp, vp], None),
+        "orc_ca_init": ([cl, vp, ci, ci, ci, vp, vp, vp], None),
         "orc_ca_redistribute": ([ci, ci, ci, ci, vp, vp, vp, vp, vp, ci, ci, ci, ci, cd, cd, ci,
-                                 ctypes.c_uint64, ctypes.c_uint64, vp, vp], None),
+                                 ctypes.c_uint64, ctypes.c_uint64, vp, vp, vp], None),
         "orc_ca_update_age": ([ci, ci, ci, vp, vp, vp, cd, ci], ci),
-        "orc_fd_biomass": ([ci, ci, ci, ci, vp, vp, vp, ci, ci, ci, ci, cd, cd, vp, vp], None),
+        "orc_fd_biomass": ([ci, ci, ci, ci, vp, vp, vp, ci, ci, ci, ci, cd, cd, vp, vp, vp], None),
         "orc_classify_solid": ([ci, ci, ci, vp, vp, ci, ci], None),
         "orc_mlp_forward": ([ci, ci, ci, vp, cl, vp, vp], None),
         "orc_mlp_grad": ([ci, ci, ci, vp, cl, vp, vp, vp], cd),
@@ -87,7 +87,19 @@ class AdeParams(ctypes.Structure):
         ("dissol", ctypes.c_void_p), ("phase", ctypes.c_void_p), ("phase_sub", ctypes.c_void_p),
         ("dissol_surface", ctypes.c_int),
         ("srg", ctypes.c_void_p),
+        ("mask_rule", ctypes.c_void_p),
     ]
+
+
+def mask_rule(pore_codes, bio_codes, sum_labels=True):
+    """The rule of several material types for the oracle: {npore, pore[8], bio[4], sum_labels}
+    (int32), as the step and the CA functions take it."""
+    v = np.zeros(14, dtype=np.int32)
+    v[0] = len(pore_codes)
+    v[1:1 + len(pore_codes)] = pore_codes
+    v[9:9 + len(bio_codes)] = bio_codes
+    v[13] = 1 if sum_labels else 0
+    return v
 
 
 class SrgNetParams(ctypes.Structure):
@@ -244,13 +256,14 @@ def node_conversion(mask, phase, mineral, pore_code, solid_code, bb_code, precip
     return m, ph, n
 
 
-def ca_init(mask, solid=0, bb=1, pore=2):
-    """createAgeDomain3D and the distance field's sign: (age int32, dpos uint8)."""
+def ca_init(mask, solid=0, bb=1, pore=2, rule=None):
+    """createAgeDomain3D and the distance field's sign: (age int32, dpos uint8). rule: a
+    mask_rule(...) of several pore types, or None (pore)."""
     lib = load()
     m = np.ascontiguousarray(mask, dtype=np.uint8).ravel()
     age = np.zeros(m.size, dtype=np.int32)
     dpos = np.zeros(m.size, dtype=np.uint8)
-    lib.orc_ca_init(m.size, _p(m), solid, bb, pore, _p(age), _p(dpos))
+    lib.orc_ca_init(m.size, _p(m), solid, bb, pore, _p(age), _p(dpos), None if rule is None else _p(rule))
     return age, dpos
 
 
@@ -258,11 +271,12 @@ class CaState:
     """The CA's lattices for the oracle: biomass (nb, N) FP64, mask, age, distance sign and
     total biomass, advanced in place by redistribute()."""
 
-    def __init__(self, nx, ny, nz, b, mask, solid=0, bb=1, pore=2, biofilm=3, bthr=10.0):
+    def __init__(self, nx, ny, nz, b, mask, solid=0, bb=1, pore=2, biofilm=3, bthr=10.0, rule=None):
         self.nx, self.ny, self.nz = nx, ny, nz
         self.b = np.ascontiguousarray(np.atleast_2d(b), dtype=np.float64).copy()
         self.mask = np.ascontiguousarray(mask, dtype=np.uint8).ravel().copy()
-        self.age, self.dpos = ca_init(self.mask, solid, bb, pore)
+        self.rule = rule
+        self.age, self.dpos = ca_init(self.mask, solid, bb, pore, rule)
         self.btot = np.zeros(self.mask.size)
         self.codes = (solid, bb, pore, biofilm)
         self.bthr = bthr
@@ -276,7 +290,7 @@ class CaState:
         s, bb, p, bf = self.codes
         lib.orc_ca_redistribute(self.nx, self.ny, self.nz, self.b.shape[0], _p(self.b), _p(self.mask), _p(self.age),
                                 _p(self.dpos), _p(self.btot), s, bb, p, bf, self.bthr, bmax, half, seed, self.sweep,
-                                _p(rep), _p(mx))
+                                _p(rep), _p(mx), None if self.rule is None else _p(self.rule))
         self.sweep += int(rep[0])
         return int(rep[0]), int(rep[1]), int(rep[2]), int(rep[3]), float(mx[0]), float(mx[1])
 
@@ -287,7 +301,8 @@ class CaState:
         mx = np.zeros(2)
         s, bb, p, bf = self.codes
         load().orc_fd_biomass(self.nx, self.ny, self.nz, self.b.shape[0], _p(self.b), _p(self.mask), _p(self.btot),
-                              s, bb, p, bf, self.bthr, nu, _p(rep), _p(mx))
+                              s, bb, p, bf, self.bthr, nu, _p(rep), _p(mx),
+                              None if self.rule is None else _p(self.rule))
         return int(rep[0]), float(mx[0]), float(mx[1])
 
     def update_age(self, bmax, max_passes=1000):
@@ -300,7 +315,8 @@ class AdeRun:
 
     def __init__(self, nx, ny, nz, mask, kind, omega, bc_type, bc_value, u, c0, solid_code=0, bb_code=1,
                  kinetics=False, dt=1.0, biomass=None, tableau=None, eq_tol=1e-10, eq_max_iter=200,
-                 mask_update=None, immobile=None, abiotic=0, dissolution=None, surrogate=None, library=None):
+                 mask_update=None, immobile=None, abiotic=0, dissolution=None, surrogate=None, library=None,
+                 rule=None):
         self.lib = load(library)
         self.nx, self.ny, self.nz = nx, ny, nz
         self.N = N = nx * ny * nz
@@ -337,6 +353,9 @@ class AdeRun:
         if mask_update is not None:
             thr, pore, bio = mask_update
             P.mask_update, P.bmass_threshold, P.pore_code, P.biofilm_code = 1, thr, pore, bio
+        self.rule = rule  # mask_rule(...) of several material types, kept alive with P
+        if rule is not None:
+            P.mask_rule = _p(rule).value
         self.immobile = np.zeros(self.S, dtype=np.int32) if immobile is None else \
             np.ascontiguousarray(immobile, dtype=np.int32)
         P.immobile = _p(self.immobile).value
