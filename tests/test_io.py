@@ -125,3 +125,55 @@ def test_cpp_read_geometry_and_vti(tmp_path, order):
     r = subprocess.run([exe, str(tmp_path / "none.dat"), "3", "3", "3", "0", str(mb), str(vti)], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 3
+
+
+# ---- the reference's own geometry files (examples/input_shared, copied unchanged under
+# tests/golden/) at the size their CompLaB.xml gives: nx = ny = 24, nz = 6 (e.g.
+# examples/08_two_microbes/CompLaB.xml:39-52, material numbers pore 2, solid 0, bounce-back
+# 1, microbe0 3, microbe1 4). The checks of examples/validateExamples.py:150-178 (value count,
+# every microbe code present), then the layout examples/makeGeometry.py:33-109 builds, which
+# fixes the file order: bounce-back rows at y = 0 and y = 23, the patches, every z plane alike.
+GOLDEN = os.path.join(HERE, "golden")
+REF_DECLARED = {0, 1, 2, 3, 4}
+
+
+def ref_layout(kind):
+    g = np.full((6, 24, 24), 2, dtype=np.int64)
+    g[:, 0, :] = 1
+    g[:, 23, :] = 1
+    if kind == "two_microbes":
+        g[:, 1:4, 4:10] = 3
+        g[:, 20:23, 14:20] = 4
+    else:
+        yy, xx = np.mgrid[0:24, 0:24]
+        for cx, cy in ((7, 8), (7, 16), (16, 8), (16, 16)):
+            g[:, (xx - cx) ** 2 + (yy - cy) ** 2 <= 9] = 0
+    return g
+
+
+@pytest.mark.parametrize("kind", ["two_microbes", "grains"])
+def test_reference_geometry_files(kind):
+    path = os.path.join(GOLDEN, f"{kind}_24x24x6.dat")
+    vals = [int(v) for v in open(path).read().split()]
+    assert len(vals) == 24 * 24 * 6
+    assert set(vals) <= REF_DECLARED
+    if kind == "two_microbes":
+        assert {3, 4} <= set(vals)                     # every microbe is seeded somewhere
+    m = clb.read_geometry(path, 24, 24, 6, clb.GEOM_X_FASTEST).reshape(6, 24, 26)
+    g = m[:, :, 1:-1]
+    assert np.array_equal(g, ref_layout(kind))
+    assert np.array_equal(m[:, :, 0], g[:, :, 0]) and np.array_equal(m[:, :, -1], g[:, :, -1])
+    assert all(np.array_equal(g[z], g[0]) for z in range(6))
+    counts = np.bincount(np.asarray(vals), minlength=5)
+    if kind == "two_microbes":
+        assert counts.tolist() == [0, 288, 3456 - 288 - 216, 108, 108]
+    # the C++ reader gives the same lattice mask
+    exe = os.path.join(HERE, "cpp", "io_check")
+    if not os.path.exists(exe):
+        subprocess.run(["make", "-C", os.path.join(HERE, "cpp"), "io_check"], check=True, capture_output=True)
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        mb, vti = os.path.join(td, "mask.bin"), os.path.join(td, "out.vti")
+        r = subprocess.run([exe, path, "24", "24", "6", "0", mb, vti], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 0, r.stderr
+        assert np.array_equal(np.fromfile(mb, dtype=np.uint8), m.ravel())
