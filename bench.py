@@ -173,6 +173,17 @@ def full_coupled(clb, args):
     p = lat.perf()
     lat.set_timing(False)
     s1 = lat.speciation_stats() - s0
+    # coupled-step voxel states for the warm-start comparison: the solutes of the reactive
+    # voxels after the last step (pore or biofilm, 1 <= x <= nx-2), a bounded sample
+    states = None
+    if not args.skip_warm_coupled:
+        cf = lat.conc()
+        mf = lat.download_mask()
+        xs = np.arange(N) % nx
+        sel = np.flatnonzero(((mf == 2) | (mf == 3)) & (xs >= 1) & (xs <= nx - 2))
+        sel = np.random.default_rng(5).choice(sel, min(sel.size, args.warm_sample), replace=False)
+        states = np.ascontiguousarray(cf[:, sel].T)
+        del cf
     lat.close()
     ade_ms = p.ade_ms / max(p.ade_launches, 1)
     spec_ms = p.spec_ms / max(p.spec_launches, 1)
@@ -193,7 +204,60 @@ def full_coupled(clb, args):
                 run_5000_ms_per_step=run_ms / run_steps, run_5000_mlups=N * run_steps / (run_ms * 1e-3) / 1e6,
                 solves_per_step=solves_step, stragglers_per_step=p.stragglers / 20.0,
                 converged=int(s1[1]), solves=int(s1[0]), mean_evals=float(s1[2] / max(s1[0], 1)),
-                coupled_solves_per_s=solves_step / (wall / steps), autotune=tune)
+                coupled_solves_per_s=solves_step / (wall / steps), autotune=tune, states=states, tableau=(S, lk))
+
+
+def coupled_warm_start(clb, S, lk, C, train_solves, epochs):
+    """Warm starts for the coupled step's speciation, compared on voxel states of the coupled
+    run: C holds the solutes (= the 3 / 6 tableau's species) of a sample of reactive voxels
+    after a step. Each start solves the same totals (calc_component_totals) to the 1e-10 judge
+    with the batch kernel:
+      * component_slot: log10 of each component's species (the reference's start in
+        run_equilibrium_biotic, src/complab3d_processors_part4_eqsolver.hh:977-994, which the
+        fused step uses);
+      * plain: log10 of the component totals;
+      * mlp: the warm-start network trained on plain solves over the sample's box of totals.
+    Reported: mean evaluations, kernel time (the network's inference included), and the same
+    on the voxels that need more than 4 evaluations from the component slots (those the fused
+    step hands to the persistent kernel)."""
+    nc = S.shape[1]
+    comp = list(range(nc))
+    Cc = np.clip(C, 1e-30, 10.0)
+    T = np.maximum(Cc @ S, 1e-30)
+    x_slot = np.log10(Cc[:, comp])
+    lat = clb.Lattice(4, 1, 1)
+    lat.set_tableau(S, lk, comp, tol=1e-10, max_iter=200)
+    rng = np.random.default_rng(17)
+    lo, hi = np.log10(T.min(axis=0)), np.log10(T.max(axis=0))
+    lat.batch_load(10.0 ** rng.uniform(lo, hi, (train_solves, nc)))
+    lat.batch_solve(clb.WARM_REF)
+    lat.mlp_config(nc, 128, nc)
+    t0 = time.perf_counter()
+    loss = lat.mlp_train_from_batch(epochs=epochs, batch=512, lr=2e-3, seed=7)
+    train_s = time.perf_counter() - t0
+    lat.batch_load(T)
+    out = dict(n=int(T.shape[0]), train_solves=train_solves, epochs=epochs, train_s=train_s, train_loss=loss)
+    its = {}
+    for name, mode, guess in (("component_slot", clb.WARM_USER, x_slot), ("plain", clb.WARM_REF, None),
+                              ("mlp", clb.WARM_MLP, None)):
+        lat.batch_solve(mode, guess)           # warm-up
+        lat.sync()
+        lat.reset_perf()
+        lat.set_timing(True)
+        lat.batch_solve(mode, guess)
+        lat.sync()
+        p = lat.perf()
+        lat.set_timing(False)
+        x, it, r, cv = lat.batch_fetch()
+        its[name] = it
+        out[name] = dict(mean_evals=float(it.mean()), kernel_ms=float(p.spec_ms + p.mlp_ms), mlp_ms=float(p.mlp_ms),
+                         converged=float(cv.mean()), max_resid=float(r.max()))
+    lat.close()
+    hard = its["component_slot"] > 4
+    out["hard_fraction"] = float(hard.mean())
+    for name in its:
+        out[name]["mean_evals_hard"] = float(its[name][hard].mean()) if hard.any() else None
+    return out
 
 
 def kinetics_case(clb, nx, ny, nz, prec, flow_steps):
@@ -484,6 +548,10 @@ def main():
     ap.add_argument("--skip-ca", action="store_true")
     ap.add_argument("--skip-surrogate", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--skip-warm-coupled", action="store_true")
+    ap.add_argument("--warm-sample", type=int, default=2000000, help="voxel states for the coupled warm-start A/B")
+    ap.add_argument("--warm-train", type=int, default=1000000, help="training solves of its network")
+    ap.add_argument("--warm-epochs", type=int, default=30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the library runs on one GPU; --gpus must be 1 (the multi-GPU slab "
@@ -518,7 +586,16 @@ def main():
         f"(warm-up: {fc['warmup_resolve_steps']}); 5000-step run {fc['run_5000_ms_per_step']:.3f} ms/step; "
         f"initial flow {fc['flow0_steps']} steps (converged {fc['flow0_converged']}); {fc['conversions']} voxels "
         f"converted in the timed segments")
+    states, (S6, lk6) = fc.pop("states"), fc.pop("tableau")
     extra = dict(full_coupled=fc)
+    if states is not None:
+        extra["coupled_warm_start"] = coupled_warm_start(clb, S6, lk6, states, args.warm_train, args.warm_epochs)
+        cw = extra["coupled_warm_start"]
+        log(f"coupled warm start ({cw['n']} voxel states): evaluations component slots "
+            f"{cw['component_slot']['mean_evals']:.2f} ({cw['component_slot']['kernel_ms']:.2f} ms), plain "
+            f"{cw['plain']['mean_evals']:.2f} ({cw['plain']['kernel_ms']:.2f} ms), MLP {cw['mlp']['mean_evals']:.2f} "
+            f"({cw['mlp']['kernel_ms']:.2f} ms); voxels beyond 4 evaluations {cw['hard_fraction']:.4f}")
+        del states
     fl_ms = fc["flow_ms"]
     extra["flow_fp32"] = dict(ms_per_step=fl_ms, mlups=N / (fl_ms * 1e-3) / 1e6,
                               gbs_dense=BYTES_D3Q19 * N / (fl_ms * 1e-3) / 1e9,

@@ -36,6 +36,12 @@ __global__ void __launch_bounds__(256) ade_step_kernel(AdeArgs<R> a) {
     for (int s = 0; s < S; ++s) rho_in[s] = pop_sum(&p[s * 7]) + a.c_ref[s];
     ade_node<R, S, KIN, CBUF>(a, x, y, z, n, code, p, nullptr, n, g.N);
     store7<R, S, ODD, false>(a, nb, n, p);
+  } else if (a.stream_none && a.kind[code] == CLB_KIND_NONE) {
+    // NoDynamics: no collision; the AA pattern's load and store alone are the streaming
+    const Nbr nb(x, y, z, g);
+    R p[S * 7];
+    load7<R, S, ODD, false>(a, nb, n, p);
+    store7<R, S, ODD, false>(a, nb, n, p);
   }
   if (x == 1 || x == g.nx - 2) {
     const int xb = (x == 1) ? 0 : g.nx - 1;

@@ -120,6 +120,11 @@ struct clb_ctx {
   int pore_code = 2, biofilm_code = 3;
   MaskRule mrule = single_rule(2, 3);
   uint8_t* d_mask_prev = nullptr;  // the mask before a pass of the rule with several pore types
+  // whether a fluid voxel neighbours a no-dynamics voxel (flow: D3Q19 stencil, solutes: D3Q7),
+  // whose populations then stream through it (refresh_leak); recomputed after mask or table
+  // changes
+  bool flow_leaky = false, ade_leaky = false, leak_dirty = true;
+  unsigned int* d_leak = nullptr;
 
   // coupled speciation
   bool have_tableau = false;
@@ -297,6 +302,7 @@ FlowArgs<R> flow_args(clb_ctx* c, double* energy) {
   a.g = c->geo;
   a.rhobar_in = (R)(c->rho_in - 1.0);
   a.rhobar_out = (R)(c->rho_out - 1.0);
+  a.stream_none = c->flow_leaky ? 1 : 0;
   a.rref0 = flow_ref0(c);
   a.rrdx = flow_rdx<R>(c);
   a.rdx = (R)a.rrdx;
@@ -343,6 +349,7 @@ AdeArgs<R> ade_args(clb_ctx* c) {
   a.dt = c->dt;
   a.solid_code = c->solid_code;
   a.bb_code = c->bb_code;
+  a.stream_none = c->ade_leaky ? 1 : 0;
   // several pore types: the rule runs after the step, over a copy of the mask
   a.mask_update = (c->mask_update && c->mrule.npore <= 1) ? 1 : 0;
   a.bthreshold = c->bthr;
@@ -478,7 +485,9 @@ int build_reactive_list(clb_ctx* c) {
 // Whether the coupled step runs the speciation inside the transport kernel: a compiled
 // shape exists for (S solutes = species, nc <= 4 components) and the mode asks for it.
 bool fused_spec(const clb_ctx* c) {
-  if (!c->have_tableau || c->spec_mode != CLB_SPEC_FUSED) return false;
+  // populations streaming through no-dynamics voxels would carry deferred increments there:
+  // the speciation then runs as its own kernel, which applies them at once
+  if (!c->have_tableau || c->spec_mode != CLB_SPEC_FUSED || c->ade_leaky) return false;
   const int nc = c->tb.nc, S = c->S;
   return c->tb.ns == S && nc >= 1 && nc <= 4 && (nc < 2 ? 2 : nc) <= S;
 }
@@ -486,7 +495,7 @@ bool fused_spec(const clb_ctx* c) {
 void drop_graph(clb_ctx* c);
 
 bool compact_ok(const clb_ctx* c) {
-  if (c->layout != CLB_LAYOUT_COMPACT || !c->ade_ready || c->S < 1) return false;
+  if (c->layout != CLB_LAYOUT_COMPACT || !c->ade_ready || c->S < 1 || c->ade_leaky) return false;
   if (c->srg_on || c->dissol_on) return false;
   if (c->have_tableau && !fused_spec(c)) return false;
   if (c->mask_update) {
@@ -946,8 +955,45 @@ int srg_prepare(clb_ctx* c) {
 // increment buffer, the dissolution water, the fused step's buffers or the separate
 // speciation's reactive list. Every caller of enqueue_step runs this first (outside any
 // capture), so no launch ever sees a buffer that is not there yet.
+// Whether populations stream into no-dynamics voxels: Palabos' NoDynamics does not collide,
+// but the lattice's streaming still moves its populations, so one next to a fluid voxel
+// passes them on (its neighbours get them a step later) instead of reflecting them. When no
+// fluid voxel has such a neighbour the no-dynamics voxels never exchange a population with
+// the fluid, and the kernels skip them. Recomputed (one pass over the mask) after a change of
+// the mask or of a kind table.
+int refresh_leak(clb_ctx* c) {
+  if (!c->leak_dirty) return CLB_OK;
+  if (!c->d_leak) {
+    int rc = dalloc(c, &c->d_leak, 2);
+    if (rc) return rc;
+  }
+  launch_fill_u32(c->d_leak, 2, 0u, c->stream);
+  launch_none_reach(c->d_mask, c->d_kind, c->geo, 19, c->d_leak, c->stream);
+  launch_none_reach(c->d_mask, c->d_ade_kind, c->geo, 7, c->d_leak + 1, c->stream);
+  unsigned int h[2] = {0, 0};
+  CLB_CHECK_HIP(c, hipMemcpyAsync(h, c->d_leak, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  CLB_CHECK_HIP(c, hipGetLastError());
+  const bool fl = h[0] != 0, al = h[1] != 0;
+  if (al != c->ade_leaky) {
+    // the step's kernels change (fused or separate speciation, compact or dense layout):
+    // deferred increments go into the populations first
+    int rc = flush_pending(c);
+    if (rc) return rc;
+  }
+  if (fl != c->flow_leaky || al != c->ade_leaky) drop_graph(c);
+  c->flow_leaky = fl;
+  c->ade_leaky = al;
+  c->leak_dirty = false;
+  return CLB_OK;
+}
+
 template <typename R>
 int prepare_step(clb_ctx* c) {
+  {
+    int rc = refresh_leak(c);
+    if (rc) return rc;
+  }
   if (c->srg_on) {
     int rc = srg_prepare(c);
     if (rc) return rc;
@@ -1223,7 +1269,8 @@ int clb_destroy(clb_ctx* c) {
                   c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
                   c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared,
                   c->d_caflag, c->d_water, c->d_srg, c->d_srg_inc, c->d_cnode, c->d_cnbr, c->d_cnf, c->d_cg,
-                  c->d_cu, c->d_cpend, c->cb.flag, c->cb.cj, c->cb.lk, c->cb.cg, c->cb.scan_tmp, c->d_mask_prev};
+                  c->d_cu, c->d_cpend, c->cb.flag, c->cb.cj, c->cb.lk, c->cb.cg, c->cb.scan_tmp, c->d_mask_prev,
+                  c->d_leak};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1257,6 +1304,7 @@ int clb_upload_mask(clb_ctx* c, const uint8_t* mask) {
   for (long long n = 0; n < c->geo.N; ++n) fluid += (c->h_kind[mask[n]] == CLB_KIND_FLUID);
   c->perf.fluid_nodes = fluid;
   c->list_dirty = true;
+  c->leak_dirty = true;
   c->cmp_dirty = true;
   return CLB_OK;
 }
@@ -1287,6 +1335,7 @@ int clb_set_dynamics(clb_ctx* c, const int* kind, const double* flow_omega) {
     if (flow_omega) c->h_flow_omega[i] = flow_omega[i];
   }
   c->list_dirty = true;
+  c->leak_dirty = true;
   c->cmp_dirty = true;
   return upload_tables(c);
 }
@@ -1307,6 +1356,7 @@ int clb_set_solute_dynamics(clb_ctx* c, const int* kind) {
   for (int i = 0; i < 256; ++i) c->h_ade_kind[i] = kind[i];
   c->ade_kind_set = true;
   c->list_dirty = true;
+  c->leak_dirty = true;
   c->cmp_dirty = true;
   return upload_tables(c);
 }
@@ -1333,6 +1383,8 @@ int clb_flow_init(clb_ctx* c, double rho_in, double rho_out) {
 int clb_flow_steps(clb_ctx* c, int nsteps, double* energy) {
   if (!c || nsteps < 0) return CLB_ERR_ARG;
   if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "clb_flow_init first");
+  int rc = refresh_leak(c);
+  if (rc) return rc;
   return c->fp32 ? flow_steps_t<float>(c, nsteps, energy) : flow_steps_t<double>(c, nsteps, energy);
 }
 
@@ -1530,6 +1582,7 @@ int clb_set_kinetics(clb_ctx* c, int kind, double dt, int solid_code, int bb_cod
   c->solid_code = solid_code;
   c->bb_code = bb_code;
   c->list_dirty = true;
+  c->leak_dirty = true;
   c->cmp_dirty = true;
   return CLB_OK;
 }
@@ -1590,6 +1643,7 @@ int clb_set_tableau(clb_ctx* c, int nc, int ns, const double* S, const double* l
     if (rc) return rc;
   }
   c->list_dirty = true;
+  c->leak_dirty = true;
   c->cmp_dirty = true;
   return CLB_OK;
 }
@@ -1647,7 +1701,10 @@ int clb_step(clb_ctx* c, int nsteps) {
   if (!c->ade_ready) return fail(c, CLB_ERR_STATE, "clb_solutes_init first");
   if (c->kin_kind == CLB_KINETICS_PLUGIN && c->nb > 0 && !c->d_b) return fail(c, CLB_ERR_STATE, "no biomass");
   if (c->have_tableau && c->tb.ns != c->S) return fail(c, CLB_ERR_ARG, "tableau species count != solute count");
-  return c->fp32 ? step_t<float>(c, nsteps) : step_t<double>(c, nsteps);
+  const int rc = c->fp32 ? step_t<float>(c, nsteps) : step_t<double>(c, nsteps);
+  // the pore <-> biofilm rule may have changed codes whose flow dynamics differ
+  if (c->mask_update && nsteps > 0) c->leak_dirty = true;
+  return rc;
 }
 
 int clb_speciation_stats(clb_ctx* c, long long* stats) {
@@ -1677,6 +1734,7 @@ int clb_set_speciation_mode(clb_ctx* c, int mode) {
   if (mode != c->spec_mode) drop_graph(c);
   c->spec_mode = mode;
   c->list_dirty = true;
+  c->leak_dirty = true;
   c->cmp_dirty = true;
   return CLB_OK;
 }
@@ -2224,6 +2282,7 @@ int clb_node_conversion(clb_ctx* c, int mineral_solute, int pore_code, int preci
     // the captured step reads the mask on the device; only the separate speciation
     // path's host-side reactive list has to be rebuilt (step_t does it)
     c->list_dirty = true;
+    c->leak_dirty = true;
     c->cmp_dirty = true;
     c->fluid_dirty = true;
   }
@@ -2502,7 +2561,8 @@ int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
   }
   c->perf = s.perf;
   c->timing = timing;
-  c->list_dirty = true;   // the mask rule may have run during the trials
+  c->list_dirty = true;
+  c->leak_dirty = true;   // the mask rule may have run during the trials
   c->cmp_dirty = true;
   c->fluid_dirty = true;
   (void)hipEventDestroy(e0);
@@ -2523,6 +2583,7 @@ int clb_autotune(clb_ctx* c, int steps, clb_tune_report* report) {
     if (c->have_tableau && c->tb.ns != c->S) return fail(c, CLB_ERR_ARG, "tableau species count != solute count");
   }
   int rc = flush_pending(c);
+  if (!rc) rc = refresh_leak(c);
   if (rc) return rc;
   drop_graph(c);  // captured launches hold the old shapes
   return c->fp32 ? autotune_t<float>(c, steps, report) : autotune_t<double>(c, steps, report);
@@ -2682,6 +2743,7 @@ int clb_fd_biomass(clb_ctx* c, double nu, clb_ca_report* report) {
   rc = c->fp32 ? fd_biomass_t<float>(c, nu, report) : fd_biomass_t<double>(c, nu, report);
   if (report->mask_changes > 0) {
     c->list_dirty = true;
+    c->leak_dirty = true;
     c->cmp_dirty = true;
     c->fluid_dirty = true;
   }
@@ -2715,6 +2777,7 @@ int clb_ca_redistribute(clb_ctx* c, double bmax, int half, unsigned long long se
                : ca_redistribute_t<double>(c, bmax, half, seed, report);
   if (report->mask_changes > 0) {
     c->list_dirty = true;
+    c->leak_dirty = true;
     c->cmp_dirty = true;
     c->fluid_dirty = true;
   }
@@ -2883,6 +2946,7 @@ int clb_load_state(clb_ctx* c, const char* path) {
   c->ade_t = h.ade_t;
   c->ca_sweep = h.ca_sweep;
   c->list_dirty = true;
+  c->leak_dirty = true;
   c->cmp_dirty = true;
   c->fluid_dirty = true;
   drop_graph(c);  // a captured segment starts on the parity it was captured for

@@ -140,12 +140,19 @@ __global__ void __launch_bounds__(256) flow_step_kernel(FlowArgs<R> a) {
   if (x < g.nx && y < g.ny) {
     const int n = x + g.nx * (y + g.ny * z);
     const uint8_t code = a.mask[n];
-    if (a.kind[code] == CLB_KIND_FLUID) {
+    const uint8_t k = a.kind[code];
+    if (k == CLB_KIND_FLUID) {
       double ek;
       if (x == 0) ek = flow_node<R, ODD, -1>(a, x, y, z, n, code);
       else if (x == g.nx - 1) ek = flow_node<R, ODD, 1>(a, x, y, z, n, code);
       else ek = flow_node<R, ODD, 0>(a, x, y, z, n, code);
       if (ENERGY) { e = ek; cnt = 1.0; }
+    } else if (k == CLB_KIND_NONE && a.stream_none && x > 0 && x < g.nx - 1) {
+      // NoDynamics: no collision; the AA pattern's load and store alone are the streaming
+      const Nbr nb(x, y, z, g);
+      R p[19];
+      load19<R, ODD, false>(a.f, g.N, nb, n, p, a.rdx);
+      store19<R, ODD, false>(a.f, g.N, nb, n, p, a.rdx);
     }
   }
   if (ENERGY) {
@@ -204,6 +211,22 @@ __global__ void __launch_bounds__(kEnergyThreads) energy_reduce_kernel(double* e
   if (t == 0) {
     energy[0] = se[0];
     energy[1] = sc[0];
+  }
+}
+
+__global__ void none_reach_kernel(const uint8_t* mask, const uint8_t* kind, Geom g, int q, unsigned int* flag) {
+  const long long n = blockIdx.x * 256LL + threadIdx.x;
+  if (n >= g.N || kind[mask[n]] != CLB_KIND_NONE) return;
+  const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
+  for (int i = 1; i < 19; ++i) {
+    if (q == 7 && Q19::cx[i] * Q19::cx[i] + Q19::cy[i] * Q19::cy[i] + Q19::cz[i] * Q19::cz[i] != 1) continue;
+    const int xx = x + Q19::cx[i];
+    if (xx < 0 || xx >= g.nx) continue;
+    const int yy = (y + Q19::cy[i] + g.ny) % g.ny, zz = (z + Q19::cz[i] + g.nz) % g.nz;
+    if (kind[mask[xx + (long long)g.nx * (yy + (long long)g.ny * zz)]] == CLB_KIND_FLUID) {
+      flag[0] = 1u;
+      return;
+    }
   }
 }
 
@@ -320,6 +343,11 @@ template <typename R>
 void launch_flow_init(const Geom& g, R* f, hipStream_t s) {
   const int bs = 256;
   hipLaunchKernelGGL(flow_init_kernel<R>, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, f, g);
+}
+
+void launch_none_reach(const uint8_t* mask, const uint8_t* kind, const Geom& g, int q, unsigned int* flag,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(none_reach_kernel, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, mask, kind, g, q, flag);
 }
 
 template <typename R>

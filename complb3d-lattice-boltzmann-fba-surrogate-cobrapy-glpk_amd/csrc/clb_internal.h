@@ -43,6 +43,10 @@ struct FlowArgs {
   R dface_in, dface_out;    // the same, less rref at x = 0 and x = nx-1
   R rdx;                    // rref(x + 1) - rref(x)
   double rref0, rrdx;       // rref(0) and rdx in FP64, for the reported densities
+  // 1: no-dynamics voxels (kind NONE) of 1 <= x <= nx-2 stream their populations unchanged
+  // (Palabos' NoDynamics: no collision, the lattice's streaming still moves them), because a
+  // fluid voxel neighbours one; 0: no fluid voxel does, and they are skipped
+  int stream_none;
   double* energy;         // [0..1] sum |u|^2/2 and count; [2 + 2b..] per-block partials; or null
 };
 
@@ -106,6 +110,7 @@ struct AdeArgs {
   int abiotic;            // 0 none, 1 plugin everywhere, 2 plugin at pore voxels next to a wall
   double dt;
   int solid_code, bb_code;
+  int stream_none;        // as FlowArgs::stream_none, for the solute lattices' kinds
   int mask_update;        // the rule inside the transport kernel (one pore type only)
   double bthreshold;
   int pore_code, biofilm_code;
@@ -152,6 +157,10 @@ struct FusedSpecIO {
 // ------------------------------------------------------------------ launchers
 template <typename R>
 void launch_flow_init(const Geom& g, R* f, hipStream_t s);
+// flag[0] = 1 when a voxel of kind NONE has a voxel of kind FLUID among its neighbours in
+// the D3Q19 (q = 19) or D3Q7 (q = 7) stencil; flag[0] is left alone otherwise.
+void launch_none_reach(const uint8_t* mask, const uint8_t* kind, const Geom& g, int q, unsigned int* flag,
+                       hipStream_t s);
 // Per-block energy partials a launch of this block shape writes (the energy buffer holds
 // 2 + 2 * flow_blocks doubles).
 long long flow_blocks(const Geom& g, dim3 block);

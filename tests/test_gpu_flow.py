@@ -114,28 +114,3 @@ def test_energy_any_block_shape(clb, pack, oracle_flow, block):
     _, eo = oracle_flow[301]
     assert np.all(np.abs(e - eo) <= 1e-9 * eo + 1e-30)
 
-
-@pytest.mark.gpu
-def test_flow_on_reference_geometry(clb, oracle):
-    """The reference's own grains geometry (examples/input_shared/grains.dat, copied under
-    tests/golden/), read at its XML size 24 x 24 x 6 into the 26 x 24 x 6 lattice: the GPU flow
-    (FP64) against the oracle over 500 steps, and the flow goes round the four grains."""
-    import os
-    print("case: flow fp64 on the reference grains geometry 26x24x6", flush=True)
-    path = os.path.join(os.path.dirname(__file__), "golden", "grains_24x24x6.dat")
-    mask = clb.read_geometry(path, 24, 24, 6, clb.GEOM_X_FASTEST)
-    nx, ny, nz = 26, 24, 6
-    kind, om = d3q19_kind_omega(0.8)
-    lat = clb.Lattice(nx, ny, nz, precision=clb.PREC_FP64)
-    lat.set_dynamics(kind, om)
-    lat.upload_mask(mask)
-    lat.flow_init(1.0, 0.999)
-    lat.flow_steps(500)
-    u, r = lat.velocity()
-    lat.close()
-    f, _ = oracle.flow_run(nx, ny, nz, mask, kind, om, 1.0, 0.999, 500)
-    uo, ro = oracle.flow_velocity(nx, ny, nz, mask, kind, 1.0, 0.999, f)
-    assert rel_l2(u, uo) < 1e-10
-    assert np.max(np.abs(r - ro)) < 1e-12
-    ux = u[:, 0].reshape(nz, ny, nx)
-    assert ux[:, 1:23, 1:-1].mean() > 0 and np.all(ux[:, 0, :] == 0) and np.all(ux[:, 23, :] == 0)
