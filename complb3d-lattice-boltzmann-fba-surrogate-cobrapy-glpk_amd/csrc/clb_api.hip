@@ -955,6 +955,22 @@ int srg_prepare(clb_ctx* c) {
 // increment buffer, the dissolution water, the fused step's buffers or the separate
 // speciation's reactive list. Every caller of enqueue_step runs this first (outside any
 // capture), so no launch ever sees a buffer that is not there yet.
+// Whether every code the pore <-> biofilm rule writes has the flow kind of the first pore type
+// (then the rule never changes which voxels are fluid for the flow).
+bool rule_keeps_flow_kind(const clb_ctx* c) {
+  const MaskRule& r = c->mrule;
+  const int k0 = c->h_kind[r.pore[0] & 255];
+  for (int k = 0; k < r.npore; ++k)
+    if (c->h_kind[r.pore[k] & 255] != k0) return false;
+  for (int set = 1; set < (1 << CLB_MAX_BIOMASS); ++set) {
+    int lab = 0;
+    for (int b = 0; b < CLB_MAX_BIOMASS; ++b)
+      if ((set >> b) & 1) lab += r.sum_labels ? r.bio[b] : (b == 0 ? r.bio[0] : 0);
+    if (lab > 0 && lab < 256 && c->h_kind[lab] != k0) return false;
+  }
+  return true;
+}
+
 // Whether populations stream into no-dynamics voxels: Palabos' NoDynamics does not collide,
 // but the lattice's streaming still moves its populations, so one next to a fluid voxel
 // passes them on (its neighbours get them a step later) instead of reflecting them. When no
@@ -1703,7 +1719,7 @@ int clb_step(clb_ctx* c, int nsteps) {
   if (c->have_tableau && c->tb.ns != c->S) return fail(c, CLB_ERR_ARG, "tableau species count != solute count");
   const int rc = c->fp32 ? step_t<float>(c, nsteps) : step_t<double>(c, nsteps);
   // the pore <-> biofilm rule may have changed codes whose flow dynamics differ
-  if (c->mask_update && nsteps > 0) c->leak_dirty = true;
+  if (c->mask_update && nsteps > 0 && !rule_keeps_flow_kind(c)) c->leak_dirty = true;
   return rc;
 }
 
