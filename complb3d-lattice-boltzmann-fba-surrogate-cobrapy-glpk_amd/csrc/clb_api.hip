@@ -136,6 +136,7 @@ struct clb_ctx {
   unsigned int* d_counter = nullptr;  // [0] work queue, [1] node conversion, [2] stragglers
   unsigned long long* d_stats = nullptr;
   int spec_grid = 0;
+  int spec_grid8 = 0;  // the batch kernel's for more than four components
   // speciation fused into the transport kernel (CLB_SPEC_FUSED) or after it (SEPARATE)
   int spec_mode = CLB_SPEC_FUSED;
   int* d_slist = nullptr;                   // voxels left to the persistent kernel
@@ -1271,6 +1272,7 @@ int clb_create(const clb_config* cfg, clb_ctx** out) {
   }
   launch_fill_u64(c->d_stats, 4, 0ull, c->stream);
   c->spec_grid = spec_occupancy_grid(8, cfg->device);
+  c->spec_grid8 = spec_group_grid(cfg->device);
   *out = c;
   return CLB_OK;
 }
@@ -1994,7 +1996,8 @@ int clb_batch_solve(clb_ctx* c, int warm, const double* logc_in) {
   launch_fill_u32(c->d_counter, 1, 0u, c->stream);
   {
     Timed tm(c, 2);
-    launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->stream);
+    launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->spec_grid8,
+                      c->stream);
   }
   count_launch(c, 2);
   CLB_CHECK_HIP(c, hipGetLastError());
@@ -2036,7 +2039,8 @@ int clb_batch_solve_noalloc(clb_ctx* c, long n, const double* d_totals, int nvar
     return fail(c, CLB_ERR_ARG, "bad warm-start mode");
   }
   launch_fill_u32(c->d_scratch, 1, 0u, c->stream);  // a fill kernel, no memset node
-  launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->stream);
+  launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->spec_grid8,
+                      c->stream);
   CLB_CHECK_HIP(c, hipGetLastError());
   return CLB_OK;
 }

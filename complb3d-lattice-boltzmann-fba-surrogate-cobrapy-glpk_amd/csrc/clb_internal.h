@@ -250,8 +250,11 @@ long long fused_waves(const Geom& g, dim3 block);
 template <typename R>
 void launch_pending_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s);
 
-void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, hipStream_t s);
+// grid: spec_kernel's persistent grid; grid8: spec_group_kernel's (tableaux of more than
+// four components)
+void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, int grid8, hipStream_t s);
 int spec_occupancy_grid(int nc, int device);
+int spec_group_grid(int device);
 
 // Mineral dissolution (clb_dissol.hip): surfaceDissolutionKinetics3D with the compiled
 // defineDissolutionRate hook. phase_sub[id] = the immobile solute holding phase id's

@@ -1,12 +1,31 @@
 // clb_spec_batch.hip -- batched speciation launcher (totals in, log activities out),
 // occupancy sizing of the persistent grid, and small fill kernels.
+#include "clb_spec_group.h"
 #include "clb_spec_kernel.h"
 
 namespace clb {
 
-void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, hipStream_t s) {
+void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, int grid8, hipStream_t s) {
+  if (tb.nc > 4) {
+    // one voxel per group of eight lanes (clb_spec_group.h)
+    const dim3 blk(256);
+    if (tb.ns <= 16) hipLaunchKernelGGL((spec_group_kernel<16>), dim3(grid8), blk, 0, s, tb, sa, warm);
+    else if (tb.ns <= 24) hipLaunchKernelGGL((spec_group_kernel<24>), dim3(grid8), blk, 0, s, tb, sa, warm);
+    else hipLaunchKernelGGL((spec_group_kernel<32>), dim3(grid8), blk, 0, s, tb, sa, warm);
+    return;
+  }
   CoupledIO io{};
   dispatch_spec<double, 0>(tb, sa, warm, io, grid, s);
+}
+
+int spec_group_grid(int device) {
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
+  int blocks = 0;
+  // the largest instantiation bounds the others
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, spec_group_kernel<32>, 256, 0) != hipSuccess || blocks < 1)
+    blocks = 2;
+  return prop.multiProcessorCount * blocks;
 }
 
 int spec_occupancy_grid(int nc, int device) {

@@ -106,9 +106,12 @@ def test_warm_start_gain_at_full_size(clb):
     """The warm-start setting at full size: 2 M plain-Newton training solves, 60 epochs, then
     128^3 fresh samples solved from the reference-rule start and from the network's start,
     both to the 1e-10 judge. The polish needs at least 3x fewer evaluations (deterministic:
-    fixed seeds, a fixed minibatch order and fixed-order gradient sums), and the measured
-    kernel time, inference included, is at least 2.9x lower (3.07x measured; the margin
-    covers timing noise)."""
+    fixed seeds, a fixed minibatch order and fixed-order gradient sums: 3.20x), and the
+    measured kernel time, inference included, is at least 2.5x lower. The time ratio sits
+    below the evaluation ratio because a polish of four evaluations pays the fixed costs of a
+    solve (the inference, the item's loads and stores) over fewer evaluations: 2.74x measured
+    against the eight-lane batch kernel (3.07x against the earlier one-lane kernel, which was
+    half as fast on both starts)."""
     print("case: mlp warm start gain, 2 M training solves, 128^3 samples", flush=True)
     S, lk = clb.gen_tableau(8, 24, seed=3)
     var = [8, 9, 10, 11]
@@ -147,7 +150,7 @@ def test_warm_start_gain_at_full_size(clb):
     print(f"evaluations {out['plain'][1]:.2f} -> {out['mlp'][1]:.2f} ({eval_ratio:.2f}x); "
           f"kernel time {out['plain'][0]:.2f} -> {out['mlp'][0]:.2f} ms ({time_ratio:.2f}x)", flush=True)
     assert eval_ratio >= 3.0
-    assert time_ratio >= 2.9
+    assert time_ratio >= 2.5
 
 
 @pytest.mark.gpu
