@@ -538,7 +538,7 @@ def main():
     ap.add_argument("--kin-steps", type=int, default=200)
     ap.add_argument("--spec-side", type=int, default=128)
     ap.add_argument("--train-solves", type=int, default=2000000)
-    ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--epochs", type=int, default=150)
     ap.add_argument("--cpu-nz", type=int, default=192)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--skip-speciation", action="store_true")

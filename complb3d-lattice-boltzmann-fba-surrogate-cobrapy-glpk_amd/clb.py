@@ -25,6 +25,7 @@ KIND_NONE, KIND_BB, KIND_FLUID = 0, 1, 2
 BC_DIRICHLET, BC_NEUMANN = 0, 1
 KINETICS_NONE, KINETICS_PLUGIN = 0, 1
 WARM_REF, WARM_USER, WARM_MLP = 0, 1, 2
+BATCH_AUTO, BATCH_FULL, BATCH_POLISH = 0, 1, 2
 SPEC_FUSED, SPEC_SEPARATE = 0, 1
 CA_DONE, CA_STALLED, CA_SWEEP_CAP, CA_AGE_CAP, CA_NONFINITE = 0, 1, 2, 3, 4
 KERNEL_FLOW, KERNEL_ADE, KERNEL_FUSED = 0, 1, 2
@@ -106,6 +107,7 @@ def load(path=None):
         "clb_speciation_stats": [vp, vp],
         "clb_batch_load": [vp, c_long, vp, c_int, vp, vp],
         "clb_batch_solve": [vp, c_int, vp],
+        "clb_set_batch_method": [vp, c_int],
         "clb_batch_fetch": [vp, vp, vp, vp, vp],
         "clb_mlp_config": [vp, c_int, c_int, c_int],
         "clb_mlp_set_params": [vp, vp],
@@ -772,6 +774,10 @@ class Lattice:
     def batch_solve(self, warm=WARM_REF, logc_in=None):
         lc = None if logc_in is None else _f64(logc_in).ravel()
         self._chk(self.lib.clb_batch_solve(self.h, warm, _ptr(lc)), "batch_solve")
+
+    def set_batch_method(self, method):
+        """BATCH_AUTO (the polish from caller / network starts), BATCH_FULL or BATCH_POLISH."""
+        self._chk(self.lib.clb_set_batch_method(self.h, int(method)), "set_batch_method")
 
     def batch_fetch(self):
         n, nc = self.bn, self.nc

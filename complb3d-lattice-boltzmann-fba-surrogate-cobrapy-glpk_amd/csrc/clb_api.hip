@@ -137,6 +137,7 @@ struct clb_ctx {
   unsigned long long* d_stats = nullptr;
   int spec_grid = 0;
   int spec_grid8 = 0;  // the batch kernel's for more than four components
+  int batch_method = CLB_BATCH_AUTO;
   // speciation fused into the transport kernel (CLB_SPEC_FUSED) or after it (SEPARATE)
   int spec_mode = CLB_SPEC_FUSED;
   int* d_slist = nullptr;                   // voxels left to the persistent kernel
@@ -1173,14 +1174,21 @@ int mlp_train_device(clb_ctx* c, long long n, const float* dX, const float* dT, 
   if (batch < 1) batch = 1;
   if (batch > n) batch = (int)n;
   const long long P = c->mlp_P;
-  // the weight gradients reduce over the batch in chunks of 32 rows (at most 16 chunks),
-  // which spreads each gradient GEMM over many workgroups
-  const int splits = (int)std::min<long long>(16, std::max<long long>(1, batch / 32));
-  const size_t nfl = (mlp_fwd_bwd_floats(d, batch) + splits * P + 1) & ~(size_t)1;  // keep the double 8-byte aligned
+  // The warm-start network's shape trains with the fused step kernel (one launch for the
+  // forward and backward passes, a partial gradient per workgroup of mlp_step_rows() rows,
+  // stride gs). Other shapes run the layer GEMMs, whose weight gradients reduce over the
+  // batch in chunks of 32 rows (at most 16 chunks), which spreads each gradient GEMM over
+  // many workgroups.
+  const bool fused = mlp_step_fits(d);
+  const long long gs = fused ? ((P + 3) & ~3ll) : P;
+  const int splits = fused ? (int)((batch + mlp_step_rows() - 1) / mlp_step_rows())
+                           : (int)std::min<long long>(16, std::max<long long>(1, batch / 32));
+  const size_t act = fused ? 0 : mlp_fwd_bwd_floats(d, batch);
+  const size_t nfl = (act + splits * gs + 1) & ~(size_t)1;  // keep the double 8-byte aligned
   int rc = ensure_work(c, nfl * sizeof(float) + sizeof(double) * 2);
   if (rc) return rc;
   float* w = c->d_work;
-  float* grad = w + mlp_fwd_bwd_floats(d, batch);
+  float* grad = w + act;
   double* d_loss = (double*)(w + nfl);
   launch_fill_f64(d_loss, 1, 0.0, c->stream);
   (void)hipMemsetAsync(c->d_adam_m, 0, sizeof(float) * P, c->stream);
@@ -1200,10 +1208,14 @@ int mlp_train_device(clb_ctx* c, long long n, const float* dX, const float* dT, 
       const long long r0 = order[bi] * batch;
       const float* Xp = dX + r0 * d.din;
       const float* Tp = dT + r0 * d.dout;
-      mlp_fwd_bwd(c, batch, Xp, Tp, grad, d_loss, w, splits);
+      if (fused)
+        launch_mlp_step(d, batch, r0, dX, dT, c->d_theta, grad, gs, 2.0f / (float)(batch * d.dout), d_loss,
+                        c->stream);
+      else
+        mlp_fwd_bwd(c, batch, Xp, Tp, grad, d_loss, w, splits);
       ++tstep;
       const float b1 = 0.9f, b2 = 0.999f;
-      launch_adam(P, c->d_theta, grad, splits, c->d_adam_m, c->d_adam_v, (float)lr_ep, b1, b2, 1e-8f,
+      launch_adam(P, c->d_theta, grad, splits, gs, c->d_adam_m, c->d_adam_v, (float)lr_ep, b1, b2, 1e-8f,
                   (float)std::pow(0.9, (double)tstep), (float)std::pow(0.999, (double)tstep), c->stream);
     }
     double h = 0;
@@ -1996,11 +2008,20 @@ int clb_batch_solve(clb_ctx* c, int warm, const double* logc_in) {
   launch_fill_u32(c->d_counter, 1, 0u, c->stream);
   {
     Timed tm(c, 2);
-    launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->spec_grid8,
+    launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->batch_method, c->spec_grid,
+                      c->spec_grid8,
                       c->stream);
   }
   count_launch(c, 2);
   CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+int clb_set_batch_method(clb_ctx* c, int method) {
+  if (!c) return CLB_ERR_ARG;
+  if (method != CLB_BATCH_AUTO && method != CLB_BATCH_FULL && method != CLB_BATCH_POLISH)
+    return fail(c, CLB_ERR_ARG, "clb_set_batch_method: CLB_BATCH_AUTO, CLB_BATCH_FULL or CLB_BATCH_POLISH");
+  c->batch_method = method;
   return CLB_OK;
 }
 
@@ -2039,7 +2060,8 @@ int clb_batch_solve_noalloc(clb_ctx* c, long n, const double* d_totals, int nvar
     return fail(c, CLB_ERR_ARG, "bad warm-start mode");
   }
   launch_fill_u32(c->d_scratch, 1, 0u, c->stream);  // a fill kernel, no memset node
-  launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->spec_grid, c->spec_grid8,
+  launch_spec_batch(c->tb, sa, warm == CLB_WARM_REF ? CLB_WARM_REF : CLB_WARM_USER, c->batch_method, c->spec_grid,
+                      c->spec_grid8,
                       c->stream);
   CLB_CHECK_HIP(c, hipGetLastError());
   return CLB_OK;
@@ -2142,18 +2164,35 @@ int clb_mlp_grad(clb_ctx* c, long n, const float* X, const float* T, float* grad
   if (!c || n < 1 || !X || !T || !grad) return CLB_ERR_ARG;
   if (!c->d_theta) return fail(c, CLB_ERR_STATE, "clb_mlp_config first");
   const MlpDims d = c->md;
-  const size_t fl = ((size_t)n * (d.din + d.dout) + (size_t)c->mlp_P + mlp_fwd_bwd_floats(d, n) + 1) & ~(size_t)1;
+  const long long P = c->mlp_P;
+  // the fused step kernel for the warm-start shape, over row blocks of at most kBlk
+  // workgroups whose partials are summed in a fixed order; the layer GEMMs otherwise
+  const bool fused = mlp_step_fits(d);
+  const long long gs = (P + 3) & ~3ll;
+  const long long rows = mlp_step_rows(), kBlk = 256;
+  const size_t work = fused ? (size_t)(std::min<long long>(kBlk, (n + rows - 1) / rows) * gs)
+                            : mlp_fwd_bwd_floats(d, n);
+  const size_t fl = ((size_t)n * (d.din + d.dout) + (size_t)P + work + 1) & ~(size_t)1;
   int rc = ensure_work(c, fl * sizeof(float) + 2 * sizeof(double));
   if (rc) return rc;
   float* dX = c->d_work;
   float* dT = dX + (size_t)n * d.din;
   float* dg = dT + (size_t)n * d.dout;
-  float* w = dg + c->mlp_P;
+  float* w = dg + P;
   double* dl = (double*)(c->d_work + fl);
   CLB_CHECK_HIP(c, hipMemcpyAsync(dX, X, sizeof(float) * n * d.din, hipMemcpyHostToDevice, c->stream));
   CLB_CHECK_HIP(c, hipMemcpyAsync(dT, T, sizeof(float) * n * d.dout, hipMemcpyHostToDevice, c->stream));
   launch_fill_f64(dl, 1, 0.0, c->stream);
-  mlp_fwd_bwd(c, n, dX, dT, dg, dl, w);
+  if (fused) {
+    const float scale = 2.0f / (float)(n * d.dout);
+    for (long long r0 = 0; r0 < n; r0 += kBlk * rows) {
+      const long long b = std::min(kBlk * rows, n - r0);
+      launch_mlp_step(d, b, r0, dX, dT, c->d_theta, w, gs, scale, dl, c->stream);
+      launch_partial_sum(P, w, (int)((b + rows - 1) / rows), gs, dg, r0 > 0 ? 1 : 0, c->stream);
+    }
+  } else {
+    mlp_fwd_bwd(c, n, dX, dT, dg, dl, w);
+  }
   double h = 0;
   CLB_CHECK_HIP(c, hipMemcpyAsync(grad, dg, sizeof(float) * c->mlp_P, hipMemcpyDeviceToHost, c->stream));
   CLB_CHECK_HIP(c, hipMemcpyAsync(&h, dl, sizeof(double), hipMemcpyDeviceToHost, c->stream));

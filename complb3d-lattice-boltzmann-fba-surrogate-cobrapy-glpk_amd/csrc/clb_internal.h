@@ -252,7 +252,7 @@ void launch_pending_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s);
 
 // grid: spec_kernel's persistent grid; grid8: spec_group_kernel's (tableaux of more than
 // four components)
-void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, int grid8, hipStream_t s);
+void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int method, int grid, int grid8, hipStream_t s);
 int spec_occupancy_grid(int nc, int device);
 int spec_group_grid(int device);
 
@@ -329,6 +329,16 @@ void launch_fd_biomass(const R* b_old, R* b, int nb, const uint8_t* mask, const 
 struct MlpDims {
   int din, H, dout;
 };
+// Fused training step (forward, loss gradient, backward) for H = 128, din <= 16, dout <= 16:
+// one workgroup per mlp_step_rows() rows of X / T from row0 on (B rows), each writing its
+// partial gradient to grad + z gs (gs >= P and a multiple of 4).
+bool mlp_step_fits(const MlpDims& d);
+int mlp_step_rows();
+void launch_mlp_step(const MlpDims& d, long long B, long long row0, const float* X, const float* T,
+                     const float* theta, float* grad, long long gs, float scale, double* loss, hipStream_t s);
+// out[i] (+)= sum_z g[z gs + i] for i < P, z ascending
+void launch_partial_sum(long long P, const float* g, int splits, long long gs, float* out, int accumulate,
+                        hipStream_t s);
 void launch_gemm(int transA, int transB, int M, int N, int K, const float* A, int lda, const float* B, int ldb,
                  float* C, int ldc, const float* bias, int act, float beta, hipStream_t s);
 // K split in `splits` chunks, chunk z written to C + z sc
@@ -340,9 +350,9 @@ void launch_tanh_backward(long long n, float* dz, const float* h, hipStream_t s)
 void launch_colsum(int rows, int cols, const float* A, float* out, int splits, long long so, hipStream_t s);
 void launch_mse_grad(long long n, const float* Y, const float* T, float scale, float* dY, double* loss,
                      hipStream_t s);
-// gradient = sum of `splits` partials at g + z n
-void launch_adam(long long n, float* p, const float* g, int splits, float* m, float* v, float lr, float b1, float b2,
-                 float eps, float b1t, float b2t, hipStream_t s);
+// gradient = sum of `splits` partials at g + z gs
+void launch_adam(long long n, float* p, const float* g, int splits, long long gs, float* m, float* v, float lr,
+                 float b1, float b2, float eps, float b1t, float b2t, hipStream_t s);
 void launch_mlp_infer_bf16(const MlpDims& d, long long n, const float* theta, const float* X, float* Y,
                            hipStream_t s);
 void launch_mlp_warm(const MlpDims& d, long long n, const float* theta, const double* xmin, const double* xmax,

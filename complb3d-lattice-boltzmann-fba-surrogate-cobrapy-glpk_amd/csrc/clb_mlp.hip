@@ -135,19 +135,288 @@ __global__ void mse_grad_kernel(long long n, const float* Y, const float* T, flo
   }
 }
 
-// Adam step; the gradient is the sum of `splits` partial gradients g[z n + i], z ascending.
-__global__ void adam_kernel(long long n, float* p, const float* g, int splits, float* m, float* v, float lr,
-                            float b1, float b2, float eps, float b1t, float b2t) {
+// The sum of `splits` partials g[z gs + i] in a fixed order: partial z goes to accumulator
+// z mod 8, and the eight are added pairwise; eight loads are in flight at a time.
+__device__ __forceinline__ float sum_partials(const float* g, int splits, long long gs, long long i) {
+  float a[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  int z = 0;
+  for (; z + 8 <= splits; z += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += g[(z + u) * gs + i];
+  }
+#pragma unroll
+  for (int u = 0; u < 8; ++u)
+    if (z + u < splits) a[u] += g[(z + u) * gs + i];
+  return ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+}
+
+// Adam step; the gradient is the sum of `splits` partial gradients (sum_partials).
+__global__ void adam_kernel(long long n, float* p, const float* g, int splits, long long gs, float* m, float* v,
+                            float lr, float b1, float b2, float eps, float b1t, float b2t) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float gi = g[i];
-  for (int z = 1; z < splits; ++z) gi += g[z * n + i];
+  const float gi = sum_partials(g, splits, gs, i);
   const float mi = b1 * m[i] + (1.f - b1) * gi;
   const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
   const float mh = mi / (1.f - b1t), vh = vi / (1.f - b2t);
   p[i] -= lr * mh / (sqrtf(vh) + eps);
+}
+
+
+// ------------------------------------------------------------------ fused training step
+// One minibatch's forward pass, loss gradient and backward pass in one launch, for H = 128,
+// din <= 16 and dout <= 16 (the warm-start network). Workgroup z takes rows
+// [z kStepRows, (z + 1) kStepRows) of the minibatch, stages W1, W2, W3 and its rows'
+// activations in LDS, and writes its partial gradient, in the layout of theta, to
+// grad + z gs (gs >= P, a multiple of 4); the Adam kernel sums the partials in ascending z,
+// so training stays deterministic. Each workgroup adds its squared errors to *loss with one atomic.
+//
+// The layers are small (a workgroup's rows hold 8 x 128 activations), so the launches of a
+// GEMM per layer and pass cost more than the arithmetic; this kernel replaces the 14
+// launches of a step by one.
+constexpr int kStepRows = 8;
+constexpr int FH = 128;          // hidden width
+constexpr int FI = 16;           // padded input / output width
+constexpr int W2S = FH + 1;      // LDS row strides: rows read by consecutive lanes and
+constexpr int W1S = FI + 1;      // columns read by consecutive lanes are both free of
+constexpr int W3S = FH + 1;      // bank conflicts
+
+struct StepSmem {
+  float w2[FH * W2S];
+  float w1[FH * W1S];
+  float w3[FI * W3S];
+  float x[kStepRows][FI];
+  float h1[kStepRows][FH];
+  float h2[kStepRows][FH];  // H2, then dZ1
+  float dz[kStepRows][FH];  // dZ2
+  float dy[kStepRows][FI];
+  double red[4];
+};
+
+__global__ void __launch_bounds__(256) mlp_step_kernel(int din, int dout, long long B, long long row0,
+                                                       const float* __restrict__ X, const float* __restrict__ T,
+                                                       const float* __restrict__ theta, float* __restrict__ grad,
+                                                       long long gs, float scale, double* loss) {
+  constexpr int R = kStepRows, RH = R / 2;
+  extern __shared__ __align__(16) unsigned char step_smem[];
+  StepSmem& sm = *reinterpret_cast<StepSmem*>(step_smem);
+  const int t = threadIdx.x;
+  const long long r0 = row0 + (long long)blockIdx.x * R;  // first row (of X and T) of this workgroup
+  const long long rend = row0 + B;
+  const float* W1 = theta;
+  const float* b1 = W1 + FH * din;
+  const float* W2 = b1 + FH;
+  const float* b2 = W2 + FH * FH;
+  const float* W3 = b2 + FH;
+  const float* b3 = W3 + dout * FH;
+  float* g = grad + (long long)blockIdx.x * gs;  // gs: a multiple of 4 floats (16-byte stores)
+  float* gW1 = g;
+  float* gb1 = gW1 + FH * din;
+  float* gW2 = gb1 + FH;
+  float* gb2 = gW2 + FH * FH;
+  float* gW3 = gb2 + FH;
+  float* gb3 = gW3 + dout * FH;
+
+  // Stage the weights: every load is issued before the first LDS store, so the loads'
+  // latencies overlap (one load / store pair at a time would pay ~100 of them in turn).
+  // W2 starts at a multiple of 4 floats (128 (din + 1)), so its float4 loads are aligned.
+  {
+    float4 v[FH * FH / 4 / 256];
+#pragma unroll
+    for (int u = 0; u < FH * FH / 4 / 256; ++u) v[u] = reinterpret_cast<const float4*>(W2)[t + 256 * u];
+    float w1v[FH * FI / 256], w3v[FI * FH / 256];
+#pragma unroll
+    for (int u = 0; u < FH * FI / 256; ++u) {
+      const int q = t + 256 * u, j = q / FI, i = q % FI;
+      w1v[u] = (i < din) ? W1[j * din + i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < FI * FH / 256; ++u) {
+      const int q = t + 256 * u, o = q / FH, k = q % FH;
+      w3v[u] = (o < dout) ? W3[o * FH + k] : 0.f;
+    }
+    float xv = 0.f;
+    if (t < R * FI) {
+      const int r = t / FI, i = t % FI;
+      if (r0 + r < rend && i < din) xv = X[(r0 + r) * din + i];
+    }
+#pragma unroll
+    for (int u = 0; u < FH * FH / 4 / 256; ++u) {
+      const int q = 4 * (t + 256 * u), j = q >> 7, k = q & (FH - 1);
+      sm.w2[j * W2S + k] = v[u].x;
+      sm.w2[j * W2S + k + 1] = v[u].y;
+      sm.w2[j * W2S + k + 2] = v[u].z;
+      sm.w2[j * W2S + k + 3] = v[u].w;
+    }
+#pragma unroll
+    for (int u = 0; u < FH * FI / 256; ++u) {
+      const int q = t + 256 * u;
+      sm.w1[(q / FI) * W1S + q % FI] = w1v[u];
+    }
+#pragma unroll
+    for (int u = 0; u < FI * FH / 256; ++u) {
+      const int q = t + 256 * u;
+      sm.w3[(q / FH) * W3S + q % FH] = w3v[u];
+    }
+    if (t < R * FI) sm.x[t / FI][t % FI] = xv;
+  }
+  __syncthreads();
+
+  const int j = t & (FH - 1);  // this thread's unit
+  const int rh = t >> 7;       // and rows rh, rh + 2, ...
+  {
+    // layer 1: h1 = tanh(W1 x + b1)
+    float z[RH];
+#pragma unroll
+    for (int q = 0; q < RH; ++q) z[q] = b1[j];
+#pragma unroll
+    for (int i = 0; i < FI; ++i) {
+      const float w = sm.w1[j * W1S + i];
+#pragma unroll
+      for (int q = 0; q < RH; ++q) z[q] = fmaf(sm.x[rh + 2 * q][i], w, z[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < RH; ++q) sm.h1[rh + 2 * q][j] = tanhf(z[q]);
+  }
+  __syncthreads();
+  {
+    // layer 2: h2 = tanh(W2 h1 + b2)
+    float z[RH];
+#pragma unroll
+    for (int q = 0; q < RH; ++q) z[q] = b2[j];
+#pragma unroll 8
+    for (int k = 0; k < FH; ++k) {
+      const float w = sm.w2[j * W2S + k];
+#pragma unroll
+      for (int q = 0; q < RH; ++q) z[q] = fmaf(sm.h1[rh + 2 * q][k], w, z[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < RH; ++q) sm.h2[rh + 2 * q][j] = tanhf(z[q]);
+  }
+  __syncthreads();
+  // output layer and the loss gradient dY = scale (Y - T)
+  double e2 = 0.0;
+  if (t < R * FI) {
+    const int r = t / FI, o = t % FI;
+    float dyv = 0.f;
+    if (r0 + r < rend && o < dout) {
+      float y = b3[o];
+#pragma unroll 8
+      for (int k = 0; k < FH; ++k) y = fmaf(sm.h2[r][k], sm.w3[o * W3S + k], y);
+      const float e = y - T[(r0 + r) * dout + o];
+      dyv = scale * e;
+      e2 = (double)e * (double)e;
+    }
+    sm.dy[r][o] = dyv;
+  }
+  __syncthreads();
+  // output layer's gradient; dZ2 = (W3^T dY) (1 - h2^2)
+  for (int q = t; q < dout * FH; q += 256) {
+    const int o = q / FH, k = q % FH;
+    float a = 0.f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a = fmaf(sm.dy[r][o], sm.h2[r][k], a);
+    gW3[o * FH + k] = a;
+  }
+  if (t < dout) {
+    float a = 0.f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a += sm.dy[r][t];
+    gb3[t] = a;
+  }
+#pragma unroll
+  for (int q = 0; q < RH; ++q) {
+    const int r = rh + 2 * q;
+    float a = 0.f;
+#pragma unroll
+    for (int o = 0; o < FI; ++o) a = fmaf(sm.dy[r][o], sm.w3[o * W3S + j], a);
+    const float h = sm.h2[r][j];
+    sm.dz[r][j] = a * (1.f - h * h);
+  }
+  __syncthreads();
+  {
+    // layer 2's gradient: thread t owns units [8 (t >> 4), + 8) x inputs [8 (t & 15), + 8)
+    const int j0 = (t >> 4) * 8, k0 = (t & 15) * 8;
+    float acc[8][8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+      for (int b = 0; b < 8; ++b) acc[a][b] = 0.f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const float4 a0 = *reinterpret_cast<const float4*>(&sm.dz[r][j0]);
+      const float4 a1 = *reinterpret_cast<const float4*>(&sm.dz[r][j0 + 4]);
+      const float4 c0 = *reinterpret_cast<const float4*>(&sm.h1[r][k0]);
+      const float4 c1 = *reinterpret_cast<const float4*>(&sm.h1[r][k0 + 4]);
+      const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+      const float cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+#pragma unroll
+      for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int b = 0; b < 8; ++b) acc[a][b] = fmaf(av[a], cv[b], acc[a][b]);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      float4* dst = reinterpret_cast<float4*>(gW2 + (j0 + a) * FH + k0);
+      dst[0] = make_float4(acc[a][0], acc[a][1], acc[a][2], acc[a][3]);
+      dst[1] = make_float4(acc[a][4], acc[a][5], acc[a][6], acc[a][7]);
+    }
+  }
+  if (t < FH) {
+    float a = 0.f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a += sm.dz[r][t];
+    gb2[t] = a;
+  }
+  {
+    // dZ1 = (W2^T dZ2) (1 - h1^2), into the h2 rows
+    float a[RH];
+#pragma unroll
+    for (int q = 0; q < RH; ++q) a[q] = 0.f;
+#pragma unroll 8
+    for (int u = 0; u < FH; ++u) {
+      const float w = sm.w2[u * W2S + j];
+#pragma unroll
+      for (int q = 0; q < RH; ++q) a[q] = fmaf(sm.dz[rh + 2 * q][u], w, a[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < RH; ++q) {
+      const float h = sm.h1[rh + 2 * q][j];
+      sm.h2[rh + 2 * q][j] = a[q] * (1.f - h * h);
+    }
+  }
+  __syncthreads();
+  // layer 1's gradient
+  for (int q = t; q < FH * din; q += 256) {
+    const int u = q / din, i = q % din;
+    float a = 0.f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a = fmaf(sm.h2[r][u], sm.x[r][i], a);
+    gW1[u * din + i] = a;
+  }
+  if (t < FH) {
+    float a = 0.f;
+#pragma unroll
+    for (int r = 0; r < R; ++r) a += sm.h2[r][t];
+    gb1[t] = a;
+  }
+  if (loss) {
+    for (int off = 32; off > 0; off >>= 1) e2 += __shfl_xor(e2, off);
+    if ((t & 63) == 0) sm.red[t >> 6] = e2;
+    __syncthreads();
+    if (t == 0) atomicAdd(loss, (sm.red[0] + sm.red[1]) + (sm.red[2] + sm.red[3]));
+  }
+}
+
+// out[i] (+)= sum of the partials g[z gs + i] (sum_partials).
+__global__ void partial_sum_kernel(long long P, const float* g, int splits, long long gs, float* out,
+                                   int accumulate) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const float a = sum_partials(g, splits, gs, i);
+  out[i] = accumulate ? out[i] + a : a;
 }
 
 // ------------------------------------------------------------------ BF16 MFMA inference
@@ -477,10 +746,39 @@ void launch_mse_grad(long long n, const float* Y, const float* T, float scale, f
                      hipStream_t s) {
   hipLaunchKernelGGL(mse_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, Y, T, scale, dY, loss);
 }
-void launch_adam(long long n, float* p, const float* g, int splits, float* m, float* v, float lr, float b1, float b2,
-                 float eps, float b1t, float b2t, hipStream_t s) {
-  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, p, g, splits, m, v, lr, b1,
-                     b2, eps, b1t, b2t);
+void launch_adam(long long n, float* p, const float* g, int splits, long long gs, float* m, float* v, float lr,
+                 float b1, float b2, float eps, float b1t, float b2t, hipStream_t s) {
+  // 64-thread blocks spread the partial sums over more CUs
+  hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, n, p, g, splits, gs, m, v, lr,
+                     b1, b2, eps, b1t, b2t);
+}
+
+// The fused step's LDS (about 96 KB) is above the default limit; allow it once.
+static void mlp_step_attr() {
+  static const bool done = [] {
+    (void)hipFuncSetAttribute((const void*)mlp_step_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)sizeof(StepSmem));
+    return true;
+  }();
+  (void)done;
+}
+
+bool mlp_step_fits(const MlpDims& d) { return d.H == FH && d.din <= FI && d.dout <= FI; }
+
+int mlp_step_rows() { return kStepRows; }
+
+void launch_mlp_step(const MlpDims& d, long long B, long long row0, const float* X, const float* T,
+                     const float* theta, float* grad, long long gs, float scale, double* loss, hipStream_t s) {
+  mlp_step_attr();
+  const unsigned blocks = (unsigned)((B + kStepRows - 1) / kStepRows);
+  hipLaunchKernelGGL(mlp_step_kernel, dim3(blocks), dim3(256), sizeof(StepSmem), s, d.din, d.dout, B, row0, X, T,
+                     theta, grad, gs, scale, loss);
+}
+
+void launch_partial_sum(long long P, const float* g, int splits, long long gs, float* out, int accumulate,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(partial_sum_kernel, dim3((unsigned)((P + 63) / 64)), dim3(64), 0, s, P, g, splits, gs, out,
+                     accumulate);
 }
 
 // The inference kernels stage ~50 KB of BF16 weights in LDS per block (three blocks per

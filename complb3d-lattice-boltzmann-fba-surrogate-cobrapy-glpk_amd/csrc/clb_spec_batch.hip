@@ -5,13 +5,20 @@
 
 namespace clb {
 
-void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int grid, int grid8, hipStream_t s) {
+void launch_spec_batch(const Tableau& tb, SpecArgs sa, int warm, int method, int grid, int grid8, hipStream_t s) {
   if (tb.nc > 4) {
     // one voxel per group of eight lanes (clb_spec_group.h)
     const dim3 blk(256);
-    if (tb.ns <= 16) hipLaunchKernelGGL((spec_group_kernel<16>), dim3(grid8), blk, 0, s, tb, sa, warm);
-    else if (tb.ns <= 24) hipLaunchKernelGGL((spec_group_kernel<24>), dim3(grid8), blk, 0, s, tb, sa, warm);
-    else hipLaunchKernelGGL((spec_group_kernel<32>), dim3(grid8), blk, 0, s, tb, sa, warm);
+    const bool pol = method == CLB_BATCH_POLISH || (method == CLB_BATCH_AUTO && warm != CLB_WARM_REF);
+    if (pol) {
+      if (tb.ns <= 16) hipLaunchKernelGGL((spec_group_kernel<16, true>), dim3(grid8), blk, 0, s, tb, sa, warm, method);
+      else if (tb.ns <= 24) hipLaunchKernelGGL((spec_group_kernel<24, true>), dim3(grid8), blk, 0, s, tb, sa, warm, method);
+      else hipLaunchKernelGGL((spec_group_kernel<32, true>), dim3(grid8), blk, 0, s, tb, sa, warm, method);
+    } else {
+      if (tb.ns <= 16) hipLaunchKernelGGL((spec_group_kernel<16, false>), dim3(grid8), blk, 0, s, tb, sa, warm, method);
+      else if (tb.ns <= 24) hipLaunchKernelGGL((spec_group_kernel<24, false>), dim3(grid8), blk, 0, s, tb, sa, warm, method);
+      else hipLaunchKernelGGL((spec_group_kernel<32, false>), dim3(grid8), blk, 0, s, tb, sa, warm, method);
+    }
     return;
   }
   CoupledIO io{};
@@ -23,7 +30,8 @@ int spec_group_grid(int device) {
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 1024;
   int blocks = 0;
   // the largest instantiation bounds the others
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, spec_group_kernel<32>, 256, 0) != hipSuccess || blocks < 1)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, spec_group_kernel<32, true>, 256, 0) != hipSuccess ||
+      blocks < 1)
     blocks = 2;
   return prop.multiProcessorCount * blocks;
 }

@@ -14,10 +14,11 @@
 // whose every step adds the same two partial sums in every lane, so all eight lanes hold
 // bit-identical values and take the same decisions.
 //
-// The iteration is spec_kernel's: Levenberg-Marquardt on the smooth log residual with
-// Nielsen damping, Newton on the convex potential G when it stalls, Levenberg-Marquardt
-// again near the solution; the judge is the reference's PCF residual |f|_2 < tol
-// (src/complab3d_processors_part4_eqsolver.hh:664-726, 1033-1049).
+// From the reference's start the iteration is spec_kernel's: Levenberg-Marquardt on the
+// smooth log residual with Nielsen damping, Newton on the convex potential G when it stalls,
+// Levenberg-Marquardt again near the solution. From a caller's or the network's start it is
+// the polish below, which falls back to that iteration. The judge is the reference's PCF
+// residual |f|_2 < tol (src/complab3d_processors_part4_eqsolver.hh:664-726, 1033-1049).
 #pragma once
 
 #include "clb_spec_kernel.h"
@@ -52,6 +53,17 @@ __device__ __forceinline__ bool gall(bool pred, int base) {
   return ((__ballot(!pred) >> base) & 0xffull) == 0ull;
 }
 
+// The polish (the start of a solve from a caller's or the network's log activities, close to
+// the solution): Newton on G with the third-order correction of steps shorter than
+// kPolishCubic, the judge only at points reached by a step shorter than kJudgeStep, at most
+// kPolishEvals evaluations; a polish that runs out of them, or backtracks below
+// kPolishMinAlpha, continues as the full iteration from its current point, and so does one
+// whose first Newton step is longer than kPolishFar (log10 units).
+constexpr int kPolishEvals = 10;
+constexpr double kPolishFar = 1.0;
+constexpr double kPolishMinAlpha = 1.0 / 64.0;
+constexpr double kPolishCubic = 0.05;
+
 // Per-group exchange area, in doubles: species cr[NSM], cc[NSM], then an 8 x 8 matrix
 // (Jacobian rows or Cholesky rows) and fs[8]; one spare double keeps the groups' areas on
 // different LDS banks.
@@ -61,9 +73,11 @@ constexpr int grp_area() { return 2 * NSM + 64 + 8 + 1; }
 // Held to three wavefronts per SIMD (168 VGPRs, a few spills): on the 128^3 field of the
 // 8 / 24 tableau it ran 105.8 M solves/s at the compiler's two (198 VGPRs), 115.3 M at three
 // and 105.4 M at four (128 VGPRs, 84 spilled).
-template <int NSM>
+// POLISH: whether the kernel carries the polish (a grid launched for the reference's starts
+// alone is built without it, which leaves the full iteration its registers).
+template <int NSM, bool POLISH>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
-spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
+spec_group_kernel(Tableau tb, SpecArgs sa, int warm, int method) {
   constexpr int NC = 8, NT = NSM / kGrp;
   constexpr int AREA = grp_area<NSM>();
   constexpr int OJ = 2 * NSM, OF = 2 * NSM + 64;
@@ -103,9 +117,9 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
 #pragma unroll
   for (int k = 0; k < NC; ++k) arow[k] = 0.0;
   unsigned int live = 0;  // bit k: component k has a positive total
-  double merit = 0.0, lam = -1.0, nu = 2.0, alpha = 1.0, best = 0.0, judge2 = 0.0;
+  double merit = 0.0, lam = -1.0, nu = 2.0, alpha = 1.0, best = 0.0, judge2 = 0.0, dmax = 0.0;
   int evals = 0, since = 0;
-  bool trial = false, gmode = false, lm_final = false;
+  bool trial = false, gmode = false, lm_final = false, pol = false, have_judge = false;
   unsigned long long my_solves = 0, my_conv = 0, my_iters = 0;
 
   while (true) {
@@ -139,14 +153,15 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
           item = v;
           have = true;
           trial = false;
-          gmode = false;
           lm_final = false;
+          have_judge = false;
           evals = 0;
           since = 0;
           lam = -1.0;
           nu = 2.0;
           alpha = 1.0;
           dc = 0.0;
+          dmax = 0.0;
 #pragma unroll
           for (int q = 0; q < 4; ++q) lkv[q] = (q < nvar) ? sa.logk_var[v * nvar + q] : 0.0;
           Tc = (c < tb.nc) ? sa.totals[v * tb.nc + c] : 0.0;
@@ -156,6 +171,8 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
           if (c >= tb.nc) xc = 0.0;
           else if (user) xc = clampd(sa.logc[v * tb.nc + c], kMinLog, kMaxLog);
           else xc = fast_log10(clampd(Tc, kMinConc, kMaxConc));
+          pol = POLISH && (method == CLB_BATCH_POLISH || (method == CLB_BATCH_AUTO && user));
+          gmode = pol;
         } else if (drained) {
           exhausted = true;
         }
@@ -165,7 +182,10 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
     if (__ballot(have) == 0) break;
     if (!have) continue;
 
-    // ---- one evaluation per group
+    // ---- one evaluation per group: the species, the merit of the current system and, where
+    // the point can end the solve, the reference judge. The system's matrix (J^T J, or the
+    // Hessian of G) is assembled below only when a new direction is needed, so the
+    // evaluation that meets the judge builds none.
     const double pc = trial ? clampd(xc + alpha * dc, kMinLog, kMaxLog) : xc;
     double pall[NC];
 #pragma unroll
@@ -199,68 +219,35 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
       sps = fma(fmax(s, 0.0), cr, sps);
       sns = fma(fmax(-s, 0.0), cr, sns);
     }
-    // the reference judge (clamped species, floors, +-10 clamp, zero-total guard) and the
-    // smooth residual on the raw species, for component c
     const bool livec = Tc > 1e-18;
     const double imu = smu[c];
-    double fj;
-    {
+    // the polish judges a point reached by a short step, and the last point of its budget:
+    // after a longer step the residual cannot meet the tolerance yet, so its logs are skipped
+    const bool judged = !pol || (trial && alpha * dmax < kJudgeStep) || evals + 1 >= kPolishEvals;
+    double e_judge2 = 0.0;
+    if (judged) {
+      // the reference judge (clamped species, floors, +-10 clamp, zero-total guard)
       const double SR = (Tc >= 0.0) ? pos : -Tc + pos;
       const double SP = (Tc >= 0.0) ? Tc + neg : neg;
-      fj = imu * fast_log10(fmax(SP, kMinConc) * fast_rcp(fmax(SR, kMinConc)));
+      double fj = imu * fast_log10(fmax(SP, kMinConc) * fast_rcp(fmax(SR, kMinConc)));
       fj = clampd(fj, -10.0, 10.0);
       if (!isfinite(fj) || !livec) fj = 0.0;
+      e_judge2 = gsum(fj * fj);
     }
-    double SRs = (Tc >= 0.0) ? sps : -Tc + sps;
-    double SPs = (Tc >= 0.0) ? Tc + sns : sns;
-    SRs = fmax(SRs, 1e-300);
-    SPs = fmax(SPs, 1e-300);
-    const double fsc = livec ? imu * (fast_log10(SPs) - fast_log10(SRs)) : 0.0;
-    const double rP = livec ? imu / SPs : 0.0;
-    const double rR = livec ? imu / SRs : 0.0;
-    const double e_judge2 = gsum(fj * fj);
-    const double e_fsn = sqrt(gsum(fsc * fsc));
-    // row c of the Jacobian of the chosen system
-    double jrow[NC];
-#pragma unroll
-    for (int k = 0; k < NC; ++k) jrow[k] = 0.0;
-#pragma unroll 2
-    for (int i = 0; i < NSM; ++i) {
-      const double s = sS[i][c];
-      const double cr = X[i];
-      const double w = gmode ? kLn10 * s * cr : -s * cr * ((s < 0.0) ? rP : rR);
-#pragma unroll
-      for (int k = 0; k < NC; ++k) jrow[k] = fma(w, sS[i][k], jrow[k]);
-    }
-    double e_merit, e_rhs, e_arow[NC];
+    double fsc = 0.0, rP = 0.0, rR = 0.0, e_fsn = 0.0, e_merit;
     if (gmode) {
       e_merit = gsum(gpart) / kLn10 - gsum(Tc * pc);
-#pragma unroll
-      for (int k = 0; k < NC; ++k)
-        e_arow[k] = (livec && ((live >> k) & 1u)) ? jrow[k] : ((k == c) ? 1.0 : 0.0);
-      e_rhs = livec ? -(sps - sns - Tc) : 0.0;
     } else {
+      // the smooth residual on the raw species, for component c
+      double SRs = (Tc >= 0.0) ? sps : -Tc + sps;
+      double SPs = (Tc >= 0.0) ? Tc + sns : sns;
+      SRs = fmax(SRs, 1e-300);
+      SPs = fmax(SPs, 1e-300);
+      fsc = livec ? imu * (fast_log10(SPs) - fast_log10(SRs)) : 0.0;
+      rP = livec ? imu / SPs : 0.0;
+      rR = livec ? imu / SRs : 0.0;
+      e_fsn = sqrt(gsum(fsc * fsc));
       e_merit = 0.5 * e_fsn * e_fsn;
-      // a component under the zero-total guard keeps its starting activity: drop its column
-#pragma unroll
-      for (int k = 0; k < NC; ++k) {
-        if (!((live >> k) & 1u)) jrow[k] = 0.0;
-        X[OJ + c * NC + k] = jrow[k];
-      }
-      X[OF + c] = fsc;
-      wave_sync();
-      // row c of J^T J and -(J^T fs)_c
-      double g = 0.0;
-#pragma unroll
-      for (int k = 0; k < NC; ++k) e_arow[k] = 0.0;
-#pragma unroll
-      for (int r = 0; r < NC; ++r) {
-        const double jrc = X[OJ + r * NC + c];
-        g = fma(jrc, X[OF + r], g);
-#pragma unroll
-        for (int k = 0; k < NC; ++k) e_arow[k] = fma(jrc, X[OJ + r * NC + k], e_arow[k]);
-      }
-      e_rhs = -g;
     }
     ++evals;
 
@@ -285,27 +272,36 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
       } else {
         const double slope = -gsum(rhsc * dc);
         accept = e_merit <= merit + 1e-4 * alpha * slope;
+        // near the solution G changes below its rounding while the judge still falls
+        if (!accept && pol && judged) accept = e_judge2 < (have_judge ? judge2 : tol2);
         if (!accept) alpha *= 0.5;
       }
     }
-    bool done = false, converged = false, resolve = true;
+    bool done = false, converged = false, resolve = true, lam_init = false;
+    // leave the polish for the full iteration, which starts over at the current point
+    auto leave_polish = [&]() {
+      pol = false;
+      gmode = false;
+      lm_final = false;
+      lam = -1.0;
+      nu = 2.0;
+      since = 0;
+      trial = false;
+      resolve = false;
+    };
     if (accept) {
       xc = pc;
-      rhsc = e_rhs;
-#pragma unroll
-      for (int k = 0; k < NC; ++k) arow[k] = e_arow[k];
       merit = e_merit;
-      judge2 = e_judge2;
-      if (judge2 < tol2) {
+      have_judge = judged;
+      if (judged) judge2 = e_judge2;
+      if (judged && judge2 < tol2) {
         done = true;
         converged = true;
+      } else if (pol) {
+        if (evals >= kPolishEvals) leave_polish();
       } else if (!gmode) {
         if (lam < 0.0) {
-          double dg = 0.0;
-#pragma unroll
-          for (int k = 0; k < NC; ++k)
-            if (k == c) dg = arow[k];
-          lam = fmax(1e-3 * gmax(dg) * fmin(1.0, e_fsn * e_fsn), 1e-300);
+          lam_init = true;
           best = e_fsn;
           since = 0;
         } else if (e_fsn < 0.95 * best) {
@@ -323,6 +319,13 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
         nu = 2.0;
         trial = false;
         resolve = false;
+      }
+    } else if (pol) {
+      if (alpha < kPolishMinAlpha || evals >= kPolishEvals) {
+        leave_polish();
+      } else {
+        resolve = false;
+        trial = true;
       }
     } else if (!gmode) {
       since += 1;
@@ -344,6 +347,54 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
     }
     if (!done && evals >= tb.max_iter) done = true;
     if (!done && resolve) {
+      if (accept) {
+        // row c of the system at the new point: the Hessian of G, or J^T J of the log residual
+        double jrow[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) jrow[k] = 0.0;
+#pragma unroll 2
+        for (int i = 0; i < NSM; ++i) {
+          const double s = sS[i][c];
+          const double cr = X[i];
+          const double w = gmode ? kLn10 * s * cr : -s * cr * ((s < 0.0) ? rP : rR);
+#pragma unroll
+          for (int k = 0; k < NC; ++k) jrow[k] = fma(w, sS[i][k], jrow[k]);
+        }
+        if (gmode) {
+#pragma unroll
+          for (int k = 0; k < NC; ++k)
+            arow[k] = (livec && ((live >> k) & 1u)) ? jrow[k] : ((k == c) ? 1.0 : 0.0);
+          rhsc = livec ? -(sps - sns - Tc) : 0.0;
+        } else {
+          // a component under the zero-total guard keeps its starting activity: drop its column
+#pragma unroll
+          for (int k = 0; k < NC; ++k) {
+            if (!((live >> k) & 1u)) jrow[k] = 0.0;
+            X[OJ + c * NC + k] = jrow[k];
+          }
+          X[OF + c] = fsc;
+          wave_sync();
+          // row c of J^T J and -(J^T fs)_c
+          double g = 0.0;
+#pragma unroll
+          for (int k = 0; k < NC; ++k) arow[k] = 0.0;
+#pragma unroll
+          for (int r = 0; r < NC; ++r) {
+            const double jrc = X[OJ + r * NC + c];
+            g = fma(jrc, X[OF + r], g);
+#pragma unroll
+            for (int k = 0; k < NC; ++k) arow[k] = fma(jrc, X[OJ + r * NC + k], arow[k]);
+          }
+          rhsc = -g;
+          if (lam_init) {
+            double dg = 0.0;
+#pragma unroll
+            for (int k = 0; k < NC; ++k)
+              if (k == c) dg = arow[k];
+            lam = fmax(1e-3 * gmax(dg) * fmin(1.0, e_fsn * e_fsn), 1e-300);
+          }
+        }
+      }
       // (A + lam I) d = rhs: Cholesky by rows across the group. Lane c holds row c; for
       // column k the pivot comes from lane k, then lane m's multiplier L_mk updates the
       // rows below.
@@ -371,33 +422,65 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm) {
           if (c >= m) a[m] -= lck * lmk;
         }
       }
-      // forward: L y = rhs
-      double acc = rhsc;
-#pragma unroll
-      for (int m = 0; m < NC; ++m) {
-        const double ym = __shfl(acc * invc, base + m);
-        if (c > m) acc -= a[m] * ym;
-      }
-      const double yc = acc * invc;
-      // backward: L^T d = y, with column c of L from the group's rows
+      // the group's rows of L in the exchange area, for the backward substitutions
       wave_sync();
 #pragma unroll
       for (int k = 0; k < NC; ++k) X[OJ + c * NC + k] = (k <= c) ? a[k] : 0.0;
       wave_sync();
-      acc = yc;
+      // L L^T d = r for component c's entry r: forward L y = r with row c of L in registers,
+      // backward L^T d = y with column c of L from the group's rows
+      auto chol_solve = [&](double r) {
+        double acc = r;
 #pragma unroll
-      for (int m = NC - 1; m >= 0; --m) {
-        const double dm = __shfl(acc * invc, base + m);
-        if (c < m) acc -= X[OJ + m * NC + c] * dm;
-      }
-      const double dnew = acc * invc;
+        for (int m = 0; m < NC; ++m) {
+          const double ym = __shfl(acc * invc, base + m);
+          if (c > m) acc -= a[m] * ym;
+        }
+        acc *= invc;
+#pragma unroll
+        for (int m = NC - 1; m >= 0; --m) {
+          const double dm = __shfl(acc * invc, base + m);
+          if (c < m) acc -= X[OJ + m * NC + c] * dm;
+        }
+        return acc * invc;
+      };
+      double dnew = chol_solve(rhsc);
       ok = gall(ok && isfinite(dnew), base);
+      if (ok && pol && gmax(fabs(dnew)) < kPolishCubic) {
+        // third-order (Chebyshev) correction of a short Newton step on G: c_i(x + d) =
+        // c_i(x) 10^(S_i.d) exactly, so the gradient at x + d is g + H d +
+        // (ln10^2 / 2) sum_i c_i S_i (S_i.d)^2 + O(d^3); solving H e = -(ln10^2 / 2)
+        // sum_i c_i S_i (S_i.d0)^2 leaves d0 + e an error of order |d0|^3
+        double dall[NC];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) dall[k] = __shfl(dnew, base + k);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+          const int i = c + kGrp * t;
+          double u = 0.0;
+#pragma unroll
+          for (int k = 0; k < NC; ++k) u = fma(sS[i][k], dall[k], u);
+          X[NSM + i] = X[i] * u * u;
+        }
+        wave_sync();
+        double t3 = 0.0;
+#pragma unroll 4
+        for (int i = 0; i < NSM; ++i) t3 = fma(sS[i][c], X[NSM + i], t3);
+        t3 = livec ? (-0.5 * kLn10 * kLn10) * t3 : 0.0;
+        const double e = chol_solve(t3);
+        if (gall(isfinite(dnew + e), base)) dnew += e;
+      }
       if (ok) {
         dc = dnew;
         const double mx = gmax(fabs(dc));
         if (mx > kMaxStep) dc *= kMaxStep / mx;
+        dmax = fmin(mx, kMaxStep);
         trial = true;
         alpha = 1.0;
+        // a first Newton step this long: the start is far from the solution
+        if (pol && evals == 1 && mx > kPolishFar) leave_polish();
+      } else if (pol) {
+        leave_polish();  // no usable Newton direction: the full iteration from here
       } else {
         gmode = true;  // no usable direction: take the convex route from here
         trial = false;

@@ -47,6 +47,11 @@ extern "C" {
 #define CLB_WARM_USER 1      /* caller-provided log activities */
 #define CLB_WARM_MLP 2       /* MLP prediction (BF16 MFMA), reference rule outside the training box */
 
+#define CLB_BATCH_AUTO 0     /* batched solve: the polish from caller / MLP starts, the full iteration
+                                from the reference rule's (clb_set_batch_method) */
+#define CLB_BATCH_FULL 1     /* the full iteration from every start */
+#define CLB_BATCH_POLISH 2   /* the polish from every start */
+
 #define CLB_SPEC_FUSED 0     /* speciation inside the transport kernel (clb_set_speciation_mode) */
 #define CLB_SPEC_SEPARATE 1  /* speciation in its own kernel after transport */
 
@@ -349,6 +354,16 @@ int clb_fd_biomass(clb_ctx* ctx, double nu, clb_ca_report* report);
 int clb_batch_load(clb_ctx* ctx, long n, const double* totals, int nvar, const int* var_species,
                    const double* logk_var);
 int clb_batch_solve(clb_ctx* ctx, int warm, const double* logc_in);
+/* Iteration of the batched solve for tableaux of more than four components (the solve of
+ * src/complab3d_processors_part4_eqsolver.hh:952-1172 restated; judge and tolerance are the
+ * reference's either way). CLB_BATCH_FULL: Levenberg-Marquardt on the log residual, Newton
+ * on the convex potential when it stalls. CLB_BATCH_POLISH: Newton on the convex potential
+ * with a third-order correction of short steps, the judge evaluated only after short steps,
+ * falling back to the full iteration after 10 evaluations; for starts near the solution.
+ * CLB_BATCH_AUTO (default): the polish from CLB_WARM_USER / CLB_WARM_MLP starts, the full
+ * iteration from the reference rule's. Tableaux of at most four components always use the
+ * full iteration. */
+int clb_set_batch_method(clb_ctx* ctx, int method);
 int clb_batch_fetch(clb_ctx* ctx, double* logc, int* iters, double* resid, uint8_t* conv);
 /* The batched solve on caller-owned device buffers, allocating nothing. No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code under HIP

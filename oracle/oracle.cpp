@@ -500,6 +500,23 @@ ORC_API void orc_pcf_residual(int nc, int ns, const double* S, const double* log
   for (long v = 0; v < n; ++v) pcf_residual(tb, logC + v * nc, T + v * nc, init.data(), f + v * nc);
 }
 
+// The same judge with a per-voxel logK for nvar species (var_species[k] takes
+// logk_var[v * nvar + k] at voxel v), as the batched solve's varying-logK inputs give it.
+ORC_API void orc_pcf_residual_var(int nc, int ns, const double* S, const double* logK, int nvar,
+                                  const int* var_species, const double* logk_var, long n, const double* logC,
+                                  const double* T, double* f) {
+#pragma omp parallel
+  {
+    std::vector<double> lk(logK, logK + ns), init(ns, 0.0);
+    Tableau tb{nc, ns, S, lk.data(), nullptr};
+#pragma omp for schedule(static)
+    for (long v = 0; v < n; ++v) {
+      for (int k = 0; k < nvar; ++k) lk[var_species[k]] = logk_var[v * nvar + k];
+      pcf_residual(tb, logC + v * nc, T + v * nc, init.data(), f + v * nc);
+    }
+  }
+}
+
 // Component totals of species vectors (calc_component_totals, :520-538).
 ORC_API void orc_component_totals(int nc, int ns, const double* S, long n, const double* conc, double* T) {
   std::vector<double> lk(ns, 0.0);

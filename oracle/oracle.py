@@ -36,6 +36,7 @@ def load(path=None):
         "orc_flow_velocity": ([ci, ci, ci, vp, vp, cd, cd, vp, vp, vp], None),
         "orc_speciate_pcf": ([ci, ci, vp, vp, vp, cl, vp, cd, ci, vp, vp, vp, vp, vp], None),
         "orc_pcf_residual": ([ci, ci, vp, vp, cl, vp, vp, vp], None),
+        "orc_pcf_residual_var": ([ci, ci, vp, vp, ci, vp, vp, cl, vp, vp, vp], None),
         "orc_component_totals": ([ci, ci, vp, cl, vp, vp], None),
         "orc_speciate_newton_reserve": ([ci, ci, vp, vp, vp, cl, vp, cd, ci, vp, vp, vp, vp], None),
         "orc_ade_init": ([cl, ci, vp, vp], None),
@@ -182,7 +183,9 @@ def speciate_newton_reserve(S, logK, comp_species, conc, tol=1e-12, max_iter=100
     return out, it, r, cv
 
 
-def pcf_residual(S, logK, logC, T):
+def pcf_residual(S, logK, logC, T, var_species=None, logk_var=None):
+    """The reference's judge per voxel; var_species / logk_var[n, nvar] give a per-voxel logK
+    to those species."""
     lib = load()
     S = _f(S)
     ns, nc = S.shape
@@ -190,7 +193,14 @@ def pcf_residual(S, logK, logC, T):
     T = _f(T)
     n = logC.shape[0]
     f = np.zeros((n, nc))
-    lib.orc_pcf_residual(nc, ns, _p(S), _p(_f(logK)), n, _p(logC), _p(T), _p(f))
+    if var_species is None:
+        lib.orc_pcf_residual(nc, ns, _p(S), _p(_f(logK)), n, _p(logC), _p(T), _p(f))
+    else:
+        vs = np.ascontiguousarray(var_species, dtype=np.int32)
+        lkv = _f(logk_var)
+        assert lkv.shape == (n, len(vs))
+        lib.orc_pcf_residual_var(nc, ns, _p(S), _p(_f(logK)), len(vs), _p(vs), _p(lkv), n, _p(logC), _p(T),
+                                 _p(f))
     return f
 
 

@@ -102,16 +102,14 @@ def test_warm_start_gain(clb, oracle):
 
 
 @pytest.mark.gpu
-def test_warm_start_gain_at_full_size(clb):
+def test_warm_start_gain_at_full_size(clb, oracle):
     """The warm-start setting at full size: 2 M plain-Newton training solves, 60 epochs, then
-    128^3 fresh samples solved from the reference-rule start and from the network's start,
-    both to the 1e-10 judge. The polish needs at least 3x fewer evaluations (deterministic:
-    fixed seeds, a fixed minibatch order and fixed-order gradient sums: 3.20x), and the
-    measured kernel time, inference included, is at least 2.5x lower. The time ratio sits
-    below the evaluation ratio because a polish of four evaluations pays the fixed costs of a
-    solve (the inference, the item's loads and stores) over fewer evaluations: 2.74x measured
-    against the eight-lane batch kernel (3.07x against the earlier one-lane kernel, which was
-    half as fast on both starts)."""
+    128^3 fresh samples solved from the reference-rule start (the full iteration) and from the
+    network's start (the polish), both to the 1e-10 judge, every answer judged again by the
+    oracle with its sample's logK. The network's start needs at least 3.5x fewer evaluations
+    (deterministic: fixed seeds, a fixed minibatch order and fixed-order gradient sums:
+    13.6 -> 3.33, 4.09x) and at least 3x less kernel time, inference included (3.37x to 3.49x
+    measured)."""
     print("case: mlp warm start gain, 2 M training solves, 128^3 samples", flush=True)
     S, lk = clb.gen_tableau(8, 24, seed=3)
     var = [8, 9, 10, 11]
@@ -142,6 +140,8 @@ def test_warm_start_gain_at_full_size(clb):
         lat.set_timing(False)
         x, it, r, cv = lat.batch_fetch()
         assert cv.all() and np.all(r < 1e-10)
+        f = np.linalg.norm(oracle.pcf_residual(S, lk, x, Tte, var_species=var, logk_var=Lte), axis=1)
+        assert f.max() < 1e-10, f"{name}: worst sample {int(f.argmax())}: {f.max():.3e}"
         out[name] = (p.spec_ms + p.mlp_ms, it.mean(), x)
     lat.close()
     assert np.max(np.abs(out["mlp"][2] - out["plain"][2])) < 1e-7
@@ -149,8 +149,8 @@ def test_warm_start_gain_at_full_size(clb):
     time_ratio = out["plain"][0] / out["mlp"][0]
     print(f"evaluations {out['plain'][1]:.2f} -> {out['mlp'][1]:.2f} ({eval_ratio:.2f}x); "
           f"kernel time {out['plain'][0]:.2f} -> {out['mlp'][0]:.2f} ms ({time_ratio:.2f}x)", flush=True)
-    assert eval_ratio >= 3.0
-    assert time_ratio >= 2.5
+    assert eval_ratio >= 3.5
+    assert time_ratio >= 3.0
 
 
 @pytest.mark.gpu

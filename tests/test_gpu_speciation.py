@@ -177,3 +177,52 @@ def test_coupled_far_from_equilibrium_converges(clb, oracle, mode):
     assert stats[0] == stats[1] == run.eq_stats[0]
     # every fluid voxel keeps positive concentrations (wall voxels report 0)
     assert np.all(np.isfinite(c)) and c[:, pack == 2].min() > 0
+
+
+@pytest.mark.gpu
+def test_batch_methods_from_caller_starts(clb, oracle, tableau, totals):
+    """The polish (Newton on the convex potential with the third-order correction of short
+    steps, judge after short steps, full iteration after 10 evaluations) against the full
+    iteration, from caller starts near the solution (0.03 log units of noise), far from it
+    (up to 3 log units) and non-finite (the reference rule is taken), and from the reference
+    rule's start. Every answer meets the oracle's judge and equals the full iteration's from
+    the reference start; near starts take fewer evaluations under the polish."""
+    print("case: batch methods 8/24, 32^3 voxels, caller starts", flush=True)
+    S, lk = tableau
+    n = totals.shape[0]
+    lat = clb.Lattice(4, 1, 1)
+    lat.set_tableau(S, lk, list(range(8)), tol=TOL, max_iter=200)
+    lat.batch_load(totals)
+    lat.set_batch_method(clb.BATCH_FULL)
+    lat.batch_solve(clb.WARM_REF)
+    x0, it0, _, cv0 = lat.batch_fetch()
+    assert cv0.all()
+    rng = np.random.default_rng(31)
+    near = x0 + rng.normal(0.0, 0.03, x0.shape)
+    far = x0 + rng.uniform(-3.0, 3.0, x0.shape)
+    far[::97, 3] = np.nan                       # these rows fall back to the reference rule
+    mean_evals = {}
+    for start_name, warm, start in (("ref", clb.WARM_REF, None), ("near", clb.WARM_USER, near),
+                                    ("far", clb.WARM_USER, far)):
+        for meth_name, meth in (("auto", clb.BATCH_AUTO), ("full", clb.BATCH_FULL), ("polish", clb.BATCH_POLISH)):
+            lat.set_batch_method(meth)
+            lat.batch_solve(warm, start)
+            x, it, r, cv = lat.batch_fetch()
+            case = f"{start_name}/{meth_name}"
+            assert cv.all(), f"{case}: {(~cv.astype(bool)).sum()} voxels did not converge"
+            f = np.linalg.norm(oracle.pcf_residual(S, lk, x, totals), axis=1)
+            assert f.max() < TOL, f"{case}: worst voxel {int(f.argmax())}: {f.max():.3e}"
+            assert np.max(np.abs(x - x0)) < 1e-7, case
+            mean_evals[case] = it.mean()
+    lat.close()
+    print({k: round(v, 2) for k, v in mean_evals.items()}, flush=True)
+    assert mean_evals["near/auto"] == mean_evals["near/polish"]
+    assert mean_evals["ref/auto"] == mean_evals["ref/full"]
+    assert mean_evals["near/polish"] < mean_evals["near/full"]
+    assert mean_evals["near/polish"] < 4.0
+    with pytest.raises(clb.ClbError):
+        lat2 = clb.Lattice(4, 1, 1)
+        try:
+            lat2.set_batch_method(7)
+        finally:
+            lat2.close()

@@ -124,3 +124,23 @@ def test_pcf_residual_is_the_judge(oracle):
     assert np.linalg.norm(f) < 1e-11
     f2 = oracle.pcf_residual(CARB_S, CARB_LOGK, x + np.array([[0.0, 1e-3]]), Tc)
     assert np.linalg.norm(f2) > 1e-4
+
+
+def test_pcf_residual_with_per_voxel_logk(oracle, clb):
+    """The judge with a per-voxel logK for some species (the batched solve's varying-logK
+    inputs) equals the shared-logK judge called voxel by voxel with that voxel's logK; a
+    wrong stride or species index in the per-voxel form fails it."""
+    S, lk = clb.gen_tableau(8, 24, seed=3)
+    rng = np.random.default_rng(21)
+    n, var = 40, [8, 13, 19]
+    T = 10 ** rng.uniform(-6, -2, (n, 8))
+    x = rng.uniform(-8, -2, (n, 8))
+    lkv = lk[var][None, :] + rng.uniform(-1, 1, (n, len(var)))
+    f = oracle.pcf_residual(S, lk, x, T, var_species=var, logk_var=lkv)
+    for v in range(n):
+        lkv_v = lk.copy()
+        lkv_v[var] = lkv[v]
+        fv = oracle.pcf_residual(S, lkv_v, x[v:v + 1], T[v:v + 1])
+        assert np.array_equal(f[v], fv[0])
+    # the varying logK matters: the shared-logK judge differs
+    assert not np.allclose(f, oracle.pcf_residual(S, lk, x, T))
