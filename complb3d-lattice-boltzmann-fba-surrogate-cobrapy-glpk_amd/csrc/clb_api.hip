@@ -133,7 +133,9 @@ struct clb_ctx {
   int* d_list = nullptr;
   long long n_reactive = 0;
   bool list_dirty = true;
-  unsigned int* d_counter = nullptr;  // [0] work queue, [1] node conversion, [2] stragglers
+  unsigned int* d_counter = nullptr;  // [0] work queue, [1] node conversion, [2] stragglers,
+                                      // [3] stragglers left by their first pass (flag, in a
+                                      // step), or the fluid count (count_fluid, outside)
   unsigned long long* d_stats = nullptr;
   int spec_grid = 0;
   int spec_grid8 = 0;  // the batch kernel's for more than four components
@@ -659,8 +661,10 @@ int prepare_fused(clb_ctx* c) {
   if (!rc && !c->d_sx) rc = dalloc(c, &c->d_sx, (size_t)4 * (size_t)c->geo.N);
   if (rc) return rc;
   const int bt = (int)(c->spec_block.x * c->spec_block.y);
-  const long long nw = std::max(fused_waves(c->geo, c->spec_block),
-                                c->ccap > 0 ? fused_waves_compact(c->ccap, bt) : 0ll);
+  // (the stragglers' first pass adds its statistics to the pair of its block index, so
+  // there are at least as many pairs as the persistent grid has blocks)
+  const long long nw = std::max({fused_waves(c->geo, c->spec_block),
+                                 c->ccap > 0 ? fused_waves_compact(c->ccap, bt) : 0ll, (long long)c->spec_grid});
   if (c->blk_stats_n < nw) {
     // a new block shape: fold the fused solves counted so far into the totals first
     if (c->d_blk_stats && c->blk_stats_n > 0) {
@@ -781,7 +785,7 @@ void enqueue_step_compact(clb_ctx* c) {
     a.pend_r = cpend_buf<R>(c, odd, false);
     a.pend_w = cpend_buf<R>(c, odd, true);
     c->pend_dirty = true;
-    launch_fill_u32(c->d_counter + 2, 1, 0u, c->stream);
+    launch_fill_u32(c->d_counter + 2, 2, 0u, c->stream);  // stragglers, and the flag of the ones left
     const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats, c->d_sx, c->fused_evals};
     {
       Timed tm(c, 1);
@@ -798,6 +802,9 @@ void enqueue_step_compact(clb_ctx* c) {
     sa.logc = c->d_sx;
     sa.counter = c->d_counter;
     sa.stats = c->d_stats;
+    sa.blk_stats = c->d_blk_stats;
+    sa.blk_stats_n = c->blk_stats_n;
+    sa.more = c->d_counter + 3;
     AdeArgs<R> as = a;
     as.geo.N = c->ccap;
     {
@@ -842,7 +849,7 @@ void enqueue_step(clb_ctx* c) {
   }
   if (fused) {
     c->pend_dirty = true;
-    launch_fill_u32(c->d_counter + 2, 1, 0u, c->stream);
+    launch_fill_u32(c->d_counter + 2, 2, 0u, c->stream);  // stragglers, and the flag of the ones left
     const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats, c->d_sx, c->fused_evals};
     {
       Timed tm(c, 1);
@@ -858,6 +865,9 @@ void enqueue_step(clb_ctx* c) {
     sa.logc = c->d_sx;  // each continues from where its fused solve stopped
     sa.counter = c->d_counter;
     sa.stats = c->d_stats;
+    sa.blk_stats = c->d_blk_stats;
+    sa.blk_stats_n = c->blk_stats_n;
+    sa.more = c->d_counter + 3;
     {
       Timed tm(c, 2);
       launch_spec_coupled(a, odd, c->tb, sa, c->spec_grid, c->stream);

@@ -143,6 +143,9 @@ struct SpecArgs {
   unsigned int* counter;     // work-queue head
   unsigned long long* stats; // solves, converged, iterations
   const unsigned int* n_dev; // item count in device memory (overrides n when set)
+  unsigned long long* blk_stats;  // fused step: per-wavefront statistics pairs, and their number
+  long long blk_stats_n;
+  unsigned int* more;             // fused step: cleared before it; set when a straggler is left
 };
 
 // Transport step with the speciation fused in (clb_ade_spec.h).

@@ -129,20 +129,28 @@ spec_group_kernel(Tableau tb, SpecArgs sa, int warm, int method) {
     const unsigned long long want = __ballot(need && c == 0);
     if (want) {
       if (qcur >= qend && !drained) {
+        // a wavefront's first range is fixed (wavefront w of the grid takes items
+        // [w kChunk, (w + 1) kChunk)); the queue hands out ranges after those, and is never
+        // touched when the fixed ranges cover every item (atomics on one address serialise
+        // across the device)
         const int leader = __ffsll((long long)want) - 1;
-        unsigned int qb = 0;
-        if (lane == leader) {
-          // at a wavefront's first grab a drained queue is seen with a plain load
-          qb = first ? __hip_atomic_load(sa.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-          if ((long long)qb < n_items) qb = atomicAdd(sa.counter, kChunk);
+        const long long fixed = (long long)gridDim.x * (blockDim.x >> 6) * kChunk;
+        long long qb;
+        if (first) {
+          qb = ((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * kChunk;
+        } else if (fixed >= n_items) {
+          qb = n_items;
+        } else {
+          unsigned int q = 0;
+          if (lane == leader) q = atomicAdd(sa.counter, kChunk);
+          qb = fixed + (long long)__shfl(q, leader);
         }
         first = false;
-        qb = __shfl(qb, leader);
-        if ((long long)qb >= n_items) {
+        if (qb >= n_items) {
           drained = true;
         } else {
           qcur = qb;
-          qend = (long long)qb + kChunk < n_items ? (long long)qb + kChunk : n_items;
+          qend = qb + kChunk < n_items ? qb + kChunk : n_items;
         }
       }
       const long long avail = qend - qcur;

@@ -286,7 +286,10 @@ __device__ __forceinline__ double species_conc(const Tableau& tb, int i, const d
 //
 // The weighted species concentrations are left in crs[i * BT] (LDS), for the correction
 // of the Newton step.
-template <int NC, int NSM, int BT>
+// UNR: unroll count of the species loops (1 inside the transport kernel, whose registers
+// are spoken for; the stragglers' kernel unrolls them fully, so the species' exp10 chains
+// overlap).
+template <int NC, int NSM, int BT, int UNR = 1>
 __device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const double* T, bool JUDGE, double& G,
                                        double& judge2, double* A, double* rhs, double* crs) {
   constexpr int NP = NC * (NC + 1) / 2;
@@ -303,7 +306,7 @@ __device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const
   // one species at a time: the index is uniform, so the tableau entries come from scalar
   // loads and only one species' temporaries are live (the solve shares the kernel's
   // register budget with the transport phase)
-#pragma unroll 1
+#pragma unroll UNR
   for (int i = 0; i < NSM; ++i) {
     const double eqw = tb.eqw[i];
     double lc = tb.logK[i];
@@ -352,7 +355,7 @@ __device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const
 // third-order correction when the step is short, limited to kMaxStep. A component under
 // the zero-total guard (identity row, zero rhs) keeps d_j = 0. Returns false when A is
 // not positive definite.
-template <int NC, int NSM, int BT>
+template <int NC, int NSM, int BT, int UNR = 1>
 __device__ __forceinline__ bool newton_dir(const Tableau& tb, const double* A, const double* rhs, const double* T,
                                            const double* crs, double* d, double& dmax) {
   double L[NC * (NC + 1) / 2], inv[NC];
@@ -364,7 +367,7 @@ __device__ __forceinline__ bool newton_dir(const Tableau& tb, const double* A, c
     double t3[NC], e[NC];
 #pragma unroll
     for (int j = 0; j < NC; ++j) t3[j] = 0.0;
-#pragma unroll 1
+#pragma unroll UNR
     for (int i = 0; i < NSM; ++i) {
       double u = 0.0;
 #pragma unroll
@@ -469,6 +472,61 @@ __device__ __forceinline__ bool spec_fused(const Tableau& tb, const double* c, d
   return conv;
 }
 
+// Continuation of spec_fused from the trial point it handed over, for a voxel that had not
+// met the judge within the fused step's budget: the same Newton iteration on G with the
+// third-order correction of short steps, but judged from its first point, which may already
+// meet the judge. T: the voxel's component totals as spec_fused formed them; x: the trial
+// point in, and on failure the next trial point out. At most max_evals evaluations. Returns
+// true when the judge is met, with the species of the judged point in crs.
+template <int NC, int NSM, int BT, int UNR = 1>
+__device__ __forceinline__ bool spec_fused_continue(const Tableau& tb, const double* T, double* x, int& evals,
+                                                    int max_evals, double* crs) {
+  constexpr int NP = NC * (NC + 1) / 2;
+  const double tol2 = tb.tol * tb.tol;
+  double G = 0.0, judge2 = 0.0, slope = 0.0, alpha = 1.0, dmax = 0.0, d[NC];
+  {
+    double A[NP], rhs[NC];
+    eval_g<NC, NSM, BT, UNR>(tb, x, T, true, G, judge2, A, rhs, crs);
+    evals = 1;
+    if (judge2 < tol2) return true;
+    if (!newton_dir<NC, NSM, BT, UNR>(tb, A, rhs, T, crs, d, dmax)) return false;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) slope -= rhs[j] * d[j];
+  }
+  // trial points as in spec_fused: Armijo on G, or a judge below the current point's; the
+  // judge at points reached by a short step and at the budget's last point
+  bool conv = false;
+#pragma unroll 1
+  for (int it = 1; it < max_evals; ++it) {
+    double p[NC], A[NP], rhs[NC], Gp, jp = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) p[j] = clampd(x[j] + alpha * d[j], kMinLog, kMaxLog);
+    const bool judged = alpha * dmax < kJudgeStep || it + 1 == max_evals;
+    eval_g<NC, NSM, BT, UNR>(tb, p, T, judged, Gp, jp, A, rhs, crs);
+    ++evals;
+    const bool armijo = Gp <= G + 1e-4 * alpha * slope;
+    if (!(armijo || (judged && jp < judge2))) {
+      alpha *= 0.5;
+      continue;
+    }
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x[j] = p[j];
+    G = Gp;
+    if (judged) judge2 = jp;
+    if (judged && judge2 < tol2) { conv = true; break; }
+    if (!newton_dir<NC, NSM, BT, UNR>(tb, A, rhs, T, crs, d, dmax)) { alpha = 0.0; break; }
+    slope = 0.0;
+#pragma unroll
+    for (int j = 0; j < NC; ++j) slope -= rhs[j] * d[j];
+    alpha = 1.0;
+  }
+  if (!conv) {
+#pragma unroll
+    for (int j = 0; j < NC; ++j) x[j] = clampd(x[j] + alpha * d[j], kMinLog, kMaxLog);
+  }
+  return conv;
+}
+
 // Limited increment of a species whose equilibrium concentration is ceq (NaN guard and
 // clamp of :1187-1214, then the 10 % limiter and 1e-20 floor of :1434-1447); c is its
 // concentration before the solve. A species outside the equilibrium set keeps c.
@@ -498,6 +556,9 @@ struct CoupledIO {
   int odd_done;         // parity of the step whose post-collision values are in g
   int immobile[CLB_MAX_SOLUTES];
   void* pend_w;         // (R*) deferred increments: dC is written here instead of into g
+  unsigned long long* blk_stats;  // the fused step's statistics pairs (solves, evaluations)
+  long long blk_stats_n;          // and their number
+  unsigned int* more;             // set by the stragglers' first pass when it leaves a voxel
 };
 
 template <typename R>
@@ -536,6 +597,8 @@ __device__ __forceinline__ void add_increment(const CoupledIO& io, int s, int n,
 template <typename R, int NC, int NSM, int MODE>
 __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int warm, CoupledIO io) {
   constexpr int NP = NC * (NC + 1) / 2;
+  // the stragglers' first pass left nothing: no item to finish
+  if (MODE == 1 && io.more && *io.more == 0u) return;
   const int lane = threadIdx.x & 63;
   const long long n_items = sa.n_dev ? (long long)*sa.n_dev : sa.n;
   int vsp[4];
@@ -556,24 +619,33 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
   unsigned long long my_solves = 0, my_conv = 0, my_iters = 0;
 
   while (true) {
-    // ---- refill idle lanes from the work queue (one atomic per wavefront)
+    // ---- refill idle lanes. Every lane's first item is fixed (lane l of the grid takes item
+    // l); the work queue hands out the items from the grid's lane count on, one atomic per
+    // wavefront and refill, and is never touched when the grid covers every item. Atomics on
+    // one address serialise across the device, so a grid launched for few items (the fused
+    // step's stragglers) must not queue up one per wavefront.
     const bool need = !have && !exhausted;
     const unsigned long long want = __ballot(need);
     if (want) {
       const int leader = __ffsll((long long)want) - 1;
-      unsigned int base = 0;
-      if (lane == leader) {
-        // at a wavefront's first grab a drained queue is seen with a plain load, so a grid
-        // launched for few items does not queue up atomics on the one counter
-        base = first ? __hip_atomic_load(sa.counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-        if ((long long)base < n_items) base = atomicAdd(sa.counter, (unsigned int)__popcll(want));
+      const long long lanes = (long long)gridDim.x * blockDim.x;
+      long long base;
+      if (first) {
+        base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~63);  // every lane wants
+      } else if (lanes >= n_items) {
+        base = n_items;
+      } else {
+        unsigned int q = 0;
+        if (lane == leader) q = atomicAdd(sa.counter, (unsigned int)__popcll(want));
+        base = lanes + (long long)__shfl(q, leader);
       }
       first = false;
-      base = __shfl(base, leader);
       if (need) {
         const unsigned long long below = want & ((1ull << lane) - 1ull);
-        const long long v = (long long)base + __popcll(below);
-        if (v < n_items) {
+        const long long v = base + __popcll(below);
+        if (MODE == 1 && v < n_items && io.list[v] < 0) {
+          // solved by the first straggler pass: this lane takes another item next trip
+        } else if (v < n_items) {
           item = v;
           have = true;
           trial = false;
@@ -639,7 +711,8 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
         }
       }
     }
-    if (__ballot(have) == 0) break;
+    // a lane neither busy nor exhausted drew an entry the straggler pass had solved: refill
+    if (__ballot(have || !exhausted) == 0) break;
     if (!have) continue;
 
     // ---- one evaluation on every busy lane
@@ -802,9 +875,105 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
     atomicAdd(&sa.stats[0], my_solves);
     atomicAdd(&sa.stats[1], my_conv);
     atomicAdd(&sa.stats[2], my_iters);
-    // voxels continued from the fused step's trial points: its stragglers
-    if (MODE == 1 && sa.logc) atomicAdd(&sa.stats[3], my_solves);
   }
+}
+
+// The fused step's stragglers (io.list[v]: their nodes; sa.logc[4 v ..]: the trial points
+// their solve handed over; io.cbuf: their post-kinetics concentrations), first pass: each
+// continues the fused solve (spec_fused_continue) for at most kStragglerEvals evaluations
+// and, when it meets the judge, writes its limited increments to pend_w as fused_solve_phase
+// does and marks its list entry done (-1). A voxel still short of the judge keeps its
+// latest point in sa.logc for spec_kernel MODE 1, which skips the entries marked done, and
+// sets io.more; with io.more clear MODE 1 returns at once.
+// The lean solve holds few registers, so these few voxels finish in a few evaluations'
+// latency. No atomics: the number of stragglers is added to sa.stats[3] once per launch,
+// and each block adds its converged solves and evaluations to its own pair of io.blk_stats
+// (the per-wavefront accumulators of the fused step, summed when statistics are read; the
+// fused step has finished with them).
+constexpr int kStragglerEvals = 16;
+
+template <typename R, int NC, int NSM>
+__global__ void __launch_bounds__(256) spec_straggler_kernel(Tableau tb, SpecArgs sa, CoupledIO io) {
+  __shared__ double crs[NSM][256];
+  const long long n = *sa.n_dev;
+  const long long N = io.geo.N;
+  int* list = const_cast<int*>(io.list);
+  R* pend = (R*)io.pend_w;
+  unsigned long long my_solves = 0, my_evals = 0;
+  for (long long v = blockIdx.x * 256ll + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
+    const int node = list[v];
+    double c[NSM], T[NC], x[NC];
+#pragma unroll
+    for (int i = 0; i < NSM; ++i) c[i] = (i < tb.ns) ? fmax(io.cbuf[(long long)i * N + node], kMinConc) : kMinConc;
+    // calc_component_totals (:520-538), formed as spec_fused forms them
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < NSM; ++i)
+        t = fma(tb.eqw[i] * tb.S[i * CLB_MAX_COMPONENTS + j], clampd(c[i], kMinConc, kMaxConc), t);
+      T[j] = (j < tb.nc) ? fmax(t, kMinConc) : 0.0;
+      x[j] = (j < tb.nc) ? clampd(sa.logc[v * 4 + j], kMinLog, kMaxLog) : 0.0;
+    }
+    int evals = 0;
+    const bool conv =
+        spec_fused_continue<NC, NSM, 256, NSM>(tb, T, x, evals, kStragglerEvals, &crs[0][threadIdx.x]);
+    my_evals += (unsigned long long)evals;
+    if (conv) {
+      my_solves += 1;
+#pragma unroll
+      for (int s = 0; s < NSM; ++s) {
+        if (s >= tb.ns) continue;
+        double d = spec_increment_eq(tb.is_eq[s], crs[s][threadIdx.x], c[s]);
+        if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
+        pend[(long long)s * N + node] = (R)d;
+      }
+      list[v] = -1;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) sa.logc[v * 4 + j] = (j < NC && j < tb.nc) ? x[j] : 0.0;
+      if (io.more) *io.more = 1u;  // any writer: the flag only says "not none"
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    my_solves += __shfl_xor(my_solves, off);
+    my_evals += __shfl_xor(my_evals, off);
+  }
+  __shared__ unsigned long long red[2][4];
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = my_solves;
+    red[1][threadIdx.x >> 6] = my_evals;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned long long bs = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    const unsigned long long be = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+    if (io.blk_stats && (long long)blockIdx.x < io.blk_stats_n && (bs || be)) {
+      io.blk_stats[2 * blockIdx.x] += bs;
+      io.blk_stats[2 * blockIdx.x + 1] += be;
+    }
+    if (blockIdx.x == 0 && sa.stats && n > 0) atomicAdd(&sa.stats[3], (unsigned long long)n);
+  }
+}
+
+// The first straggler pass for the compiled fused shapes (nc padded up to 2, species up to
+// 4, 6 or 8); false when the tableau has none.
+template <typename R>
+bool dispatch_straggler(const Tableau& tb, const SpecArgs& sa, const CoupledIO& io, int grid, hipStream_t s) {
+  const int nc = tb.nc, ns = tb.ns;
+  if (nc < 1 || nc > 4 || ns > 8 || !sa.n_dev || !sa.logc) return false;
+  const int NC = (nc <= 2) ? 2 : nc;
+  const int NSM = (ns <= 4) ? 4 : ((ns <= 6) ? 6 : 8);
+  const dim3 blk(256);
+#define CLB_STRAG(NC_, NSM_) \
+  case NC_ * 10 + NSM_: hipLaunchKernelGGL((spec_straggler_kernel<R, NC_, NSM_>), dim3(grid), blk, 0, s, tb, sa, io); return true;
+  switch (NC * 10 + (NSM < NC ? NC : NSM)) {
+    CLB_STRAG(2, 4) CLB_STRAG(2, 6) CLB_STRAG(2, 8)
+    CLB_STRAG(3, 4) CLB_STRAG(3, 6) CLB_STRAG(3, 8)
+    CLB_STRAG(4, 4) CLB_STRAG(4, 6) CLB_STRAG(4, 8)
+    default: return false;
+  }
+#undef CLB_STRAG
 }
 
 template <typename R, int MODE>
