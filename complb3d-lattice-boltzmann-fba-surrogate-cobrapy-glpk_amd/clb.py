@@ -124,6 +124,7 @@ def load(path=None):
         "clb_node_conversion": [vp, c_int, c_int, c_int, c_int, c_double, c_double, vp],
         "clb_set_speciation_mode": [vp, c_int],
         "clb_set_fused_budget": [vp, c_int],
+        "clb_set_straggler_budget": [vp, c_int],
         "clb_set_layout": [vp, c_int],
         "clb_set_compact_order": [vp, c_int, c_int, c_int],
         "clb_set_block_shape": [vp, c_int, c_int, c_int],
@@ -653,6 +654,10 @@ class Lattice:
 
     def set_fused_budget(self, evals):
         self._chk(self.lib.clb_set_fused_budget(self.h, int(evals)), "set_fused_budget")
+
+    def set_straggler_budget(self, evals):
+        """Budget of the stragglers' first pass (0: the full iteration takes every straggler)."""
+        self._chk(self.lib.clb_set_straggler_budget(self.h, int(evals)), "set_straggler_budget")
 
     def set_block_shape(self, kernel, bx, by):
         self._chk(self.lib.clb_set_block_shape(self.h, int(kernel), int(bx), int(by)), "set_block_shape")

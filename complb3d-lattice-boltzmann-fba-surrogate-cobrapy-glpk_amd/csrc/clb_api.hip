@@ -145,6 +145,7 @@ struct clb_ctx {
   int* d_slist = nullptr;                   // voxels left to the persistent kernel
   double* d_sx = nullptr;                   // their next trial points (4 per straggler)
   int fused_evals = 3;                      // evaluation budget of the fused solve
+  int straggler_evals = 16;                 // and of the stragglers' first pass (0: none)
   unsigned long long* d_blk_stats = nullptr;  // per transport wavefront: solves, evaluations
   long long blk_stats_n = 0;                  // wavefronts covered
   // deferred speciation increments of the fused step: two S*N buffers alternating by step
@@ -662,9 +663,9 @@ int prepare_fused(clb_ctx* c) {
   if (rc) return rc;
   const int bt = (int)(c->spec_block.x * c->spec_block.y);
   // (the stragglers' first pass adds its statistics to the pair of its block index, so
-  // there are at least as many pairs as the persistent grid has blocks)
+  // there are at least as many pairs as its grid has blocks: 4 x the persistent grid)
   const long long nw = std::max({fused_waves(c->geo, c->spec_block),
-                                 c->ccap > 0 ? fused_waves_compact(c->ccap, bt) : 0ll, (long long)c->spec_grid});
+                                 c->ccap > 0 ? fused_waves_compact(c->ccap, bt) : 0ll, 4ll * c->spec_grid});
   if (c->blk_stats_n < nw) {
     // a new block shape: fold the fused solves counted so far into the totals first
     if (c->d_blk_stats && c->blk_stats_n > 0) {
@@ -805,6 +806,7 @@ void enqueue_step_compact(clb_ctx* c) {
     sa.blk_stats = c->d_blk_stats;
     sa.blk_stats_n = c->blk_stats_n;
     sa.more = c->d_counter + 3;
+    sa.straggler_evals = c->straggler_evals;
     AdeArgs<R> as = a;
     as.geo.N = c->ccap;
     {
@@ -868,6 +870,7 @@ void enqueue_step(clb_ctx* c) {
     sa.blk_stats = c->d_blk_stats;
     sa.blk_stats_n = c->blk_stats_n;
     sa.more = c->d_counter + 3;
+    sa.straggler_evals = c->straggler_evals;
     {
       Timed tm(c, 2);
       launch_spec_coupled(a, odd, c->tb, sa, c->spec_grid, c->stream);
@@ -1953,6 +1956,13 @@ int clb_set_fused_budget(clb_ctx* c, int evals) {
   if (!c || evals < 2 || evals > 64) return CLB_ERR_ARG;
   if (evals != c->fused_evals) drop_graph(c);  // captured launches hold the old budget
   c->fused_evals = evals;
+  return CLB_OK;
+}
+
+int clb_set_straggler_budget(clb_ctx* c, int evals) {
+  if (!c || evals < 0 || evals > 64) return CLB_ERR_ARG;
+  if (evals != c->straggler_evals) drop_graph(c);  // captured launches hold the old budget
+  c->straggler_evals = evals;
   return CLB_OK;
 }
 

@@ -146,6 +146,7 @@ struct SpecArgs {
   unsigned long long* blk_stats;  // fused step: per-wavefront statistics pairs, and their number
   long long blk_stats_n;
   unsigned int* more;             // fused step: cleared before it; set when a straggler is left
+  int straggler_evals;            // fused step: budget of the stragglers' first pass (0: none)
 };
 
 // Transport step with the speciation fused in (clb_ade_spec.h).

@@ -287,8 +287,8 @@ __device__ __forceinline__ double species_conc(const Tableau& tb, int i, const d
 // The weighted species concentrations are left in crs[i * BT] (LDS), for the correction
 // of the Newton step.
 // UNR: unroll count of the species loops (1 inside the transport kernel, whose registers
-// are spoken for; the stragglers' kernel unrolls them fully, so the species' exp10 chains
-// overlap).
+// are spoken for; a full unroll lets the species' exp10 chains overlap where registers
+// are free).
 template <int NC, int NSM, int BT, int UNR = 1>
 __device__ __forceinline__ void eval_g(const Tableau& tb, const double* x, const double* T, bool JUDGE, double& G,
                                        double& judge2, double* A, double* rhs, double* crs) {
@@ -880,17 +880,18 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
 
 // The fused step's stragglers (io.list[v]: their nodes; sa.logc[4 v ..]: the trial points
 // their solve handed over; io.cbuf: their post-kinetics concentrations), first pass: each
-// continues the fused solve (spec_fused_continue) for at most kStragglerEvals evaluations
+// continues the fused solve (spec_fused_continue) for at most sa.straggler_evals evaluations
 // and, when it meets the judge, writes its limited increments to pend_w as fused_solve_phase
 // does and marks its list entry done (-1). A voxel still short of the judge keeps its
 // latest point in sa.logc for spec_kernel MODE 1, which skips the entries marked done, and
 // sets io.more; with io.more clear MODE 1 returns at once.
-// The lean solve holds few registers, so these few voxels finish in a few evaluations'
-// latency. No atomics: the number of stragglers is added to sa.stats[3] once per launch,
+// The lean solve holds few registers (four wavefronts per SIMD), and the grid gives every
+// straggler a lane of its own (kStragglerGridScale times the persistent grid: 262 144 lanes
+// on 256 CUs), so a step's stragglers finish in a few evaluations' latency. No atomics: the number of stragglers is added to sa.stats[3] once per launch,
 // and each block adds its converged solves and evaluations to its own pair of io.blk_stats
 // (the per-wavefront accumulators of the fused step, summed when statistics are read; the
 // fused step has finished with them).
-constexpr int kStragglerEvals = 16;
+constexpr int kStragglerGridScale = 4;
 
 template <typename R, int NC, int NSM>
 __global__ void __launch_bounds__(256) spec_straggler_kernel(Tableau tb, SpecArgs sa, CoupledIO io) {
@@ -917,7 +918,7 @@ __global__ void __launch_bounds__(256) spec_straggler_kernel(Tableau tb, SpecArg
     }
     int evals = 0;
     const bool conv =
-        spec_fused_continue<NC, NSM, 256, NSM>(tb, T, x, evals, kStragglerEvals, &crs[0][threadIdx.x]);
+        spec_fused_continue<NC, NSM, 256>(tb, T, x, evals, sa.straggler_evals, &crs[0][threadIdx.x]);
     my_evals += (unsigned long long)evals;
     if (conv) {
       my_solves += 1;
@@ -961,12 +962,15 @@ __global__ void __launch_bounds__(256) spec_straggler_kernel(Tableau tb, SpecArg
 template <typename R>
 bool dispatch_straggler(const Tableau& tb, const SpecArgs& sa, const CoupledIO& io, int grid, hipStream_t s) {
   const int nc = tb.nc, ns = tb.ns;
-  if (nc < 1 || nc > 4 || ns > 8 || !sa.n_dev || !sa.logc) return false;
+  if (nc < 1 || nc > 4 || ns > 8 || !sa.n_dev || !sa.logc || sa.straggler_evals < 1) return false;
   const int NC = (nc <= 2) ? 2 : nc;
   const int NSM = (ns <= 4) ? 4 : ((ns <= 6) ? 6 : 8);
   const dim3 blk(256);
 #define CLB_STRAG(NC_, NSM_) \
-  case NC_ * 10 + NSM_: hipLaunchKernelGGL((spec_straggler_kernel<R, NC_, NSM_>), dim3(grid), blk, 0, s, tb, sa, io); return true;
+  case NC_ * 10 + NSM_:                                                                                       \
+    hipLaunchKernelGGL((spec_straggler_kernel<R, NC_, NSM_>), dim3(kStragglerGridScale * grid), blk, 0, s, tb, sa, \
+                       io);                                                                                           \
+    return true;
   switch (NC * 10 + (NSM < NC ? NC : NSM)) {
     CLB_STRAG(2, 4) CLB_STRAG(2, 6) CLB_STRAG(2, 8)
     CLB_STRAG(3, 4) CLB_STRAG(3, 6) CLB_STRAG(3, 8)

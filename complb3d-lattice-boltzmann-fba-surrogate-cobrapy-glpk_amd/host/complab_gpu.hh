@@ -299,6 +299,7 @@ class Lattice {
   // Execution choices with no reference counterpart (clb.h documents each).
   void setSpeciationMode(int mode) { chk(clb_set_speciation_mode(c_, mode), "clb_set_speciation_mode"); }
   void setFusedBudget(int evals) { chk(clb_set_fused_budget(c_, evals), "clb_set_fused_budget"); }
+  void setStragglerBudget(int evals) { chk(clb_set_straggler_budget(c_, evals), "clb_set_straggler_budget"); }
   void setLayout(int layout) { chk(clb_set_layout(c_, layout), "clb_set_layout"); }
   void setCompactOrder(int bx, int by, int bz) { chk(clb_set_compact_order(c_, bx, by, bz), "clb_set_compact_order"); }
   void setBlockShape(int kernel, int bx, int by) { chk(clb_set_block_shape(c_, kernel, bx, by), "clb_set_block_shape"); }

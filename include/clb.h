@@ -183,6 +183,12 @@ int clb_set_compact_order(clb_ctx* ctx, int bx, int by, int bz);
  * kernel in the same step, from the point its fused solve reached; a small budget keeps the
  * transport kernel's wavefronts from waiting on their slowest lane. */
 int clb_set_fused_budget(clb_ctx* ctx, int evals);
+/* Evaluation budget of the stragglers' first pass, 0..64 (no reference counterpart: a tuning
+ * choice, default 16). The first pass continues each straggler's fused solve (Newton on the
+ * convex potential with the third-order step) within this budget; the full iteration
+ * finishes what it leaves. 0: no first pass, the full iteration takes every straggler.
+ * Results meet the same judge either way. */
+int clb_set_straggler_budget(clb_ctx* ctx, int evals);
 #define CLB_KERNEL_FLOW 0    /* D3Q19 collide + stream (at most 256 threads) */
 #define CLB_KERNEL_ADE 1     /* D3Q7 transport without fused speciation (at most 256 threads) */
 #define CLB_KERNEL_FUSED 2   /* D3Q7 transport with the fused speciation (at most 512 threads) */

@@ -15,7 +15,7 @@ from conftest import d3q19_kind_omega, rel_l2
 from test_oracle_speciation import CARB_COMP, CARB_LOGK, CARB_S, carbonate_exact
 
 
-def run(clb, oracle, mode, graph, nseg=4, seg=20, budget=None, block=None):
+def run(clb, oracle, mode, graph, nseg=4, seg=20, budget=None, block=None, straggler_budget=None):
     nx, ny, nz = 32, 16, 16
     pack = clb.gen_sphere_pack(nx, ny, nz, 3.0, 5.0, 0.5, seed=0)
     kind, om = d3q19_kind_omega(0.8)
@@ -27,6 +27,8 @@ def run(clb, oracle, mode, graph, nseg=4, seg=20, budget=None, block=None):
     lat.set_speciation_mode(mode)
     if budget is not None:
         lat.set_fused_budget(budget)
+    if straggler_budget is not None:
+        lat.set_straggler_budget(straggler_budget)
     if block is not None:
         lat.set_block_shape(clb.KERNEL_FUSED, *block)
     lat.set_dynamics(kind, om)
@@ -72,12 +74,34 @@ def test_fused_graph_matches_separate_with_node_conversion(clb, oracle, budget, 
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("straggler_budget", [0, 1, 16])
+def test_straggler_passes_match_separate(clb, oracle, straggler_budget):
+    """The stragglers of a fused budget of 2 (about a quarter of the voxels) finished by the
+    first pass alone (16), mostly by the full iteration after a one-evaluation first pass
+    (1: the full iteration skips the entries the first pass solved), or by the full iteration
+    alone (0): the same fields as the separate path, to the solver tolerance."""
+    print(f"case: fused budget 2, straggler first-pass budget {straggler_budget}", flush=True)
+    cf, mf, sf, chf = run(clb, oracle, 0, True, budget=2, block=(64, 4), straggler_budget=straggler_budget)
+    if not _separate:
+        _separate["r"] = run(clb, oracle, 1, False)
+    cs, ms, ss, chs = _separate["r"]
+    assert sum(chf) > 0 and chf == chs
+    assert np.array_equal(mf, ms)
+    assert sf[0] == sf[1] and sf[0] == ss[0]
+    for s in range(4):
+        assert rel_l2(cf[s], cs[s]) < 2e-6
+
+
+@pytest.mark.gpu
 def test_launch_settings_rejected(clb):
     print("case: budget / block shape argument checks", flush=True)
     lat = clb.Lattice(32, 16, 8, precision=0)
     for bad in (1, 0, 65):
         with pytest.raises(clb.ClbError):
             lat.set_fused_budget(bad)
+    for bad in (-1, 65):
+        with pytest.raises(clb.ClbError):
+            lat.set_straggler_budget(bad)
     for kern, bx, by in ((clb.KERNEL_FUSED, 64, 16), (clb.KERNEL_ADE, 64, 8), (clb.KERNEL_FLOW, 32, 3), (7, 64, 4)):
         with pytest.raises(clb.ClbError):
             lat.set_block_shape(kern, bx, by)
