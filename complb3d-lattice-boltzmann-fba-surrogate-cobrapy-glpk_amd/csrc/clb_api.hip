@@ -739,8 +739,24 @@ int flush_pending_compact_t(clb_ctx* c) {
   return CLB_OK;
 }
 
+// Leaving the compact layout with the deferred increments of an even next step folded in on
+// the way (one pass instead of a flush pass, a zero fill and the scatter).
+template <typename R>
+int flush_to_dense_t(clb_ctx* c) {
+  AdeArgs<R> ca = compact_args<R>(c);
+  ca.pend_r = cpend_buf<R>(c, false, false);
+  ca.pend_w = cpend_buf<R>(c, false, true);
+  launch_compact_move_flush<R>(ca, (R*)c->d_g, (R*)c->d_pend, (R*)c->d_cpend, c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->cmp_active = false;
+  c->pend_dirty = false;
+  return CLB_OK;
+}
+
 int flush_pending(clb_ctx* c) {
   if (c->cmp_active && c->pend_dirty && c->d_pend && c->ade_ready) {
+    if ((c->ade_t & 1) == 0)  // the next step is even: the flush touches each node's own slots only
+      return c->fp32 ? flush_to_dense_t<float>(c) : flush_to_dense_t<double>(c);
     const int rc = c->fp32 ? flush_pending_compact_t<float>(c) : flush_pending_compact_t<double>(c);
     if (rc) return rc;
   }

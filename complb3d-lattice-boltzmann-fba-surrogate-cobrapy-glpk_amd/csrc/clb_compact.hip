@@ -93,7 +93,10 @@ __global__ void compact_table_kernel(const unsigned int* flag, const unsigned in
 // to_compact: dense -> compact; else compact -> dense. Populations of every fluid node and
 // the dense wall slot behind each of its bounce-back links; both deferred-increment buffers;
 // towards the compact layout, the velocity.
-template <typename R, bool TO_COMPACT>
+// FLUSH (compact -> dense, the next step even): the deferred increments (ca.pend_r, ca.pend_w)
+// are folded in on the way, as compact_flush_kernel<R, false> would fold them in first (each
+// node's own slots and its ghost slots, the same sums), and the dense increments are zero.
+template <typename R, bool TO_COMPACT, bool FLUSH = false>
 __global__ void compact_move_kernel(AdeArgs<R> ca, R* gd, const R* ud, R* pd, R* pc) {
   const long long j = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (j >= (long long)*ca.cnf) return;
@@ -102,13 +105,32 @@ __global__ void compact_move_kernel(AdeArgs<R> ca, R* gd, const R* ud, R* pd, R*
   const int n = ca.cnode[j];
   const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
   if (gd) {
+    CSlots L;
+    if (FLUSH) L = cslots(ca, (int)j);
     for (int s = 0; s < ca.S; ++s) {
+      R own = R(0), prev = R(0);  // the node's increments of the previous step and the one before
+      if (FLUSH) {
+        own = ca.pend_r[(long long)s * cap + j];
+        prev = ca.pend_w[(long long)s * cap + j];
+      }
+      const bool imm = ca.immobile[s] != 0;
 #pragma unroll
       for (int i = 0; i < 7; ++i) {
         R* dv = &gd[(long long)(s * 7 + i) * N + n];
         R* cv = &ca.g[(long long)(s * 7 + i) * st + j];
-        if (TO_COMPACT) *cv = *dv;
-        else *dv = *cv;
+        if (TO_COMPACT) {
+          *cv = *dv;
+        } else if (FLUSH) {
+          // add_pending's increment of the population read from slot (i, j) next step (slots it
+          // leaves alone are copied as they are)
+          const int src = L.s[opp_dir(i)];
+          const bool has = imm ? (i == 0) : (src >= 0);
+          const R v = (i == 0) ? own : (src >= 0 && src < cap ? ca.pend_r[(long long)s * cap + src] : prev);
+          const R add = imm ? v : R(Q7::w(i)) * v;
+          *dv = has ? *cv + add : *cv;
+        } else {
+          *dv = *cv;
+        }
       }
 #pragma unroll
       for (int d = 1; d < 7; ++d) {
@@ -117,6 +139,7 @@ __global__ void compact_move_kernel(AdeArgs<R> ca, R* gd, const R* ud, R* pd, R*
         R* dv = &gd[(long long)(s * 7 + d) * N + nb_index(x, y, z, d, g)];
         R* cv = &ca.g[(long long)(s * 7 + d) * st + v];
         if (TO_COMPACT) *cv = *dv;
+        else if (FLUSH && !imm) *dv = *cv + R(Q7::w(d)) * own;  // sent towards the wall last step
         else *dv = *cv;
       }
     }
@@ -131,7 +154,7 @@ __global__ void compact_move_kernel(AdeArgs<R> ca, R* gd, const R* ud, R* pd, R*
       R* dv = &pd[(long long)q * N + n];
       R* cv = &pc[(long long)q * cap + j];
       if (TO_COMPACT) *cv = *dv;
-      else *dv = *cv;
+      else *dv = FLUSH ? R(0) : *cv;
     }
   }
 }
@@ -224,6 +247,13 @@ void launch_compact_move(bool to_compact, const AdeArgs<R>& ca, R* g_dense, cons
 }
 
 template <typename R>
+void launch_compact_move_flush(const AdeArgs<R>& ca, R* g_dense, R* pend_dense, R* pend_compact, hipStream_t s) {
+  const unsigned blocks = (unsigned)((ca.ccap + 255) / 256);
+  hipLaunchKernelGGL((compact_move_kernel<R, false, true>), dim3(blocks), dim3(256), 0, s, ca, g_dense, nullptr,
+                     pend_dense, pend_compact);
+}
+
+template <typename R>
 void launch_compact_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s) {
   const unsigned blocks = (unsigned)((a.ccap + 255) / 256);
   if (odd_next) hipLaunchKernelGGL((compact_flush_kernel<R, true>), dim3(blocks), dim3(256), 0, s, a);
@@ -249,6 +279,8 @@ template void launch_compact_move<float>(bool, const AdeArgs<float>&, float*, co
                                          hipStream_t);
 template void launch_compact_move<double>(bool, const AdeArgs<double>&, double*, const double*, double*, double*,
                                           hipStream_t);
+template void launch_compact_move_flush<float>(const AdeArgs<float>&, float*, float*, float*, hipStream_t);
+template void launch_compact_move_flush<double>(const AdeArgs<double>&, double*, double*, double*, hipStream_t);
 template void launch_compact_flush<float>(const AdeArgs<float>&, bool, hipStream_t);
 template void launch_compact_flush<double>(const AdeArgs<double>&, bool, hipStream_t);
 template void launch_ade_step_compact<float>(const AdeArgs<float>&, bool, int, hipStream_t);

@@ -214,20 +214,37 @@ __global__ void __launch_bounds__(kEnergyThreads) energy_reduce_kernel(double* e
   }
 }
 
-__global__ void none_reach_kernel(const uint8_t* mask, const uint8_t* kind, Geom g, int q, unsigned int* flag) {
+// flag[0] = 1 when a no-dynamics voxel has a fluid neighbour among the Q - 1 of the stencil.
+// The kind table sits in LDS, and every neighbour's mask byte is loaded before any is tested,
+// so a voxel's loads overlap instead of following one another (most voxels test them all).
+template <int Q>
+__global__ void __launch_bounds__(256) none_reach_kernel(const uint8_t* mask, const uint8_t* kind, Geom g,
+                                                         unsigned int* flag) {
+  __shared__ uint8_t sk[256];
+  sk[threadIdx.x] = kind[threadIdx.x];
+  __syncthreads();
   const long long n = blockIdx.x * 256LL + threadIdx.x;
-  if (n >= g.N || kind[mask[n]] != CLB_KIND_NONE) return;
+  if (n >= g.N || sk[mask[n]] != CLB_KIND_NONE) return;
   const int x = (int)(n % g.nx), y = (int)((n / g.nx) % g.ny), z = (int)(n / ((long long)g.nx * g.ny));
+  const int ym = (y == 0) ? g.ny - 1 : y - 1, yp = (y == g.ny - 1) ? 0 : y + 1;
+  const int zm = (z == 0) ? g.nz - 1 : z - 1, zp = (z == g.nz - 1) ? 0 : z + 1;
+  uint8_t m[19];
+  bool ok[19];
+#pragma unroll
   for (int i = 1; i < 19; ++i) {
-    if (q == 7 && Q19::cx[i] * Q19::cx[i] + Q19::cy[i] * Q19::cy[i] + Q19::cz[i] * Q19::cz[i] != 1) continue;
+    ok[i] = false;
+    m[i] = 0;
+    if (Q == 7 && Q19::cx[i] * Q19::cx[i] + Q19::cy[i] * Q19::cy[i] + Q19::cz[i] * Q19::cz[i] != 1) continue;
     const int xx = x + Q19::cx[i];
-    if (xx < 0 || xx >= g.nx) continue;
-    const int yy = (y + Q19::cy[i] + g.ny) % g.ny, zz = (z + Q19::cz[i] + g.nz) % g.nz;
-    if (kind[mask[xx + (long long)g.nx * (yy + (long long)g.ny * zz)]] == CLB_KIND_FLUID) {
-      flag[0] = 1u;
-      return;
-    }
+    const int yy = (Q19::cy[i] == 0) ? y : ((Q19::cy[i] > 0) ? yp : ym);
+    const int zz = (Q19::cz[i] == 0) ? z : ((Q19::cz[i] > 0) ? zp : zm);
+    ok[i] = xx >= 0 && xx < g.nx;
+    m[i] = mask[ok[i] ? xx + (long long)g.nx * (yy + (long long)g.ny * zz) : n];
   }
+  bool hit = false;
+#pragma unroll
+  for (int i = 1; i < 19; ++i) hit = hit || (ok[i] && sk[m[i]] == CLB_KIND_FLUID);
+  if (hit) flag[0] = 1u;
 }
 
 // The initial state, at rest with rho linear in x from rho_in to rho_out, is the reference
@@ -347,7 +364,9 @@ void launch_flow_init(const Geom& g, R* f, hipStream_t s) {
 
 void launch_none_reach(const uint8_t* mask, const uint8_t* kind, const Geom& g, int q, unsigned int* flag,
                        hipStream_t s) {
-  hipLaunchKernelGGL(none_reach_kernel, dim3((unsigned)((g.N + 255) / 256)), dim3(256), 0, s, mask, kind, g, q, flag);
+  const dim3 grid((unsigned)((g.N + 255) / 256));
+  if (q == 7) hipLaunchKernelGGL((none_reach_kernel<7>), grid, dim3(256), 0, s, mask, kind, g, flag);
+  else hipLaunchKernelGGL((none_reach_kernel<19>), grid, dim3(256), 0, s, mask, kind, g, flag);
 }
 
 template <typename R>

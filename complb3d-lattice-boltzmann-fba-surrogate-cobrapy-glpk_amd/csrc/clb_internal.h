@@ -230,6 +230,10 @@ void launch_compact_table(const CompactBuild& b, const Geom& g, const BrickOrder
 template <typename R>
 void launch_compact_move(bool to_compact, const AdeArgs<R>& ca, R* g_dense, const R* u_dense, R* pend_dense,
                          R* pend_compact, hipStream_t s);
+// compact -> dense with the deferred increments (ca.pend_r / pend_w, the compact buffers)
+// folded in, for a next step of even parity; the dense increments come out zero
+template <typename R>
+void launch_compact_move_flush(const AdeArgs<R>& ca, R* g_dense, R* pend_dense, R* pend_compact, hipStream_t s);
 // pending_flush on the compact layout (a.pend_r / pend_w: the compact buffers, as for the
 // next step of parity odd_next)
 template <typename R>
