@@ -45,8 +45,8 @@ constexpr int fused_waves_per_eu() { return (sizeof(R) == 4 && S <= 6 && NC <= 3
 // threads of the block each solve one slot (post-kinetics concentrations sc, node index
 // snode), write the limited increments to pend_w (stride: N dense, ccap compact), or, for a
 // voxel that has not met the judge within io.evals evaluations, zeros there, its
-// concentrations to cbuf and its next trial point to io.sx, and append it to the straggler
-// list; then the per-wavefront statistics of block bid.
+// concentrations to cbuf and its next trial point to io.sx, and append it to block bid's
+// segment of the straggler list; then the per-wavefront statistics of block bid.
 template <typename R, int S, int NC, int BT>
 __device__ __forceinline__ void fused_solve_phase(const AdeArgs<R>& a, const Tableau& tb, const FusedSpecIO& io,
                                                   int lane, int count, double (&sc)[S][BT], double (&scr)[S][BT],
@@ -79,16 +79,18 @@ __device__ __forceinline__ void fused_solve_phase(const AdeArgs<R>& a, const Tab
       a.pend_w[(long long)s * stride + node] = (R)d;
     }
   }
+  // appended to the block's segment of the list (one counter per segment)
   const unsigned long long sb = __ballot(straggler);
   if (sb) {
+    const int seg = (int)(bid % kStragglerSegments);
     unsigned int base = 0;
-    if (wlane == 0) base = atomicAdd(io.scount, (unsigned int)__popcll(sb));
+    if (wlane == 0) base = atomicAdd(&io.scount[seg], (unsigned int)__popcll(sb));
     base = __shfl(base, 0);
     if (straggler) {
-      const unsigned int k = base + __popcll(sb & ((1ull << wlane) - 1ull));
+      const long long k = seg * io.segcap + base + __popcll(sb & ((1ull << wlane) - 1ull));
       io.slist[k] = node;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) io.sx[(long long)k * 4 + j] = (j < NC) ? xs[j] : 0.0;
+      for (int j = 0; j < 4; ++j) io.sx[k * 4 + j] = (j < NC) ? xs[j] : 0.0;
     }
   }
   // ---- statistics: per-wavefront accumulators (solves, evaluations), no atomics

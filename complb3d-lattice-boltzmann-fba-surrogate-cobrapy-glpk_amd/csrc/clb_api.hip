@@ -133,17 +133,20 @@ struct clb_ctx {
   int* d_list = nullptr;
   long long n_reactive = 0;
   bool list_dirty = true;
-  unsigned int* d_counter = nullptr;  // [0] work queue, [1] node conversion, [2] stragglers,
-                                      // [3] stragglers left by their first pass (flag, in a
-                                      // step), or the fluid count (count_fluid, outside)
+  unsigned int* d_counter = nullptr;  // [0] work queue, [1] node conversion, [3] the fluid count
+                                      // (count_fluid)
   unsigned long long* d_stats = nullptr;
   int spec_grid = 0;
   int spec_grid8 = 0;  // the batch kernel's for more than four components
   int batch_method = CLB_BATCH_AUTO;
   // speciation fused into the transport kernel (CLB_SPEC_FUSED) or after it (SEPARATE)
   int spec_mode = CLB_SPEC_FUSED;
-  int* d_slist = nullptr;                   // voxels left to the persistent kernel
+  int* d_slist = nullptr;                   // the fused step's stragglers, by segment
   double* d_sx = nullptr;                   // their next trial points (4 per straggler)
+  unsigned int* d_scnt = nullptr;           // [kStragglerSegments] per-segment counts, then the failures'
+  int* d_flist = nullptr;                   // stragglers the first pass leaves to the full iteration
+  double* d_fx = nullptr;                   // and their points
+  long long scap = 0;                       // capacity of the four straggler buffers
   int fused_evals = 3;                      // evaluation budget of the fused solve
   int straggler_evals = 16;                 // and of the stragglers' first pass (0: none)
   unsigned long long* d_blk_stats = nullptr;  // per transport wavefront: solves, evaluations
@@ -655,12 +658,39 @@ int to_compact(clb_ctx* c) {
   return CLB_OK;
 }
 
+// Segment capacity of the fused step's straggler list (kStragglerSegments segments; block b
+// appends to segment b mod kStragglerSegments): the voxels of the blocks of one segment.
+long long straggler_segcap(long long blocks, long long threads) {
+  return ((blocks + kStragglerSegments - 1) / kStragglerSegments) * threads;
+}
+long long segcap_dense(const clb_ctx* c) {
+  const dim3 b = c->spec_block;
+  const long long blocks = (long long)((c->geo.nx + b.x - 1) / b.x) * ((c->geo.ny + b.y - 1) / b.y) * c->geo.nz;
+  return straggler_segcap(blocks, (long long)b.x * b.y);
+}
+long long segcap_compact(const clb_ctx* c) {
+  const long long bt = (long long)c->spec_block.x * c->spec_block.y;
+  return straggler_segcap((c->ccap + bt - 1) / bt, bt);
+}
+
 // Buffers of the fused step (allocated outside any capture).
 int prepare_fused(clb_ctx* c) {
   int rc = CLB_OK;
-  if (!c->d_slist) rc = dalloc(c, &c->d_slist, (size_t)c->geo.N);
-  if (!rc && !c->d_sx) rc = dalloc(c, &c->d_sx, (size_t)4 * (size_t)c->geo.N);
+  if (!c->d_scnt) rc = dalloc(c, &c->d_scnt, (size_t)kStragglerSegments + 1);
   if (rc) return rc;
+  const long long scap = kStragglerSegments * std::max(segcap_dense(c), c->ccap > 0 ? segcap_compact(c) : 0ll);
+  if (c->scap < scap) {
+    drop_graph(c);  // captured launches hold the old buffers
+    rc = dalloc(c, &c->d_slist, (size_t)scap);
+    if (!rc) rc = dalloc(c, &c->d_sx, (size_t)4 * (size_t)scap);
+    if (!rc) rc = dalloc(c, &c->d_flist, (size_t)scap);
+    if (!rc) rc = dalloc(c, &c->d_fx, (size_t)4 * (size_t)scap);
+    if (rc) {
+      c->scap = 0;
+      return rc;
+    }
+    c->scap = scap;
+  }
   const int bt = (int)(c->spec_block.x * c->spec_block.y);
   // (the stragglers' first pass adds its statistics to the pair of its block index, so
   // there are at least as many pairs as its grid has blocks: 4 x the persistent grid)
@@ -793,6 +823,31 @@ void enqueue_mask_rule(clb_ctx* c) {
                       c->bb_code, c->mrule, c->bthr, c->stream);
 }
 
+// Arguments of the stragglers' kernels after a fused step whose list segments hold segcap:
+// the first pass reads the segments and leaves its failures to the full iteration, which
+// takes them (sa.n_dev, sa.iters, sa.logc: the failure count, list and points) from where
+// the first pass left them.
+SpecArgs straggler_args(clb_ctx* c, long long segcap) {
+  SpecArgs sa{};
+  sa.n = 0;
+  sa.n_dev = c->d_scnt + kStragglerSegments;
+  sa.iters = c->d_flist;
+  sa.logc = c->d_fx;
+  sa.counter = c->d_counter;
+  sa.stats = c->d_stats;
+  sa.blk_stats = c->d_blk_stats;
+  sa.blk_stats_n = c->blk_stats_n;
+  sa.straggler_evals = c->straggler_evals;
+  sa.seg_list = c->d_slist;
+  sa.seg_x = c->d_sx;
+  sa.seg_n = c->d_scnt;
+  sa.segcap = segcap;
+  sa.fail_list = c->d_flist;
+  sa.fail_x = c->d_fx;
+  sa.fail_n = c->d_scnt + kStragglerSegments;
+  return sa;
+}
+
 // One coupled step on the compact layout (the state was gathered by prepare_step).
 template <typename R>
 void enqueue_step_compact(clb_ctx* c) {
@@ -802,8 +857,9 @@ void enqueue_step_compact(clb_ctx* c) {
     a.pend_r = cpend_buf<R>(c, odd, false);
     a.pend_w = cpend_buf<R>(c, odd, true);
     c->pend_dirty = true;
-    launch_fill_u32(c->d_counter + 2, 2, 0u, c->stream);  // stragglers, and the flag of the ones left
-    const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats, c->d_sx, c->fused_evals};
+    const long long segcap = segcap_compact(c);
+    launch_fill_u32(c->d_scnt, kStragglerSegments + 1, 0u, c->stream);  // segment counts, failures
+    const FusedSpecIO io{c->d_slist, c->d_scnt, c->d_blk_stats, c->d_sx, c->fused_evals, segcap};
     {
       Timed tm(c, 1);
       launch_ade_spec_step_compact<R>(a, odd, (int)(c->spec_block.x * c->spec_block.y), c->tb, io, c->stream);
@@ -812,17 +868,7 @@ void enqueue_step_compact(clb_ctx* c) {
     // the stragglers, in the same step; their concentrations and increments use the compact
     // stride
     launch_fill_u32(c->d_counter, 1, 0u, c->stream);
-    SpecArgs sa{};
-    sa.n = 0;
-    sa.n_dev = c->d_counter + 2;
-    sa.iters = c->d_slist;
-    sa.logc = c->d_sx;
-    sa.counter = c->d_counter;
-    sa.stats = c->d_stats;
-    sa.blk_stats = c->d_blk_stats;
-    sa.blk_stats_n = c->blk_stats_n;
-    sa.more = c->d_counter + 3;
-    sa.straggler_evals = c->straggler_evals;
+    SpecArgs sa = straggler_args(c, segcap);
     AdeArgs<R> as = a;
     as.geo.N = c->ccap;
     {
@@ -867,26 +913,18 @@ void enqueue_step(clb_ctx* c) {
   }
   if (fused) {
     c->pend_dirty = true;
-    launch_fill_u32(c->d_counter + 2, 2, 0u, c->stream);  // stragglers, and the flag of the ones left
-    const FusedSpecIO io{c->d_slist, c->d_counter + 2, c->d_blk_stats, c->d_sx, c->fused_evals};
+    const long long segcap = segcap_dense(c);
+    launch_fill_u32(c->d_scnt, kStragglerSegments + 1, 0u, c->stream);  // segment counts, failures
+    const FusedSpecIO io{c->d_slist, c->d_scnt, c->d_blk_stats, c->d_sx, c->fused_evals, segcap};
     {
       Timed tm(c, 1);
       launch_ade_spec_step<R>(a, odd, c->spec_block, c->tb, io, c->stream);
     }
     count_launch(c, 1);
-    // voxels whose fused solve did not converge: the persistent kernel, same step
+    // voxels whose fused solve did not converge, in the same step: the first pass, then the
+    // full iteration for what it leaves
     launch_fill_u32(c->d_counter, 1, 0u, c->stream);
-    SpecArgs sa{};
-    sa.n = 0;
-    sa.n_dev = c->d_counter + 2;
-    sa.iters = c->d_slist;
-    sa.logc = c->d_sx;  // each continues from where its fused solve stopped
-    sa.counter = c->d_counter;
-    sa.stats = c->d_stats;
-    sa.blk_stats = c->d_blk_stats;
-    sa.blk_stats_n = c->blk_stats_n;
-    sa.more = c->d_counter + 3;
-    sa.straggler_evals = c->straggler_evals;
+    SpecArgs sa = straggler_args(c, segcap);
     {
       Timed tm(c, 2);
       launch_spec_coupled(a, odd, c->tb, sa, c->spec_grid, c->stream);
@@ -1326,6 +1364,7 @@ int clb_destroy(clb_ctx* c) {
                   c->d_u, c->d_b, c->d_cbuf, c->d_list, c->d_counter, c->d_stats, c->d_bt, c->d_blk,
                   c->d_blogc, c->d_biters, c->d_bres, c->d_bconv, c->d_theta, c->d_adam_m, c->d_adam_v,
                   c->d_box, c->d_work, c->d_immobile, c->d_phase, c->d_fstats, c->d_slist, c->d_blk_stats, c->d_pend, c->d_sx,
+                  c->d_scnt, c->d_flist, c->d_fx,
                   c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared,
                   c->d_caflag, c->d_water, c->d_srg, c->d_srg_inc, c->d_cnode, c->d_cnbr, c->d_cnf, c->d_cg,
                   c->d_cu, c->d_cpend, c->cb.flag, c->cb.cj, c->cb.lk, c->cb.cg, c->cb.scan_tmp, c->d_mask_prev,

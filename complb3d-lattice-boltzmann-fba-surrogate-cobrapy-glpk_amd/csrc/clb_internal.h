@@ -145,17 +145,34 @@ struct SpecArgs {
   const unsigned int* n_dev; // item count in device memory (overrides n when set)
   unsigned long long* blk_stats;  // fused step: per-wavefront statistics pairs, and their number
   long long blk_stats_n;
-  unsigned int* more;             // fused step: cleared before it; set when a straggler is left
   int straggler_evals;            // fused step: budget of the stragglers' first pass (0: none)
+  // fused step: the stragglers as the transport kernel left them, by segment (segment s holds
+  // seg_n[s] of them from s * segcap on), and where the first pass leaves what it does not
+  // solve for the full iteration (fail_n of them)
+  const int* seg_list;
+  const double* seg_x;
+  const unsigned int* seg_n;
+  long long segcap;
+  int* fail_list;
+  double* fail_x;
+  unsigned int* fail_n;
 };
+
+// The fused step's stragglers are appended to kStragglerSegments segments of the list, block b
+// of the transport kernel to segment b mod kStragglerSegments, each with a counter of its own:
+// atomics on one address serialise across the device, and one counter for every wavefront with
+// a straggler made the appends a bottleneck of the transport kernel. A segment holds at most
+// the voxels of its blocks: segcap = ceil(blocks / kStragglerSegments) x threads per block.
+constexpr int kStragglerSegments = 64;
 
 // Transport step with the speciation fused in (clb_ade_spec.h).
 struct FusedSpecIO {
-  int* slist;                     // voxels left to the persistent kernel (capacity N)
-  unsigned int* scount;           // their number (cleared before the step)
+  int* slist;                     // voxels left to the persistent kernel, by segment
+  unsigned int* scount;           // kStragglerSegments counters (cleared before the step)
   unsigned long long* blk_stats;  // per wavefront of the launch: solves, evaluations
   double* sx;                     // per straggler: log activities its solve continues from (stride 4)
   int evals;                      // evaluation budget of the fused solve (>= 2)
+  long long segcap;               // capacity of a segment
 };
 
 // ------------------------------------------------------------------ launchers

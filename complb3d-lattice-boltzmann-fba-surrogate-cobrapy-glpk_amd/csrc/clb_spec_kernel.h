@@ -558,7 +558,6 @@ struct CoupledIO {
   void* pend_w;         // (R*) deferred increments: dC is written here instead of into g
   unsigned long long* blk_stats;  // the fused step's statistics pairs (solves, evaluations)
   long long blk_stats_n;          // and their number
-  unsigned int* more;             // set by the stragglers' first pass when it leaves a voxel
 };
 
 template <typename R>
@@ -597,8 +596,6 @@ __device__ __forceinline__ void add_increment(const CoupledIO& io, int s, int n,
 template <typename R, int NC, int NSM, int MODE>
 __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int warm, CoupledIO io) {
   constexpr int NP = NC * (NC + 1) / 2;
-  // the stragglers' first pass left nothing: no item to finish
-  if (MODE == 1 && io.more && *io.more == 0u) return;
   const int lane = threadIdx.x & 63;
   const long long n_items = sa.n_dev ? (long long)*sa.n_dev : sa.n;
   int vsp[4];
@@ -643,9 +640,7 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
       if (need) {
         const unsigned long long below = want & ((1ull << lane) - 1ull);
         const long long v = base + __popcll(below);
-        if (MODE == 1 && v < n_items && io.list[v] < 0) {
-          // solved by the first straggler pass: this lane takes another item next trip
-        } else if (v < n_items) {
+        if (v < n_items) {
           item = v;
           have = true;
           trial = false;
@@ -711,8 +706,7 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
         }
       }
     }
-    // a lane neither busy nor exhausted drew an entry the straggler pass had solved: refill
-    if (__ballot(have || !exhausted) == 0) break;
+    if (__ballot(have) == 0) break;
     if (!have) continue;
 
     // ---- one evaluation on every busy lane
@@ -878,31 +872,38 @@ __global__ void __launch_bounds__(256) spec_kernel(Tableau tb, SpecArgs sa, int 
   }
 }
 
-// The fused step's stragglers (io.list[v]: their nodes; sa.logc[4 v ..]: the trial points
-// their solve handed over; io.cbuf: their post-kinetics concentrations), first pass: each
-// continues the fused solve (spec_fused_continue) for at most sa.straggler_evals evaluations
-// and, when it meets the judge, writes its limited increments to pend_w as fused_solve_phase
-// does and marks its list entry done (-1). A voxel still short of the judge keeps its
-// latest point in sa.logc for spec_kernel MODE 1, which skips the entries marked done, and
-// sets io.more; with io.more clear MODE 1 returns at once.
-// The lean solve holds few registers (four wavefronts per SIMD), and the grid gives every
-// straggler a lane of its own (kStragglerGridScale times the persistent grid: 262 144 lanes
-// on 256 CUs), so a step's stragglers finish in a few evaluations' latency. No atomics: the number of stragglers is added to sa.stats[3] once per launch,
-// and each block adds its converged solves and evaluations to its own pair of io.blk_stats
-// (the per-wavefront accumulators of the fused step, summed when statistics are read; the
-// fused step has finished with them).
+// The fused step's stragglers, first pass. The transport kernel left them by segment
+// (segment s: sa.seg_n[s] nodes from sa.seg_list[s * segcap] on, with the trial points its
+// solve handed over in sa.seg_x, stride 4; their post-kinetics concentrations in io.cbuf).
+// Each continues the fused solve (spec_fused_continue) for at most sa.straggler_evals
+// evaluations and, when it meets the judge, writes its limited increments to pend_w as
+// fused_solve_phase does. A voxel still short of the judge goes with its latest point to the
+// failure list (sa.fail_list, sa.fail_x, sa.fail_n), which spec_kernel MODE 1 finishes.
+// Wavefront w of the grid works on segment w mod kStragglerSegments, in turn with that
+// segment's other wavefronts. The lean solve holds few registers (four wavefronts per SIMD)
+// and the grid is kStragglerGridScale times the persistent one, so most stragglers have a
+// lane of their own and a step's stragglers finish in a few evaluations' latency. Atomics:
+// one per wavefront that leaves a voxel (rare), and the straggler count once per launch;
+// each block adds its solves and evaluations to its own pair of io.blk_stats (the fused
+// step's per-wavefront accumulators, summed when statistics are read; the fused step has
+// finished with them).
 constexpr int kStragglerGridScale = 4;
 
 template <typename R, int NC, int NSM>
 __global__ void __launch_bounds__(256) spec_straggler_kernel(Tableau tb, SpecArgs sa, CoupledIO io) {
   __shared__ double crs[NSM][256];
-  const long long n = *sa.n_dev;
   const long long N = io.geo.N;
-  int* list = const_cast<int*>(io.list);
   R* pend = (R*)io.pend_w;
+  const int lane = threadIdx.x & 63;
+  const long long gw = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;  // wavefront of the grid
+  const long long nw = ((long long)gridDim.x * blockDim.x) >> 6;
+  const int seg = (int)(gw % kStragglerSegments);
+  const long long wseg = nw / kStragglerSegments + ((long long)seg < nw % kStragglerSegments ? 1 : 0);
+  const long long ns = (long long)sa.seg_n[seg];
   unsigned long long my_solves = 0, my_evals = 0;
-  for (long long v = blockIdx.x * 256ll + threadIdx.x; v < n; v += (long long)gridDim.x * 256) {
-    const int node = list[v];
+  for (long long k = (gw / kStragglerSegments) * 64 + lane; k < ns; k += wseg * 64) {
+    const long long v = seg * sa.segcap + k;
+    const int node = sa.seg_list[v];
     double c[NSM], T[NC], x[NC];
 #pragma unroll
     for (int i = 0; i < NSM; ++i) c[i] = (i < tb.ns) ? fmax(io.cbuf[(long long)i * N + node], kMinConc) : kMinConc;
@@ -914,11 +915,12 @@ __global__ void __launch_bounds__(256) spec_straggler_kernel(Tableau tb, SpecArg
       for (int i = 0; i < NSM; ++i)
         t = fma(tb.eqw[i] * tb.S[i * CLB_MAX_COMPONENTS + j], clampd(c[i], kMinConc, kMaxConc), t);
       T[j] = (j < tb.nc) ? fmax(t, kMinConc) : 0.0;
-      x[j] = (j < tb.nc) ? clampd(sa.logc[v * 4 + j], kMinLog, kMaxLog) : 0.0;
+      x[j] = (j < tb.nc) ? clampd(sa.seg_x[v * 4 + j], kMinLog, kMaxLog) : 0.0;
     }
     int evals = 0;
-    const bool conv =
-        spec_fused_continue<NC, NSM, 256>(tb, T, x, evals, sa.straggler_evals, &crs[0][threadIdx.x]);
+    bool conv = false;
+    if (sa.straggler_evals > 0)
+      conv = spec_fused_continue<NC, NSM, 256>(tb, T, x, evals, sa.straggler_evals, &crs[0][threadIdx.x]);
     my_evals += (unsigned long long)evals;
     if (conv) {
       my_solves += 1;
@@ -929,11 +931,20 @@ __global__ void __launch_bounds__(256) spec_straggler_kernel(Tableau tb, SpecArg
         if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
         pend[(long long)s * N + node] = (R)d;
       }
-      list[v] = -1;
-    } else {
+    }
+    // voxels left to the full iteration, with the point they reached
+    const unsigned long long fb = __ballot(!conv);
+    if (fb) {
+      const int leader = __ffsll((long long)fb) - 1;
+      unsigned int fbase = 0;
+      if (lane == leader) fbase = atomicAdd(sa.fail_n, (unsigned int)__popcll(fb));
+      fbase = __shfl(fbase, leader);
+      if (!conv) {
+        const long long f = (long long)fbase + __popcll(fb & ((1ull << lane) - 1ull));
+        sa.fail_list[f] = node;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) sa.logc[v * 4 + j] = (j < NC && j < tb.nc) ? x[j] : 0.0;
-      if (io.more) *io.more = 1u;  // any writer: the flag only says "not none"
+        for (int j = 0; j < 4; ++j) sa.fail_x[f * 4 + j] = (j < NC && j < tb.nc) ? x[j] : 0.0;
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -953,16 +964,20 @@ __global__ void __launch_bounds__(256) spec_straggler_kernel(Tableau tb, SpecArg
       io.blk_stats[2 * blockIdx.x] += bs;
       io.blk_stats[2 * blockIdx.x + 1] += be;
     }
-    if (blockIdx.x == 0 && sa.stats && n > 0) atomicAdd(&sa.stats[3], (unsigned long long)n);
+    if (blockIdx.x == 0 && sa.stats) {
+      unsigned long long total = 0;
+      for (int q = 0; q < kStragglerSegments; ++q) total += sa.seg_n[q];
+      if (total) atomicAdd(&sa.stats[3], total);
+    }
   }
 }
 
-// The first straggler pass for the compiled fused shapes (nc padded up to 2, species up to
-// 4, 6 or 8); false when the tableau has none.
+// The first straggler pass for the fused shapes (nc padded up to 2, species up to 4, 6 or 8:
+// every tableau the fused step takes); false for a tableau outside them.
 template <typename R>
 bool dispatch_straggler(const Tableau& tb, const SpecArgs& sa, const CoupledIO& io, int grid, hipStream_t s) {
   const int nc = tb.nc, ns = tb.ns;
-  if (nc < 1 || nc > 4 || ns > 8 || !sa.n_dev || !sa.logc || sa.straggler_evals < 1) return false;
+  if (nc < 1 || nc > 4 || ns > 8 || !sa.seg_n || !sa.fail_n) return false;
   const int NC = (nc <= 2) ? 2 : nc;
   const int NSM = (ns <= 4) ? 4 : ((ns <= 6) ? 6 : 8);
   const dim3 blk(256);
