@@ -134,6 +134,8 @@ def full_coupled(clb, args):
         lat.set_block_shape(clb.KERNEL_FUSED, args.fused_block, 1)
     if args.fused_budget:
         lat.set_fused_budget(args.fused_budget)
+    if args.straggler_budget >= 0:
+        lat.set_straggler_budget(args.straggler_budget)
     lat.set_graph(True, args.interval)
     tracer = clb.ValueTracer(1.0, 1000.0, args.ns_converge)  # the driver's ns_convg2
     st = dict(conversions=0, resolves=0, resolve_steps=0, unconverged=0, percolation=False)
@@ -534,6 +536,8 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=10, help="steps per autotuner trial (0: default shapes)")
     ap.add_argument("--fused-block", type=int, default=0, help="fused kernel threads per block (0: tuned/default)")
     ap.add_argument("--fused-budget", type=int, default=0, help="fused evaluation budget (0: tuned/default)")
+    ap.add_argument("--straggler-budget", type=int, default=-1,
+                    help="evaluations of the stragglers' first pass (-1: the library's default)")
     ap.add_argument("--brick", default="", help="bx,by,bz: order of the compact layout's voxels (default: the library's)")
     ap.add_argument("--kin-steps", type=int, default=200)
     ap.add_argument("--spec-side", type=int, default=128)
