@@ -157,8 +157,8 @@ __device__ __forceinline__ void write_rest_state(const AdeArgs<R>& a, bool next_
 // neither solid nor bounce-back and whose mineral reaches precip_max becomes precip_code,
 // stamped with phase_id; any other voxel stamped with phase_id whose mineral falls below
 // reopen becomes pore_code with phase 0, whatever its mask was. changed[0] counts the
-// voxels whose mask code changed (a flow re-solve trigger); a wall voxel that reopens
-// restarts at rest. The flow's slots are stored relative to the reference profile
+// voxels whose mask code changed (a flow re-solve trigger), changed[1] those of them that
+// were walls for the flow and are fluid now; a wall voxel that reopens restarts at rest. The flow's slots are stored relative to the reference profile
 // 1 + fref0 + x * frdx of their own voxel (clb_flow.hip), so a population at rest at
 // density 1 in a slot of voxel x is -t_i (fref0 + x * frdx).
 template <typename R, bool ODD>
@@ -167,7 +167,7 @@ __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int
                                        R* flowf, bool flow_next_odd, double fref0, double frdx) {
   const long long n64 = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const Geom& g = a.geo;
-  bool flip = false;
+  bool flip = false, flow_open = false;
   if (n64 < g.N) {
     const int n = (int)n64;
     const int x = n % g.nx, y = (n / g.nx) % g.ny, z = n / (g.nx * g.ny);
@@ -191,6 +191,7 @@ __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int
       // judges "wall" by its own dynamics table
       if (a.kind[m] != CLB_KIND_FLUID) write_rest_state(a, ODD, x, y, z, n, nullptr);
       if (flowf && a.flow_kind[m] != CLB_KIND_FLUID) {
+        flow_open = true;
         const Nbr nb(x, y, z, g);
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
@@ -207,6 +208,8 @@ __global__ void node_conversion_kernel(AdeArgs<R> a, uint8_t* phase, int ms, int
   }
   const unsigned long long bal = __ballot(flip);
   if ((threadIdx.x & 63) == 0 && bal) atomicAdd(changed, (unsigned int)__popcll(bal));
+  const unsigned long long fo = __ballot(flow_open);
+  if ((threadIdx.x & 63) == 0 && fo) atomicAdd(changed + 1, (unsigned int)__popcll(fo));
 }
 
 // updateLocalMaskNtotalLattices3D with several pore types, after the step: every solute-fluid

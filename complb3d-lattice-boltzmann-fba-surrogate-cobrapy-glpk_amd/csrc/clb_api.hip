@@ -7,6 +7,7 @@
 // kinetics -> speciation -> mask update -> stream, done as one fused transport/kinetics
 // kernel followed by the speciation kernel.
 #include <algorithm>
+#include <climits>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -69,6 +70,19 @@ struct clb_ctx {
   double* d_energy = nullptr;   // energy sums + per-block partials of the flow step
   long long energy_n = 0;
   dim3 flow_block{64, 4, 1};
+  // compact (flow-fluid only) flow layout (clb_flow_compact.hip): the flow steps run on it when
+  // the layout is compact and no no-dynamics voxel streams (flow_compact_ok); the dense array
+  // stays the layout the other flow kernels read, brought up to date on demand
+  bool fcmp_active = false;           // the flow state lives in the compact arrays
+  bool fcmp_mirror = false;           // and the dense array holds the same state
+  bool fcmp_dirty = true;             // mask or flow dynamics changed since the table was built
+  long long fcap = 0, fgcap = 0, fstride = 0;  // node capacity, ghost capacity, plane stride
+  int* d_fnode = nullptr;
+  int* d_fnbr = nullptr;
+  unsigned int* d_fnf = nullptr;
+  void* d_fcf = nullptr;              // compact populations (19 * fstride)
+  CompactBuild fcb{};                 // build buffers (flags, scans)
+  long long fcb_N = 0;                // keys they were sized for
 
   // solutes
   int S = 0;
@@ -317,8 +331,37 @@ FlowArgs<R> flow_args(clb_ctx* c, double* energy) {
   a.dface_in = (R)(c->rho_in - 1.0 - a.rref0);
   a.dface_out = (R)(c->rho_out - 1.0 - (a.rref0 + a.rrdx * (double)(c->geo.nx - 1)));
   a.energy = energy;
+  a.fnode = nullptr;
+  a.fnbr = nullptr;
+  a.fnf = nullptr;
+  a.fcap = a.fstride = 0;
   return a;
 }
+
+// The same on the compact flow layout (its populations and table).
+template <typename R>
+FlowArgs<R> flow_cargs(clb_ctx* c, double* energy) {
+  FlowArgs<R> a = flow_args<R>(c, energy);
+  a.f = (R*)c->d_fcf;
+  a.fnode = c->d_fnode;
+  a.fnbr = c->d_fnbr;
+  a.fnf = c->d_fnf;
+  a.fcap = c->fcap;
+  a.fstride = c->fstride;
+  return a;
+}
+
+// Whether the flow steps run on the compact layout: requested, and no no-dynamics voxel
+// streams (those would need slots of their own in the table).
+// The scans of the table build count 18 links per brick-order key in an int.
+BrickOrder brick_order(const clb_ctx* c);
+bool flow_compact_ok(const clb_ctx* c) {
+  return c->layout == CLB_LAYOUT_COMPACT && !c->flow_leaky && 18 * brick_order(c).nkey + 1 < (long long)INT_MAX;
+}
+template <typename R>
+int flow_to_compact(clb_ctx* c);
+int flow_sync_dense(clb_ctx* c);
+int flow_to_dense(clb_ctx* c);
 
 template <typename R>
 AdeArgs<R> ade_args(clb_ctx* c) {
@@ -404,20 +447,28 @@ int upload_tables(clb_ctx* c) {
 
 template <typename R>
 int flow_steps_t(clb_ctx* c, int nsteps, double* energy) {
+  const bool cmp = flow_compact_ok(c);
+  {
+    const int rc = cmp ? flow_to_compact<R>(c) : flow_to_dense(c);
+    if (rc) return rc;
+  }
   if (energy) {
-    const long long need = 2 + 2 * flow_blocks(c->geo, c->flow_block);
-    if (c->energy_n != need) {
+    const long long nblk = cmp ? flow_compact_blocks(c->fcap, (int)(c->flow_block.x * c->flow_block.y)) : flow_blocks(c->geo, c->flow_block);
+    const long long need = 2 + 2 * nblk;
+    if (c->energy_n < need) {
       int rc = dalloc(c, &c->d_energy, (size_t)need);
       c->energy_n = rc ? 0 : need;
       if (rc) return rc;
     }
   }
+  if (cmp && nsteps > 0) c->fcmp_mirror = false;
   for (int t = 0; t < nsteps; ++t) {
     double* e = nullptr;
     if (energy) e = c->d_energy;  // every block writes its partial; the reduction writes [0..1]
     {
       Timed tm(c, 0);
-      launch_flow_step<R>(flow_args<R>(c, e), (c->flow_t & 1) != 0, c->flow_block, c->stream);
+      if (cmp) launch_flow_step_compact<R>(flow_cargs<R>(c, e), (c->flow_t & 1) != 0, (int)(c->flow_block.x * c->flow_block.y), c->stream);
+      else launch_flow_step<R>(flow_args<R>(c, e), (c->flow_t & 1) != 0, c->flow_block, c->stream);
     }
     count_launch(c, 0);
     c->flow_t++;
@@ -655,6 +706,98 @@ int to_compact(clb_ctx* c) {
   CLB_CHECK_HIP(c, hipGetLastError());
   c->cmp_active = true;
   c->cmp_u_dirty = false;
+  return CLB_OK;
+}
+
+// ---- compact flow layout
+// The dense flow array brought up to date with the compact state, which stays the state.
+int flow_sync_dense(clb_ctx* c) {
+  if (!c->fcmp_active || c->fcmp_mirror) return CLB_OK;
+  if (c->fp32) launch_flow_compact_move<float>(false, flow_cargs<float>(c, nullptr), (float*)c->d_f, c->stream);
+  else launch_flow_compact_move<double>(false, flow_cargs<double>(c, nullptr), (double*)c->d_f, c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->fcmp_mirror = true;
+  return CLB_OK;
+}
+
+// The dense flow array as the state (before a caller rewrites it, or a step on the dense layout).
+int flow_to_dense(clb_ctx* c) {
+  const int rc = flow_sync_dense(c);
+  c->fcmp_active = false;
+  return rc;
+}
+
+// Node table of the current mask and flow dynamics; buffers (re)allocated when the flow nodes
+// or wall links outgrow them.
+template <typename R>
+int build_flow_compact(clb_ctx* c) {
+  const BrickOrder o = brick_order(c);
+  const long long K = o.nkey;
+  CompactBuild& b = c->fcb;
+  int rc = CLB_OK;
+  if (c->fcb_N != K) {
+    rc = dalloc(c, &b.flag, (size_t)K);
+    if (!rc) rc = dalloc(c, &b.cj, (size_t)K);
+    if (!rc) rc = dalloc(c, &b.lk, (size_t)(18 * K + 1));
+    if (!rc) rc = dalloc(c, &b.cg, (size_t)(18 * K + 1));
+    const size_t tb = flow_compact_scan_bytes(K);
+    if (!rc) rc = dalloc(c, (uint8_t**)&b.scan_tmp, tb);
+    if (rc) {
+      c->fcb_N = 0;
+      return rc;
+    }
+    b.scan_tmp_bytes = tb;
+    launch_fill_u32(b.lk, 18 * K + 1, 0u, c->stream);  // the flags kernel leaves lk[18 K] zero
+    c->fcb_N = K;
+  }
+  launch_flow_compact_build(c->d_mask, c->d_kind, c->geo, o, b, c->stream);
+  unsigned int hf[2], hg[19];
+  CLB_CHECK_HIP(c, hipMemcpyAsync(&hf[0], b.flag + (K - 1), sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  CLB_CHECK_HIP(c, hipMemcpyAsync(&hf[1], b.cj + (K - 1), sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream));
+  for (int d = 0; d <= 18; ++d)
+    CLB_CHECK_HIP(c, hipMemcpyAsync(&hg[d], b.cg + (long long)d * K, sizeof(unsigned int), hipMemcpyDeviceToHost,
+                                    c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  const long long nf = (long long)hf[0] + hf[1];
+  long long maxlink = 0;
+  for (int d = 0; d < 18; ++d) maxlink = std::max<long long>(maxlink, (long long)hg[d + 1] - hg[d]);
+  if (nf > c->fcap || maxlink > c->fgcap || !c->d_fcf) {
+    // headroom, so that a few reopened voxels do not force a reallocation
+    const long long cap = std::max<long long>(256, ((nf + nf / 16 + 1024) + 255) / 256 * 256);
+    const long long gcap = ((maxlink + maxlink / 4 + 1024) + 63) / 64 * 64;
+    c->fcap = cap;
+    c->fgcap = gcap;
+    c->fstride = cap + gcap;
+    rc = dalloc(c, &c->d_fnode, (size_t)cap);
+    if (!rc) rc = dalloc(c, &c->d_fnbr, (size_t)(18 * cap));
+    if (!rc && !c->d_fnf) rc = dalloc(c, &c->d_fnf, 1);
+    if (!rc) rc = dalloc(c, (uint8_t**)&c->d_fcf, (size_t)19 * (size_t)c->fstride * sizeof(R));
+    if (rc) {
+      c->fcap = c->fgcap = c->fstride = 0;
+      return rc;
+    }
+  }
+  launch_flow_compact_table(b, c->geo, o, c->d_fnode, c->d_fnbr, c->fcap, c->d_fnf, c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->fcmp_dirty = false;
+  return CLB_OK;
+}
+
+// The flow state in the compact arrays: rebuild the table if the mask or the flow dynamics
+// changed (scattering first with the table it was gathered with), then gather.
+template <typename R>
+int flow_to_compact(clb_ctx* c) {
+  if (c->fcmp_active && !c->fcmp_dirty) return CLB_OK;
+  int rc = flow_to_dense(c);
+  if (rc) return rc;
+  if (c->fcmp_dirty || !c->d_fcf) {
+    rc = build_flow_compact<R>(c);
+    if (rc) return rc;
+  }
+  launch_flow_compact_move<R>(true, flow_cargs<R>(c, nullptr), (R*)c->d_f, c->stream);
+  CLB_CHECK_HIP(c, hipGetLastError());
+  c->fcmp_active = true;
+  c->fcmp_mirror = true;
   return CLB_OK;
 }
 
@@ -1368,7 +1511,8 @@ int clb_destroy(clb_ctx* c) {
                   c->d_age, c->d_age2, c->d_dpos, c->d_btot, c->d_caout, c->d_bold, c->d_cared,
                   c->d_caflag, c->d_water, c->d_srg, c->d_srg_inc, c->d_cnode, c->d_cnbr, c->d_cnf, c->d_cg,
                   c->d_cu, c->d_cpend, c->cb.flag, c->cb.cj, c->cb.lk, c->cb.cg, c->cb.scan_tmp, c->d_mask_prev,
-                  c->d_leak};
+                  c->d_leak, c->d_fnode, c->d_fnbr, c->d_fnf, c->d_fcf, c->fcb.flag, c->fcb.cj, c->fcb.lk,
+                  c->fcb.cg, c->fcb.scan_tmp};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1403,6 +1547,7 @@ int clb_upload_mask(clb_ctx* c, const uint8_t* mask) {
   c->perf.fluid_nodes = fluid;
   c->list_dirty = true;
   c->leak_dirty = true;
+  c->fcmp_dirty = true;
   c->cmp_dirty = true;
   return CLB_OK;
 }
@@ -1434,6 +1579,7 @@ int clb_set_dynamics(clb_ctx* c, const int* kind, const double* flow_omega) {
   }
   c->list_dirty = true;
   c->leak_dirty = true;
+  c->fcmp_dirty = true;
   c->cmp_dirty = true;
   return upload_tables(c);
 }
@@ -1472,6 +1618,7 @@ int clb_flow_init(clb_ctx* c, double rho_in, double rho_out) {
   c->fref_out = rho_out;
   if (c->fp32) launch_flow_init<float>(c->geo, (float*)c->d_f, c->stream);
   else launch_flow_init<double>(c->geo, (double*)c->d_f, c->stream);
+  c->fcmp_active = false;  // the dense array is the state again
   c->flow_t = 0;
   c->flow_ready = true;
   CLB_CHECK_HIP(c, hipGetLastError());
@@ -1489,6 +1636,10 @@ int clb_flow_steps(clb_ctx* c, int nsteps, double* energy) {
 int clb_flow_download_velocity(clb_ctx* c, double* u, double* rho) {
   if (!c || (!u && !rho)) return CLB_ERR_ARG;
   if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "clb_flow_init first");
+  {
+    const int rc0 = flow_sync_dense(c);
+    if (rc0) return rc0;
+  }
   double *du = nullptr, *dr = nullptr;
   int rc = dalloc(c, &du, (size_t)3 * c->geo.N);
   if (!rc) rc = dalloc(c, &dr, (size_t)c->geo.N);
@@ -1588,11 +1739,21 @@ int clb_couple_velocity(clb_ctx* c) {
   if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "no flow field");
   if (!c->d_u) return fail(c, CLB_ERR_STATE, "clb_set_solutes first");
   const bool odd_next = (c->flow_t & 1) != 0;
-  if (c->fp32)
-    launch_flow_velocity<float>(flow_args<float>(c, nullptr), odd_next, nullptr, nullptr, (float*)c->d_u, c->stream);
-  else
-    launch_flow_velocity<double>(flow_args<double>(c, nullptr), odd_next, nullptr, nullptr, (double*)c->d_u,
-                                 c->stream);
+  if (c->fcmp_active && !c->fcmp_dirty) {
+    // from the compact state: zero everywhere, then the velocity of every fluid node
+    const size_t rs = c->fp32 ? sizeof(float) : sizeof(double);
+    launch_fill_u32((unsigned int*)c->d_u, (long long)(3 * (size_t)c->geo.N * rs / 4), 0u, c->stream);
+    if (c->fp32) launch_flow_velocity_compact<float>(flow_cargs<float>(c, nullptr), odd_next, (float*)c->d_u, c->stream);
+    else launch_flow_velocity_compact<double>(flow_cargs<double>(c, nullptr), odd_next, (double*)c->d_u, c->stream);
+  } else {
+    const int rc = flow_sync_dense(c);
+    if (rc) return rc;
+    if (c->fp32)
+      launch_flow_velocity<float>(flow_args<float>(c, nullptr), odd_next, nullptr, nullptr, (float*)c->d_u, c->stream);
+    else
+      launch_flow_velocity<double>(flow_args<double>(c, nullptr), odd_next, nullptr, nullptr, (double*)c->d_u,
+                                   c->stream);
+  }
   CLB_CHECK_HIP(c, hipGetLastError());
   c->cmp_u_dirty = true;
   return CLB_OK;
@@ -1801,7 +1962,10 @@ int clb_step(clb_ctx* c, int nsteps) {
   if (c->have_tableau && c->tb.ns != c->S) return fail(c, CLB_ERR_ARG, "tableau species count != solute count");
   const int rc = c->fp32 ? step_t<float>(c, nsteps) : step_t<double>(c, nsteps);
   // the pore <-> biofilm rule may have changed codes whose flow dynamics differ
-  if (c->mask_update && nsteps > 0 && !rule_keeps_flow_kind(c)) c->leak_dirty = true;
+  if (c->mask_update && nsteps > 0 && !rule_keeps_flow_kind(c)) {
+    c->leak_dirty = true;
+    c->fcmp_dirty = true;  // a voxel may have become fluid for the flow
+  }
   return rc;
 }
 
@@ -1989,18 +2153,21 @@ int clb_set_block_shape(clb_ctx* c, int kernel, int bx, int by) {
 
 int clb_set_compact_order(clb_ctx* c, int bx, int by, int bz) {
   if (!c || bx < 0 || by < 0 || bz < 0) return CLB_ERR_ARG;
-  const int rc = to_dense(c);  // gathered in the old order
+  int rc = to_dense(c);  // gathered in the old order
+  if (!rc) rc = flow_to_dense(c);
   if (rc) return rc;
   c->brick[0] = bx;
   c->brick[1] = by;
   c->brick[2] = bz;
   c->cmp_dirty = true;
+  c->fcmp_dirty = true;
   return CLB_OK;
 }
 
 int clb_set_layout(clb_ctx* c, int layout) {
   if (!c || (layout != CLB_LAYOUT_COMPACT && layout != CLB_LAYOUT_DENSE)) return CLB_ERR_ARG;
-  const int rc = to_dense(c);
+  int rc = to_dense(c);
+  if (!rc) rc = flow_to_dense(c);
   if (rc) return rc;
   if (layout != c->layout) drop_graph(c);
   c->layout = layout;
@@ -2395,7 +2562,13 @@ int clb_node_conversion(clb_ctx* c, int mineral_solute, int pore_code, int preci
     if (rc) return rc;
     CLB_CHECK_HIP(c, hipMemset(c->d_phase, 0, (size_t)c->geo.N));
   }
-  launch_fill_u32(c->d_counter + 1, 1, 0u, c->stream);
+  // a reopened voxel's flow populations are written into the dense array: bring it up to date
+  // first (only when the call can reopen one)
+  if (c->flow_ready && reopen > 0.0 && phase_id > 0) {
+    const int rc = flow_sync_dense(c);
+    if (rc) return rc;
+  }
+  launch_fill_u32(c->d_counter + 1, 2, 0u, c->stream);  // [1] changed, [2] reopened for the flow
   const bool odd_next = (c->ade_t & 1) != 0;
   const bool flow_odd = (c->flow_t & 1) != 0;
   if (c->fp32)
@@ -2408,10 +2581,17 @@ int clb_node_conversion(clb_ctx* c, int mineral_solute, int pore_code, int preci
                                    precip_code, phase_id, precip_max, reopen, c->d_counter + 1,
                                    c->flow_ready ? (double*)c->d_f : nullptr, flow_odd, flow_ref0(c), flow_rdx<double>(c),
                                    c->stream);
-  unsigned int h = 0;
-  CLB_CHECK_HIP(c, hipMemcpyAsync(&h, c->d_counter + 1, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  unsigned int hc[2] = {0, 0};
+  CLB_CHECK_HIP(c, hipMemcpyAsync(hc, c->d_counter + 1, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
   CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  const unsigned int h = hc[0];
   if (changed) *changed = h;
+  if (hc[1] > 0) {
+    // voxels that were walls for the flow are fluid now, with their populations written into
+    // the dense array: it is the state, and the compact table needs them
+    c->fcmp_active = false;
+    c->fcmp_dirty = true;
+  }
   if (h > 0) {
     // the captured step reads the mask on the device; only the separate speciation
     // path's host-side reactive list has to be rebuilt (step_t does it)
@@ -2428,6 +2608,10 @@ int clb_flow_stats(clb_ctx* c, double* stats) {
   if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "clb_flow_init first");
   if (!c->d_fstats) {
     int rc = dalloc(c, &c->d_fstats, 4);
+    if (rc) return rc;
+  }
+  {
+    const int rc = flow_sync_dense(c);
     if (rc) return rc;
   }
   launch_fill_f64(c->d_fstats, 4, 0.0, c->stream);
@@ -2561,7 +2745,8 @@ int snap_restore(clb_ctx* c, const TuneSnapshot& s) {
   c->flow_t = s.flow_t;
   c->ade_t = s.ade_t;
   c->pend_dirty = s.pend_dirty;
-  c->cmp_active = false;  // the restored dense copy is the state
+  c->cmp_active = false;   // the restored dense copy is the state
+  c->fcmp_active = false;  // for the flow too
   return CLB_OK;
 }
 
@@ -2570,13 +2755,21 @@ template <typename R>
 int tune_trial(clb_ctx* c, const TuneSnapshot& s, bool flow, int steps, hipEvent_t e0, hipEvent_t e1, double& ms) {
   int rc = snap_restore(c, s);
   if (rc) return rc;
+  const bool fcmp = flow && flow_compact_ok(c);
   if (!flow) {
     rc = prepare_step<R>(c);
     if (rc) return rc;
+  } else if (fcmp) {
+    rc = flow_to_compact<R>(c);
+    if (rc) return rc;
+    c->fcmp_mirror = false;
   }
   auto run = [&](int n) {
     for (int t = 0; t < n; ++t) {
-      if (flow) {
+      if (fcmp) {
+        launch_flow_step_compact<R>(flow_cargs<R>(c, nullptr), (c->flow_t & 1) != 0, (int)(c->flow_block.x * c->flow_block.y), c->stream);
+        c->flow_t++;
+      } else if (flow) {
         launch_flow_step<R>(flow_args<R>(c, nullptr), (c->flow_t & 1) != 0, c->flow_block, c->stream);
         c->flow_t++;
       } else {
@@ -2606,7 +2799,8 @@ int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
   const bool cmp = compact_ok(c);
   static const int budgets[] = {2, 3, 4};
   TuneSnapshot s;
-  int rc = snap_take(c, s);
+  int rc = flow_to_dense(c);  // the snapshot holds the dense flow array
+  if (!rc) rc = snap_take(c, s);
   if (rc) return rc;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -2617,7 +2811,23 @@ int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
   c->timing = false;
   *rep = clb_tune_report{};
   auto body = [&]() -> int {
-    if (c->flow_ready) {
+    if (c->flow_ready && flow_compact_ok(c)) {
+      // the compact flow kernel: 1-D blocks (only the thread count matters)
+      double best = 1e300;
+      dim3 pick = c->flow_block;
+      for (auto& sh : cshapes256) {
+        c->flow_block = dim3(sh[0], sh[1], 1);
+        double ms = 0;
+        int r = tune_trial<R>(c, s, true, steps, e0, e1, ms);
+        if (r) return r;
+        rep->trials++;
+        if (ms < best) { best = ms; pick = c->flow_block; }
+      }
+      c->flow_block = pick;
+      rep->flow_bx = (int)pick.x;
+      rep->flow_by = (int)pick.y;
+      rep->flow_ms = best;
+    } else if (c->flow_ready) {
       double best = 1e300;
       dim3 pick = c->flow_block;
       for (auto& sh : shapes256) {
@@ -2696,7 +2906,8 @@ int autotune_t(clb_ctx* c, int steps, clb_tune_report* rep) {
   c->perf = s.perf;
   c->timing = timing;
   c->list_dirty = true;
-  c->leak_dirty = true;   // the mask rule may have run during the trials
+  c->leak_dirty = true;
+  c->fcmp_dirty = true;   // the mask rule may have run during the trials
   c->cmp_dirty = true;
   c->fluid_dirty = true;
   (void)hipEventDestroy(e0);
@@ -2878,6 +3089,7 @@ int clb_fd_biomass(clb_ctx* c, double nu, clb_ca_report* report) {
   if (report->mask_changes > 0) {
     c->list_dirty = true;
     c->leak_dirty = true;
+    c->fcmp_dirty = true;
     c->cmp_dirty = true;
     c->fluid_dirty = true;
   }
@@ -2912,6 +3124,7 @@ int clb_ca_redistribute(clb_ctx* c, double bmax, int half, unsigned long long se
   if (report->mask_changes > 0) {
     c->list_dirty = true;
     c->leak_dirty = true;
+    c->fcmp_dirty = true;
     c->cmp_dirty = true;
     c->fluid_dirty = true;
   }
@@ -3014,6 +3227,7 @@ extern "C" {
 int clb_save_state(clb_ctx* c, const char* path) {
   if (!c || !path) return CLB_ERR_ARG;
   int rc = flush_pending(c);  // the deferred increments belong in the populations
+  if (!rc) rc = flow_sync_dense(c);
   if (rc) return rc;
   CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
   const ChkHeader h = chk_header_of(c);
@@ -3054,6 +3268,7 @@ int clb_load_state(clb_ctx* c, const char* path) {
                                   "precision, or which of flow / solutes / phase / CA are set up)");
   }
   int rc = clear_pending(c);  // the populations are about to be overwritten
+  c->fcmp_active = false;     // and the dense flow array
   if (rc) {
     std::fclose(f);
     return rc;
@@ -3081,6 +3296,8 @@ int clb_load_state(clb_ctx* c, const char* path) {
   c->ca_sweep = h.ca_sweep;
   c->list_dirty = true;
   c->leak_dirty = true;
+  c->fcmp_dirty = true;
+  c->fcmp_active = false;  // the loaded dense array is the flow state
   c->cmp_dirty = true;
   c->fluid_dirty = true;
   drop_graph(c);  // a captured segment starts on the parity it was captured for

@@ -15,6 +15,7 @@
 //  3. the node table: dense index per node, and per direction the slot at +c_d.
 #include <hipcub/hipcub.hpp>
 
+#include "clb_brick.h"
 #include "clb_compact.h"
 
 namespace clb {
@@ -33,20 +34,6 @@ __device__ __forceinline__ long long nb_index(int x, int y, int z, int d, const 
   if (zz < 0) zz += g.nz;
   if (zz >= g.nz) zz -= g.nz;
   return (long long)(x + Q7::cx[d]) + (long long)g.nx * (yy + (long long)g.ny * zz);
-}
-
-// Brick-order keys: key(x, y, z) and its inverse; voxels of the padding are outside the lattice.
-__device__ __forceinline__ long long brick_key(int x, int y, int z, const BrickOrder& o) {
-  const long long b = (long long)(x / o.bx) + (long long)o.nbx * ((y / o.by) + (long long)o.nby * (z / o.bz));
-  return b * o.vol + (x % o.bx) + (long long)o.bx * ((y % o.by) + (long long)o.by * (z % o.bz));
-}
-
-__device__ __forceinline__ bool brick_voxel(long long k, const BrickOrder& o, const Geom& g, int& x, int& y, int& z) {
-  const long long b = k / o.vol, r = k % o.vol;
-  x = (int)((b % o.nbx) * o.bx + r % o.bx);
-  y = (int)(((b / o.nbx) % o.nby) * o.by + (r / o.bx) % o.by);
-  z = (int)((b / ((long long)o.nbx * o.nby)) * o.bz + r / ((long long)o.bx * o.by));
-  return x < g.nx && y < g.ny && z < g.nz;
 }
 
 __global__ void compact_flags_kernel(const uint8_t* mask, const uint8_t* kind, Geom g, BrickOrder o, unsigned int* flag,

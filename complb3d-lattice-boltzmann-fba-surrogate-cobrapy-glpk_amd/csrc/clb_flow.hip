@@ -16,53 +16,11 @@
 // that moves one voxel along x changes reference by t_i c_ix rdx. The collision is unchanged
 // in form: with q_i = f_i - t_i (1 + rref(x)), sum q_i = rho - 1 - rref(x) and
 // sum c_i q_i = j (sum c_i t_i = 0), and q_eq has the f_eq - t_i form with that density.
-#include "clb_lattice.h"
+#include "clb_flow_node.h"
 
 namespace clb {
 
 namespace {
-
-// f_eq,i - t_i (1 + r) = t_i [(rho - 1 - r) + 3 c.j + 4.5 (c.j)^2 - 1.5 j^2]   (rho0 = 1), for
-// any reference r: called with rhobar = rho - 1 - rref(x)
-template <typename R>
-__device__ __forceinline__ R feq19(int i, R rhobar, R jx, R jy, R jz, R jsq) {
-  const R cj = R(Q19::cx[i]) * jx + R(Q19::cy[i]) * jy + R(Q19::cz[i]) * jz;
-  return R(Q19::w(i)) * (rhobar + R(3) * cj + R(4.5) * cj * cj - R(1.5) * jsq);
-}
-
-// Regularized pressure boundary on an x face (side -1: x = 0, side +1: x = nx-1).
-// j_x from the mass relation over the known populations, tangential momentum zero,
-// Pi_neq from on-wall (once) and normal (twice) populations, then all 19 populations
-// rebuilt as f_eq + t_i * 9/2 * Q_i:Pi_neq.
-template <typename R, int SIDE>
-__device__ __forceinline__ void regularize(R* p, R rhobar) {
-  R onwall = 0, normal = 0;
-#pragma unroll
-  for (int i = 0; i < 19; ++i) {
-    if (Q19::cx[i] == 0) onwall += p[i];
-    else if (Q19::cx[i] == SIDE) normal += p[i];
-  }
-  const R jx = (SIDE < 0) ? (rhobar - onwall - R(2) * normal) : -(rhobar - onwall - R(2) * normal);
-  const R jsq = jx * jx;
-  R pxx = 0, pxy = 0, pxz = 0, pyy = 0, pyz = 0, pzz = 0;
-#pragma unroll
-  for (int i = 0; i < 19; ++i) {
-    if (Q19::cx[i] != 0 && Q19::cx[i] != SIDE) continue;
-    const R wgt = (Q19::cx[i] == 0) ? R(1) : R(2);
-    const R fneq = wgt * (p[i] - feq19<R>(i, rhobar, jx, R(0), R(0), jsq));
-    const R cx = Q19::cx[i], cy = Q19::cy[i], cz = Q19::cz[i];
-    pxx += cx * cx * fneq; pxy += cx * cy * fneq; pxz += cx * cz * fneq;
-    pyy += cy * cy * fneq; pyz += cy * cz * fneq; pzz += cz * cz * fneq;
-  }
-#pragma unroll
-  for (int i = 0; i < 19; ++i) {
-    const R cx = Q19::cx[i], cy = Q19::cy[i], cz = Q19::cz[i];
-    const R third = R(1) / R(3);
-    const R qpi = (cx * cx - third) * pxx + (cy * cy - third) * pyy + (cz * cz - third) * pzz +
-                  R(2) * (cx * cy * pxy + cx * cz * pxz + cy * cz * pyz);
-    p[i] = feq19<R>(i, rhobar, jx, R(0), R(0), jsq) + R(Q19::w(i) * 4.5) * qpi;
-  }
-}
 
 // XCHK = false for a node with 1 <= x <= nx-2, whose x neighbours always exist: no checks,
 // so the loads and stores need no branches. Odd steps move populations between voxels: one
@@ -78,7 +36,7 @@ __device__ __forceinline__ void load19(const R* __restrict__ f, long long N, con
       const int o = opp_dir(i);
       if (!XCHK || nb.xok(-Q19::cx[i])) {
         p[i] = f[(long long)o * N + nb.at(-Q19::cx[i], -Q19::cy[i], -Q19::cz[i])];
-        if (Q19::cx[i] != 0) p[i] -= R(Q19::w(i) * Q19::cx[i]) * rdx;
+        if (Q19::cx[i] != 0) p[i] = fma(R(-Q19::w(i) * Q19::cx[i]), rdx, p[i]);
       } else {
         p[i] = R(0);
       }
@@ -97,7 +55,7 @@ __device__ __forceinline__ void store19(R* __restrict__ f, long long N, const Nb
       __builtin_nontemporal_store(p[i], &f[(long long)opp_dir(i) * N + n]);
     } else {
       if (!XCHK || nb.xok(Q19::cx[i])) {
-        const R v = (Q19::cx[i] != 0) ? p[i] - R(Q19::w(i) * Q19::cx[i]) * rdx : p[i];
+        const R v = (Q19::cx[i] != 0) ? fma(R(-Q19::w(i) * Q19::cx[i]), rdx, p[i]) : p[i];
         __builtin_nontemporal_store(v, &f[(long long)i * N + nb.at(Q19::cx[i], Q19::cy[i], Q19::cz[i])]);
       }
     }
@@ -114,18 +72,7 @@ __device__ __forceinline__ double flow_node(const FlowArgs<R>& a, int x, int y, 
   load19<R, ODD, FACE != 0>(a.f, g.N, nb, n, p, a.rdx);
   if (FACE == -1) regularize<R, -1>(p, a.dface_in);
   if (FACE == 1) regularize<R, 1>(p, a.dface_out);
-  R rhobar = 0, jx = 0, jy = 0, jz = 0;
-#pragma unroll
-  for (int i = 0; i < 19; ++i) {
-    rhobar += p[i];
-    jx += R(Q19::cx[i]) * p[i];
-    jy += R(Q19::cy[i]) * p[i];
-    jz += R(Q19::cz[i]) * p[i];
-  }
-  const R om = a.omega[code];
-  const R jsq = jx * jx + jy * jy + jz * jz;
-#pragma unroll
-  for (int i = 0; i < 19; ++i) p[i] = (R(1) - om) * p[i] + om * feq19<R>(i, rhobar, jx, jy, jz, jsq);
+  const R jsq = collide19<R>(p, a.omega[code]);
   store19<R, ODD, FACE != 0>(a.f, g.N, nb, n, p, a.rdx);
   return 0.5 * (double)jsq;
 }
@@ -346,6 +293,10 @@ __global__ void flow_stats_kernel(FlowArgs<R> a, double* acc) {
 
 }  // namespace
 
+void launch_energy_reduce(double* energy, long long nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(kEnergyThreads), 0, s, energy, nblocks);
+}
+
 template <typename R>
 void launch_flow_stats(const FlowArgs<R>& a, bool odd_next, double* acc, hipStream_t s) {
   const int bs = 256;
@@ -375,8 +326,7 @@ void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s)
   if (a.energy) {
     if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((flow_step_kernel<R, false, true>), grid, block, 0, s, a);
-    hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(kEnergyThreads), 0, s, a.energy,
-                       (long long)grid.x * grid.y * grid.z);
+    launch_energy_reduce(a.energy, (long long)grid.x * grid.y * grid.z, s);
   } else {
     if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, false>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((flow_step_kernel<R, false, false>), grid, block, 0, s, a);

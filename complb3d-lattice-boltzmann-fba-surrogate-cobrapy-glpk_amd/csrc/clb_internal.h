@@ -48,6 +48,13 @@ struct FlowArgs {
   // fluid voxel neighbours one; 0: no fluid voxel does, and they are skipped
   int stream_none;
   double* energy;         // [0..1] sum |u|^2/2 and count; [2 + 2b..] per-block partials; or null
+  // the compact layout (clb_flow_compact.hip), when f holds its populations: node j's voxel
+  // fnode[j], its slot at +c_d fnbr[(d-1) fcap + j] (d = 1..18), the node count *fnf, plane
+  // stride fstride = fcap + ghost capacity; unused by the dense kernels
+  const int* fnode;
+  const int* fnbr;
+  const unsigned int* fnf;
+  long long fcap, fstride;
 };
 
 // The pore <-> biofilm rule of updateLocalMaskNtotalLattices3D
@@ -185,6 +192,8 @@ void launch_none_reach(const uint8_t* mask, const uint8_t* kind, const Geom& g, 
 // Per-block energy partials a launch of this block shape writes (the energy buffer holds
 // 2 + 2 * flow_blocks doubles).
 long long flow_blocks(const Geom& g, dim3 block);
+// energy[0..1] = the sums of the nblocks per-block partials, in a fixed order
+void launch_energy_reduce(double* energy, long long nblocks, hipStream_t s);
 template <typename R>
 void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s);
 template <typename R>
@@ -196,6 +205,8 @@ void launch_ade_init(const Geom& g, int S, R* gpop, const double* c0, const R* c
 template <typename R>
 void launch_mask_rule(const R* b, int nb, const uint8_t* prev, uint8_t* mask, const uint8_t* kind, const Geom& g,
                       int solid, int bb, const MaskRule& r, double thr, hipStream_t s);
+// changed[0] += voxels whose code changed; changed[1] += those of them that were walls for the
+// flow and are fluid now (their flow populations written into flowf, the dense array)
 template <typename R>
 void launch_node_conversion(const AdeArgs<R>& a, bool odd_next, uint8_t* phase, int ms, int pore_code,
                             int precip_code, int phase_id, double precip_max, double reopen, unsigned int* changed,
@@ -257,6 +268,28 @@ template <typename R>
 void launch_compact_flush(const AdeArgs<R>& a, bool odd_next, hipStream_t s);
 template <typename R>
 void launch_ade_step_compact(const AdeArgs<R>& a, bool odd, int bt, hipStream_t s);
+
+// Compact flow layout (clb_flow_compact.hip): the voxels of flow kind FLUID in brick order,
+// with per direction (d = 1..18) the slot at +c_d: a flow node's index, a ghost slot (>= fcap,
+// plane d) where the neighbour is a wall, -1 beyond an x face. The build buffers are
+// CompactBuild's with lk / cg of 18 nkey + 1 entries.
+size_t flow_compact_scan_bytes(long long nkey);
+void launch_flow_compact_build(const uint8_t* mask, const uint8_t* kind, const Geom& g, const BrickOrder& o,
+                               const CompactBuild& b, hipStream_t s);
+void launch_flow_compact_table(const CompactBuild& b, const Geom& g, const BrickOrder& o, int* fnode, int* fnbr,
+                               long long fcap, unsigned int* fnf, hipStream_t s);
+// dense <-> compact copies of the flow populations (a.f: the compact array; f_dense: 19 N):
+// every node's own slots and the wall slots behind its links
+template <typename R>
+void launch_flow_compact_move(bool to_compact, const FlowArgs<R>& a, R* f_dense, hipStream_t s);
+// per-block energy partials of a compact launch of bt threads
+long long flow_compact_blocks(long long fcap, int bt);
+template <typename R>
+void launch_flow_step_compact(const FlowArgs<R>& a, bool odd, int bt, hipStream_t s);
+// the advecting velocity of the solutes (latticeToPassiveAdvDiff) from the compact flow
+// state: every node of the table writes ade_u at its voxel (the caller zeroes the rest)
+template <typename R>
+void launch_flow_velocity_compact(const FlowArgs<R>& a, bool odd_next, R* ade_u, hipStream_t s);
 template <typename R>
 bool launch_ade_spec_step_compact(const AdeArgs<R>& a, bool odd, int bt, const Tableau& tb, const FusedSpecIO& io,
                                   hipStream_t s);

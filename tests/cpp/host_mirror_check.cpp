@@ -92,8 +92,8 @@ int main(int argc, char** argv) {
     lat.step(10);
     // the autotuner keeps the state: the 30 steps still match the oracle's
     const clb_tune_report tr = lat.autotune(4);
-    // flow: 6 block shapes; transport on the compact layout: 3 block sizes
-    if (tr.trials != 9 || tr.ade_bx * tr.ade_by < 64) throw std::runtime_error("autotune report");
+    // flow and transport on the compact layouts: 3 block sizes each
+    if (tr.trials != 6 || tr.ade_bx * tr.ade_by < 64) throw std::runtime_error("autotune report");
     lat.step(20);
     write_bin(dir + "/conc.bin", lat.concentrations());
     write_bin(dir + "/bmass.bin", lat.biomass());

@@ -61,9 +61,10 @@ def test_autotune_restores_state_and_matches(clb, mode, prec, layout):
         assert rep["ade_bx"] == 0 and rep["fused_ms"] > 0
     else:
         assert rep["ade_bx"] * rep["ade_by"] in (64, 128, 256) and rep["fused_bx"] == 0
-    # flow: 6 shapes; fused on the compact layout: 2 block sizes x 3 budgets, on the dense one
-    # 6 shapes x 3 budgets; the separate path's transport (dense): 6 shapes
-    assert rep["trials"] == 6 + ((6 if layout == 0 else 18) if mode == 0 else 6)
+    # flow: 3 block sizes on the compact layout, 6 shapes on the dense one; fused on the compact
+    # layout: 2 block sizes x 3 budgets, on the dense one 6 shapes x 3 budgets; the separate
+    # path's transport (dense): 6 shapes
+    assert rep["trials"] == (3 if layout == 0 else 6) + ((6 if layout == 0 else 18) if mode == 0 else 6)
     # continue both runs: tuned shapes (and budget) against the defaults
     for l in (lat, ref):
         l.step(40)
@@ -91,6 +92,10 @@ def test_autotune_arguments(clb):
     for bad in (0, 3, -2):
         with pytest.raises(clb.ClbError):
             lat.autotune(bad)
+    rep = lat.autotune(2)
+    # the flow alone, on the compact layout: 3 block sizes
+    assert rep["trials"] == 3 and rep["fused_bx"] == 0 and rep["ade_bx"] == 0
+    lat.set_layout(clb.LAYOUT_DENSE)
     rep = lat.autotune(2)
     assert rep["trials"] == 6 and rep["fused_bx"] == 0 and rep["ade_bx"] == 0
     lat.close()
