@@ -57,7 +57,7 @@ FLOP_SRG_NET = 2 * 2 + 2 * (2 * 10 + 3 * 100) + 4 * 10 * (20 + 4) + 2 * 10 + 4
 MEASURED_TRAFFIC_SOURCE = "profiles/hbm_traffic_full_coupled_384x192x192_kinetics_256x128x128.md"
 MEASURED_TRAFFIC = {
     "full_coupled_384x192x192": (3.863e9 + 2.958e9) / 2,
-    "flow_384x192x192": (1.977e9 + 1.867e9) / 2,
+    "flow_384x192x192": (1.817e9 + 0.857e9) / 2,
     "kinetics_256x128x128": (0.793e9 + 0.568e9) / 2,
 }
 
@@ -114,7 +114,11 @@ def full_coupled(clb, args):
     lat.flow_init(1.0, 0.999)
     # set-up: the initial flow solve of the driver (src/complab.cpp:593, 614-620), to its own
     # tracer ns_convg1 = ValueTracer(1, 1000, ns_converge_iT1), at most ns_max_iT1 steps
+    lat.sync()
+    t_flow0 = time.perf_counter()
     flow0_steps, flow0_conv = lat.flow_resolve(clb.ValueTracer(1.0, 1000.0, args.ns_converge1), args.ns_max1)
+    lat.sync()
+    flow0_s = time.perf_counter() - t_flow0
     lat.set_solutes(np.full((6, 256), 1.0 / 0.8), [0, 1] * 6, sum([[cin[s], 0.0] for s in range(6)], []))
     lat.couple_velocity()
     lat.set_biomass(b)
@@ -195,7 +199,7 @@ def full_coupled(clb, args):
     # the whole configured run (5000 coupled steps) with the warm-up's first re-solve in it
     run_steps = 5000
     run_ms = (wall * 1e3 / steps) * run_steps + warm["resolve_steps"] * flow_ms
-    return dict(N=N, flow0_steps=int(flow0_steps), flow0_converged=bool(flow0_conv), threshold=thr,
+    return dict(N=N, flow0_steps=int(flow0_steps), flow0_converged=bool(flow0_conv), flow0_s=flow0_s, threshold=thr,
                 segments=args.steps, steps=steps, wall=wall, ms_per_segment=wall * 1e3 / args.steps,
                 ms_per_step=wall * 1e3 / steps, mlups=N * steps / wall / 1e6,
                 ade_ms=ade_ms, straggler_ms=spec_ms, flow_ms=flow_ms, fluid=fluid, reactive=int(p.reactive_nodes),
@@ -588,7 +592,8 @@ def main():
         f"{fc['straggler_ms']:.3f} ms, {fc['solves_per_step']:.0f} solves/step, {fc['mean_evals']:.2f} "
         f"evaluations/solve; {fc['resolves']} re-solves of {fc['resolve_steps']} flow steps in the timed segments "
         f"(warm-up: {fc['warmup_resolve_steps']}); 5000-step run {fc['run_5000_ms_per_step']:.3f} ms/step; "
-        f"initial flow {fc['flow0_steps']} steps (converged {fc['flow0_converged']}); {fc['conversions']} voxels "
+        f"initial flow {fc['flow0_steps']} steps in {fc['flow0_s']:.2f} s (converged {fc['flow0_converged']}); "
+        f"{fc['conversions']} voxels "
         f"converted in the timed segments")
     states, (S6, lk6) = fc.pop("states"), fc.pop("tableau")
     extra = dict(full_coupled=fc)
@@ -601,9 +606,9 @@ def main():
             f"({cw['mlp']['kernel_ms']:.2f} ms); voxels beyond 4 evaluations {cw['hard_fraction']:.4f}")
         del states
     fl_ms = fc["flow_ms"]
-    extra["flow_fp32"] = dict(ms_per_step=fl_ms, mlups=N / (fl_ms * 1e-3) / 1e6,
-                              gbs_dense=BYTES_D3Q19 * N / (fl_ms * 1e-3) / 1e9,
-                              frac_dense=BYTES_D3Q19 * N / (fl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+    # the compact flow kernel moves the fluid voxels only: its roofline counts their bytes
+    extra["flow_fp32"] = dict(ms_per_step=fl_ms, mlups=N / (fl_ms * 1e-3) / 1e6, layout="compact",
+                              kernel="flow_step_compact_kernel<float,*,*>",
                               gbs_fluid=BYTES_D3Q19 * nf / (fl_ms * 1e-3) / 1e9,
                               frac_fluid=BYTES_D3Q19 * nf / (fl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                               traffic=(MEASURED_TRAFFIC["flow_384x192x192"]

@@ -162,14 +162,15 @@ int clb_speciation_stats(clb_ctx* ctx, long long* stats /*3: solves, converged, 
  * separate path runs. CLB_SPEC_SEPARATE: transport kernel, then the persistent kernel over
  * every reactive voxel. Both meet the same judge. */
 int clb_set_speciation_mode(clb_ctx* ctx, int mode);
-/* Layout of the solute lattices in the step kernels (no reference counterpart: a memory
- * layout choice). CLB_LAYOUT_COMPACT (default): the step runs on the ADE-fluid voxels only,
- * stored contiguously in natural order with a neighbour table (the dense layout moves the
+/* Layout of the solute and flow lattices in the step kernels (no reference counterpart: a
+ * memory layout choice). CLB_LAYOUT_COMPACT (default): the steps run on the fluid voxels
+ * only, stored contiguously in brick order with a neighbour table (the dense layout moves the
  * cache lines of walls too: in a pack of porosity 0.35 nearly every line holds a fluid
- * voxel); used when neither the surrogate processor nor dissolution is on, the speciation
- * (if any) is fused, and the pore <-> biofilm rule keeps every voxel fluid for the solutes,
- * else the dense layout runs. The state moves between the layouts on its own; results are
- * the same. CLB_LAYOUT_DENSE: always the dense layout. */
+ * voxel). Solutes: used when neither the surrogate processor nor dissolution is on, the
+ * speciation (if any) is fused, and the pore <-> biofilm rule keeps every voxel fluid for the
+ * solutes. Flow (clb_flow_steps): used unless no-dynamics voxels stream. Otherwise the dense
+ * layout runs. The state moves between the layouts on its own; results are the same (the
+ * FP32 flow to its store's rounding). CLB_LAYOUT_DENSE: always the dense layout. */
 #define CLB_LAYOUT_COMPACT 0
 #define CLB_LAYOUT_DENSE 1
 int clb_set_layout(clb_ctx* ctx, int layout);
