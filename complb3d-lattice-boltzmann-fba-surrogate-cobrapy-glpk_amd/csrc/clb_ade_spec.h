@@ -14,6 +14,7 @@
 #pragma once
 
 #include "clb_ade_node.h"
+#include "clb_brick.h"
 #include "clb_compact.h"
 #include "clb_internal.h"
 #include "clb_spec_kernel.h"
@@ -42,14 +43,16 @@ template <typename R, int S, int NC>
 constexpr int fused_waves_per_eu() { return (sizeof(R) == 4 && S <= 6 && NC <= 3) ? 4 : 1; }
 
 // Phase 2 of the fused step, shared by the dense and compact kernels: the first `count`
-// threads of the block each solve one slot (post-kinetics concentrations sc, node index
-// snode), write the limited increments to pend_w (stride: N dense, ccap compact), or, for a
+// threads of the block each solve one slot (the post-kinetics population sums sps, so the
+// concentration is max(sps + c_ref, kMinConc) exactly as the transport formed it, in
+// R to halve the LDS the block holds at FP32; node index snode), write the limited
+// increments to pend_w (stride: N dense, ccap compact), or, for a
 // voxel that has not met the judge within io.evals evaluations, zeros there, its
 // concentrations to cbuf and its next trial point to io.sx, and append it to block bid's
 // segment of the straggler list; then the per-wavefront statistics of block bid.
 template <typename R, int S, int NC, int BT>
 __device__ __forceinline__ void fused_solve_phase(const AdeArgs<R>& a, const Tableau& tb, const FusedSpecIO& io,
-                                                  int lane, int count, double (&sc)[S][BT], double (&scr)[S][BT],
+                                                  int lane, int count, R (&sps)[S][BT], double (&scr)[S][BT],
                                                   const int (&snode)[BT], long long stride, long long bid) {
   constexpr int NW = BT / 64;
   const int wlane = lane & 63, wave = lane >> 6;
@@ -61,20 +64,21 @@ __device__ __forceinline__ void fused_solve_phase(const AdeArgs<R>& a, const Tab
   if (lane < count) {
     double c[S];
 #pragma unroll
-    for (int s = 0; s < S; ++s) c[s] = sc[s][lane];
+    for (int s = 0; s < S; ++s) c[s] = fmax((double)sps[s][lane] + (double)a.c_ref[s], kMinConc);
     node = snode[lane];
     solved = spec_fused<NC, S, BT>(tb, c, xs, evals, io.evals, &scr[0][lane]);
     straggler = !solved;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       double d = 0.0;
+      const double cs = fmax((double)sps[s][lane] + (double)a.c_ref[s], kMinConc);
       if (solved) {
         // the judged evaluation that converged was at xs: its species concentrations are
         // still in scr (clamping them equals calc_species' clamp of log c to [-30, 1])
-        d = spec_increment_eq(tb.is_eq[s], scr[s][lane], sc[s][lane]);
+        d = spec_increment_eq(tb.is_eq[s], scr[s][lane], cs);
         if (!(d > 1e-12 || d < -1e-12)) d = 0.0;
       } else {
-        a.cbuf[(long long)s * stride + node] = sc[s][lane];
+        a.cbuf[(long long)s * stride + node] = cs;
       }
       a.pend_w[(long long)s * stride + node] = (R)d;
     }
@@ -110,7 +114,7 @@ template <typename R, int S, bool ODD, int NC, int BT>
 __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(fused_waves_per_eu<R, S, NC>(), 8)))
 ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
   constexpr int NW = BT / 64;
-  __shared__ double sc[S][BT];         // post-kinetics concentrations of the block's reactive voxels
+  __shared__ R sps[S][BT];             // post-kinetics population sums of the block's reactive voxels
   __shared__ double scr[S][BT];        // species concentrations of the solve's latest evaluation
   __shared__ int snode[BT];            // node indices of the slots
   __shared__ unsigned int woff[NW + 1];  // per-wavefront slot offsets, [NW] = block total
@@ -155,7 +159,7 @@ ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
     ade_node<R, S, true, false>(a, x, y, z, n, code, p, nullptr, n, g.N);
     if (react) {
 #pragma unroll
-      for (int s = 0; s < S; ++s) sc[s][slot] = fmax((double)pop_sum(&p[s * 7]) + (double)a.c_ref[s], kMinConc);
+      for (int s = 0; s < S; ++s) sps[s][slot] = pop_sum(&p[s * 7]);
       snode[slot] = n;
     }
     store7<R, S, ODD, false>(a, nb, n, p);
@@ -180,7 +184,7 @@ ade_spec_step_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
     }
   }
   __syncthreads();
-  fused_solve_phase<R, S, NC, BT>(a, tb, io, lane, count, sc, scr, snode, g.N,
+  fused_solve_phase<R, S, NC, BT>(a, tb, io, lane, count, sps, scr, snode, g.N,
                                   blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z));
 }
 
@@ -191,7 +195,7 @@ template <typename R, int S, bool ODD, int NC, int BT>
 __global__ void __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(fused_waves_per_eu<R, S, NC>(), 8)))
 ade_spec_step_compact_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
   constexpr int NW = BT / 64;
-  __shared__ double sc[S][BT];
+  __shared__ R sps[S][BT];
   __shared__ double scr[S][BT];
   __shared__ int snode[BT];
   __shared__ unsigned int woff[NW + 1];
@@ -199,7 +203,8 @@ ade_spec_step_compact_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
   const int lane = threadIdx.x;
   const int wlane = lane & 63, wave = lane >> 6;
   const int nwaves = (blockDim.x + 63) >> 6;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int j = lb * blockDim.x + threadIdx.x;
   const bool valid = j < (int)*a.cnf;
   CPos q{0, 0, 0, 0};
   uint8_t code = (uint8_t)a.solid_code;
@@ -251,7 +256,7 @@ ade_spec_step_compact_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
         ade_node<R, S, true, false>(a, q.x, q.y, q.z, q.n, code, p, nullptr, j, a.ccap);
         if (react) {
 #pragma unroll
-          for (int s = 0; s < S; ++s) sc[s][slot] = fmax((double)pop_sum(&p[s * 7]) + (double)a.c_ref[s], kMinConc);
+          for (int s = 0; s < S; ++s) sps[s][slot] = pop_sum(&p[s * 7]);
           snode[slot] = j;
         }
         cstore7<R, S, ODD, false>(a, L, p);
@@ -274,7 +279,7 @@ ade_spec_step_compact_kernel(AdeArgs<R> a, Tableau tb, FusedSpecIO io) {
     }
   }
   __syncthreads();
-  fused_solve_phase<R, S, NC, BT>(a, tb, io, lane, count, sc, scr, snode, a.ccap, blockIdx.x);
+  fused_solve_phase<R, S, NC, BT>(a, tb, io, lane, count, sps, scr, snode, a.ccap, lb);
 }
 
 template <typename R, int S, int NC>

@@ -25,5 +25,21 @@ __device__ __forceinline__ bool brick_voxel(long long k, const BrickOrder& o, co
   return x < g.nx && y < g.ny && z < g.nz;
 }
 
+// The device dispatches workgroups to its eight XCDs in turn (workgroup b to XCD b mod 8), and
+// each XCD has its own L2. A compact-layout kernel's block b processes the nodes of logical block
+// xcd_block(b, n) instead: runs of kXcdChunk consecutive logical blocks go to one XCD, so the lines
+// that neighbouring blocks share (a node's y and z neighbours lie in the blocks next to its own)
+// are fetched into one L2 rather than several. Blocks past the last whole round of
+// 8 * kXcdChunk keep their number. A bijection on [0, n): results do not depend on it. Measured on
+// the 384 x 192 x 192 coupled step (fused kernel, mean of odd and even steps): 0.829-0.840 ms with
+// the plain order, 0.815-0.820 ms with runs of 8, 0.818 with 16, 0.813 with 32, 0.817 with 64.
+constexpr int kXcdChunk = 32;
+__device__ __forceinline__ int xcd_block(int b, int n) {
+  const int full = (n / (8 * kXcdChunk)) * (8 * kXcdChunk);
+  if (b >= full) return b;
+  const int x = b & 7, r = b >> 3;
+  return (r / kXcdChunk) * (8 * kXcdChunk) + x * kXcdChunk + (r % kXcdChunk);
+}
+
 }  // namespace
 }  // namespace clb

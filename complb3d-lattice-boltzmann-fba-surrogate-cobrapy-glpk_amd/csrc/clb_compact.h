@@ -16,6 +16,7 @@
 #pragma once
 
 #include "clb_ade_node.h"
+#include "clb_brick.h"
 
 namespace clb {
 
@@ -140,7 +141,7 @@ __device__ __forceinline__ CPos cpos(const AdeArgs<R>& a, int j) {
 // value); a face node next to a wall updates itself with its own density.
 template <typename R, int S, bool ODD, bool KIN>
 __global__ void __launch_bounds__(256) ade_step_compact_kernel(AdeArgs<R> a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = xcd_block((int)blockIdx.x, (int)gridDim.x) * blockDim.x + threadIdx.x;
   if (j >= (int)*a.cnf) return;
   const Geom& g = a.geo;
   const CPos q = cpos(a, j);

@@ -183,7 +183,8 @@ __device__ __forceinline__ double fc_node(const FlowArgs<R>& a, int j, uint8_t c
 // dense kernel's (per-block partials, summed by energy_reduce_kernel in a fixed order).
 template <typename R, bool ODD, bool ENERGY>
 __global__ void __launch_bounds__(256) flow_step_compact_kernel(FlowArgs<R> a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
+  const int j = lb * blockDim.x + threadIdx.x;
   double e = 0.0, cnt = 0.0;
   if (j < (int)*a.fnf) {
     const int n = a.fnode[j];
@@ -210,8 +211,8 @@ __global__ void __launch_bounds__(256) flow_step_compact_kernel(FlowArgs<R> a) {
     if (t == 0) {
       double es = 0.0, cs = 0.0;
       for (int w = 0; w < nw; ++w) { es += se[w]; cs += sc[w]; }
-      a.energy[2 + 2 * (long long)blockIdx.x] = es;
-      a.energy[3 + 2 * (long long)blockIdx.x] = cs;
+      a.energy[2 + 2 * (long long)lb] = es;
+      a.energy[3 + 2 * (long long)lb] = cs;
     }
   }
 }
