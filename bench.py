@@ -116,7 +116,8 @@ def full_coupled(clb, args):
     # tracer ns_convg1 = ValueTracer(1, 1000, ns_converge_iT1), at most ns_max_iT1 steps
     lat.sync()
     t_flow0 = time.perf_counter()
-    flow0_steps, flow0_conv = lat.flow_resolve(clb.ValueTracer(1.0, 1000.0, args.ns_converge1), args.ns_max1)
+    lat.flow_tracer(1.0, 1000.0, args.ns_converge1)  # on the device: no host round trip per step
+    flow0_steps, flow0_conv = lat.flow_resolve_device(args.ns_max1)
     lat.sync()
     flow0_s = time.perf_counter() - t_flow0
     lat.set_solutes(np.full((6, 256), 1.0 / 0.8), [0, 1] * 6, sum([[cin[s], 0.0] for s in range(6)], []))
@@ -141,7 +142,7 @@ def full_coupled(clb, args):
     if args.straggler_budget >= 0:
         lat.set_straggler_budget(args.straggler_budget)
     lat.set_graph(True, args.interval)
-    tracer = clb.ValueTracer(1.0, 1000.0, args.ns_converge)  # the driver's ns_convg2
+    lat.flow_tracer(1.0, 1000.0, args.ns_converge)  # the driver's ns_convg2, on the device
     st = dict(conversions=0, resolves=0, resolve_steps=0, unconverged=0, percolation=False)
 
     def segments(n):
@@ -150,7 +151,7 @@ def full_coupled(clb, args):
             ch = lat.node_conversion(5, 2, 1, 9, thr, 0.5 * thr)
             st["conversions"] += ch
             if ch and not st["percolation"]:
-                k, conv = lat.flow_resolve(tracer, args.ns_max)
+                k, conv = lat.flow_resolve_device(args.ns_max)
                 st["resolves"] += 1
                 st["resolve_steps"] += k
                 if not conv:

@@ -91,6 +91,8 @@ def load(path=None):
         "clb_set_solute_dynamics": [vp, vp],
         "clb_flow_init": [vp, c_double, c_double],
         "clb_flow_steps": [vp, c_int, vp],
+        "clb_flow_tracer": [vp, c_double, c_double, c_double],
+        "clb_flow_resolve": [vp, c_int, vp, vp, vp],
         "clb_flow_download_velocity": [vp, vp, vp],
         "clb_set_solutes": [vp, c_int, vp, vp, vp],
         "clb_solutes_init": [vp, vp],
@@ -630,6 +632,23 @@ class Lattice:
             if tracer.has_converged():
                 return done, True
         return done, False
+
+    def flow_tracer(self, u=1.0, L=1000.0, epsilon=1e-4):
+        """Start the device tracer (Palabos' ValueTracer(u, L, epsilon), as ValueTracer above)
+        with an empty window; clb_flow_resolve's steps feed it until the next call."""
+        self._chk(self.lib.clb_flow_tracer(self.h, float(u), float(L), float(epsilon)), "flow_tracer")
+
+    def flow_resolve_device(self, max_steps, energy=False):
+        """flow_resolve with the tracer on the device (flow_tracer): the same steps and
+        convergence as the host loop with the same tracer, without a host round trip per step.
+        Returns (steps run, converged) or, with energy, (steps run, converged, energies)."""
+        steps, conv = ctypes.c_int(), ctypes.c_int()
+        e = np.zeros(max(int(max_steps), 1)) if energy else None
+        self._chk(self.lib.clb_flow_resolve(self.h, int(max_steps), ctypes.byref(steps), ctypes.byref(conv),
+                                            _ptr(e) if energy else None), "flow_resolve")
+        if energy:
+            return steps.value, bool(conv.value), e[:steps.value].copy()
+        return steps.value, bool(conv.value)
 
     def percolates(self, thrd=1e-12):
         """The percolation test after a re-solve that used all its steps (src/complab.cpp:

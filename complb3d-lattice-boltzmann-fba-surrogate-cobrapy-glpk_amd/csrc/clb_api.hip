@@ -83,6 +83,13 @@ struct clb_ctx {
   void* d_fcf = nullptr;              // compact populations (19 * fstride)
   CompactBuild fcb{};                 // build buffers (flags, scans)
   long long fcb_N = 0;                // keys they were sized for
+  // device ValueTracer of clb_flow_resolve
+  FlowTracer* d_tr = nullptr;
+  double* d_trw = nullptr;            // its window
+  int tr_w = 0;                       // window length
+  int tr_fill = 0;                    // values it holds (the host's count)
+  double* d_elog = nullptr;           // energies of the current resolve
+  long long elog_n = 0;
 
   // solutes
   int S = 0;
@@ -335,6 +342,7 @@ FlowArgs<R> flow_args(clb_ctx* c, double* energy) {
   a.fnbr = nullptr;
   a.fnf = nullptr;
   a.fcap = a.fstride = 0;
+  a.stop = nullptr;
   return a;
 }
 
@@ -480,6 +488,74 @@ int flow_steps_t(clb_ctx* c, int nsteps, double* energy) {
     }
   }
   CLB_CHECK_HIP(c, hipGetLastError());
+  return CLB_OK;
+}
+
+// The driver's flow solve with the tracer on the device: batches of steps with no host round
+// trip in between. While the window is filling no step can end the solve, so those run as
+// one batch; then batches of 1, 2, 4, ... 64 steps, so a solve that ends at once (a re-solve
+// whose window is full) launches one step, and a long one reads the device every 64 steps.
+// Steps launched after the tracer converged do nothing; flow_t advances by the steps taken.
+template <typename R>
+int flow_resolve_t(clb_ctx* c, int max_steps, int* steps_out, int* conv_out, double* energy) {
+  const bool cmp = flow_compact_ok(c);
+  {
+    const int rc = cmp ? flow_to_compact<R>(c) : flow_to_dense(c);
+    if (rc) return rc;
+  }
+  const int bt = (int)(c->flow_block.x * c->flow_block.y);
+  const long long nblk = cmp ? flow_compact_blocks(c->fcap, bt) : flow_blocks(c->geo, c->flow_block);
+  if (c->energy_n < 2 + 2 * nblk) {
+    int rc = dalloc(c, &c->d_energy, (size_t)(2 + 2 * nblk));
+    c->energy_n = rc ? 0 : 2 + 2 * nblk;
+    if (rc) return rc;
+  }
+  if (energy && c->elog_n < max_steps) {
+    int rc = dalloc(c, &c->d_elog, (size_t)max_steps);
+    c->elog_n = rc ? 0 : max_steps;
+    if (rc) return rc;
+  }
+  double* elog = energy ? c->d_elog : nullptr;
+  launch_tracer_begin(c->d_tr, c->stream);
+  if (cmp && max_steps > 0) c->fcmp_mirror = false;
+  FlowArgs<R> a = cmp ? flow_cargs<R>(c, c->d_energy) : flow_args<R>(c, c->d_energy);
+  a.stop = &c->d_tr->stop;
+  long long done = 0, grow = 1;
+  bool conv = false;
+  FlowTracer h{};
+  while (done < max_steps && !conv) {
+    long long k;
+    if (c->tr_fill < c->tr_w) {
+      k = c->tr_w - c->tr_fill;
+    } else {
+      k = grow;
+      grow = std::min<long long>(2 * grow, 64);
+    }
+    k = std::min<long long>(k, max_steps - done);
+    for (long long i = 0; i < k; ++i) {
+      const bool odd = ((c->flow_t + i) & 1) != 0;
+      {
+        Timed tm(c, 0);
+        if (cmp) launch_flow_step_compact<R>(a, odd, bt, c->stream);
+        else launch_flow_step<R>(a, odd, c->flow_block, c->stream);
+      }
+      count_launch(c, 0);
+      launch_tracer_step(c->d_energy, nblk, c->d_tr, c->d_trw, elog, c->stream);
+    }
+    CLB_CHECK_HIP(c, hipMemcpyAsync(&h, c->d_tr, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+    const long long taken = h.steps - h.call0 - done;  // steps of this batch that ran
+    c->flow_t += taken;
+    c->tr_fill = std::min<long long>(c->tr_w, c->tr_fill + taken);
+    done += taken;
+    conv = h.stop != 0;
+    if (taken < k && !conv) return fail(c, CLB_ERR_HIP, "flow resolve: steps did not run");
+  }
+  CLB_CHECK_HIP(c, hipGetLastError());
+  if (energy && done > 0)
+    CLB_CHECK_HIP(c, hipMemcpy(energy, c->d_elog, sizeof(double) * (size_t)done, hipMemcpyDeviceToHost));
+  *steps_out = (int)done;
+  *conv_out = conv ? 1 : 0;
   return CLB_OK;
 }
 
@@ -1512,7 +1588,7 @@ int clb_destroy(clb_ctx* c) {
                   c->d_caflag, c->d_water, c->d_srg, c->d_srg_inc, c->d_cnode, c->d_cnbr, c->d_cnf, c->d_cg,
                   c->d_cu, c->d_cpend, c->cb.flag, c->cb.cj, c->cb.lk, c->cb.cg, c->cb.scan_tmp, c->d_mask_prev,
                   c->d_leak, c->d_fnode, c->d_fnbr, c->d_fnf, c->d_fcf, c->fcb.flag, c->fcb.cj, c->fcb.lk,
-                  c->fcb.cg, c->fcb.scan_tmp};
+                  c->fcb.cg, c->fcb.scan_tmp, c->d_tr, c->d_trw, c->d_elog};
   for (void* p : ptrs) dfree(p);
   for (auto& p : c->pools)
     for (hipEvent_t e : p.ev) (void)hipEventDestroy(e);
@@ -1631,6 +1707,39 @@ int clb_flow_steps(clb_ctx* c, int nsteps, double* energy) {
   int rc = refresh_leak(c);
   if (rc) return rc;
   return c->fp32 ? flow_steps_t<float>(c, nsteps, energy) : flow_steps_t<double>(c, nsteps, energy);
+}
+
+int clb_flow_tracer(clb_ctx* c, double u, double L, double epsilon) {
+  if (!c || !(u > 0.0) || !(L > 0.0) || !(epsilon > 0.0)) return CLB_ERR_ARG;
+  const double wd = L / u / 2.0;
+  if (!(wd >= 2.0 && wd <= 1e6)) return fail(c, CLB_ERR_ARG, "tracer window L / u / 2 must be 2 .. 1e6 values");
+  const int w = (int)wd;
+  if (!c->d_tr) {
+    int rc = dalloc(c, (uint8_t**)&c->d_tr, sizeof(FlowTracer));
+    if (rc) return rc;
+  }
+  if (c->tr_w != w || !c->d_trw) {
+    int rc = dalloc(c, &c->d_trw, (size_t)w);
+    c->tr_w = rc ? 0 : w;
+    if (rc) return rc;
+  }
+  FlowTracer h{};
+  h.eps = epsilon;
+  h.w = w;
+  CLB_CHECK_HIP(c, hipMemcpyAsync(c->d_tr, &h, sizeof(h), hipMemcpyHostToDevice, c->stream));
+  CLB_CHECK_HIP(c, hipStreamSynchronize(c->stream));
+  c->tr_fill = 0;
+  return CLB_OK;
+}
+
+int clb_flow_resolve(clb_ctx* c, int max_steps, int* steps, int* converged, double* energy) {
+  if (!c || max_steps < 0 || !steps || !converged) return CLB_ERR_ARG;
+  if (!c->flow_ready) return fail(c, CLB_ERR_STATE, "clb_flow_init first");
+  if (!c->d_tr) return fail(c, CLB_ERR_STATE, "clb_flow_tracer first");
+  int rc = refresh_leak(c);
+  if (rc) return rc;
+  return c->fp32 ? flow_resolve_t<float>(c, max_steps, steps, converged, energy)
+                 : flow_resolve_t<double>(c, max_steps, steps, converged, energy);
 }
 
 int clb_flow_download_velocity(clb_ctx* c, double* u, double* rho) {

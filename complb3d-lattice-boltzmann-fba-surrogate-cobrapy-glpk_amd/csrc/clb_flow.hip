@@ -79,6 +79,7 @@ __device__ __forceinline__ double flow_node(const FlowArgs<R>& a, int x, int y, 
 
 template <typename R, bool ODD, bool ENERGY>
 __global__ void __launch_bounds__(256) flow_step_kernel(FlowArgs<R> a) {
+  if (a.stop && *a.stop) return;  // the device tracer has converged
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
@@ -129,8 +130,7 @@ __global__ void __launch_bounds__(256) flow_step_kernel(FlowArgs<R> a) {
 // fixed order (one block of 1024 threads: thread t sums the partials t, t + 1024, ... into
 // four accumulators in turn, so four loads are in flight; then a tree over the threads).
 constexpr int kEnergyThreads = 1024;
-__global__ void __launch_bounds__(kEnergyThreads) energy_reduce_kernel(double* energy, long long nblocks) {
-  __shared__ double se[kEnergyThreads], sc[kEnergyThreads];
+__device__ __forceinline__ void reduce_partials(const double* energy, long long nblocks, double* se, double* sc) {
   double e[4] = {0.0, 0.0, 0.0, 0.0}, c[4] = {0.0, 0.0, 0.0, 0.0};
   const long long t = threadIdx.x;
   long long b = t;
@@ -155,10 +155,95 @@ __global__ void __launch_bounds__(kEnergyThreads) energy_reduce_kernel(double* e
     }
     __syncthreads();
   }
-  if (t == 0) {
+}
+
+__global__ void __launch_bounds__(kEnergyThreads) energy_reduce_kernel(double* energy, long long nblocks) {
+  __shared__ double se[kEnergyThreads], sc[kEnergyThreads];
+  reduce_partials(energy, nblocks, se, sc);
+  if (threadIdx.x == 0) {
     energy[0] = se[0];
     energy[1] = sc[0];
   }
+}
+
+// The device ValueTracer (Palabos' util::ValueTracer as the driver's flow loops use it,
+// src/complab.cpp:607-620, 1794-1800, 1836-1840; the host restatements clb.ValueTracer and
+// complab_gpu::ValueTracer do the same): after a flow step, the step's stored average energy
+// (the per-block partials summed as energy_reduce_kernel sums them) joins a window of the
+// last tr->w values; once the window is full the tracer has converged when |stddev / mean| <
+// eps, stddev with w - 1, or when that ratio is NaN. The sums run in the window's order, one
+// thread, with no fused multiply-adds, as the host restatements compute them. Once it has
+// converged (tr->stop) the flow steps and this kernel do nothing until the next resolve.
+constexpr int kTracerLds = 4096;  // windows up to this many values are staged in LDS
+__global__ void __launch_bounds__(kEnergyThreads) tracer_step_kernel(const double* energy, long long nblocks,
+                                                                      FlowTracer* tr, double* win, double* elog) {
+#pragma clang fp contract(off)
+  if (tr->stop) return;
+  __shared__ double se[kEnergyThreads], sc[kEnergyThreads];
+  __shared__ double sw[kTracerLds];
+  reduce_partials(energy, nblocks, se, sc);
+  const int t = threadIdx.x;
+  const int w = tr->w;
+  if (t == 0) {
+    const double v = sc[0] > 0 ? se[0] / sc[0] : 0.0;
+    if (tr->n < w) {
+      win[(tr->head + tr->n) % w] = v;
+      tr->n += 1;
+    } else {
+      win[tr->head] = v;
+      tr->head = (tr->head + 1) % w;
+    }
+    if (elog) elog[tr->steps - tr->call0] = v;
+    tr->steps += 1;
+  }
+  __syncthreads();
+  if (tr->n < w) return;
+  const int head = tr->head;
+  const bool lds = w <= kTracerLds;
+  if (lds) {
+    for (int k = t; k < w; k += kEnergyThreads) sw[k] = win[(head + k) % w];
+    __syncthreads();
+  }
+  if (t != 0) return;
+  // one thread, in the window's order; from LDS the values are loaded 32 at a time so that
+  // the sequential adds do not wait on each load
+  double s = 0.0, s2 = 0.0, avg = 0.0;
+  if (lds) {
+    int k = 0;
+    for (; k + 32 <= w; k += 32) {
+      double v[32];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) v[q] = sw[k + q];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) s += v[q];
+    }
+    for (; k < w; ++k) s += sw[k];
+    avg = s / (double)w;
+    for (k = 0; k + 32 <= w; k += 32) {
+      double v[32];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) v[q] = sw[k + q];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) s2 += (v[q] - avg) * (v[q] - avg);
+    }
+    for (; k < w; ++k) s2 += (sw[k] - avg) * (sw[k] - avg);
+  } else {
+    for (int k = 0; k < w; ++k) s += win[(head + k) % w];
+    avg = s / (double)w;
+    for (int k = 0; k < w; ++k) {
+      const double v = win[(head + k) % w];
+      s2 += (v - avg) * (v - avg);
+    }
+  }
+  const double ratio = sqrt(s2 / (double)(w - 1)) / avg;
+  if (isnan(ratio) || fabs(ratio) < tr->eps) tr->stop = 1;
+}
+
+// Start of a resolve: the steps run again (at least one, as the driver's loop takes one
+// before it asks the tracer), counted from here.
+__global__ void tracer_begin_kernel(FlowTracer* tr) {
+  tr->stop = 0;
+  tr->call0 = tr->steps;
 }
 
 // flag[0] = 1 when a no-dynamics voxel has a fluid neighbour among the Q - 1 of the stencil.
@@ -295,6 +380,15 @@ __global__ void flow_stats_kernel(FlowArgs<R> a, double* acc) {
 
 void launch_energy_reduce(double* energy, long long nblocks, hipStream_t s) {
   hipLaunchKernelGGL(energy_reduce_kernel, dim3(1), dim3(kEnergyThreads), 0, s, energy, nblocks);
+}
+
+void launch_tracer_step(const double* energy, long long nblocks, FlowTracer* tr, double* win, double* elog,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(tracer_step_kernel, dim3(1), dim3(kEnergyThreads), 0, s, energy, nblocks, tr, win, elog);
+}
+
+void launch_tracer_begin(FlowTracer* tr, hipStream_t s) {
+  hipLaunchKernelGGL(tracer_begin_kernel, dim3(1), dim3(1), 0, s, tr);
 }
 
 template <typename R>

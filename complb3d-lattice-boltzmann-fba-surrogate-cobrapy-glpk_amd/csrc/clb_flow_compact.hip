@@ -183,6 +183,7 @@ __device__ __forceinline__ double fc_node(const FlowArgs<R>& a, int j, uint8_t c
 // dense kernel's (per-block partials, summed by energy_reduce_kernel in a fixed order).
 template <typename R, bool ODD, bool ENERGY>
 __global__ void __launch_bounds__(256) flow_step_compact_kernel(FlowArgs<R> a) {
+  if (a.stop && *a.stop) return;  // the device tracer has converged
   const int lb = xcd_block((int)blockIdx.x, (int)gridDim.x);
   const int j = lb * blockDim.x + threadIdx.x;
   double e = 0.0, cnt = 0.0;

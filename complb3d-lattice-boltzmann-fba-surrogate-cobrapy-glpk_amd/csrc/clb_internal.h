@@ -55,6 +55,19 @@ struct FlowArgs {
   const int* fnbr;
   const unsigned int* fnf;
   long long fcap, fstride;
+  // when set and *stop != 0 the step does nothing (the device tracer has converged)
+  const int* stop;
+};
+
+// The device ValueTracer of clb_flow_resolve; its window (w doubles) is a separate array.
+struct FlowTracer {
+  double eps;
+  int w;            // window length, (int)(L / u / 2)
+  int n;            // values held
+  int head;         // slot of the oldest
+  int stop;         // 1 once converged: the steps launched after it do nothing
+  long long steps;  // energies taken since clb_flow_tracer
+  long long call0;  // steps at the start of the current resolve
 };
 
 // The pore <-> biofilm rule of updateLocalMaskNtotalLattices3D
@@ -194,6 +207,11 @@ void launch_none_reach(const uint8_t* mask, const uint8_t* kind, const Geom& g, 
 long long flow_blocks(const Geom& g, dim3 block);
 // energy[0..1] = the sums of the nblocks per-block partials, in a fixed order
 void launch_energy_reduce(double* energy, long long nblocks, hipStream_t s);
+// the device tracer after a flow step (energy: that step's partials; elog: the energies of the
+// current resolve, or null) and at the start of a resolve
+void launch_tracer_step(const double* energy, long long nblocks, FlowTracer* tr, double* win, double* elog,
+                        hipStream_t s);
+void launch_tracer_begin(FlowTracer* tr, hipStream_t s);
 template <typename R>
 void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s);
 template <typename R>

@@ -185,6 +185,18 @@ class Lattice {
     if (converged) *converged = false;
     return done;
   }
+  // The same solves with the tracer on the device (clb_flow_tracer / clb_flow_resolve): no
+  // host round trip per step. deviceTracer starts a tracer with an empty window; each
+  // flowResolveDevice feeds it (the window persists across calls, as ns_convg2's does).
+  void deviceTracer(double u, double L, double epsilon) {
+    detail::check(clb_flow_tracer(c_, u, L, epsilon), c_, "clb_flow_tracer");
+  }
+  long long flowResolveDevice(int maxSteps, bool* converged = nullptr) {
+    int steps = 0, conv = 0;
+    detail::check(clb_flow_resolve(c_, maxSteps, &steps, &conv, nullptr), c_, "clb_flow_resolve");
+    if (converged) *converged = conv != 0;
+    return steps;
+  }
   // The percolation test after a re-solve that used all its steps (src/complab.cpp:1801-1805):
   // the mean x velocity over the plane x = nx-2 above thrd.
   bool percolates(double thrd = 1e-12) const { return flowStats()[2] > thrd; }

@@ -110,6 +110,19 @@ int clb_set_solute_dynamics(clb_ctx* ctx, const int* kind /*256*/);
  *      src/complab.cpp:597-667). Density rho_in on x = 0, rho_out on x = nx-1. */
 int clb_flow_init(clb_ctx* ctx, double rho_in, double rho_out);
 int clb_flow_steps(clb_ctx* ctx, int nsteps, double* energy /* nsteps values or NULL */);
+/* The driver's flow solves with the convergence tracer on the device: the collideAndStream
+ * loops of src/complab.cpp:607-620 (ns_convg1), 1794-1800 and 1836-1840 (ns_convg2), each
+ * step's stored average energy into Palabos' util::ValueTracer(u, L, epsilon) (a window of
+ * (int)(L/u/2) values, converged once full and |stddev/mean| < epsilon, stddev with n - 1, or
+ * a NaN ratio), restated on the device. clb_flow_tracer starts a tracer with an empty window;
+ * clb_flow_resolve runs steps, at least one, until it has converged or max_steps ran, and the
+ * window persists across calls until the next clb_flow_tracer (the driver declares ns_convg2
+ * once and never resets it). *steps: steps run; *converged: 1 if the tracer ended the solve;
+ * energy: the steps' energies (max_steps values) or NULL. The steps are launched in batches
+ * without a host round trip in between (the host loop of clb_flow_steps reads the energy back
+ * after every step); the results equal that loop's with the same tracer. */
+int clb_flow_tracer(clb_ctx* ctx, double u, double L, double epsilon);
+int clb_flow_resolve(clb_ctx* ctx, int max_steps, int* steps, int* converged, double* energy /* or NULL */);
 int clb_flow_download_velocity(clb_ctx* ctx, double* u /*3N, interleaved*/, double* rho /*N or NULL*/);
 
 /* ---- D3Q7 solutes (soluteDomainSetup, src/complab_functions.hh:296-340; omega per

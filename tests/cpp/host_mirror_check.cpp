@@ -98,6 +98,25 @@ int main(int argc, char** argv) {
     write_bin(dir + "/conc.bin", lat.concentrations());
     write_bin(dir + "/bmass.bin", lat.biomass());
 
+    // ---- the flow solve with the tracer on the host and on the device: same steps, same state
+    {
+      complab_gpu::Lattice fa(nx, ny, nz, CLB_PREC_FP64), fb(nx, ny, nz, CLB_PREC_FP64);
+      for (complab_gpu::Lattice* f : {&fa, &fb}) {
+        f->setDynamics(kind, om);
+        f->uploadMask(mask);
+        f->flowInit(1.0, 0.999);
+      }
+      complab_gpu::ValueTracer tracer(1.0, 1000.0, 1e-4);
+      fb.deviceTracer(1.0, 1000.0, 1e-4);
+      for (int cap : {3000, 1}) {
+        bool ca = false, cb = false;
+        const long long na = fa.flowResolve(tracer, cap, &ca);
+        const long long nb = fb.flowResolveDevice(cap, &cb);
+        if (na != nb || ca != cb || na < 1) throw std::runtime_error("device tracer steps");
+        if (fa.velocity() != fb.velocity()) throw std::runtime_error("device tracer state");
+      }
+    }
+
     // ---- surrogate processor: uniform field at rest, constant growth 0.4 / h, no Vmax.
     // One step: every reactive voxel's biomass is B (1 + 0.4 dt / 3600); a voxel whose
     // neighbours all reacted holds C - C^2 / (Kc + C) (the supply-limited draw).

@@ -100,3 +100,55 @@ def test_clogged_outlet_does_not_percolate(clb):
     assert steps == 300 and not conv
     assert lat.percolates()
     lat.close()
+
+
+def _resolve_pair(clb, prec):
+    """Two identical lattices: one solved with the host tracer loop, one with the device
+    tracer."""
+    pack = clb.gen_sphere_pack(NX, NY, NZ, 3.0, 5.0, 0.45, seed=0)
+    kind, om = d3q19_kind_omega(0.8)
+    lats = []
+    for _ in range(2):
+        lat = clb.Lattice(NX, NY, NZ, precision=prec)
+        lat.set_dynamics(kind, om)
+        lat.upload_mask(pack)
+        lat.flow_init(1.0, 0.999)
+        lats.append(lat)
+    return pack, lats
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", [0, 1])
+def test_device_tracer_stops_where_the_host_tracer_does(clb, prec):
+    """clb_flow_resolve runs the driver's loop with the tracer on the device: the same energies,
+    the same stopping step and the same velocities as the host loop (flow_resolve with
+    clb.ValueTracer), for the initial solve, a capped solve, re-solves whose window persists
+    (ending after one step), and a fresh tracer."""
+    print(f"case: device tracer vs host tracer, 64x32x32 prec={prec}", flush=True)
+    pack, (host, dev) = _resolve_pair(clb, prec)
+    tracer = clb.ValueTracer(1.0, 1000.0, EPS)
+    dev.flow_tracer(1.0, 1000.0, EPS)
+    e_host = []
+    for max_steps in (700, 5000, 3, 1, 50):  # capped, to convergence, then re-solves
+        t0 = len(tracer.values)
+        n_h, c_h = host.flow_resolve(tracer, max_steps)
+        n_d, c_d, e_d = dev.flow_resolve_device(max_steps, energy=True)
+        assert (n_d, c_d) == (n_h, c_h)
+        assert n_d >= 1
+        if max_steps == 5000:
+            assert c_d and n_d < 5000
+        assert np.array_equal(host.velocity()[0], dev.velocity()[0])
+        e_host.append(e_d)
+    # a re-solve with a full window that still meets the threshold ends after one step
+    assert len(e_host[2]) == 1 and len(e_host[3]) == 1
+    # a fresh tracer: an empty window again
+    tracer = clb.ValueTracer(1.0, 1000.0, 1e-6)
+    dev.flow_tracer(1.0, 1000.0, 1e-6)
+    n_h, c_h = host.flow_resolve(tracer, 800)
+    n_d, c_d = dev.flow_resolve_device(800)
+    assert (n_d, c_d) == (n_h, c_h) and n_d >= 500
+    assert np.array_equal(host.velocity()[0], dev.velocity()[0])
+    with pytest.raises(clb.ClbError):
+        dev.flow_tracer(1.0, 2.0, 1e-4)  # a window of one value
+    host.close()
+    dev.close()
