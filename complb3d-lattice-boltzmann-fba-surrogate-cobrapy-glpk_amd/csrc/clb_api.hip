@@ -536,8 +536,8 @@ int flow_resolve_t(clb_ctx* c, int max_steps, int* steps_out, int* conv_out, dou
       const bool odd = ((c->flow_t + i) & 1) != 0;
       {
         Timed tm(c, 0);
-        if (cmp) launch_flow_step_compact<R>(a, odd, bt, c->stream);
-        else launch_flow_step<R>(a, odd, c->flow_block, c->stream);
+        if (cmp) launch_flow_step_compact<R>(a, odd, bt, c->stream, false);
+        else launch_flow_step<R>(a, odd, c->flow_block, c->stream, false);
       }
       count_launch(c, 0);
       launch_tracer_step(c->d_energy, nblk, c->d_tr, c->d_trw, elog, c->stream);

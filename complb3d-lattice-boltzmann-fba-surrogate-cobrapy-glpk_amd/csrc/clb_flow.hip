@@ -415,12 +415,12 @@ void launch_none_reach(const uint8_t* mask, const uint8_t* kind, const Geom& g, 
 }
 
 template <typename R>
-void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s) {
+void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s, bool reduce) {
   dim3 grid((a.g.nx + block.x - 1) / block.x, (a.g.ny + block.y - 1) / block.y, a.g.nz);
   if (a.energy) {
     if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, true>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((flow_step_kernel<R, false, true>), grid, block, 0, s, a);
-    launch_energy_reduce(a.energy, (long long)grid.x * grid.y * grid.z, s);
+    if (reduce) launch_energy_reduce(a.energy, (long long)grid.x * grid.y * grid.z, s);
   } else {
     if (odd) hipLaunchKernelGGL((flow_step_kernel<R, true, false>), grid, block, 0, s, a);
     else hipLaunchKernelGGL((flow_step_kernel<R, false, false>), grid, block, 0, s, a);
@@ -441,8 +441,8 @@ long long flow_blocks(const Geom& g, dim3 block) {
   return (long long)((g.nx + block.x - 1) / block.x) * ((g.ny + block.y - 1) / block.y) * g.nz;
 }
 
-template void launch_flow_step<float>(const FlowArgs<float>&, bool, dim3, hipStream_t);
-template void launch_flow_step<double>(const FlowArgs<double>&, bool, dim3, hipStream_t);
+template void launch_flow_step<float>(const FlowArgs<float>&, bool, dim3, hipStream_t, bool);
+template void launch_flow_step<double>(const FlowArgs<double>&, bool, dim3, hipStream_t, bool);
 template void launch_flow_velocity<float>(const FlowArgs<float>&, bool, double*, double*, float*, hipStream_t);
 template void launch_flow_velocity<double>(const FlowArgs<double>&, bool, double*, double*, double*, hipStream_t);
 

@@ -295,12 +295,12 @@ void launch_flow_compact_move(bool to_compact, const FlowArgs<R>& a, R* f_dense,
 long long flow_compact_blocks(long long fcap, int bt) { return (fcap + bt - 1) / bt; }
 
 template <typename R>
-void launch_flow_step_compact(const FlowArgs<R>& a, bool odd, int bt, hipStream_t s) {
+void launch_flow_step_compact(const FlowArgs<R>& a, bool odd, int bt, hipStream_t s, bool reduce) {
   const long long blocks = flow_compact_blocks(a.fcap, bt);
   if (a.energy) {
     if (odd) hipLaunchKernelGGL((flow_step_compact_kernel<R, true, true>), dim3((unsigned)blocks), dim3(bt), 0, s, a);
     else hipLaunchKernelGGL((flow_step_compact_kernel<R, false, true>), dim3((unsigned)blocks), dim3(bt), 0, s, a);
-    launch_energy_reduce(a.energy, blocks, s);
+    if (reduce) launch_energy_reduce(a.energy, blocks, s);
   } else {
     if (odd) hipLaunchKernelGGL((flow_step_compact_kernel<R, true, false>), dim3((unsigned)blocks), dim3(bt), 0, s, a);
     else hipLaunchKernelGGL((flow_step_compact_kernel<R, false, false>), dim3((unsigned)blocks), dim3(bt), 0, s, a);
@@ -316,8 +316,8 @@ void launch_flow_velocity_compact(const FlowArgs<R>& a, bool odd_next, R* ade_u,
 
 template void launch_flow_compact_move<float>(bool, const FlowArgs<float>&, float*, hipStream_t);
 template void launch_flow_compact_move<double>(bool, const FlowArgs<double>&, double*, hipStream_t);
-template void launch_flow_step_compact<float>(const FlowArgs<float>&, bool, int, hipStream_t);
-template void launch_flow_step_compact<double>(const FlowArgs<double>&, bool, int, hipStream_t);
+template void launch_flow_step_compact<float>(const FlowArgs<float>&, bool, int, hipStream_t, bool);
+template void launch_flow_step_compact<double>(const FlowArgs<double>&, bool, int, hipStream_t, bool);
 template void launch_flow_velocity_compact<float>(const FlowArgs<float>&, bool, float*, hipStream_t);
 template void launch_flow_velocity_compact<double>(const FlowArgs<double>&, bool, double*, hipStream_t);
 

@@ -212,8 +212,10 @@ void launch_energy_reduce(double* energy, long long nblocks, hipStream_t s);
 void launch_tracer_step(const double* energy, long long nblocks, FlowTracer* tr, double* win, double* elog,
                         hipStream_t s);
 void launch_tracer_begin(FlowTracer* tr, hipStream_t s);
+// with a.energy: the per-block partials, summed into energy[0..1] when reduce (the device tracer
+// sums them itself)
 template <typename R>
-void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s);
+void launch_flow_step(const FlowArgs<R>& a, bool odd, dim3 block, hipStream_t s, bool reduce = true);
 template <typename R>
 void launch_flow_velocity(const FlowArgs<R>& a, bool odd_next, double* u, double* rho, R* ade_u, hipStream_t s);
 
@@ -303,7 +305,7 @@ void launch_flow_compact_move(bool to_compact, const FlowArgs<R>& a, R* f_dense,
 // per-block energy partials of a compact launch of bt threads
 long long flow_compact_blocks(long long fcap, int bt);
 template <typename R>
-void launch_flow_step_compact(const FlowArgs<R>& a, bool odd, int bt, hipStream_t s);
+void launch_flow_step_compact(const FlowArgs<R>& a, bool odd, int bt, hipStream_t s, bool reduce = true);
 // the advecting velocity of the solutes (latticeToPassiveAdvDiff) from the compact flow
 // state: every node of the table writes ade_u at its voxel (the caller zeroes the rest)
 template <typename R>
