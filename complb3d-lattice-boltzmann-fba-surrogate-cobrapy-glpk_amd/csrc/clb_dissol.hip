@@ -161,7 +161,7 @@ void launch_dissol_shape(const AdeArgs<R>& a, const DissolArgs& d, const Coupled
 
 template <typename R>
 void launch_dissolution(const AdeArgs<R>& a, const DissolArgs& d, bool odd_done, hipStream_t s) {
-  CoupledIO io;
+  CoupledIO io{};  // value-initialized: fields a launcher does not set are null / zero
   io.g = (void*)a.g;
   io.mask = a.mask;
   io.list = nullptr;

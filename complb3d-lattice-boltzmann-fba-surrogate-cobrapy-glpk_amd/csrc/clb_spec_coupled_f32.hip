@@ -6,7 +6,7 @@ namespace clb {
 
 void launch_spec_coupled(const AdeArgs<float>& a, bool odd_done, const Tableau& tb, SpecArgs sa, int grid,
                              hipStream_t s) {
-  CoupledIO io;
+  CoupledIO io{};  // value-initialized: fields a launcher does not set are null / zero
   io.g = (void*)a.g;
   io.mask = a.mask;
   io.list = (const int*)sa.iters;  // node list (reactive voxels, or the first pass's failures)
