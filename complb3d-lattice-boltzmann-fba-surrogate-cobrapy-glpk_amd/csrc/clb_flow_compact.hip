@@ -150,16 +150,18 @@ __device__ __forceinline__ void fc_store19(const FlowArgs<R>& a, int j, const in
   const long long st = a.fstride;
 #pragma unroll
   for (int i = 0; i < 19; ++i) {
-    // non-temporal, as the dense kernel's: the next step reads the line long after this one
+    // plain stores: with the blocks in runs per XCD, neighbouring wavefronts' stores into the
+    // same lines meet in one L2 (non-temporal stores, which the dense kernel keeps, measured
+    // 0.236 against 0.233 ms here)
     if (!ODD) {
-      __builtin_nontemporal_store(p[i], &a.f[(long long)opp_dir(i) * st + j]);
+      a.f[(long long)opp_dir(i) * st + j] = p[i];
     } else if (i == 0) {
-      __builtin_nontemporal_store(p[0], &a.f[j]);
+      a.f[j] = p[0];
     } else {
       const int dst = L[i];
       if (!XCHK || dst >= 0) {
         const R v = (Q19::cx[i] != 0) ? fma(R(-Q19::w(i) * Q19::cx[i]), a.rdx, p[i]) : p[i];
-        __builtin_nontemporal_store(v, &a.f[(long long)i * st + dst]);
+        a.f[(long long)i * st + dst] = v;
       }
     }
   }
