@@ -56,8 +56,8 @@ FLOP_SRG_NET = 2 * 2 + 2 * (2 * 10 + 3 * 100) + 4 * 10 * (20 + 4) + 2 * 10 + 4
 # Reported only when the workload is the default one.
 MEASURED_TRAFFIC_SOURCE = "profiles/hbm_traffic_full_coupled_384x192x192_kinetics_256x128x128.md"
 MEASURED_TRAFFIC = {
-    "full_coupled_384x192x192": (3.301e9 + 2.766e9) / 2,
-    "flow_384x192x192": (1.583e9 + 0.826e9) / 2,
+    "full_coupled_384x192x192": (3.301e9 + 2.773e9) / 2,
+    "flow_384x192x192": (1.533e9 + 0.826e9) / 2,
     "kinetics_256x128x128": (0.699e9 + 0.548e9) / 2,
 }
 
